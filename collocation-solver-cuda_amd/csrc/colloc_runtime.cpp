@@ -1,0 +1,687 @@
+// libcolloc.so -- host runtime of the MI355X collocation solver.
+//
+// Owns the device workspace of one problem instance, loads the problem's
+// hiprtc-compiled code object and sequences the kernels of one damped Newton
+// iteration on a single HIP stream.  The C ABI is declared in include/colloc.h.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "colloc.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) return fail(CS_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+struct Seg4 {
+  long long off[4];
+  long long len[4];
+};
+
+enum Fn {
+  F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_COND, F_CR_PAIR, F_CR_FINAL, F_CR_BACK, F_RECOVER,
+  F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
+};
+const char* kFnNames[F_COUNT] = {
+  "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian",
+  "colloc_condense", "colloc_cr_pair", "colloc_cr_final", "colloc_cr_back", "colloc_recover",
+  "colloc_axpy4", "colloc_init_input", "colloc_segres", "colloc_recover_z"};
+
+}  // namespace
+
+struct cs_handle {
+  int ny, nz, np, m, n_max, nc;
+  int N = 0;
+  int init_N = 0;   // mesh size of the saved initial input (y0buf)
+  // launch shapes read from the module
+  int cw = 0, fw = 0;
+  hipStream_t stream = nullptr;
+  hipModule_t module = nullptr;
+  hipFunction_t fn[F_COUNT] = {};
+  // sizes derived from dims
+  int nj, nv, ns, recn, crw, recw, jstage;
+  size_t st_y, st_yd, st_z, st_len;   // packed state offsets: [y | ydot | ztil | p]
+  // device buffers
+  double* t = nullptr;
+  double* state = nullptr;   // current state
+  double* trial = nullptr;   // line-search trial state
+  double* dir = nullptr;     // Newton direction (same packing)
+  double* c = nullptr;
+  double* ytil = nullptr;
+  double* fa = nullptr;
+  double* fb = nullptr;
+  double* rbc = nullptr;
+  double* jac = nullptr;
+  double* Dr = nullptr;
+  double* rows0 = nullptr;
+  double* rows1 = nullptr;
+  double* rec = nullptr;
+  double* crrec = nullptr;
+  double* segres = nullptr;
+  double* zbuf = nullptr;
+  double* y0buf = nullptr;
+  unsigned long long* scal = nullptr;  // [0]: norm, [1]: max residual
+  int* status = nullptr;
+  // profiling
+  bool prof = false;
+  hipEvent_t ev0[CS_KERNEL_COUNT][2] = {};
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[CS_KERNEL_COUNT];
+  double prof_ms[CS_KERNEL_COUNT] = {};
+  int prof_n[CS_KERNEL_COUNT] = {};
+  std::vector<hipEvent_t> pool;
+
+  double* y() const { return state + st_y; }
+  double* yd() const { return state + st_yd; }
+  double* zt() const { return state + st_z; }
+  double* p() const { return state + st_len - np; }
+};
+
+namespace {
+
+hipEvent_t take_event(cs_handle* h) {
+  if (!h->pool.empty()) {
+    hipEvent_t e = h->pool.back();
+    h->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  return e;
+}
+
+struct ProfScope {
+  cs_handle* h;
+  int k;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(cs_handle* h_, int k_) : h(h_), k(k_) {
+    if (h->prof) {
+      a = take_event(h);
+      b = take_event(h);
+      hipEventRecord(a, h->stream);
+    }
+  }
+  ~ProfScope() {
+    if (a) {
+      hipEventRecord(b, h->stream);
+      h->pending[k].push_back({a, b});
+    }
+  }
+};
+
+void drain_prof(cs_handle* h) {
+  for (int k = 0; k < CS_KERNEL_COUNT; ++k) {
+    for (auto& pr : h->pending[k]) {
+      float ms = 0.f;
+      hipEventSynchronize(pr.second);
+      hipEventElapsedTime(&ms, pr.first, pr.second);
+      h->prof_ms[k] += ms;
+      h->prof_n[k] += 1;
+      h->pool.push_back(pr.first);
+      h->pool.push_back(pr.second);
+    }
+    h->pending[k].clear();
+  }
+}
+
+int launch(cs_handle* h, Fn f, unsigned grid, unsigned block, void** args) {
+  if (!h->module) return fail(CS_E_NOMODULE, "no problem module loaded");
+  if (grid == 0) return CS_OK;
+  HIPCHK(hipModuleLaunchKernel(h->fn[f], grid, 1, 1, block, 1, 1, 0, h->stream, args, nullptr));
+  return CS_OK;
+}
+
+#define TRY(expr)              \
+  do {                         \
+    int _r = (expr);           \
+    if (_r != CS_OK) return _r; \
+  } while (0)
+
+int dmalloc(double** p, size_t n) {
+  HIPCHK(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(double)));
+  return CS_OK;
+}
+
+int fill_scalars(cs_handle* h) {
+  unsigned long long zero = 0ull;
+  int n = 2;
+  void* args[] = {&h->scal, &zero, &n};
+  return launch(h, F_FILL, 1, 64, args);
+}
+
+int read_u64_double(cs_handle* h, int slot, double* out) {
+  unsigned long long v = 0;
+  HIPCHK(hipMemcpyAsync(&v, h->scal + slot, sizeof v, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::memcpy(out, &v, sizeof v);
+  return CS_OK;
+}
+
+int need_mesh(cs_handle* h) {
+  if (h->N < 2) return fail(CS_E_ARG, "mesh not set");
+  return CS_OK;
+}
+
+// Residual of a packed state; writes f_a/f_b/y_tilde/r_bc when full.
+int residual(cs_handle* h, const double* st, double alpha, bool full, double* norm) {
+  TRY(need_mesh(h));
+  ProfScope ps(h, CS_KERNEL_RESIDUAL);
+  TRY(fill_scalars(h));
+  int nint = h->N - 1;
+  const double* y = st + h->st_y;
+  const double* yd = st + h->st_yd;
+  const double* zt = st + h->st_z;
+  const double* p = st + h->st_len - h->np;
+  unsigned long long* slot = h->scal;
+  void* args[] = {&nint, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &h->ytil, &h->fa, &h->fb, &h->rbc, &slot};
+  unsigned nthreads = (unsigned)(nint * h->m + 1);
+  TRY(launch(h, full ? F_RES_FULL : F_RES_NORM, (nthreads + 255) / 256, 256, args));
+  return read_u64_double(h, 0, norm);
+}
+
+int jacobian(cs_handle* h, double alpha) {
+  ProfScope ps(h, CS_KERNEL_JACOBIAN);
+  int nint = h->N - 1;
+  double* y = h->y();
+  double* p = h->p();
+  double* zt = h->zt();
+  void* args[] = {&nint, &y, &h->ytil, &zt, &p, &h->c, &alpha, &h->jac, &h->Dr};
+  unsigned nthreads = (unsigned)(nint * h->m + 1);
+  return launch(h, F_JAC, (nthreads + 255) / 256, 256, args);
+}
+
+int condense(cs_handle* h) {
+  ProfScope ps(h, CS_KERNEL_CONDENSE);
+  int nint = h->N - 1;
+  void* args[] = {&nint, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status};
+  return launch(h, F_COND, (unsigned)nint, (unsigned)(h->cw * 64), args);
+}
+
+int babd(cs_handle* h) {
+  ProfScope ps(h, CS_KERNEL_BABD);
+  int nint = h->N - 1;
+  double* dy = h->dir + h->st_y;
+  double* dp = h->dir + h->st_len - h->np;
+  // forward: cyclic reduction levels
+  std::vector<int> lev_rows;
+  std::vector<size_t> lev_base;
+  int nrows = nint;
+  size_t base = 0;
+  double* rin = h->rows0;
+  double* rout = h->rows1;
+  unsigned pair_block = (unsigned)((h->recw + 63) / 64 * 64);
+  while (nrows > 1) {
+    int npairs = nrows / 2;
+    double* recs = h->crrec + base * (size_t)h->ny * h->recw;
+    void* args[] = {&nrows, &rin, &rout, &recs};
+    TRY(launch(h, F_CR_PAIR, (unsigned)(npairs + (nrows & 1)), pair_block, args));
+    lev_rows.push_back(nrows);
+    lev_base.push_back(base);
+    base += npairs;
+    nrows = npairs + (nrows & 1);
+    std::swap(rin, rout);
+  }
+  {
+    void* args[] = {&nint, &rin, &h->Dr, &h->rbc, &dy, &dp, &h->status};
+    TRY(launch(h, F_CR_FINAL, 1, (unsigned)(h->fw * 64), args));
+  }
+  for (int l = (int)lev_rows.size() - 1; l >= 0; --l) {
+    int nr = lev_rows[l];
+    double* recs = h->crrec + lev_base[l] * (size_t)h->ny * h->recw;
+    void* args[] = {&l, &nr, &nint, &recs, &dy, &dp};
+    TRY(launch(h, F_CR_BACK, (unsigned)(nr / 2), 64, args));
+  }
+  return CS_OK;
+}
+
+int recover(cs_handle* h) {
+  ProfScope ps(h, CS_KERNEL_RECOVER);
+  int nint = h->N - 1;
+  double* dy = h->dir + h->st_y;
+  double* dp = h->dir + h->st_len - h->np;
+  double* dyd = h->dir + h->st_yd;
+  double* dz = h->dir + h->st_z;
+  void* args[] = {&nint, &h->rec, &dy, &dp, &dyd, &dz};
+  return launch(h, F_RECOVER, (unsigned)nint, 256, args);
+}
+
+Seg4 state_segs(cs_handle* h) {
+  Seg4 s;
+  int nint = h->N - 1;
+  s.off[0] = (long long)h->st_y;  s.len[0] = (long long)h->N * h->ny;
+  s.off[1] = (long long)h->st_yd; s.len[1] = (long long)nint * h->m * h->ny;
+  s.off[2] = (long long)h->st_z;  s.len[2] = (long long)nint * h->m * h->nz;
+  s.off[3] = (long long)(h->st_len - h->np); s.len[3] = h->np;
+  return s;
+}
+
+int axpy_state(cs_handle* h, const double* x, const double* d, double a, double* out) {
+  Seg4 s = state_segs(h);
+  long long total = s.len[0] + s.len[1] + s.len[2] + s.len[3];
+  void* args[] = {&s, &x, &d, &a, &out};
+  return launch(h, F_AXPY, (unsigned)((total + 255) / 256), 256, args);
+}
+
+int copy_out(cs_handle* h, double* host, const double* dev, size_t n) {
+  if (!host || n == 0) return CS_OK;
+  HIPCHK(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  return CS_OK;
+}
+
+int copy_in(cs_handle* h, double* dev, const double* host, size_t n) {
+  if (!host || n == 0) return CS_OK;
+  HIPCHK(hipMemcpyAsync(dev, host, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  return CS_OK;
+}
+
+int sync(cs_handle* h) {
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+int init_from_saved(cs_handle* h);
+
+}  // namespace
+
+extern "C" {
+
+const char* cs_last_error(void) { return g_err.c_str(); }
+
+const char* cs_version(void) { return "colloc-mi355x 0.1"; }
+
+void cs_free(void* ptr) { std::free(ptr); }
+
+int cs_jit_compile(const char* source, const char* name, const char* include_dir, void** code, size_t* size,
+                   char** log) {
+  if (!source || !code || !size) return fail(CS_E_ARG, "null argument");
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, source, name ? name : "colloc_problem.hip", 0, nullptr, nullptr) !=
+      HIPRTC_SUCCESS)
+    return fail(CS_E_COMPILE, "hiprtcCreateProgram failed");
+  std::string inc = std::string("-I") + (include_dir ? include_dir : ".");
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", inc.c_str(),
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+  size_t lsz = 0;
+  hiprtcGetProgramLogSize(prog, &lsz);
+  std::string lg(lsz + 1, '\0');
+  if (lsz) hiprtcGetProgramLog(prog, &lg[0]);
+  if (log) {
+    *log = (char*)std::malloc(lg.size() + 1);
+    std::memcpy(*log, lg.c_str(), lg.size() + 1);
+  }
+  if (rc != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    return fail(CS_E_COMPILE, "hiprtc compile failed: %.900s", lg.c_str());
+  }
+  size_t csz = 0;
+  hiprtcGetCodeSize(prog, &csz);
+  void* buf = std::malloc(csz);
+  hiprtcGetCode(prog, (char*)buf);
+  hiprtcDestroyProgram(&prog);
+  *code = buf;
+  *size = csz;
+  return CS_OK;
+}
+
+int cs_create(const cs_dims* d, cs_handle** out) {
+  if (!d || !out) return fail(CS_E_ARG, "null argument");
+  if (d->ny < 1 || d->nz < 0 || d->np < 0 || d->m < 2 || d->n_max < 2 || d->nc < 0)
+    return fail(CS_E_SHAPE, "invalid dims ny=%d nz=%d np=%d m=%d n_max=%d", d->ny, d->nz, d->np, d->m, d->n_max);
+  cs_handle* h = new cs_handle();
+  h->ny = d->ny; h->nz = d->nz; h->np = d->np; h->m = d->m; h->n_max = d->n_max; h->nc = d->nc;
+  const int ny = h->ny, nz = h->nz, np = h->np, m = h->m;
+  h->nj = ny + nz + np;
+  h->nv = ny + np + 1;
+  h->ns = (m - 1) * ny;
+  h->recn = (h->ns + m * ny + m * nz) * h->nv;
+  h->crw = 2 * ny + np + 1;
+  h->recw = 3 * ny + np + 1;
+  h->jstage = (ny + nz) * h->nj;
+  const size_t nmax = (size_t)h->n_max, nint = nmax - 1;
+  h->st_y = 0;
+  h->st_yd = nmax * ny;
+  h->st_z = h->st_yd + nint * m * ny;
+  h->st_len = h->st_z + nint * m * nz + np;
+  int r = CS_OK;
+  auto A = [&](double** p, size_t n) { if (r == CS_OK) r = dmalloc(p, n); };
+  A(&h->t, nmax);
+  A(&h->state, h->st_len);
+  A(&h->trial, h->st_len);
+  A(&h->dir, h->st_len);
+  A(&h->c, (size_t)std::max(h->nc, 1));
+  A(&h->ytil, nint * m * ny);
+  A(&h->fa, nint * m * (ny + nz));
+  A(&h->fb, nint * ny);
+  A(&h->rbc, (size_t)(ny + np));
+  A(&h->jac, nint * m * (size_t)h->jstage);
+  A(&h->Dr, (size_t)(ny + np) * (2 * ny + np));
+  A(&h->rows0, nint * ny * (size_t)h->crw);
+  A(&h->rows1, (nint / 2 + 1) * ny * (size_t)h->crw);
+  A(&h->rec, nint * (size_t)h->recn);
+  A(&h->crrec, nint * ny * (size_t)h->recw);
+  A(&h->segres, nmax);
+  A(&h->zbuf, nmax * std::max(nz, 1));
+  A(&h->y0buf, nmax * (ny + std::max(nz, 1)) + np);
+  if (r == CS_OK) {
+    hipError_t e = hipMalloc((void**)&h->scal, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&h->status, sizeof(int));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemset(h->status, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(h->c, 0, sizeof(double) * std::max(h->nc, 1));
+    if (e != hipSuccess) r = fail(CS_E_HIP, "allocation failed: %s", hipGetErrorString(e));
+  }
+  if (r != CS_OK) {
+    std::string msg = g_err;
+    cs_destroy(h);
+    g_err = msg;
+    return r;
+  }
+  *out = h;
+  return CS_OK;
+}
+
+void cs_destroy(cs_handle* h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  double* bufs[] = {h->t, h->state, h->trial, h->dir, h->c, h->ytil, h->fa, h->fb, h->rbc, h->jac, h->Dr,
+                    h->rows0, h->rows1, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf};
+  for (double* b : bufs)
+    if (b) hipFree(b);
+  if (h->scal) hipFree(h->scal);
+  if (h->status) hipFree(h->status);
+  for (int k = 0; k < CS_KERNEL_COUNT; ++k)
+    for (auto& pr : h->pending[k]) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
+  for (hipEvent_t e : h->pool) hipEventDestroy(e);
+  if (h->module) hipModuleUnload(h->module);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int cs_load_module(cs_handle* h, const void* code, size_t size) {
+  if (!h || !code || size == 0) return fail(CS_E_ARG, "null argument");
+  if (h->module) {
+    hipModuleUnload(h->module);
+    h->module = nullptr;
+  }
+  HIPCHK(hipModuleLoadData(&h->module, code));
+  for (int f = 0; f < F_COUNT; ++f) {
+    hipError_t e = hipModuleGetFunction(&h->fn[f], h->module, kFnNames[f]);
+    if (e != hipSuccess) return fail(CS_E_HIP, "kernel %s missing: %s", kFnNames[f], hipGetErrorString(e));
+  }
+  // module metadata: sizes and launch shapes baked into the code object
+  hipDeviceptr_t gp;
+  size_t gsz;
+  HIPCHK(hipModuleGetGlobal(&gp, &gsz, h->module, "colloc_meta"));
+  int meta[8] = {};
+  HIPCHK(hipMemcpy(meta, (void*)gp, sizeof meta, hipMemcpyDeviceToHost));
+  if (meta[0] != h->ny || meta[1] != h->nz || meta[2] != h->np || meta[3] != h->m || meta[4] != h->nc)
+    return fail(CS_E_SHAPE, "module sizes (%d,%d,%d,%d,%d) differ from handle (%d,%d,%d,%d,%d)", meta[0], meta[1],
+                meta[2], meta[3], meta[4], h->ny, h->nz, h->np, h->m, h->nc);
+  h->cw = meta[5];
+  h->fw = meta[6];
+  return CS_OK;
+}
+
+int cs_set_mesh(cs_handle* h, const double* t, int N) {
+  if (!h || !t) return fail(CS_E_ARG, "null argument");
+  if (N < 2 || N > h->n_max) return fail(CS_E_SHAPE, "N=%d outside [2, %d]", N, h->n_max);
+  h->N = N;
+  TRY(copy_in(h, h->t, t, (size_t)N));
+  return sync(h);
+}
+
+int cs_set_constants(cs_handle* h, const double* c, int nc) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (nc != h->nc) return fail(CS_E_SHAPE, "nc=%d, handle has %d", nc, h->nc);
+  TRY(copy_in(h, h->c, c, (size_t)nc));
+  return sync(h);
+}
+
+int cs_set_state(cs_handle* h, const double* y, const double* y_dot, const double* z_tilde, const double* p) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  const size_t nint = h->N - 1;
+  TRY(copy_in(h, h->y(), y, (size_t)h->N * h->ny));
+  TRY(copy_in(h, h->yd(), y_dot, nint * h->m * h->ny));
+  TRY(copy_in(h, h->zt(), z_tilde, nint * h->m * h->nz));
+  TRY(copy_in(h, h->p(), p, (size_t)h->np));
+  return sync(h);
+}
+
+int cs_get_state(cs_handle* h, double* y, double* y_dot, double* z_tilde, double* p) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  const size_t nint = h->N - 1;
+  TRY(copy_out(h, y, h->y(), (size_t)h->N * h->ny));
+  TRY(copy_out(h, y_dot, h->yd(), nint * h->m * h->ny));
+  TRY(copy_out(h, z_tilde, h->zt(), nint * h->m * h->nz));
+  TRY(copy_out(h, p, h->p(), (size_t)h->np));
+  return sync(h);
+}
+
+int cs_form_initial_input(cs_handle* h, const double* y0, const double* z0, const double* p0) {
+  if (!h || !y0 || (!z0 && h->nz) || (!p0 && h->np)) return fail(CS_E_ARG, "null argument");
+  TRY(need_mesh(h));
+  const size_t N = h->N;
+  TRY(copy_in(h, h->y0buf, y0, N * h->ny));
+  TRY(copy_in(h, h->y0buf + N * h->ny, z0, N * h->nz));
+  TRY(copy_in(h, h->y0buf + N * (h->ny + h->nz), p0, (size_t)h->np));
+  h->init_N = h->N;
+  return init_from_saved(h);
+}
+
+int cs_restart(cs_handle* h) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (h->init_N != h->N || h->N < 2) return fail(CS_E_ARG, "no initial input for the current mesh");
+  return init_from_saved(h);
+}
+
+}  // extern "C"
+
+namespace {
+int init_from_saved(cs_handle* h) {
+  const size_t N = h->N;
+  double* zb = h->y0buf + N * h->ny;
+  HIPCHK(hipMemcpyAsync(h->y(), h->y0buf, N * h->ny * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  if (h->np)
+    HIPCHK(hipMemcpyAsync(h->p(), h->y0buf + N * (h->ny + h->nz), h->np * sizeof(double),
+                          hipMemcpyDeviceToDevice, h->stream));
+  int nint = h->N - 1;
+  double* y = h->y();
+  double* p = h->p();
+  double* yd = h->yd();
+  double* zt = h->zt();
+  void* args[] = {&nint, &y, &zb, &p, &h->c, &yd, &zt};
+  TRY(launch(h, F_INIT, (unsigned)((nint * h->m + 255) / 256), 256, args));
+  return sync(h);
+}
+}  // namespace
+
+extern "C" {
+
+int cs_compute_f_q(cs_handle* h, double alpha, double* norm_inf, double* y_tilde, double* f_a, double* f_b,
+                   double* r_bc) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  double nrm = 0.0;
+  TRY(residual(h, h->state, alpha, true, &nrm));
+  if (norm_inf) *norm_inf = nrm;
+  const size_t nint = h->N - 1;
+  TRY(copy_out(h, y_tilde, h->ytil, nint * h->m * h->ny));
+  TRY(copy_out(h, f_a, h->fa, nint * h->m * (h->ny + h->nz)));
+  TRY(copy_out(h, f_b, h->fb, nint * h->ny));
+  TRY(copy_out(h, r_bc, h->rbc, (size_t)(h->ny + h->np)));
+  return sync(h);
+}
+
+int cs_construct_jacobian(cs_handle* h, double alpha, double* Df, double* Dg, double* Dr) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  TRY(jacobian(h, alpha));
+  const size_t nst = (size_t)(h->N - 1) * h->m;
+  const size_t nf = (size_t)h->ny * h->nj, ng = (size_t)h->nz * h->nj;
+  if (Df || Dg) {
+    std::vector<double> buf(nst * h->jstage);
+    TRY(copy_out(h, buf.data(), h->jac, buf.size()));
+    TRY(sync(h));
+    for (size_t s = 0; s < nst; ++s) {
+      if (Df) std::memcpy(Df + s * nf, buf.data() + s * h->jstage, nf * sizeof(double));
+      if (Dg && ng) std::memcpy(Dg + s * ng, buf.data() + s * h->jstage + nf, ng * sizeof(double));
+    }
+  }
+  TRY(copy_out(h, Dr, h->Dr, (size_t)(h->ny + h->np) * (2 * h->ny + h->np)));
+  return sync(h);
+}
+
+int cs_reduce_jacobian(cs_handle* h, double* A, double* H, double* b) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  TRY(condense(h));
+  if (A || H || b) {
+    const size_t nint = h->N - 1;
+    std::vector<double> buf(nint * h->ny * h->crw);
+    TRY(copy_out(h, buf.data(), h->rows0, buf.size()));
+    TRY(sync(h));
+    for (size_t i = 0; i < nint; ++i)
+      for (int r = 0; r < h->ny; ++r) {
+        const double* row = buf.data() + (i * h->ny + r) * h->crw;
+        if (A) std::memcpy(A + (i * h->ny + r) * h->ny, row, h->ny * sizeof(double));
+        if (H && h->np) std::memcpy(H + (i * h->ny + r) * h->np, row + 2 * h->ny, h->np * sizeof(double));
+        if (b) b[i * h->ny + r] = row[h->crw - 1];
+      }
+  }
+  return sync(h);
+}
+
+int cs_solve_babd(cs_handle* h, double* delta_y, double* delta_p) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  TRY(babd(h));
+  TRY(copy_out(h, delta_y, h->dir + h->st_y, (size_t)h->N * h->ny));
+  TRY(copy_out(h, delta_p, h->dir + h->st_len - h->np, (size_t)h->np));
+  return sync(h);
+}
+
+int cs_recover_delta_k(cs_handle* h, double* delta_y_dot, double* delta_z_tilde) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  TRY(recover(h));
+  const size_t nint = h->N - 1;
+  TRY(copy_out(h, delta_y_dot, h->dir + h->st_yd, nint * h->m * h->ny));
+  TRY(copy_out(h, delta_z_tilde, h->dir + h->st_z, nint * h->m * h->nz));
+  return sync(h);
+}
+
+int cs_newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time, double* norm, double* norms,
+              int hist_len) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  if (max_iter < 1) return fail(CS_E_ARG, "max_iter must be >= 1");
+  int it = 0;
+  double nrm = 0.0;
+  for (it = 0; it < max_iter; ++it) {
+    TRY(residual(h, h->state, alpha, true, &nrm));
+    if (norms && it < hist_len) norms[it] = nrm;
+    if (nrm < tol) break;
+    TRY(jacobian(h, alpha));
+    TRY(condense(h));
+    TRY(babd(h));
+    TRY(recover(h));
+    double a0 = 1.0;
+    for (int ls = 0; ls < 20; ++ls) {
+      TRY(axpy_state(h, h->state, h->dir, a0, h->trial));
+      double nn = 0.0;
+      TRY(residual(h, h->trial, alpha, false, &nn));
+      if (nn < nrm) {
+        std::swap(h->state, h->trial);
+        break;
+      }
+      a0 *= 0.5;
+    }
+  }
+  if (it == max_iter) it = max_iter - 1;   // the reference's loop variable stops at max_iter - 1
+  if (iter_time) *iter_time = it;
+  if (norm) *norm = nrm;
+  return sync(h);
+}
+
+int cs_segment_residual(cs_handle* h, double alpha, double tol, double* residual_out, double* max_residual) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  TRY(fill_scalars(h));
+  int nint = h->N - 1;
+  double* y = h->y();
+  double* yd = h->yd();
+  double* zt = h->zt();
+  double* p = h->p();
+  unsigned long long* slot = h->scal + 1;
+  void* args[] = {&nint, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &tol, &h->segres, &slot};
+  TRY(launch(h, F_SEGRES, (unsigned)((nint + 1 + 255) / 256), 256, args));
+  double mr = 0.0;
+  TRY(read_u64_double(h, 1, &mr));
+  if (max_residual) *max_residual = mr;
+  TRY(copy_out(h, residual_out, h->segres, (size_t)h->N));
+  return sync(h);
+}
+
+int cs_recover_solution(cs_handle* h, double* z) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  int n = h->N;
+  double* zt = h->zt();
+  void* args[] = {&n, &zt, &h->zbuf};
+  TRY(launch(h, F_RECZ, (unsigned)((n + 255) / 256), 256, args));
+  TRY(copy_out(h, z, h->zbuf, (size_t)n * h->nz));
+  return sync(h);
+}
+
+int cs_profile(cs_handle* h, int enable) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(sync(h));
+  drain_prof(h);
+  for (int k = 0; k < CS_KERNEL_COUNT; ++k) { h->prof_ms[k] = 0.0; h->prof_n[k] = 0; }
+  h->prof = enable != 0;
+  return CS_OK;
+}
+
+int cs_profile_get(cs_handle* h, int kernel, double* total_ms, int* launches) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (kernel < 0 || kernel >= CS_KERNEL_COUNT) return fail(CS_E_ARG, "bad kernel id %d", kernel);
+  TRY(sync(h));
+  drain_prof(h);
+  if (total_ms) *total_ms = h->prof_ms[kernel];
+  if (launches) *launches = h->prof_n[kernel];
+  return CS_OK;
+}
+
+int cs_status(cs_handle* h, int* status) {
+  if (!h || !status) return fail(CS_E_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(status, h->status, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemsetAsync(h->status, 0, sizeof(int), h->stream));
+  return sync(h);
+}
+
+}  // extern "C"

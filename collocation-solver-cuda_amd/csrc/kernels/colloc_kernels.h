@@ -1,0 +1,726 @@
+// Collocation BVP-DAE kernels for gfx950, specialised per problem at JIT time.
+//
+// The including translation unit defines, before including this file:
+//   namespace prob { f, g, Df, Dg, r, Dr }   -- the generated callbacks
+//   struct Cfg { NY, NZ, NP, NC, M, CW, FW; }  -- sizes and launch shapes
+//   __constant__ tables  c_rkA[M*M], c_rkb[M], c_rkc[M],
+//                        c_gL[(M+1)*M], c_gI[(M+1)*M], c_gw[M+1], c_L0[M], c_L1[M]
+//
+// Device memory layouts (row-major, float64):
+//   t[N]; y[N][NY]; ydot[Nint*M][NY]; ztil[Nint*M][NZ]; p[NP]
+//   ytil[Nint*M][NY]; fa[Nint][M*(NY+NZ)]; fb[Nint][NY]; rbc[NY+NP]
+//   jac[Nint*M][JSTAGE]      stage block = Df (NY x NJC) then Dg (NZ x NJC)
+//   rows[Nint][NY][CRW]      BABD row  [G | C | H | b]  (G x_left + C x_right + H p = b)
+//   rec[Nint][REC]           condensation record [U (NS x NV) | T (M NY x NV) | G (M NZ x NV)]
+//   crrec[Nint-1][NY][RECW]  cyclic-reduction record [R | Gp | Ep | Jp | d]
+#pragma once
+#include "colloc_common.h"
+#include "colloc_gj.h"
+
+namespace colloc {
+
+constexpr int NY = Cfg::NY, NZ = Cfg::NZ, NP = Cfg::NP, NC = Cfg::NC, M = Cfg::M;
+constexpr int NZ1 = NZ > 0 ? NZ : 1;
+constexpr int NP1 = NP > 0 ? NP : 1;
+constexpr int NC1 = NC > 0 ? NC : 1;
+constexpr int NYZ = NY + NZ;
+constexpr int NJC = NY + NZ + NP;          // columns of Df / Dg
+constexpr int JSTAGE = (NY + NZ) * NJC;    // doubles per stage Jacobian block
+constexpr int NV = NY + NP + 1;            // unknown-coefficient columns [dy_i | dp | 1]
+constexpr int NS = (M - 1) * NY;           // size of the condensed stage system
+constexpr int REC = (NS + M * NY + M * NZ) * NV;
+constexpr int CRW = 2 * NY + NP + 1;       // BABD row width
+constexpr int RECW = 3 * NY + NP + 1;      // cyclic-reduction record width
+constexpr int NFIN = 2 * NY + NP;          // boundary (final) system size
+
+static_assert(M >= 2, "need at least two collocation stages");
+static_assert(NY <= 64, "cyclic-reduction kernels hold one row per lane");
+
+// -------------------------------------------------------------------- fill
+__device__ void fill_u64(unsigned long long* p, unsigned long long v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------- residual
+// One thread per (interval, stage); thread Nint*M evaluates r.  Restates
+// collocation_update_kernel + compute_f_q_kernel1/2 (solver:478-559) and the
+// host max-norm of f_q (solver:116) as a block max folded into `norm`.
+template <bool FULL>
+__device__ void residual(int nint, const double* __restrict__ t, const double* __restrict__ y,
+                         const double* __restrict__ ydot, const double* __restrict__ ztil,
+                         const double* __restrict__ p, const double* __restrict__ c, double alpha,
+                         double* __restrict__ ytil, double* __restrict__ fa, double* __restrict__ fb,
+                         double* __restrict__ rbc, unsigned long long* __restrict__ norm) {
+  __shared__ double sred[16];
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nst = nint * M;
+  double amax = 0.0;
+  if (gid < nst) {
+    const int i = gid / M;
+    const int j = gid - i * M;
+    const double h = t[i + 1] - t[i];
+    const double* yi = y + (size_t)i * NY;
+    const double* yd = ydot + (size_t)i * M * NY;
+    double yt[NY], zt[NZ1], fo[NY], go[NZ1];
+#pragma unroll
+    for (int l = 0; l < NY; ++l) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < M; ++k) s += c_rkA[j * M + k] * yd[k * NY + l];
+      yt[l] = yi[l] + h * s;
+    }
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) zt[l] = ztil[(size_t)gid * NZ + l];
+    prob::f(yt, zt, p, c, fo);
+    prob::g(yt, zt, p, c, alpha, go);
+    double* fai = fa + (size_t)gid * NYZ;
+#pragma unroll
+    for (int l = 0; l < NY; ++l) {
+      const double v = fo[l] - yd[j * NY + l];
+      amax = nmax(fabs(v), amax);
+      if (FULL) fai[l] = v;
+    }
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+      amax = nmax(fabs(go[l]), amax);
+      if (FULL) fai[NY + l] = go[l];
+    }
+    if (FULL) {
+#pragma unroll
+      for (int l = 0; l < NY; ++l) ytil[(size_t)gid * NY + l] = yt[l];
+    }
+    if (j == M - 1) {
+#pragma unroll
+      for (int l = 0; l < NY; ++l) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) s += c_rkb[k] * yd[k * NY + l];
+        const double v = y[(size_t)(i + 1) * NY + l] - yi[l] - h * s;
+        amax = nmax(fabs(v), amax);
+        if (FULL) fb[(size_t)i * NY + l] = v;
+      }
+    }
+  } else if (gid == nst) {
+    double ro[NY + NP];
+    prob::r(y, y + (size_t)nint * NY, p, c, ro);
+#pragma unroll
+    for (int l = 0; l < NY + NP; ++l) {
+      amax = nmax(fabs(ro[l]), amax);
+      if (FULL) rbc[l] = ro[l];
+    }
+  }
+  block_max_to(amax, norm, sred);
+}
+
+// ---------------------------------------------------------------- jacobian
+// One thread per stage writes Df, Dg densely; thread Nint*M writes Dr.
+// Restates construct_jacobian_kernel (solver:680-797) without materialising
+// J, V, D, W (the condensation kernel assembles what it needs).
+__device__ void jacobian(int nint, const double* __restrict__ y, const double* __restrict__ ytil,
+                         const double* __restrict__ ztil, const double* __restrict__ p,
+                         const double* __restrict__ c, double alpha, double* __restrict__ jac,
+                         double* __restrict__ Dr) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nst = nint * M;
+  if (gid < nst) {
+    double yt[NY], zt[NZ1];
+#pragma unroll
+    for (int l = 0; l < NY; ++l) yt[l] = ytil[(size_t)gid * NY + l];
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) zt[l] = ztil[(size_t)gid * NZ + l];
+    double* out = jac + (size_t)gid * JSTAGE;
+    prob::Df(yt, zt, p, c, out);
+    prob::Dg(yt, zt, p, c, alpha, out + NY * NJC);
+  } else if (gid == nst) {
+    prob::Dr(y, y + (size_t)nint * NY, p, c, Dr);
+  }
+}
+
+// --------------------------------------------------------------- condense
+// One workgroup per interval.  Condenses the stage unknowns out of the
+// interval's linearised collocation equations (the role of
+// reduce_jacobian_parallel_kernel, solver:887-1001) using the stage
+// structure of W instead of a dense K x K LU:
+//   per stage j:  G_j = g_z^-1 [g_y | g_p | f_a,g]        (NZ x NV)
+//                 T_j = [h_y | h_p | f_a,h] - h_z G_j      (NY x NV)
+//   so that  dz_j = -G_j [u_j; dp; 1],  dydot_j = T_j [u_j; dp; 1]
+//   with u_j the stage-state increment.  Lobatto IIIA has a_0k = 0, so
+//   u_0 = dy_i and the unknowns u_1..u_{M-1} satisfy
+//     u_j - h sum_{k>=1} a_jk M_k u_k = (E_y + h a_j0 T_0 + h sum_{k>=1} a_jk [0|N_k|s_k]) v
+//   with M_k = T_k[:, :NY], [N_k|s_k] = T_k[:, NY:], v = [dy_i; dp; 1].  That
+//   NS x NS system with NV right-hand sides is solved by register-resident
+//   Gauss-Jordan.  Because b = A[M-1, :] (c_{M-1} = 1), the continuity row is
+//   dy_{i+1} - u_{M-1} = -f_b, giving the BABD row
+//   G = -U_{M-1}[:, :NY], C = I, H = -U_{M-1}[:, NY:NY+NP], b = -f_b + U_{M-1}[:, NV-1].
+template <int CW>
+__device__ void condense(int nint, const double* __restrict__ t, const double* __restrict__ jac,
+                         const double* __restrict__ fa, const double* __restrict__ fb,
+                         double* __restrict__ rows, double* __restrict__ rec, int* __restrict__ status) {
+  using Solver = GJ<NS, NS + NV, CW>;
+  constexpr int NT = CW * WAVE;
+  __shared__ double sT[M * NY * NV];
+  __shared__ double sG[NZ1 * NV];
+  __shared__ double sgz[NZ1 * NZ1];
+  __shared__ double colbuf[Solver::COLBUF];
+  __shared__ int pivrow[NS];
+  __shared__ int invperm[Solver::RQ * WAVE];
+  __shared__ int sbad;
+
+  const int i = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (tid == 0) sbad = 0;
+  const double h = t[i + 1] - t[i];
+  double* recU = rec + (size_t)i * REC;
+  double* recT = recU + NS * NV;
+  double* recG = recT + M * NY * NV;
+  int bad = 0;
+
+  // ---- phase 1: per-stage elimination of the algebraic unknowns
+  for (int j = 0; j < M; ++j) {
+    const double* Dfj = jac + ((size_t)i * M + j) * JSTAGE;
+    const double* Dgj = Dfj + NY * NJC;
+    const double* faj = fa + (size_t)i * M * NYZ + j * NYZ;
+    if (NZ > 0) {
+      for (int e = tid; e < NZ * NZ; e += NT) sgz[e] = Dgj[(e / NZ) * NJC + NY + (e % NZ)];
+      for (int e = tid; e < NZ * NV; e += NT) {
+        const int r = e / NV, q = e - (e / NV) * NV;
+        sG[e] = q < NY ? Dgj[r * NJC + q] : (q < NY + NP ? Dgj[r * NJC + NZ + q] : faj[NY + r]);
+      }
+      __syncthreads();
+      // Gauss-Jordan on [g_z | rhs] in LDS, one thread per column.
+      for (int k = 0; k < NZ; ++k) {
+        int pr = k;
+        double best = fabs(sgz[k * NZ + k]);
+        for (int r = k + 1; r < NZ; ++r) {
+          const double v = fabs(sgz[r * NZ + k]);
+          if (v > best) { best = v; pr = r; }
+        }
+        // column k with rows k and pr exchanged (indices resolved without
+        // runtime-indexed register arrays)
+        double colk[NZ1];
+#pragma unroll
+        for (int r = 0; r < NZ; ++r) colk[r] = sgz[(r == k ? pr : (r == pr ? k : r)) * NZ + k];
+        const double piv = sgz[pr * NZ + k];
+        if (!(fabs(piv) > 0.0)) bad |= 1;
+        const double inv = 1.0 / piv;
+        __syncthreads();
+        for (int cidx = tid; cidx < NZ + NV; cidx += NT) {
+          double* base = cidx < NZ ? sgz + cidx : sG + (cidx - NZ);
+          const int ld = cidx < NZ ? NZ : NV;
+          if (pr != k) {
+            const double tmp = base[k * ld];
+            base[k * ld] = base[pr * ld];
+            base[pr * ld] = tmp;
+          }
+          const double pk = base[k * ld] * inv;
+#pragma unroll
+          for (int r = 0; r < NZ; ++r) base[r * ld] = (r == k) ? pk : fma(-colk[r], pk, base[r * ld]);
+        }
+        __syncthreads();
+      }
+      for (int e = tid; e < NZ * NV; e += NT) recG[j * NZ * NV + e] = sG[e];
+    }
+    // T_j = [h_y | h_p | f_a,h] - h_z G_j
+    for (int e = tid; e < NY * NV; e += NT) {
+      const int r = e / NV, q = e - (e / NV) * NV;
+      double v = q < NY ? Dfj[r * NJC + q] : (q < NY + NP ? Dfj[r * NJC + NZ + q] : faj[r]);
+#pragma unroll
+      for (int s = 0; s < NZ; ++s) v = fma(-Dfj[r * NJC + NY + s], sG[s * NV + q], v);
+      sT[j * NY * NV + e] = v;
+      recT[j * NY * NV + e] = v;
+    }
+    __syncthreads();
+  }
+
+  // ---- phase 2: assemble [S | RHS] into registers
+  Solver gj;
+#pragma unroll
+  for (int q = 0; q < Solver::RQ; ++q) {
+    const int r = Solver::row(q);
+    const int jr = r / NY + 1;
+    const int ar = r - (jr - 1) * NY;
+#pragma unroll
+    for (int cq = 0; cq < Solver::CQ; ++cq) {
+      const int cc = Solver::col(cq);
+      double v = 0.0;
+      if (r < NS && cc < NS + NV) {
+        if (cc < NS) {
+          const int kc = cc / NY + 1;
+          const int bc = cc - (kc - 1) * NY;
+          v = -h * c_rkA[jr * M + kc] * sT[(kc * NY + ar) * NV + bc];
+          if (jr == kc && ar == bc) v += 1.0;
+        } else {
+          const int qc = cc - NS;
+          v = h * c_rkA[jr * M] * sT[ar * NV + qc];
+          if (qc < NY) {
+            if (qc == ar) v += 1.0;
+          } else {
+            for (int k = 1; k < M; ++k) v += h * c_rkA[jr * M + k] * sT[(k * NY + ar) * NV + qc];
+          }
+        }
+      }
+      gj.a[q][cq] = v;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 3: Gauss-Jordan
+  for (int k = tid; k < Solver::RQ * WAVE; k += NT) invperm[k] = -1;
+  if (gj.eliminate(colbuf, pivrow)) bad |= 2;
+  __syncthreads();
+  for (int k = tid; k < NS; k += NT) invperm[pivrow[k]] = k;
+  if (bad) atomicOr(&sbad, bad);
+  __syncthreads();
+
+  // ---- phase 4: outputs
+  double* rowi = rows + (size_t)i * NY * CRW;
+#pragma unroll
+  for (int q = 0; q < Solver::RQ; ++q) {
+    const int r = Solver::row(q);
+    if (r >= NS) continue;
+    const int kk = invperm[r];
+    if (kk < 0 || kk >= NS) continue;   // only after a failed pivot search (non-finite column)
+    const bool last = kk >= (M - 2) * NY;
+    const int a = kk - (M - 2) * NY;
+#pragma unroll
+    for (int cq = 0; cq < Solver::CQ; ++cq) {
+      const int cc = Solver::col(cq);
+      if (cc < NS || cc >= NS + NV) continue;
+      const int qc = cc - NS;
+      const double v = gj.a[q][cq];
+      recU[kk * NV + qc] = v;
+      if (last) {
+        if (qc < NY) rowi[a * CRW + qc] = -v;
+        else if (qc < NY + NP) rowi[a * CRW + NY + qc] = -v;
+        else rowi[a * CRW + CRW - 1] = v - fb[(size_t)i * NY + a];
+      }
+    }
+  }
+  for (int e = tid; e < NY * NY; e += NT) {
+    const int r = e / NY, q = e - (e / NY) * NY;
+    rowi[r * CRW + NY + q] = (r == q) ? 1.0 : 0.0;
+  }
+  if (tid == 0 && sbad) atomicOr(status, sbad);
+}
+
+// ------------------------------------------------------ cyclic reduction
+// One workgroup per pair of adjacent BABD rows (level l):
+//   row a: G_a x_L + C_a x_M + H_a p = b_a,   row b: G_b x_M + C_b x_R + H_b p = b_b.
+// Householder QR of [C_a; G_b] (2NY x NY) eliminates x_M (the same 2-block
+// step as partition_factorization_kernel, solve_babd_system.py:184-246):
+// the top NY rows [R | Gp | Ep | Jp | d] are the record used to recover x_M,
+// the bottom NY rows are the combined row [G | C | H | b] of level l+1.
+// Thread c owns column c of the 2NY x RECW block in registers; the
+// Householder vector is broadcast through LDS.  An odd last row is carried.
+__device__ void cr_pair(int nrows, const double* __restrict__ rows_in, double* __restrict__ rows_out,
+                        double* __restrict__ recs) {
+  constexpr int R2 = 2 * NY;
+  __shared__ double sv[2][R2];
+  __shared__ double sbeta[2];
+  const int k = blockIdx.x;
+  const int npairs = nrows >> 1;
+  const int cidx = threadIdx.x;
+  if (k == npairs) {  // carry the odd last row
+    const double* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
+    double* dst = rows_out + (size_t)npairs * NY * CRW;
+    for (int e = cidx; e < NY * CRW; e += blockDim.x) dst[e] = src[e];
+    return;
+  }
+  const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
+  const double* rb = ra + NY * CRW;
+  // column layout: [x_M (NY) | x_L (NY) | x_R (NY) | p (NP) | rhs]
+  double a[R2];
+  if (cidx < RECW) {
+#pragma unroll
+    for (int r = 0; r < NY; ++r) {
+      double va, vb;
+      if (cidx < NY) { va = ra[r * CRW + NY + cidx]; vb = rb[r * CRW + cidx]; }
+      else if (cidx < 2 * NY) { va = ra[r * CRW + cidx - NY]; vb = 0.0; }
+      else if (cidx < 3 * NY) { va = 0.0; vb = rb[r * CRW + cidx - NY]; }
+      else { va = ra[r * CRW + cidx - NY]; vb = rb[r * CRW + cidx - NY]; }
+      a[r] = va;
+      a[NY + r] = vb;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R2; ++r) a[r] = 0.0;
+  }
+  const double eps = 2.220446049250313e-16;
+  bool stopped = false;
+#pragma unroll
+  for (int kk = 0; kk < NY; ++kk) {
+    const int buf = kk & 1;
+    if (cidx == kk) {
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+      for (int r = kk; r < R2; r += 2) {
+        s0 = fma(a[r], a[r], s0);
+        if (r + 1 < R2) s1 = fma(a[r + 1], a[r + 1], s1);
+      }
+      const double nrm = sqrt(s0 + s1);
+      const double g = a[kk] >= 0.0 ? -nrm : nrm;
+      const double v0 = a[kk] - g;
+      const double vs = s0 + s1 - a[kk] * a[kk] + v0 * v0;
+      sv[buf][kk] = v0;
+#pragma unroll
+      for (int r = kk + 1; r < R2; ++r) sv[buf][r] = a[r];
+      sbeta[buf] = vs < eps ? 0.0 : -2.0 / vs;
+      if (!(vs < eps)) {
+        a[kk] = g;
+#pragma unroll
+        for (int r = kk + 1; r < R2; ++r) a[r] = 0.0;
+      }
+    }
+    __syncthreads();
+    const double beta = sbeta[buf];
+    if (beta == 0.0) stopped = true;   // the reference stops reflecting at |v|^2 < eps
+    if (!stopped && cidx > kk && cidx < RECW) {
+      double w0 = 0.0, w1 = 0.0;
+#pragma unroll
+      for (int r = kk; r < R2; r += 2) {
+        w0 = fma(sv[buf][r], a[r], w0);
+        if (r + 1 < R2) w1 = fma(sv[buf][r + 1], a[r + 1], w1);
+      }
+      const double w = (w0 + w1) * beta;
+#pragma unroll
+      for (int r = kk; r < R2; ++r) a[r] = fma(w, sv[buf][r], a[r]);
+    }
+  }
+  if (cidx < RECW) {
+    double* rec = recs + (size_t)k * NY * RECW;
+#pragma unroll
+    for (int r = 0; r < NY; ++r) rec[r * RECW + cidx] = a[r];
+    if (cidx >= NY) {
+      double* out = rows_out + (size_t)k * NY * CRW;
+#pragma unroll
+      for (int r = 0; r < NY; ++r) out[r * CRW + cidx - NY] = a[NY + r];
+    }
+  }
+}
+
+// Boundary system on the last remaining row G x_0 + C x_N + H p = b and the
+// boundary conditions B_0 x_0 + B_n x_N + V_n p = -r_bc: unknowns
+// [x_N | x_0 | p] as in qr_decomposition's final block (solve_babd_system.py:364-381).
+template <int FW>
+__device__ void cr_final(int nint, const double* __restrict__ row, const double* __restrict__ Dr,
+                         const double* __restrict__ rbc, double* __restrict__ dy, double* __restrict__ dp,
+                         int* __restrict__ status) {
+  using Solver = GJ<NFIN, NFIN + 1, FW>;
+  __shared__ double colbuf[Solver::COLBUF];
+  __shared__ int pivrow[NFIN];
+  __shared__ int invperm[Solver::RQ * WAVE];
+  Solver gj;
+  constexpr int DRW = 2 * NY + NP;
+  for (int k = threadIdx.x; k < Solver::RQ * WAVE; k += blockDim.x) invperm[k] = -1;
+#pragma unroll
+  for (int q = 0; q < Solver::RQ; ++q) {
+    const int r = Solver::row(q);
+#pragma unroll
+    for (int cq = 0; cq < Solver::CQ; ++cq) {
+      const int cc = Solver::col(cq);
+      double v = 0.0;
+      if (r < NY) {
+        if (cc < NY) v = row[r * CRW + NY + cc];                 // C
+        else if (cc < 2 * NY) v = row[r * CRW + cc - NY];        // G
+        else if (cc < NFIN) v = row[r * CRW + cc];               // H
+        else if (cc == NFIN) v = row[r * CRW + CRW - 1];         // b
+      } else if (r < NFIN) {
+        const int rr = r - NY;
+        if (cc < NY) v = Dr[rr * DRW + NY + cc];                 // B_n
+        else if (cc < 2 * NY) v = Dr[rr * DRW + cc - NY];        // B_0
+        else if (cc < NFIN) v = Dr[rr * DRW + cc];               // V_n
+        else if (cc == NFIN) v = -rbc[rr];
+      }
+      gj.a[q][cq] = v;
+    }
+  }
+  const int bad = gj.eliminate(colbuf, pivrow);
+  __syncthreads();
+  for (int k = threadIdx.x; k < NFIN; k += blockDim.x) invperm[pivrow[k]] = k;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < Solver::RQ; ++q) {
+    const int r = Solver::row(q);
+#pragma unroll
+    for (int cq = 0; cq < Solver::CQ; ++cq) {
+      if (Solver::col(cq) != NFIN || r >= NFIN) continue;
+      const int kk = invperm[r];
+      if (kk < 0 || kk >= NFIN) continue;
+      const double v = gj.a[q][cq];
+      if (kk < NY) dy[(size_t)nint * NY + kk] = v;
+      else if (kk < 2 * NY) dy[kk - NY] = v;
+      else dp[kk - 2 * NY] = v;
+    }
+  }
+  if (bad && threadIdx.x == 0) atomicOr(status, 4);
+}
+
+// Back substitution of one cyclic-reduction level: one wave per record,
+//   x_M = R^-1 (d - Gp x_L - Ep x_R - Jp p)   (as partition_backward_substitution_kernel,
+// solve_babd_system.py:537-572, with the triangular solve done directly).
+__device__ void cr_back(int level, int nrows, int nint, const double* __restrict__ recs,
+                        double* __restrict__ dy, const double* __restrict__ dp) {
+  const int k = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int mid = (2 * k + 1) << level;
+  const int left = (2 * k) << level;
+  const int right = min((2 * k + 2) << level, nint);
+  const double* rec = recs + (size_t)k * NY * RECW;
+  const double* xl = dy + (size_t)left * NY;
+  const double* xr = dy + (size_t)right * NY;
+  double rhs = 0.0;
+  if (lane < NY) {
+    const double* rw = rec + lane * RECW;
+    rhs = rw[RECW - 1];
+    for (int cc = 0; cc < NY; ++cc) rhs = fma(-rw[NY + cc], xl[cc], rhs);
+    for (int cc = 0; cc < NY; ++cc) rhs = fma(-rw[2 * NY + cc], xr[cc], rhs);
+    for (int cc = 0; cc < NP; ++cc) rhs = fma(-rw[3 * NY + cc], dp[cc], rhs);
+  }
+  double x = 0.0;
+  for (int r = NY - 1; r >= 0; --r) {
+    double xr_ = 0.0;
+    if (lane == r) { x = rhs / rec[r * RECW + r]; xr_ = x; }
+    xr_ = readlane_d(xr_, r);
+    if (lane < r) rhs = fma(-rec[lane * RECW + r], xr_, rhs);
+  }
+  if (lane < NY) dy[(size_t)mid * NY + lane] = x;
+}
+
+// ---------------------------------------------------------------- recover
+// One workgroup per interval (recover_delta_k_kernel, solver:1619-1650):
+//   u_0 = dy_i, u_j = U_j v,  dydot_j = T_j [u_j; dp; 1],  dz_j = -G_j [u_j; dp; 1].
+__device__ void recover(int nint, const double* __restrict__ rec, const double* __restrict__ dy,
+                        const double* __restrict__ dp, double* __restrict__ dydot, double* __restrict__ dz) {
+  __shared__ double sw[M][NV];
+  const int i = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = lane_id(), wave = wave_id();
+  const int nw = blockDim.x / WAVE;
+  const double* recU = rec + (size_t)i * REC;
+  const double* recT = recU + NS * NV;
+  const double* recG = recT + M * NY * NV;
+  for (int e = tid; e < M * NV; e += blockDim.x) {
+    const int j = e / NV, q = e - (e / NV) * NV;
+    double v = 0.0;
+    if (q < NY) v = (j == 0) ? dy[(size_t)i * NY + q] : 0.0;
+    else if (q < NY + NP) v = dp[q - NY];
+    else v = 1.0;
+    sw[j][q] = v;
+  }
+  __syncthreads();
+  // u_j rows: one wave per row, lanes across the NV columns
+  for (int rr = wave; rr < NS; rr += nw) {
+    double s = 0.0;
+    for (int q = lane; q < NV; q += WAVE) s = fma(recU[rr * NV + q], sw[0][q], s);
+    s = wave_sum(s);
+    if (lane == 0) sw[rr / NY + 1][rr % NY] = s;
+  }
+  __syncthreads();
+  for (int rr = wave; rr < M * (NY + NZ); rr += nw) {
+    const bool isz = rr >= M * NY;
+    const int j = isz ? (rr - M * NY) / NZ1 : rr / NY;
+    const double* rowp = isz ? recG + (size_t)(rr - M * NY) * NV : recT + (size_t)rr * NV;
+    double s = 0.0;
+    for (int q = lane; q < NV; q += WAVE) s = fma(rowp[q], sw[j][q], s);
+    s = wave_sum(s);
+    if (lane == 0) {
+      if (isz) {
+        const int a = (rr - M * NY) - j * NZ;
+        dz[((size_t)i * M + j) * NZ + a] = -s;
+      } else {
+        const int a = rr - j * NY;
+        dydot[((size_t)i * M + j) * NY + a] = s;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------- axpy
+struct Seg4 { long long off[4]; long long len[4]; };
+__device__ void axpy4(Seg4 s, const double* __restrict__ x, const double* __restrict__ d, double a,
+                      double* __restrict__ out) {
+  long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (gid < s.len[q]) {
+      const long long e = s.off[q] + gid;
+      out[e] = fma(a, d[e], x[e]);
+      return;
+    }
+    gid -= s.len[q];
+  }
+}
+
+// -------------------------------------------------------- initial input
+// form_initial_input_kernel (solver:316-356).
+__device__ void init_input(int nint, const double* __restrict__ y0, const double* __restrict__ z0,
+                           const double* __restrict__ p, const double* __restrict__ c,
+                           double* __restrict__ ydot, double* __restrict__ ztil) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nint * M) return;
+  const int i = gid / M, j = gid - (gid / M) * M;
+  const double cj = c_rkc[j];
+  double yt[NY], zt[NZ1], fo[NY];
+#pragma unroll
+  for (int l = 0; l < NY; ++l) yt[l] = (1.0 - cj) * y0[(size_t)i * NY + l] + cj * y0[(size_t)(i + 1) * NY + l];
+#pragma unroll
+  for (int l = 0; l < NZ; ++l) {
+    zt[l] = (1.0 - cj) * z0[(size_t)i * NZ + l] + cj * z0[(size_t)(i + 1) * NZ + l];
+    ztil[(size_t)gid * NZ + l] = zt[l];
+  }
+  prob::f(yt, zt, p, c, fo);
+#pragma unroll
+  for (int l = 0; l < NY; ++l) ydot[(size_t)gid * NY + l] = fo[l];
+}
+
+// ------------------------------------------------------- segment residual
+// compute_segment_residual_kernel (solver:1833-1892), residual_type 1,
+// scale_by_time: note the scaling rho *= dt inside the component loop.
+__device__ void segres(int nint, const double* __restrict__ t, const double* __restrict__ y,
+                       const double* __restrict__ ydot, const double* __restrict__ ztil,
+                       const double* __restrict__ p, const double* __restrict__ c, double alpha, double tol,
+                       double* __restrict__ residual, unsigned long long* __restrict__ maxres) {
+  __shared__ double sred[16];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double val = 0.0;
+  if (i < nint) {
+    const double h = t[i + 1] - t[i];
+    const double* yd = ydot + (size_t)i * M * NY;
+    const double* zz = ztil + (size_t)i * M * NZ;
+    double rho_h = 0.0, rho_g = 0.0;
+    for (int gp = 0; gp < M + 1; ++gp) {
+      double ydt[NY], yt[NY], zt[NZ1], fo[NY], go[NZ1];
+#pragma unroll
+      for (int l = 0; l < NY; ++l) {
+        double s = 0.0;
+        double yy = y[(size_t)i * NY + l];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          s += c_gL[gp * M + k] * yd[k * NY + l];
+          yy += h * c_gI[gp * M + k] * yd[k * NY + l];
+        }
+        ydt[l] = s;
+        yt[l] = yy;
+      }
+#pragma unroll
+      for (int l = 0; l < NZ; ++l) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) s += c_gL[gp * M + k] * zz[k * NZ + l];
+        zt[l] = s;
+      }
+      prob::f(yt, zt, p, c, fo);
+      const double w = c_gw[gp];
+#pragma unroll
+      for (int l = 0; l < NY; ++l) {
+        rho_h += w * fabs(fo[l] - ydt[l]);
+        rho_h *= h;
+      }
+      if (NZ > 0) {
+        prob::g(yt, zt, p, c, alpha, go);
+#pragma unroll
+        for (int l = 0; l < NZ; ++l) {
+          rho_g += w * fabs(go[l]);
+          rho_g *= h;
+        }
+      }
+    }
+    val = (fabs(rho_h) + fabs(rho_g)) / tol;
+    residual[i] = val;
+  } else if (i == nint) {
+    double ro[NY + NP];
+    prob::r(y, y + (size_t)nint * NY, p, c, ro);
+    double m = 0.0;
+#pragma unroll
+    for (int l = 0; l < NY + NP; ++l) m = nmax(fabs(ro[l]), m);
+    val = m / tol;
+    residual[i] = val;
+  }
+  block_max_to(val, maxres, sred);
+}
+
+// ---------------------------------------------------------- recover z
+// recover_solution_kernel (solver:2049-2075).
+__device__ void recover_z(int nnodes, const double* __restrict__ ztil, double* __restrict__ z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnodes) return;
+  const bool last = i == nnodes - 1;
+  const int iv = last ? i - 1 : i;
+#pragma unroll
+  for (int l = 0; l < NZ; ++l) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) s += (last ? c_L1[j] : c_L0[j]) * ztil[((size_t)iv * M + j) * NZ + l];
+    z[(size_t)i * NZ + l] = s;
+  }
+}
+
+}  // namespace colloc
+
+// ------------------------------------------------------------ entry points
+extern "C" {
+__global__ void colloc_fill_u64(unsigned long long* p, unsigned long long v, int n) { colloc::fill_u64(p, v, n); }
+
+__global__ __launch_bounds__(256) void colloc_residual_full(int nint, const double* t, const double* y,
+    const double* ydot, const double* ztil, const double* p, const double* c, double alpha, double* ytil,
+    double* fa, double* fb, double* rbc, unsigned long long* norm) {
+  colloc::residual<true>(nint, t, y, ydot, ztil, p, c, alpha, ytil, fa, fb, rbc, norm);
+}
+
+__global__ __launch_bounds__(256) void colloc_residual_norm(int nint, const double* t, const double* y,
+    const double* ydot, const double* ztil, const double* p, const double* c, double alpha, double* ytil,
+    double* fa, double* fb, double* rbc, unsigned long long* norm) {
+  colloc::residual<false>(nint, t, y, ydot, ztil, p, c, alpha, ytil, fa, fb, rbc, norm);
+}
+
+__global__ __launch_bounds__(256) void colloc_jacobian(int nint, const double* y, const double* ytil,
+    const double* ztil, const double* p, const double* c, double alpha, double* jac, double* Dr) {
+  colloc::jacobian(nint, y, ytil, ztil, p, c, alpha, jac, Dr);
+}
+
+__global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const double* t, const double* jac,
+    const double* fa, const double* fb, double* rows, double* rec, int* status) {
+  colloc::condense<Cfg::CW>(nint, t, jac, fa, fb, rows, rec, status);
+}
+
+__global__ __launch_bounds__(512) void colloc_cr_pair(int nrows, const double* rows_in, double* rows_out,
+    double* recs) {
+  colloc::cr_pair(nrows, rows_in, rows_out, recs);
+}
+
+__global__ __launch_bounds__(Cfg::FW * 64) void colloc_cr_final(int nint, const double* row, const double* Dr,
+    const double* rbc, double* dy, double* dp, int* status) {
+  colloc::cr_final<Cfg::FW>(nint, row, Dr, rbc, dy, dp, status);
+}
+
+__global__ __launch_bounds__(64) void colloc_cr_back(int level, int nrows, int nint, const double* recs,
+    double* dy, const double* dp) {
+  colloc::cr_back(level, nrows, nint, recs, dy, dp);
+}
+
+__global__ __launch_bounds__(256) void colloc_recover(int nint, const double* rec, const double* dy,
+    const double* dp, double* dydot, double* dz) {
+  colloc::recover(nint, rec, dy, dp, dydot, dz);
+}
+
+__global__ __launch_bounds__(256) void colloc_axpy4(colloc::Seg4 s, const double* x, const double* d, double a,
+    double* out) {
+  colloc::axpy4(s, x, d, a, out);
+}
+
+__global__ __launch_bounds__(256) void colloc_init_input(int nint, const double* y0, const double* z0,
+    const double* p, const double* c, double* ydot, double* ztil) {
+  colloc::init_input(nint, y0, z0, p, c, ydot, ztil);
+}
+
+__global__ __launch_bounds__(256) void colloc_segres(int nint, const double* t, const double* y,
+    const double* ydot, const double* ztil, const double* p, const double* c, double alpha, double tol,
+    double* residual, unsigned long long* maxres) {
+  colloc::segres(nint, t, y, ydot, ztil, p, c, alpha, tol, residual, maxres);
+}
+
+__global__ __launch_bounds__(256) void colloc_recover_z(int nnodes, const double* ztil, double* z) {
+  colloc::recover_z(nnodes, ztil, z);
+}
+}

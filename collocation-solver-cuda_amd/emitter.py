@@ -1,0 +1,450 @@
+"""OCP -> BVP-DAE conversion and HIP code emission.
+
+Restates OCP2ABVP.make_abvp (OCP2ABVP.py:41-182): the Hamiltonian
+H = L + lambda'f (+ alpha u'u/2 when inequalities exist) + theta'c + mu'd + xi's,
+f = [H_lambda; -H_x; H_p], g = [H_u; c; smoothed complementarity], and the
+boundary conditions r = [Gamma; lambda + Gamma_x' kappa_i; gamma;
+Psi; lambda - phi_x - Psi_x' kappa_f; gamma + phi_p + Psi_p' kappa_f].
+Variable order, Jacobian column order ([y | z | p] for Df/Dg, [y0 | y1 | p]
+for Dr) and the boundary multiplier convention are the reference's.
+
+Differences from the reference's emitter (OCP2ABVP.py:184-468), by design:
+ * nonlinear function applications (tanh, sin, sqrt, ...) are hoisted into
+   their own symbols before differentiation and derivatives are taken by the
+   chain rule over those symbols, so that sympy never differentiates a
+   nested argument twice and the emitted code evaluates each such term once;
+ * constants stay symbolic until code generation (emitted as constexpr);
+ * the callbacks are HIP __device__ functions (r and Dr too), and Jacobians
+   are written densely (structural zeros included) so callers need not clear
+   them.
+"""
+import hashlib
+
+import numpy as np
+import sympy as sp
+from sympy.printing.c import C99CodePrinter
+
+from .ocp import OCP
+
+# -------------------------------------------------------------- hoisting
+class Hoister:
+    """Holds hoisted nonlinear subexpressions h_k = F(arg_k) and differentiates
+    expressions that contain them by the chain rule."""
+
+    def __init__(self, prefix="_h"):
+        self.prefix = prefix
+        self.defs = []          # list of (symbol, definition) in creation order
+        self.defn = {}          # symbol -> definition
+        self.key = {}           # definition -> symbol
+        self.deps = {}          # symbol -> frozenset of base symbols it depends on
+        self._dcache = {}
+
+    # -- structure -------------------------------------------------------
+    def _intern(self, d):
+        s = self.key.get(d)
+        if s is not None:
+            return s
+        s = sp.Symbol("%s%d" % (self.prefix, len(self.defs)), real=True)
+        self.key[d] = s
+        self.defn[s] = d
+        self.defs.append((s, d))
+        self.deps[s] = self.base_deps(d)
+        return s
+
+    def base_deps(self, e):
+        out = set()
+        for s in e.free_symbols:
+            if s in self.deps:
+                out |= self.deps[s]
+            else:
+                out.add(s)
+        return frozenset(out)
+
+    def hoist(self, e):
+        e = sp.sympify(e)
+        if e.is_Atom:
+            return e
+        if isinstance(e, sp.Function):
+            args = [self.hoist(a) for a in e.args]
+            if all(a.is_number for a in args):
+                return e.func(*args)
+            return self._intern(e.func(*args))
+        if isinstance(e, sp.Pow):
+            b = self.hoist(e.base)
+            x = self.hoist(e.exp)
+            if x.is_Integer or not (b.free_symbols or x.free_symbols):
+                return b ** x
+            return self._intern(b ** x)
+        return e.func(*[self.hoist(a) for a in e.args])
+
+    # -- differentiation ---------------------------------------------------
+    def deriv(self, e, v):
+        """d e / d v where e may contain hoisted symbols."""
+        if v not in self.base_deps(e):
+            return sp.Integer(0)
+        res = sp.diff(e, v)
+        for h in e.free_symbols:
+            if h in self.defn and v in self.deps[h]:
+                dh = self.dh(h, v)
+                if dh != 0:
+                    res = res + sp.diff(e, h) * dh
+        return res
+
+    def dh(self, h, v):
+        ck = (h, v)
+        if ck in self._dcache:
+            return self._dcache[ck]
+        d = self.defn[h]
+        if isinstance(d, sp.Pow):
+            b, x = d.base, d.exp
+            if v not in self.base_deps(x):
+                out = x * h / b * self.deriv(b, v)
+            else:
+                lb = self._intern(sp.log(b))
+                out = h * (self.deriv(x, v) * lb + x * self.deriv(b, v) / b)
+        else:
+            out = sp.Integer(0)
+            dummies = [sp.Dummy("a%d" % i, real=True) for i in range(len(d.args))]
+            fd = d.func(*dummies)
+            for k, a in enumerate(d.args):
+                da = self.deriv(a, v)
+                if da == 0:
+                    continue
+                fp = sp.diff(fd, dummies[k])
+                if fp.has(sp.Derivative) or fp.has(sp.Subs):
+                    raise ValueError("cannot differentiate %s" % d.func)
+                fp = fp.subs(dict(zip(dummies, d.args)))
+                out = out + self.hoist(fp) * da
+        self._dcache[ck] = out
+        return out
+
+    def needed(self, exprs):
+        """Hoisted symbols reachable from exprs, in definition order."""
+        need = set()
+        stack = [s for e in exprs for s in e.free_symbols if s in self.defn]
+        while stack:
+            s = stack.pop()
+            if s in need:
+                continue
+            need.add(s)
+            stack.extend(t for t in self.defn[s].free_symbols if t in self.defn)
+        return [(s, d) for s, d in self.defs if s in need]
+
+
+# ------------------------------------------------------------ BVP-DAE form
+class BvpDaeSymbolic:
+    """The BVP-DAE of an OCP: symbols, expressions and problem settings."""
+
+    def __init__(self, ocp):
+        o = ocp
+        self.ocp = o
+        nx, npar = len(o.x), len(o.p)
+        nu, nc, nd, ns = len(o.u), len(o.c), len(o.d), len(o.s)
+        ngam, npsi = len(o.Gamma), len(o.Psi)
+        R = dict(real=True)
+        lam = [sp.Symbol("_lambda%d" % (i + 1), **R) for i in range(nx)]
+        gam = [sp.Symbol("_gamma%d" % (i + 1), **R) for i in range(npar)]
+        theta = [sp.Symbol("_theta%d" % (i + 1), **R) for i in range(nc)]
+        mu = [sp.Symbol("_mu%d" % (i + 1), **R) for i in range(nd)]
+        nuv = [sp.Symbol("_nu%d" % (i + 1), **R) for i in range(nd)]
+        xi = [sp.Symbol("_xi%d" % (i + 1), **R) for i in range(ns)]
+        sig = [sp.Symbol("_sigma%d" % (i + 1), **R) for i in range(ns)]
+        kap_i = [sp.Symbol("_kappa_i%d" % (i + 1), **R) for i in range(ngam)]
+        kap_f = [sp.Symbol("_kappa_f%d" % (i + 1), **R) for i in range(npsi)]
+        alpha = sp.Symbol("_alpha", **R)
+        # user symbols are made real so that |.| differentiates to sign(.)
+        realmap = {}
+        for s in list(o.x) + list(o.u) + list(o.p) + list(o.constants):
+            realmap[s] = sp.Symbol(s.name, real=True)
+        sub = lambda e: sp.sympify(e).xreplace(realmap)
+        xs = [realmap[s] for s in o.x]
+        us = [realmap[s] for s in o.u]
+        ps = [realmap[s] for s in o.p]
+        self.constants = {realmap[s]: v for s, v in o.constants.items()}
+        self.instance_constants = [realmap[s] for s in o.instance_constants]
+
+        # variable vectors (OCP2ABVP.py:54-90)
+        self.y = xs + lam + gam
+        self.z = us + theta + mu + xi + nuv + sig
+        self.p = ps + kap_i + kap_f
+        self.alpha = alpha
+        self.size_y, self.size_z, self.size_p = len(self.y), len(self.z), len(self.p)
+        self.size_inequality, self.size_sv_inequality = nd, ns
+
+        hz = Hoister("_h")
+        L, f = hz.hoist(sub(o.L)), [hz.hoist(sub(e)) for e in o.f]
+        cc = [hz.hoist(sub(e)) for e in o.c]
+        dd = [hz.hoist(sub(e)) for e in o.d]
+        ss = [hz.hoist(sub(e)) for e in o.s]
+        if len(f) != nx:
+            raise ValueError("DifferentialEquations has %d entries for %d states" % (len(f), nx))
+        # Hamiltonian (OCP2ABVP.py:95-111)
+        H = L + sum(lam[i] * f[i] for i in range(nx))
+        if nd > 0 or ns > 0:
+            H = H + sum(alpha * u * u / 2 for u in us)
+        H = H + sum(theta[i] * cc[i] for i in range(nc))
+        H = H + sum(mu[i] * dd[i] for i in range(nd))
+        H = H + sum(xi[i] * ss[i] for i in range(ns))
+        # f = [H_lambda; -H_x; H_p] (OCP2ABVP.py:118-124)
+        fe = [hz.deriv(H, lam[i]) for i in range(nx)]
+        fe += [-hz.deriv(H, xs[i]) for i in range(nx)]
+        fe += [hz.deriv(H, ps[i]) for i in range(npar)]
+        # g (OCP2ABVP.py:130-144)
+        ge = [sp.Integer(0)] * self.size_z
+        for i in range(nu):
+            ge[i] = hz.deriv(H, us[i])
+        for i in range(nc):
+            ge[nu + i] = hz.deriv(H, theta[i])
+        for i in range(nd):
+            ge[nu + nc + i] = hz.deriv(H, mu[i]) + nuv[i] - alpha * mu[i]
+            ge[nu + nc + nd + ns + i] = mu[i] + nuv[i] - hz.hoist(sp.sqrt(mu[i] ** 2 + nuv[i] ** 2 + 2 * alpha))
+        for i in range(ns):
+            ge[nu + nc + nd + i] = hz.deriv(H, xi[i]) + sig[i] - alpha * xi[i]
+            ge[nu + nc + nd + ns + nd + i] = xi[i] + sig[i] - hz.hoist(sp.sqrt(xi[i] ** 2 + sig[i] ** 2 + 2 * alpha))
+        self.hoist_fg = hz
+        self.f_exprs = fe
+        self.g_exprs = ge
+        yz = self.y + self.z + self.p
+        self.Df_exprs = [[hz.deriv(e, v) for v in yz] for e in fe]
+        self.Dg_exprs = [[hz.deriv(e, v) for v in yz] for e in ge]
+
+        # boundary conditions (OCP2ABVP.py:146-180); separate hoisting because
+        # the initial rows bind y to y(t_i) and the final rows to y(t_f)
+        hi, hf = Hoister("_hi"), Hoister("_hf")
+        Gam = [hi.hoist(sub(e)) for e in o.Gamma]
+        Psi = [hf.hoist(sub(e)) for e in o.Psi]
+        phi = hf.hoist(sub(o.phi))
+        ri = list(Gam)
+        for i in range(nx):
+            ri.append(lam[i] + sum(hi.deriv(Gam[j], xs[i]) * kap_i[j] for j in range(ngam)))
+        for i in range(npar):
+            ri.append(gam[i])
+        rf = list(Psi)
+        for i in range(nx):
+            rf.append(lam[i] - hf.deriv(phi, xs[i]) - sum(hf.deriv(Psi[j], xs[i]) * kap_f[j] for j in range(npsi)))
+        for i in range(npar):
+            rf.append(gam[i] + hf.deriv(phi, ps[i]) + sum(hf.deriv(Psi[j], ps[i]) * kap_f[j] for j in range(npsi)))
+        if len(ri) + len(rf) != self.size_y + self.size_p:
+            raise ValueError("boundary conditions: %d rows for %d unknowns (y %d + p %d)"
+                             % (len(ri) + len(rf), self.size_y + self.size_p, self.size_y, self.size_p))
+        self.hoist_ri, self.hoist_rf = hi, hf
+        self.ri_exprs, self.rf_exprs = ri, rf
+        yp = self.y + self.p
+        self.Dri_exprs = [[hi.deriv(e, v) for v in yp] for e in ri]
+        self.Drf_exprs = [[hf.deriv(e, v) for v in yp] for e in rf]
+
+    # problem settings (the generated BvpDae.__init__, OCP2ABVP.py:329-447)
+    @property
+    def tolerance(self):
+        return self.ocp.tolerance
+
+
+# ---------------------------------------------------------------- printing
+class _Printer(C99CodePrinter):
+    def __init__(self, names):
+        super().__init__()
+        self._names = names
+
+    def _print_Symbol(self, e):
+        n = self._names.get(e)
+        return n if n is not None else super()._print_Symbol(e)
+
+    def _print_Pow(self, e):
+        b, x = e.base, e.exp
+        if x.is_Integer:
+            k = int(x)
+            if 1 <= k <= 4:
+                s = "(" + self._print(b) + ")"
+                return "(" + "*".join([s] * k) + ")"
+            if -4 <= k <= -1:
+                s = "(" + self._print(b) + ")"
+                return "(1.0/(" + "*".join([s] * (-k)) + "))"
+        if x == sp.Rational(1, 2):
+            return "sqrt(%s)" % self._print(b)
+        if x == sp.Rational(-1, 2):
+            return "(1.0/sqrt(%s))" % self._print(b)
+        return "pow(%s, %s)" % (self._print(b), self._print(x))
+
+    def _print_Float(self, e):
+        return repr(float(e))
+
+    def _print_Rational(self, e):
+        return "(%r)" % (float(e.p) / float(e.q))
+
+    def _print_Integer(self, e):
+        return "%d.0" % int(e)
+
+    def _print_sign(self, e):
+        a = self._print(e.args[0])
+        return "((%s) > 0.0 ? 1.0 : ((%s) < 0.0 ? -1.0 : 0.0))" % (a, a)
+
+
+def _cname(sym):
+    """C name of a symbol: generated symbols (leading '_') get prefix 'w', user
+    symbols 'u_', constants are printed as 'K_<name>' (distinct namespaces)."""
+    n = sym.name
+    return "w" + n if n.startswith("_") else "u_" + n
+
+
+def _emit_block(lines, exprs_out, hoister, bindings, constants, inst_consts, indent="  "):
+    """Append C statements computing exprs_out = [(target, expr), ...]."""
+    exprs = [e for _, e in exprs_out]
+    hdefs = hoister.needed(exprs) if hoister is not None else []
+    used = set()
+    for e in exprs:
+        used |= e.free_symbols
+    for _, d in hdefs:
+        used |= d.free_symbols
+    names = {}
+    for s, src in bindings:
+        names[s] = _cname(s)
+        if s in used:
+            lines.append("%sconst double %s = %s;" % (indent, names[s], src))
+    for k, s in enumerate(inst_consts):
+        names[s] = _cname(s)
+        if s in used:
+            lines.append("%sconst double %s = _c[%d];" % (indent, names[s], k))
+    for s in constants:
+        if s not in names:
+            names[s] = "K_" + s.name
+    pr = _Printer(names)
+    for s, d in hdefs:
+        names[s] = _cname(s)
+        lines.append("%sconst double %s = %s;" % (indent, names[s], pr.doprint(d)))
+    nonzero = [(tgt, e) for tgt, e in exprs_out if e != 0]
+    if nonzero:
+        reps, red = sp.cse([e for _, e in nonzero], symbols=sp.numbered_symbols("_t", real=True),
+                           optimizations=None, order="none")
+        for s, d in reps:
+            names[s] = _cname(s)
+            lines.append("%sconst double %s = %s;" % (indent, names[s], pr.doprint(d)))
+        for (tgt, _), e in zip(nonzero, red):
+            lines.append("%s%s = %s;" % (indent, tgt, pr.doprint(e)))
+    for tgt, e in exprs_out:
+        if e == 0:
+            lines.append("%s%s = 0.0;" % (indent, tgt))
+
+
+def emit_problem(sym):
+    """HIP source text of namespace prob { f, g, Df, Dg, r, Dr } for a BVP-DAE."""
+    ny, nz, np_ = sym.size_y, sym.size_z, sym.size_p
+    nj = ny + nz + np_
+    ic = sym.instance_constants
+    consts = [s for s in sym.constants if s not in ic]
+    out = ["// Generated BVP-DAE callbacks (restating the _abvp_* contract of OCP2ABVP.py:203-327).",
+           "namespace prob {"]
+    for s in consts:
+        out.append("constexpr double K_%s = %r;" % (s.name, float(sym.constants[s])))
+    yb = [(s, "_y[%d]" % k) for k, s in enumerate(sym.y)]
+    zb = [(s, "_z[%d]" % k) for k, s in enumerate(sym.z)]
+    pb = [(s, "_p[%d]" % k) for k, s in enumerate(sym.p)]
+    ab = [(sym.alpha, "_alpha")]
+    sig_fg = "const double* __restrict__ _y, const double* __restrict__ _z, const double* __restrict__ _p, " \
+             "const double* __restrict__ _c"
+    H = sym.hoist_fg
+
+    def fn(name, extra, target, rows, binds, hoister):
+        lines = ["__device__ __forceinline__ void %s(%s%s, double* __restrict__ %s) {" % (name, sig_fg, extra, target)]
+        lines.append("  (void)_y; (void)_z; (void)_p; (void)_c;")
+        if "_alpha" in extra:
+            lines.append("  (void)_alpha;")
+        _emit_block(lines, rows, hoister, binds, consts, ic)
+        lines.append("}")
+        out.extend(lines)
+
+    fn("f", "", "_f", [("_f[%d]" % i, e) for i, e in enumerate(sym.f_exprs)], yb + zb + pb, H)
+    fn("g", ", double _alpha", "_g", [("_g[%d]" % i, e) for i, e in enumerate(sym.g_exprs)], yb + zb + pb + ab, H)
+    fn("Df", "", "_Df", [("_Df[%d]" % (i * nj + j), e) for i, row in enumerate(sym.Df_exprs)
+                         for j, e in enumerate(row)], yb + zb + pb, H)
+    fn("Dg", ", double _alpha", "_Dg", [("_Dg[%d]" % (i * nj + j), e) for i, row in enumerate(sym.Dg_exprs)
+                                        for j, e in enumerate(row)], yb + zb + pb + ab, H)
+    # r and Dr: initial rows see y(t_i) = _y0, final rows y(t_f) = _y1
+    sig_r = "const double* __restrict__ _y0, const double* __restrict__ _y1, const double* __restrict__ _p, " \
+            "const double* __restrict__ _c"
+    nri = len(sym.ri_exprs)
+    y0b = [(s, "_y0[%d]" % k) for k, s in enumerate(sym.y)]
+    y1b = [(s, "_y1[%d]" % k) for k, s in enumerate(sym.y)]
+    out.append("__device__ __forceinline__ void r(%s, double* __restrict__ _r) {" % sig_r)
+    out.append("  (void)_y0; (void)_y1; (void)_p; (void)_c;")
+    out.append("  {")
+    _emit_block(out, [("_r[%d]" % i, e) for i, e in enumerate(sym.ri_exprs)], sym.hoist_ri, y0b + pb, consts, ic, "    ")
+    out.append("  }")
+    out.append("  {")
+    _emit_block(out, [("_r[%d]" % (nri + i), e) for i, e in enumerate(sym.rf_exprs)], sym.hoist_rf, y1b + pb,
+                consts, ic, "    ")
+    out.append("  }")
+    out.append("}")
+    ndr = 2 * ny + np_
+    out.append("__device__ __forceinline__ void Dr(%s, double* __restrict__ _Dr) {" % sig_r)
+    out.append("  (void)_y0; (void)_y1; (void)_p; (void)_c;")
+    rows = []
+    for i, row in enumerate(sym.Dri_exprs):
+        for j in range(ndr):
+            col = j if j < ny else (None if j < 2 * ny else j - ny)
+            rows.append(("_Dr[%d]" % (i * ndr + j), row[col] if col is not None else sp.Integer(0)))
+    out.append("  {")
+    _emit_block(out, rows, sym.hoist_ri, y0b + pb, consts, ic, "    ")
+    out.append("  }")
+    rows = []
+    for i, row in enumerate(sym.Drf_exprs):
+        for j in range(ndr):
+            col = None if j < ny else (j - ny if j < 2 * ny else j - ny)
+            rows.append(("_Dr[%d]" % ((nri + i) * ndr + j), row[col] if col is not None else sp.Integer(0)))
+    out.append("  {")
+    _emit_block(out, rows, sym.hoist_rf, y1b + pb, consts, ic, "    ")
+    out.append("  }")
+    out.append("}")
+    out.append("}  // namespace prob")
+    return "\n".join(out) + "\n"
+
+
+def _carray(name, vals):
+    return "__constant__ double %s[%d] = {%s};" % (name, len(vals), ", ".join(repr(float(v)) for v in vals))
+
+
+def choose_waves(ny, nz, np_, m):
+    """Waves per workgroup of the register-resident eliminations.
+
+    The condensed stage system is (m-1) ny x ((m-1) ny + ny + np + 1); the
+    wave count is picked so that one thread holds at most ~48 of its doubles
+    (4 waves/SIMD, 128 VGPRs), capped at 16 waves."""
+    def pick(n, ncol, cap):
+        rq = (n + 63) // 64
+        for w in range(1, cap + 1):
+            if rq * ((ncol + w - 1) // w) <= 48:
+                return w
+        return cap
+    ns = (m - 1) * ny
+    cw = pick(ns, ns + ny + np_ + 1, 16)
+    nf = 2 * ny + np_
+    fw = pick(nf, nf + 1, 16)
+    return cw, fw
+
+
+def build_unit(sym, m, tables):
+    """Complete hiprtc translation unit: callbacks, sizes, tables, kernels.
+
+    tables: object with lobatto A, b, c and gauss t, w plus the Lagrange basis
+    L(tau) and its integral I(tau) at the Gauss points and L(0), L(1)."""
+    ny, nz, np_ = sym.size_y, sym.size_z, sym.size_p
+    nc = len(sym.instance_constants)
+    cw, fw = choose_waves(ny, nz, np_, m)
+    parts = ['#include "colloc_common.h"', emit_problem(sym)]
+    parts.append("struct Cfg { static constexpr int NY = %d, NZ = %d, NP = %d, NC = %d, M = %d, CW = %d, FW = %d; };"
+                 % (ny, nz, np_, nc, m, cw, fw))
+    parts.append(_carray("c_rkA", tables.A.ravel()))
+    parts.append(_carray("c_rkb", tables.b))
+    parts.append(_carray("c_rkc", tables.c))
+    parts.append(_carray("c_gL", tables.gL.ravel()))
+    parts.append(_carray("c_gI", tables.gI.ravel()))
+    parts.append(_carray("c_gw", tables.gw))
+    parts.append(_carray("c_L0", tables.L0))
+    parts.append(_carray("c_L1", tables.L1))
+    parts.append('extern "C" __device__ int colloc_meta[8] = {%d, %d, %d, %d, %d, %d, %d, 0};'
+                 % (ny, nz, np_, m, nc, cw, fw))
+    parts.append('#include "colloc_kernels.h"')
+    return "\n".join(parts) + "\n"
+
+
+def unit_hash(text):
+    return hashlib.sha256(text.encode()).hexdigest()[:24]

@@ -1,0 +1,165 @@
+/*
+ * colloc.h -- C ABI of the MI355X collocation BVP-DAE solver (libcolloc.so).
+ *
+ * The reference solver (UW-OCP/Collocation-Solver-CUDA) exposes its hot path
+ * as a set of synchronous, numpy-in/numpy-out stage functions in
+ * collocation_solver_parallel_2d_shared.py ("solver") and
+ * solve_babd_system.py, plus a generated problem module whose callbacks
+ * _abvp_f/_g/_Df/_Dg/_r/_Dr define the BVP-DAE (OCP2ABVP.py:203-327).
+ * Each entry point below replaces one of those functions; the comment on each
+ * cites the reference function (file:line) it stands in for.
+ *
+ * Conventions
+ *  - Plain C types only; host arrays are row-major float64 in the reference's
+ *    layouts: t[N]; y[N][ny]; y_dot[(N-1)*m][ny] and z_tilde[(N-1)*m][nz]
+ *    with row i*m+j for stage j of interval i (solver:278-283); p[np].
+ *  - All solver state lives in device memory owned by the handle; host
+ *    arrays are copied in and out on the handle's stream, and every call
+ *    returns after the stream has drained.
+ *  - Return value 0 = success, otherwise one of the CS_E_* codes below;
+ *    cs_last_error() gives a message for the calling thread.
+ *  - A handle is not re-entrant: one host thread per handle.
+ *  - The problem-specific device code (the generated callbacks plus all
+ *    kernels specialised to the problem's sizes) is a HIP source text that
+ *    the caller generates; cs_jit_compile() compiles it with hiprtc and
+ *    cs_load_module() loads the resulting gfx950 code object.
+ */
+#ifndef COLLOC_H
+#define COLLOC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CS_OK 0
+#define CS_E_HIP 1          /* a HIP runtime call failed                     */
+#define CS_E_SHAPE 2        /* sizes do not match the handle / module          */
+#define CS_E_SINGULAR 3     /* a zero or non-finite pivot was met              */
+#define CS_E_ARG 4          /* invalid argument                               */
+#define CS_E_COMPILE 5      /* hiprtc compilation failed                      */
+#define CS_E_NOMODULE 6     /* no problem module loaded                       */
+#define CS_E_UNSUPPORTED 7  /* shape outside what the kernels support          */
+
+typedef struct cs_handle cs_handle;
+
+typedef struct {
+  int ny;      /* size_y  (OCP2ABVP.py:344)                         */
+  int nz;      /* size_z                                            */
+  int np;      /* size_p                                            */
+  int m;       /* collocation stages (solver -m flag, solver:2381)   */
+  int n_max;   /* largest number of time nodes N the handle holds     */
+  int nc;      /* number of per-instance constants (0 if none)        */
+} cs_dims;
+
+/* Error text of the last failed call on this thread. */
+const char* cs_last_error(void);
+
+/* Library version string. */
+const char* cs_version(void);
+
+/* Compile a HIP source text for gfx950 with hiprtc.  include_dir is added to
+ * the include path.  On success *code (malloc'ed, free with cs_free) holds the
+ * code object of *size bytes.  *log (may be NULL) receives the compiler log
+ * (malloc'ed).  Stands in for Numba's JIT of the generated bvp_problem module
+ * and the @cuda.jit kernels (solver:10-17). */
+int cs_jit_compile(const char* source, const char* name, const char* include_dir,
+                   void** code, size_t* size, char** log);
+void cs_free(void* ptr);
+
+/* Create a handle owning all device workspace for problems of the given
+ * sizes, and one HIP stream.  Replaces the per-call cuda.device_array
+ * allocations of every stage wrapper (e.g. solver:294-307, 411-445). */
+int cs_create(const cs_dims* dims, cs_handle** out);
+void cs_destroy(cs_handle* h);
+
+/* Load a code object produced by cs_jit_compile for this handle's sizes. */
+int cs_load_module(cs_handle* h, const void* code, size_t size);
+
+/* Time mesh (solver:53-55 t_span).  2 <= N <= n_max. */
+int cs_set_mesh(cs_handle* h, const double* t, int N);
+
+/* Per-instance constants of the generated problem (nc values). */
+int cs_set_constants(cs_handle* h, const double* c, int nc);
+
+/* Collocation state: y[N][ny], y_dot, z_tilde, p (solver:101 outputs). */
+int cs_set_state(cs_handle* h, const double* y, const double* y_dot,
+                 const double* z_tilde, const double* p);
+int cs_get_state(cs_handle* h, double* y, double* y_dot, double* z_tilde, double* p);
+
+/* form_initial_input_parallel (solver:287-356): y = y0, p = p0, stage values
+ * interpolated linearly from node values and y_dot = f(y_stage, z_tilde, p). */
+int cs_form_initial_input(cs_handle* h, const double* y0, const double* z0, const double* p0);
+
+/* Re-form the collocation state from the node values, mesh and parameters
+ * last passed to cs_form_initial_input (kept in device memory), without any
+ * host transfer.  Same result as repeating that call; used to restart a
+ * solve from its initial guess (e.g. repeated or batched solves). */
+int cs_restart(cs_handle* h);
+
+/* compute_f_q_parallel (solver:399-466): residual of the current state.
+ * *norm_inf = ||f_q||_inf (the host reduction of solver:116).  Any of the
+ * output arrays may be NULL; otherwise y_tilde[(N-1)*m][ny],
+ * f_a[N-1][m*(ny+nz)], f_b[N-1][ny], r_bc[ny+np]. */
+int cs_compute_f_q(cs_handle* h, double alpha, double* norm_inf, double* y_tilde,
+                   double* f_a, double* f_b, double* r_bc);
+
+/* construct_jacobian_parallel (solver:611-797): stage Jacobians at the
+ * stage values of the last cs_compute_f_q call.  Kept on the device; when
+ * non-NULL, Df[(N-1)*m][ny][ny+nz+np], Dg[(N-1)*m][nz][ny+nz+np] and
+ * Dr[ny+np][2*ny+np] (= [B_0 | B_n | V_n], solver:655-665) are copied out. */
+int cs_construct_jacobian(cs_handle* h, double alpha, double* Df, double* Dg, double* Dr);
+
+/* reduce_jacobian_parallel (solver:840-1001): per-interval condensation to
+ * A_i dy_i + C_i dy_{i+1} + H_i dp = b_i with C_i = I.  Outputs (may be
+ * NULL): A[N-1][ny][ny], H[N-1][ny][np], b[N-1][ny]. */
+int cs_reduce_jacobian(cs_handle* h, double* A, double* H, double* b);
+
+/* partition_factorization_parallel + construct_babd_mshoot + qr_decomposition
+ * + backward_substitution + recover_babd_solution +
+ * partition_backward_substitution_parallel (solve_babd_system.py:93-572,
+ * driven at solver:134-147).  Outputs (may be NULL): delta_y[N][ny],
+ * delta_p[np]. */
+int cs_solve_babd(cs_handle* h, double* delta_y, double* delta_p);
+
+/* recover_delta_k_parallel (solver:1580-1650).  Outputs (may be NULL):
+ * delta_y_dot[(N-1)*m][ny], delta_z_tilde[(N-1)*m][nz]. */
+int cs_recover_delta_k(cs_handle* h, double* delta_y_dot, double* delta_z_tilde);
+
+/* The damped Newton loop with line search (solver:109-174).  Runs until
+ * ||f_q||_inf < tol or max_iter iterations; *iter_time receives the loop
+ * counter at exit (as the reference's iter_time), *norm the last residual
+ * norm, norms[0..hist_len) the norm history (may be NULL). */
+int cs_newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
+              double* norm, double* norms, int hist_len);
+
+/* compute_segment_residual_parallel (solver:1770-1892) with residual_type 1
+ * and scale_by_time.  residual[N] (may be NULL), *max_residual. */
+int cs_segment_residual(cs_handle* h, double alpha, double tol, double* residual,
+                        double* max_residual);
+
+/* recover_solution_parallel (solver:2034-2075): z at the nodes, z[N][nz]. */
+int cs_recover_solution(cs_handle* h, double* z);
+
+/* Kernel timing with HIP events on the handle's stream.  cs_profile(h, 1)
+ * starts recording every launch of the kernels listed in CS_KERNEL_*;
+ * cs_profile_get returns the summed milliseconds and launch count. */
+#define CS_KERNEL_RESIDUAL 0
+#define CS_KERNEL_JACOBIAN 1
+#define CS_KERNEL_CONDENSE 2
+#define CS_KERNEL_BABD 3
+#define CS_KERNEL_RECOVER 4
+#define CS_KERNEL_COUNT 5
+int cs_profile(cs_handle* h, int enable);
+int cs_profile_get(cs_handle* h, int kernel, double* total_ms, int* launches);
+
+/* Device status word (bit 0: singular stage block, bit 1: singular stage
+ * system, bit 2: singular boundary system); cleared by reading. */
+int cs_status(cs_handle* h, int* status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COLLOC_H */

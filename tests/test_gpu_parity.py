@@ -1,0 +1,219 @@
+"""Parity of the HIP path (through libcolloc.so) with the CPU oracle.
+
+Stage by stage on seeded random states of a small tanh instance (n_x=4,
+Nint=32, m=5), then whole solves: the LQ configuration (n_x=4, Nint=128,
+m=4) against the oracle and its matrix-exponential closed form, and the
+n_x=20 tanh family at a reduced mesh against the oracle.  Tolerances are
+relative to the largest entry of the compared quantity.
+"""
+import numpy as np
+import pytest
+
+from colloc_amd import problems, solver
+from oracle import babd as obabd
+from oracle import families
+from oracle import solver as osolver
+from oracle import tables as otables
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def family_of(inst):
+    return families.TanhFamily(inst.A, inst.B, inst.x0, C=inst.C, s=inst.s, q=inst.q, c_u=inst.c_u)
+
+
+@pytest.fixture(scope="module")
+def small():
+    inst, m = problems.config(1, nx=4, nu=2, nint=32)
+    bvp = solver.BvpDae(inst.ocp_text())
+    say("small tanh: ny=%d nz=%d np=%d m=%d N=%d" % (bvp.size_y, bvp.size_z, bvp.size_p, m, bvp.N))
+    S = solver.CollocationSolver(bvp, m, n_max=bvp.N)
+    yield inst, m, bvp, S
+    S.close()
+
+
+def seeded_state(prob, m, N, seed, scale=0.5):
+    rng = np.random.default_rng(seed)
+    y = rng.normal(scale=scale, size=(N, prob.size_y))
+    yd = rng.normal(scale=scale, size=((N - 1) * m, prob.size_y))
+    zt = rng.normal(scale=scale, size=((N - 1) * m, prob.size_z))
+    p = rng.normal(scale=scale, size=prob.size_p)
+    return y, yd, zt, p
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_stage_parity(small, seed):
+    inst, m, bvp, S = small
+    prob = family_of(inst)
+    N = bvp.N
+    rng = np.random.default_rng(100 + seed)
+    t = np.sort(np.concatenate([[0.0, inst.T], rng.uniform(0.0, inst.T, N - 2)]))
+    rk = otables.lobatto(m)
+    y, yd, zt, p = seeded_state(prob, m, N, seed)
+    S.set_mesh(t)
+    S.set_state(y, yd, zt, p)
+    alpha = 0.1
+
+    say("compute_f_q seed=%d" % seed)
+    nrm, yt, fa, fb, rbc = S.compute_f_q(alpha)
+    f_q, yt_o, fa_o, fb_o, rbc_o = osolver.compute_f_q(prob, m, N, rk, t, y, yd, zt, p, alpha)
+    assert rel_err(yt, yt_o) < 1e-14
+    assert rel_err(fa, fa_o) < 1e-13
+    assert rel_err(fb, fb_o) < 1e-13
+    assert rel_err(rbc, rbc_o) < 1e-14
+    assert abs(nrm - np.max(np.abs(f_q))) <= 1e-13 * np.max(np.abs(f_q))
+
+    say("construct_jacobian seed=%d" % seed)
+    Df, Dg, Dr = S.construct_jacobian(alpha)
+    assert rel_err(Df, prob.Df(yt_o, zt, p)) < 1e-13
+    assert rel_err(Dg, prob.Dg(yt_o, zt, p)) < 1e-13
+    assert rel_err(Dr, prob.Dr(y[0], y[-1], p)) < 1e-14
+
+    say("reduce_jacobian seed=%d" % seed)
+    A, C, H, b = S.reduce_jacobian()
+    J, V, D, W, B0, Bn, Vn = osolver.construct_jacobian(prob, m, N, rk, t, y, yt_o, zt, p, alpha)
+    A_o, C_o, H_o, b_o = osolver.reduce_jacobian(prob, m, N, W, D, J, V, fa_o, fb_o)
+    assert rel_err(A, A_o) < 1e-11
+    assert rel_err(H, H_o) < 1e-11
+    assert rel_err(b, b_o) < 1e-11
+
+    say("solve_babd seed=%d" % seed)
+    dy, dp = S.solve_babd()
+    dy_o, dp_o = obabd.solve_babd(prob.size_y, prob.size_p, N, A_o, C_o, H_o, b_o, B0, Bn, Vn, rbc_o, M=8)
+    assert rel_err(dy, dy_o) < 1e-10
+    assert rel_err(dp, dp_o) < 1e-10
+
+    say("recover_delta_k seed=%d" % seed)
+    dyd, dz = S.recover_delta_k()
+    _, dyd_o, dz_o = osolver.recover_delta_k(prob, m, N, dy_o, dp_o, fa_o, J, V, W)
+    assert rel_err(dyd, dyd_o) < 1e-10
+    assert rel_err(dz, dz_o) < 1e-10
+    assert S.status() == 0
+
+
+def test_initial_input_and_segment_residual(small):
+    inst, m, bvp, S = small
+    prob = family_of(inst)
+    N = bvp.N
+    t = np.linspace(0.0, inst.T, N)
+    rng = np.random.default_rng(5)
+    y0 = rng.normal(size=(N, prob.size_y))
+    z0 = rng.normal(size=(N, prob.size_z))
+    p0 = rng.normal(size=prob.size_p)
+    S.set_mesh(t)
+    say("form_initial_input")
+    S.form_initial_input(y0, z0, p0)
+    y, yd, zt, p = S.get_state()
+    _, yd_o, zt_o = osolver.form_initial_input(prob, m, N, y0, z0, p0, otables.lobatto(m))
+    assert rel_err(yd, yd_o) < 1e-13
+    assert rel_err(zt, zt_o) < 1e-15
+    say("segment residual")
+    res, mx = S.segment_residual(0.1, 1e-8)
+    res_o, mx_o = osolver.compute_segment_residual(prob, m, N, t, y, yd, zt, p, 0.1, 1e-8)
+    assert rel_err(res, res_o) < 1e-12
+    assert abs(mx - mx_o) <= 1e-12 * mx_o
+    say("recover_solution")
+    z = S.recover_solution()
+    assert rel_err(z, osolver.recover_solution(m, N, zt)) < 1e-15
+
+
+def compare_solves(res, ref, tol=1e-9):
+    assert res["success"] == ref["success"] == 1
+    assert [h["iterations"] for h in res["history"]] == [h["iterations"] for h in ref["history"]]
+    assert res["N"] == ref["t"].shape[0]
+    for k in ("y", "z", "p"):
+        assert rel_err(res[k], ref[k]) < tol, k
+
+
+def test_lq_solve_parity():
+    inst, m = problems.config(0)
+    bvp = solver.BvpDae(inst.ocp_text())
+    say("LQ solve N=%d" % bvp.N)
+    res = solver.collocation_solver_parallel(bvp, m)
+    prob = family_of(inst)
+    ref = osolver.collocation_solver(prob, m, bvp.T0, bvp.Y0, bvp.Z0, bvp.P0, tol=bvp.tolerance)
+    compare_solves(res, ref)
+    assert res["history"][0]["iterations"] == 1
+    Ys, Us, P = prob.lq_exact(inst.T, res["t"])
+    assert np.max(np.abs(res["y"] - Ys)) < 1e-9
+    assert np.max(np.abs(res["z"] - Us)) < 1e-9
+    assert np.max(np.abs(res["p"] - P)) < 1e-9
+    assert res["max_residual"] <= 1.0
+
+
+def homotopy_families(inst):
+    """Oracle problems matching inst.homotopy_schedule() (tanh scale S first at 0)."""
+    out = []
+    for vals in inst.homotopy_schedule():
+        s = vals[inst.instance_constants().index("S")]
+        out.append(families.TanhFamily(inst.A, inst.B, inst.x0, C=inst.C, s=s, q=inst.q, c_u=inst.c_u))
+    return out
+
+
+def test_tanh20_plain_guess_stalls_identically():
+    """From the all-ones guess the reference's damped Newton stalls on this
+    family (no step passes the strict-decrease line search); the HIP path
+    must stall at the same merit value."""
+    inst, m = problems.config(1, nint=64)
+    bvp = solver.BvpDae(inst.ocp_text())
+    S = solver.CollocationSolver(bvp, m, n_max=bvp.N)
+    S.set_mesh(bvp.T0)
+    S.form_initial_input(bvp.Y0, bvp.Z0, bvp.P0)
+    say("tanh20 plain Newton, 30 iterations")
+    it, nrm, hist = S.newton(0.1, 1e-8, 30)
+    S.close()
+    prob = family_of(inst)
+    rk = otables.lobatto(m)
+    y, yd, zt = osolver.form_initial_input(prob, m, bvp.N, bvp.Y0, bvp.Z0, bvp.P0, rk)
+    ref = osolver.newton_solve(prob, m, rk, bvp.T0, y, yd, zt, bvp.P0.copy(), 0.1, 1e-8, 30)
+    assert it == ref[4] == 29
+    np.testing.assert_allclose(hist, ref[5], rtol=1e-6)
+
+
+@pytest.mark.parametrize("nint", [64, 200])
+def test_tanh20_solve_parity(nint):
+    inst, m = problems.config(1, nint=nint)
+    bvp = solver.BvpDae(inst.ocp_text())
+    say("tanh20 solve N=%d" % bvp.N)
+    res = solver.collocation_solver_parallel(bvp, m, homotopy=inst.homotopy_schedule())
+    ref = osolver.collocation_solver(family_of(inst), m, bvp.T0, bvp.Y0, bvp.Z0, bvp.P0, tol=bvp.tolerance,
+                                     homotopy=homotopy_families(inst))
+    compare_solves(res, ref)
+    # the oracle's residual of the GPU solution meets the tolerance
+    f_q = osolver.compute_f_q(family_of(inst), m, res["N"], otables.lobatto(m), res["t"], res["y"], res["y_dot"],
+                              res["z_tilde"], res["p"], 0.1)[0]
+    assert np.max(np.abs(f_q)) < bvp.tolerance
+
+
+def test_tanh20_full_size_converges():
+    """The n_x=20 family at its full mesh (Nint=4096, m=5): converged to the
+    tolerance by the oracle's independent residual evaluation, accepted by
+    the segment residual, and the same Newton counts as the reduced meshes."""
+    inst, m = problems.config(1)
+    bvp = solver.BvpDae(inst.ocp_text())
+    say("tanh20 full solve N=%d" % bvp.N)
+    res = solver.collocation_solver_parallel(bvp, m, homotopy=inst.homotopy_schedule())
+    assert res["success"] == 1 and res["N"] == 4097
+    assert [h["iterations"] for h in res["history"]] == [1, 6]
+    assert res["max_residual"] <= 1.0
+    prob = family_of(inst)
+    f_q = osolver.compute_f_q(prob, m, res["N"], otables.lobatto(m), res["t"], res["y"], res["y_dot"],
+                              res["z_tilde"], res["p"], 0.1)[0]
+    assert np.max(np.abs(f_q)) < bvp.tolerance
+    # boundary conditions and multiplier relations hold
+    nx = inst.nx
+    assert np.max(np.abs(res["y"][0, :nx] - inst.x0)) < 1e-8
+    assert np.max(np.abs(res["y"][-1, :nx])) < 1e-8
+    np.testing.assert_allclose(res["p"][:nx], -res["y"][0, nx:], atol=1e-8)
+    np.testing.assert_allclose(res["p"][nx:], res["y"][-1, nx:], atol=1e-8)
+    # optimality: u = -B' lambda / (2 c_u) at the nodes
+    np.testing.assert_allclose(res["z"], -(res["y"][:, nx:] @ inst.B) / (2 * inst.c_u), atol=1e-7)
