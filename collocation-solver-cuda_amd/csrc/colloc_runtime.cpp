@@ -232,7 +232,7 @@ int babd(cs_handle* h) {
   size_t base = 0;
   double* rin = h->rows0;
   double* rout = h->rows1;
-  unsigned pair_block = (unsigned)((h->recw + 63) / 64 * 64);
+  unsigned pair_block = (unsigned)((h->recw + 15) / 16 * 64);   // 4 lanes per record column
   while (nrows > 1) {
     int npairs = nrows / 2;
     double* recs = h->crrec + base * (size_t)h->ny * h->recw;

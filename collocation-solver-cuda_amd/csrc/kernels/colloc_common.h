@@ -9,13 +9,57 @@ namespace colloc {
 constexpr int WAVE = 64;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
-__device__ __forceinline__ int wave_id() { return threadIdx.x >> 6; }
+// Wave index as a uniform (scalar) value, so that tests on it compile to
+// scalar branches instead of exec-mask manipulation.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 // Broadcast a double held by lane `src` (uniform) to the whole wave.
 __device__ __forceinline__ double readlane_d(double v, int src) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
   return __hiloint2double(hi, lo);
+}
+
+// Compile-time loop: f(Int<I>{}) for I in [B, E).  Used where a loop index
+// must be a constant (register-array subscripts) even when the body is too
+// large for the unroller.
+template <int V>
+struct Int {
+  static constexpr int value = V;
+};
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(Int<B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// 64-bit DPP move within rows of 16 lanes (CTRL: a DPP control code).
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+  const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)v, (int)(unsigned)v, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(v >> 32), (int)(v >> 32), CTRL, 0xf, 0xf, false);
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+// Wave-wide max of a u64 (result uniform): butterflies within rows of 16
+// (quad_perm xor 1, xor 2, row_half_mirror, row_mirror) then the four rows.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  unsigned long long o;
+  o = dpp_u64<0xB1>(v);  v = o > v ? o : v;
+  o = dpp_u64<0x4E>(v);  v = o > v ? o : v;
+  o = dpp_u64<0x141>(v); v = o > v ? o : v;
+  o = dpp_u64<0x140>(v); v = o > v ? o : v;
+  unsigned long long m = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)v, 16 * r);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(v >> 32), 16 * r);
+    const unsigned long long x = ((unsigned long long)hi << 32) | lo;
+    m = x > m ? x : m;
+  }
+  return m;
 }
 
 // NaN-propagating max (fmax would drop a NaN and hide a diverged residual).

@@ -32,12 +32,16 @@ constexpr int REC = (NS + M * NY + M * NZ) * NV;
 constexpr int CRW = 2 * NY + NP + 1;       // BABD row width
 constexpr int RECW = 3 * NY + NP + 1;      // cyclic-reduction record width
 constexpr int NFIN = 2 * NY + NP;          // boundary (final) system size
+constexpr int CR_PARTS = 4;                          // lanes per record column in cr_pair
+constexpr int CR_COLS = WAVE / CR_PARTS;              // columns per wave
+constexpr int CR_ROWS = (2 * NY + CR_PARTS - 1) / CR_PARTS;   // rows per lane
+constexpr int CR_THREADS = (RECW + CR_COLS - 1) / CR_COLS * WAVE;
 
 static_assert(M >= 2, "need at least two collocation stages");
 static_assert(NY <= 64, "cyclic-reduction kernels hold one row per lane");
 
 // -------------------------------------------------------------------- fill
-__device__ void fill_u64(unsigned long long* p, unsigned long long v, int n) {
+__device__ __forceinline__ void fill_u64(unsigned long long* p, unsigned long long v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -47,7 +51,7 @@ __device__ void fill_u64(unsigned long long* p, unsigned long long v, int n) {
 // collocation_update_kernel + compute_f_q_kernel1/2 (solver:478-559) and the
 // host max-norm of f_q (solver:116) as a block max folded into `norm`.
 template <bool FULL>
-__device__ void residual(int nint, const double* __restrict__ t, const double* __restrict__ y,
+__device__ __forceinline__ void residual(int nint, const double* __restrict__ t, const double* __restrict__ y,
                          const double* __restrict__ ydot, const double* __restrict__ ztil,
                          const double* __restrict__ p, const double* __restrict__ c, double alpha,
                          double* __restrict__ ytil, double* __restrict__ fa, double* __restrict__ fb,
@@ -117,7 +121,7 @@ __device__ void residual(int nint, const double* __restrict__ t, const double* _
 // One thread per stage writes Df, Dg densely; thread Nint*M writes Dr.
 // Restates construct_jacobian_kernel (solver:680-797) without materialising
 // J, V, D, W (the condensation kernel assembles what it needs).
-__device__ void jacobian(int nint, const double* __restrict__ y, const double* __restrict__ ytil,
+__device__ __forceinline__ void jacobian(int nint, const double* __restrict__ y, const double* __restrict__ ytil,
                          const double* __restrict__ ztil, const double* __restrict__ p,
                          const double* __restrict__ c, double alpha, double* __restrict__ jac,
                          double* __restrict__ Dr) {
@@ -154,21 +158,21 @@ __device__ void jacobian(int nint, const double* __restrict__ y, const double* _
 //   dy_{i+1} - u_{M-1} = -f_b, giving the BABD row
 //   G = -U_{M-1}[:, :NY], C = I, H = -U_{M-1}[:, NY:NY+NP], b = -f_b + U_{M-1}[:, NV-1].
 template <int CW>
-__device__ void condense(int nint, const double* __restrict__ t, const double* __restrict__ jac,
+__device__ __forceinline__ void condense(int nint, const double* __restrict__ t, const double* __restrict__ jac,
                          const double* __restrict__ fa, const double* __restrict__ fb,
                          double* __restrict__ rows, double* __restrict__ rec, int* __restrict__ status) {
   using Solver = GJ<NS, NS + NV, CW>;
   constexpr int NT = CW * WAVE;
   __shared__ double sT[M * NY * NV];
-  __shared__ double sG[NZ1 * NV];
-  __shared__ double sgz[NZ1 * NZ1];
   __shared__ double colbuf[Solver::COLBUF];
   __shared__ int pivrow[NS];
+  __shared__ double pivval[NS];
   __shared__ int invperm[Solver::RQ * WAVE];
   __shared__ int sbad;
 
   const int i = blockIdx.x;
   const int tid = threadIdx.x;
+  const int lane = lane_id(), wave = wave_id();
   if (tid == 0) sbad = 0;
   const double h = t[i + 1] - t[i];
   double* recU = rec + (size_t)i * REC;
@@ -176,62 +180,85 @@ __device__ void condense(int nint, const double* __restrict__ t, const double* _
   double* recG = recT + M * NY * NV;
   int bad = 0;
 
-  // ---- phase 1: per-stage elimination of the algebraic unknowns
-  for (int j = 0; j < M; ++j) {
+  // ---- phase 1: wave j eliminates the algebraic unknowns of stage j.
+  // Lane l holds columns l + 64 s of [g_z | g_y g_p f_a,g] (NZ rows each);
+  // Gauss-Jordan with partial pivoting and physical row swaps, the pivot
+  // column broadcast by v_readlane, so the wave needs no barrier.  The same
+  // wave then forms T_j = [h_y | h_p | f_a,h] - h_z G_j for its columns.
+  constexpr int NCZ = NZ + NV;
+  constexpr int SL = (NCZ + WAVE - 1) / WAVE;
+  for (int j = wave; j < M; j += CW) {
     const double* Dfj = jac + ((size_t)i * M + j) * JSTAGE;
     const double* Dgj = Dfj + NY * NJC;
     const double* faj = fa + (size_t)i * M * NYZ + j * NYZ;
-    if (NZ > 0) {
-      for (int e = tid; e < NZ * NZ; e += NT) sgz[e] = Dgj[(e / NZ) * NJC + NY + (e % NZ)];
-      for (int e = tid; e < NZ * NV; e += NT) {
-        const int r = e / NV, q = e - (e / NV) * NV;
-        sG[e] = q < NY ? Dgj[r * NJC + q] : (q < NY + NP ? Dgj[r * NJC + NZ + q] : faj[NY + r]);
-      }
-      __syncthreads();
-      // Gauss-Jordan on [g_z | rhs] in LDS, one thread per column.
-      for (int k = 0; k < NZ; ++k) {
-        int pr = k;
-        double best = fabs(sgz[k * NZ + k]);
-        for (int r = k + 1; r < NZ; ++r) {
-          const double v = fabs(sgz[r * NZ + k]);
-          if (v > best) { best = v; pr = r; }
+    double gc[SL][NZ1];
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) {
+      const int cc = lane + WAVE * sl;
+#pragma unroll
+      for (int r = 0; r < NZ; ++r) {
+        double v = 0.0;
+        if (cc < NZ) v = Dgj[r * NJC + NY + cc];
+        else if (cc < NCZ) {
+          const int q = cc - NZ;
+          v = q < NY ? Dgj[r * NJC + q] : (q < NY + NP ? Dgj[r * NJC + NZ + q] : faj[NY + r]);
         }
-        // column k with rows k and pr exchanged (indices resolved without
-        // runtime-indexed register arrays)
-        double colk[NZ1];
-#pragma unroll
-        for (int r = 0; r < NZ; ++r) colk[r] = sgz[(r == k ? pr : (r == pr ? k : r)) * NZ + k];
-        const double piv = sgz[pr * NZ + k];
-        if (!(fabs(piv) > 0.0)) bad |= 1;
-        const double inv = 1.0 / piv;
-        __syncthreads();
-        for (int cidx = tid; cidx < NZ + NV; cidx += NT) {
-          double* base = cidx < NZ ? sgz + cidx : sG + (cidx - NZ);
-          const int ld = cidx < NZ ? NZ : NV;
-          if (pr != k) {
-            const double tmp = base[k * ld];
-            base[k * ld] = base[pr * ld];
-            base[pr * ld] = tmp;
-          }
-          const double pk = base[k * ld] * inv;
-#pragma unroll
-          for (int r = 0; r < NZ; ++r) base[r * ld] = (r == k) ? pk : fma(-colk[r], pk, base[r * ld]);
-        }
-        __syncthreads();
+        gc[sl][r] = v;
       }
-      for (int e = tid; e < NZ * NV; e += NT) recG[j * NZ * NV + e] = sG[e];
     }
-    // T_j = [h_y | h_p | f_a,h] - h_z G_j
-    for (int e = tid; e < NY * NV; e += NT) {
-      const int r = e / NV, q = e - (e / NV) * NV;
-      double v = q < NY ? Dfj[r * NJC + q] : (q < NY + NP ? Dfj[r * NJC + NZ + q] : faj[r]);
 #pragma unroll
-      for (int s = 0; s < NZ; ++s) v = fma(-Dfj[r * NJC + NY + s], sG[s * NV + q], v);
-      sT[j * NY * NV + e] = v;
-      recT[j * NY * NV + e] = v;
+    for (int k = 0; k < NZ; ++k) {
+      double ck[NZ1];
+#pragma unroll
+      for (int r = 0; r < NZ; ++r) ck[r] = readlane_d(gc[0][r], k);
+      int pr = k;
+      double best = fabs(ck[k]);
+#pragma unroll
+      for (int r = k + 1; r < NZ; ++r)
+        if (fabs(ck[r]) > best) { best = fabs(ck[r]); pr = r; }
+      // exchange rows k and pr by per-row conditional swaps (at most one fires)
+#pragma unroll
+      for (int r = k + 1; r < NZ; ++r) {
+        const bool sw = r == pr;
+        const double t0 = ck[r];
+        ck[r] = sw ? ck[k] : ck[r];
+        ck[k] = sw ? t0 : ck[k];
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+          const double t1 = gc[sl][r];
+          gc[sl][r] = sw ? gc[sl][k] : gc[sl][r];
+          gc[sl][k] = sw ? t1 : gc[sl][k];
+        }
+      }
+      const double cpr = ck[k];
+      if (!(fabs(cpr) > 0.0)) bad |= 1;
+      const double inv = 1.0 / cpr;
+#pragma unroll
+      for (int sl = 0; sl < SL; ++sl) {
+        const double pk = gc[sl][k] * inv;
+#pragma unroll
+        for (int r = 0; r < NZ; ++r) gc[sl][r] = (r == k) ? pk : fma(-ck[r], pk, gc[sl][r]);
+      }
     }
-    __syncthreads();
+    // gc now holds [I | G_j]; write G_j and form T_j column by column
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) {
+      const int cc = lane + WAVE * sl;
+      if (cc < NZ || cc >= NCZ) continue;
+      const int q = cc - NZ;
+#pragma unroll
+      for (int r = 0; r < NZ; ++r) recG[(j * NZ + r) * NV + q] = gc[sl][r];
+      for (int r = 0; r < NY; ++r) {
+        const double* hrow = Dfj + r * NJC;
+        double v = q < NY ? hrow[q] : (q < NY + NP ? hrow[NZ + q] : faj[r]);
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) v = fma(-hrow[NY + z], gc[sl][z], v);
+        sT[(j * NY + r) * NV + q] = v;
+        recT[(j * NY + r) * NV + q] = v;
+      }
+    }
   }
+  __syncthreads();
 
   // ---- phase 2: assemble [S | RHS] into registers
   Solver gj;
@@ -267,7 +294,7 @@ __device__ void condense(int nint, const double* __restrict__ t, const double* _
 
   // ---- phase 3: Gauss-Jordan
   for (int k = tid; k < Solver::RQ * WAVE; k += NT) invperm[k] = -1;
-  if (gj.eliminate(colbuf, pivrow)) bad |= 2;
+  if (gj.eliminate(colbuf, pivrow, pivval)) bad |= 2;
   __syncthreads();
   for (int k = tid; k < NS; k += NT) invperm[pivrow[k]] = k;
   if (bad) atomicOr(&sbad, bad);
@@ -288,7 +315,7 @@ __device__ void condense(int nint, const double* __restrict__ t, const double* _
       const int cc = Solver::col(cq);
       if (cc < NS || cc >= NS + NV) continue;
       const int qc = cc - NS;
-      const double v = gj.a[q][cq];
+      const double v = gj.a[q][cq] / pivval[kk];
       recU[kk * NV + qc] = v;
       if (last) {
         if (qc < NY) rowi[a * CRW + qc] = -v;
@@ -311,90 +338,126 @@ __device__ void condense(int nint, const double* __restrict__ t, const double* _
 // step as partition_factorization_kernel, solve_babd_system.py:184-246):
 // the top NY rows [R | Gp | Ep | Jp | d] are the record used to recover x_M,
 // the bottom NY rows are the combined row [G | C | H | b] of level l+1.
-// Thread c owns column c of the 2NY x RECW block in registers; the
-// Householder vector is broadcast through LDS.  An odd last row is carried.
-__device__ void cr_pair(int nrows, const double* __restrict__ rows_in, double* __restrict__ rows_out,
-                        double* __restrict__ recs) {
+// Each column of the 2NY x RECW block is held by CR_PARTS lanes of one wave
+// (CR_ROWS rows each), so a pair has many waves and short per-lane chains;
+// column sums over the parts are two lane shuffles.  The Householder vector
+// is broadcast through LDS, where rows above the current pivot are kept zero
+// so every loop has a fixed trip count.  An odd last row is carried.
+__device__ __forceinline__ void cr_pair(int nrows, const double* __restrict__ rows_in,
+                                        double* __restrict__ rows_out, double* __restrict__ recs) {
   constexpr int R2 = 2 * NY;
-  __shared__ double sv[2][R2];
+  __shared__ double sv[2][CR_PARTS * CR_ROWS];
   __shared__ double sbeta[2];
   const int k = blockIdx.x;
   const int npairs = nrows >> 1;
-  const int cidx = threadIdx.x;
+  const int tid = threadIdx.x;
   if (k == npairs) {  // carry the odd last row
     const double* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
     double* dst = rows_out + (size_t)npairs * NY * CRW;
-    for (int e = cidx; e < NY * CRW; e += blockDim.x) dst[e] = src[e];
+    for (int e = tid; e < NY * CRW; e += blockDim.x) dst[e] = src[e];
     return;
   }
+  for (int e = tid; e < 2 * CR_PARTS * CR_ROWS; e += blockDim.x) (&sv[0][0])[e] = 0.0;
+  const int lane = lane_id();
+  const int part = lane / CR_COLS;                 // which CR_ROWS-row slice
+  const int cidx = wave_id() * CR_COLS + (lane % CR_COLS);
+  const bool live = cidx < RECW;
   const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
   const double* rb = ra + NY * CRW;
   // column layout: [x_M (NY) | x_L (NY) | x_R (NY) | p (NP) | rhs]
-  double a[R2];
-  if (cidx < RECW) {
+  double a[CR_ROWS];
 #pragma unroll
-    for (int r = 0; r < NY; ++r) {
-      double va, vb;
-      if (cidx < NY) { va = ra[r * CRW + NY + cidx]; vb = rb[r * CRW + cidx]; }
-      else if (cidx < 2 * NY) { va = ra[r * CRW + cidx - NY]; vb = 0.0; }
-      else if (cidx < 3 * NY) { va = 0.0; vb = rb[r * CRW + cidx - NY]; }
-      else { va = ra[r * CRW + cidx - NY]; vb = rb[r * CRW + cidx - NY]; }
-      a[r] = va;
-      a[NY + r] = vb;
+  for (int i = 0; i < CR_ROWS; ++i) {
+    const int r = part * CR_ROWS + i;
+    double v = 0.0;
+    if (live && r < R2) {
+      const bool top = r < NY;
+      const double* src = top ? ra + r * CRW : rb + (r - NY) * CRW;
+      if (cidx < NY) v = top ? src[NY + cidx] : src[cidx];
+      else if (cidx < 2 * NY) v = top ? src[cidx - NY] : 0.0;
+      else if (cidx < 3 * NY) v = top ? 0.0 : src[cidx - NY];
+      else v = src[cidx - NY];
     }
-  } else {
-#pragma unroll
-    for (int r = 0; r < R2; ++r) a[r] = 0.0;
+    a[i] = v;
   }
+  // partial sum of squares over this lane's rows at or below the pivot row
+  double tail = 0.0;
+#pragma unroll
+  for (int i = 0; i < CR_ROWS; ++i) tail = fma(a[i], a[i], tail);
+  __syncthreads();
   const double eps = 2.220446049250313e-16;
   bool stopped = false;
-#pragma unroll
-  for (int kk = 0; kk < NY; ++kk) {
-    const int buf = kk & 1;
+  static_for<0, NY>([&](auto kc) {
+    constexpr int kk = decltype(kc)::value;
+    constexpr int buf = kk & 1;
+    constexpr int pk = kk / CR_ROWS, ik = kk % CR_ROWS;
+    // column norm below the pivot and the pivot entry, for every column
+    double tfull = tail + __shfl_xor(tail, CR_COLS);
+    if (CR_PARTS > 2) tfull += __shfl_xor(tfull, 2 * CR_COLS);
+    const double akk = __shfl(a[ik], pk * CR_COLS + (lane % CR_COLS));
     if (cidx == kk) {
-      double s0 = 0.0, s1 = 0.0;
+      const double nrm = sqrt(tfull);
+      const double g = akk >= 0.0 ? -nrm : nrm;
+      const double v0 = akk - g;
+      const double vs = tfull - akk * akk + v0 * v0;
+      const bool reflect = !(vs < eps);
 #pragma unroll
-      for (int r = kk; r < R2; r += 2) {
-        s0 = fma(a[r], a[r], s0);
-        if (r + 1 < R2) s1 = fma(a[r + 1], a[r + 1], s1);
+      for (int i = 0; i < CR_ROWS; ++i) {
+        const int r = part * CR_ROWS + i;
+        double v = 0.0;
+        if (r == kk) v = v0;
+        else if (r > kk && r < R2) v = a[i];
+        sv[buf][r] = v;                      // rows < kk (stale from step kk-2) become zero
+        if (reflect) {
+          if (r == kk) a[i] = g;
+          else if (r > kk) a[i] = 0.0;
+        }
       }
-      const double nrm = sqrt(s0 + s1);
-      const double g = a[kk] >= 0.0 ? -nrm : nrm;
-      const double v0 = a[kk] - g;
-      const double vs = s0 + s1 - a[kk] * a[kk] + v0 * v0;
-      sv[buf][kk] = v0;
-#pragma unroll
-      for (int r = kk + 1; r < R2; ++r) sv[buf][r] = a[r];
-      sbeta[buf] = vs < eps ? 0.0 : -2.0 / vs;
-      if (!(vs < eps)) {
-        a[kk] = g;
-#pragma unroll
-        for (int r = kk + 1; r < R2; ++r) a[r] = 0.0;
-      }
+      if (part == 0) sbeta[buf] = reflect ? -2.0 / vs : 0.0;
     }
     __syncthreads();
     const double beta = sbeta[buf];
     if (beta == 0.0) stopped = true;   // the reference stops reflecting at |v|^2 < eps
-    if (!stopped && cidx > kk && cidx < RECW) {
+    double w = 0.0;
+    if (!stopped && cidx > kk && live) {
       double w0 = 0.0, w1 = 0.0;
 #pragma unroll
-      for (int r = kk; r < R2; r += 2) {
-        w0 = fma(sv[buf][r], a[r], w0);
-        if (r + 1 < R2) w1 = fma(sv[buf][r + 1], a[r + 1], w1);
+      for (int i = 0; i < CR_ROWS; i += 2) {
+        w0 = fma(sv[buf][part * CR_ROWS + i], a[i], w0);
+        if (i + 1 < CR_ROWS) w1 = fma(sv[buf][part * CR_ROWS + i + 1], a[i + 1], w1);
       }
-      const double w = (w0 + w1) * beta;
-#pragma unroll
-      for (int r = kk; r < R2; ++r) a[r] = fma(w, sv[buf][r], a[r]);
+      w = w0 + w1;
     }
-  }
-  if (cidx < RECW) {
+    w += __shfl_xor(w, CR_COLS);
+    if (CR_PARTS > 2) w += __shfl_xor(w, 2 * CR_COLS);
+    if (!stopped && cidx > kk && live) {
+      w *= beta;
+      double t0 = 0.0;
+#pragma unroll
+      for (int i = 0; i < CR_ROWS; ++i) {
+        const int r = part * CR_ROWS + i;
+        a[i] = fma(w, sv[buf][r], a[i]);
+        t0 = (r > kk) ? fma(a[i], a[i], t0) : t0;
+      }
+      tail = t0;
+    } else {
+      double t0 = 0.0;
+#pragma unroll
+      for (int i = 0; i < CR_ROWS; ++i) {
+        const int r = part * CR_ROWS + i;
+        t0 = (r > kk) ? fma(a[i], a[i], t0) : t0;
+      }
+      tail = t0;
+    }
+  });
+  if (live) {
     double* rec = recs + (size_t)k * NY * RECW;
+    double* out = rows_out + (size_t)k * NY * CRW;
 #pragma unroll
-    for (int r = 0; r < NY; ++r) rec[r * RECW + cidx] = a[r];
-    if (cidx >= NY) {
-      double* out = rows_out + (size_t)k * NY * CRW;
-#pragma unroll
-      for (int r = 0; r < NY; ++r) out[r * CRW + cidx - NY] = a[NY + r];
+    for (int i = 0; i < CR_ROWS; ++i) {
+      const int r = part * CR_ROWS + i;
+      if (r < NY) rec[r * RECW + cidx] = a[i];
+      else if (r < R2 && cidx >= NY) out[(r - NY) * CRW + cidx - NY] = a[i];
     }
   }
 }
@@ -403,12 +466,13 @@ __device__ void cr_pair(int nrows, const double* __restrict__ rows_in, double* _
 // boundary conditions B_0 x_0 + B_n x_N + V_n p = -r_bc: unknowns
 // [x_N | x_0 | p] as in qr_decomposition's final block (solve_babd_system.py:364-381).
 template <int FW>
-__device__ void cr_final(int nint, const double* __restrict__ row, const double* __restrict__ Dr,
+__device__ __forceinline__ void cr_final(int nint, const double* __restrict__ row, const double* __restrict__ Dr,
                          const double* __restrict__ rbc, double* __restrict__ dy, double* __restrict__ dp,
                          int* __restrict__ status) {
   using Solver = GJ<NFIN, NFIN + 1, FW>;
   __shared__ double colbuf[Solver::COLBUF];
   __shared__ int pivrow[NFIN];
+  __shared__ double pivval[NFIN];
   __shared__ int invperm[Solver::RQ * WAVE];
   Solver gj;
   constexpr int DRW = 2 * NY + NP;
@@ -435,7 +499,7 @@ __device__ void cr_final(int nint, const double* __restrict__ row, const double*
       gj.a[q][cq] = v;
     }
   }
-  const int bad = gj.eliminate(colbuf, pivrow);
+  const int bad = gj.eliminate(colbuf, pivrow, pivval);
   __syncthreads();
   for (int k = threadIdx.x; k < NFIN; k += blockDim.x) invperm[pivrow[k]] = k;
   __syncthreads();
@@ -447,7 +511,7 @@ __device__ void cr_final(int nint, const double* __restrict__ row, const double*
       if (Solver::col(cq) != NFIN || r >= NFIN) continue;
       const int kk = invperm[r];
       if (kk < 0 || kk >= NFIN) continue;
-      const double v = gj.a[q][cq];
+      const double v = gj.a[q][cq] / pivval[kk];
       if (kk < NY) dy[(size_t)nint * NY + kk] = v;
       else if (kk < 2 * NY) dy[kk - NY] = v;
       else dp[kk - 2 * NY] = v;
@@ -459,7 +523,7 @@ __device__ void cr_final(int nint, const double* __restrict__ row, const double*
 // Back substitution of one cyclic-reduction level: one wave per record,
 //   x_M = R^-1 (d - Gp x_L - Ep x_R - Jp p)   (as partition_backward_substitution_kernel,
 // solve_babd_system.py:537-572, with the triangular solve done directly).
-__device__ void cr_back(int level, int nrows, int nint, const double* __restrict__ recs,
+__device__ __forceinline__ void cr_back(int level, int nrows, int nint, const double* __restrict__ recs,
                         double* __restrict__ dy, const double* __restrict__ dp) {
   const int k = blockIdx.x;
   const int lane = threadIdx.x;
@@ -469,20 +533,24 @@ __device__ void cr_back(int level, int nrows, int nint, const double* __restrict
   const double* rec = recs + (size_t)k * NY * RECW;
   const double* xl = dy + (size_t)left * NY;
   const double* xr = dy + (size_t)right * NY;
-  double rhs = 0.0;
-  if (lane < NY) {
-    const double* rw = rec + lane * RECW;
-    rhs = rw[RECW - 1];
-    for (int cc = 0; cc < NY; ++cc) rhs = fma(-rw[NY + cc], xl[cc], rhs);
-    for (int cc = 0; cc < NY; ++cc) rhs = fma(-rw[2 * NY + cc], xr[cc], rhs);
-    for (int cc = 0; cc < NP; ++cc) rhs = fma(-rw[3 * NY + cc], dp[cc], rhs);
-  }
+  // lane r holds row r of R and the right-hand side d - Gp x_L - Ep x_R - Jp p
+  const double* rw = rec + (lane < NY ? lane : 0) * RECW;
+  double Rr[NY];
+#pragma unroll
+  for (int c = 0; c < NY; ++c) Rr[c] = rw[c];
+  double rhs = rw[RECW - 1];
+#pragma unroll 8
+  for (int c = 0; c < NY; ++c) rhs = fma(-rw[NY + c], xl[c], rhs);
+#pragma unroll 8
+  for (int c = 0; c < NY; ++c) rhs = fma(-rw[2 * NY + c], xr[c], rhs);
+#pragma unroll 8
+  for (int c = 0; c < NP; ++c) rhs = fma(-rw[3 * NY + c], dp[c], rhs);
   double x = 0.0;
+#pragma unroll
   for (int r = NY - 1; r >= 0; --r) {
-    double xr_ = 0.0;
-    if (lane == r) { x = rhs / rec[r * RECW + r]; xr_ = x; }
-    xr_ = readlane_d(xr_, r);
-    if (lane < r) rhs = fma(-rec[lane * RECW + r], xr_, rhs);
+    const double xv = readlane_d(rhs / Rr[r], r);
+    if (lane == r) x = xv;
+    if (lane < r) rhs = fma(-Rr[r], xv, rhs);
   }
   if (lane < NY) dy[(size_t)mid * NY + lane] = x;
 }
@@ -490,7 +558,7 @@ __device__ void cr_back(int level, int nrows, int nint, const double* __restrict
 // ---------------------------------------------------------------- recover
 // One workgroup per interval (recover_delta_k_kernel, solver:1619-1650):
 //   u_0 = dy_i, u_j = U_j v,  dydot_j = T_j [u_j; dp; 1],  dz_j = -G_j [u_j; dp; 1].
-__device__ void recover(int nint, const double* __restrict__ rec, const double* __restrict__ dy,
+__device__ __forceinline__ void recover(int nint, const double* __restrict__ rec, const double* __restrict__ dy,
                         const double* __restrict__ dp, double* __restrict__ dydot, double* __restrict__ dz) {
   __shared__ double sw[M][NV];
   const int i = blockIdx.x;
@@ -538,7 +606,7 @@ __device__ void recover(int nint, const double* __restrict__ rec, const double* 
 
 // ------------------------------------------------------------------- axpy
 struct Seg4 { long long off[4]; long long len[4]; };
-__device__ void axpy4(Seg4 s, const double* __restrict__ x, const double* __restrict__ d, double a,
+__device__ __forceinline__ void axpy4(Seg4 s, const double* __restrict__ x, const double* __restrict__ d, double a,
                       double* __restrict__ out) {
   long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 #pragma unroll
@@ -554,7 +622,7 @@ __device__ void axpy4(Seg4 s, const double* __restrict__ x, const double* __rest
 
 // -------------------------------------------------------- initial input
 // form_initial_input_kernel (solver:316-356).
-__device__ void init_input(int nint, const double* __restrict__ y0, const double* __restrict__ z0,
+__device__ __forceinline__ void init_input(int nint, const double* __restrict__ y0, const double* __restrict__ z0,
                            const double* __restrict__ p, const double* __restrict__ c,
                            double* __restrict__ ydot, double* __restrict__ ztil) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -577,7 +645,7 @@ __device__ void init_input(int nint, const double* __restrict__ y0, const double
 // ------------------------------------------------------- segment residual
 // compute_segment_residual_kernel (solver:1833-1892), residual_type 1,
 // scale_by_time: note the scaling rho *= dt inside the component loop.
-__device__ void segres(int nint, const double* __restrict__ t, const double* __restrict__ y,
+__device__ __forceinline__ void segres(int nint, const double* __restrict__ t, const double* __restrict__ y,
                        const double* __restrict__ ydot, const double* __restrict__ ztil,
                        const double* __restrict__ p, const double* __restrict__ c, double alpha, double tol,
                        double* __restrict__ residual, unsigned long long* __restrict__ maxres) {
@@ -642,7 +710,7 @@ __device__ void segres(int nint, const double* __restrict__ t, const double* __r
 
 // ---------------------------------------------------------- recover z
 // recover_solution_kernel (solver:2049-2075).
-__device__ void recover_z(int nnodes, const double* __restrict__ ztil, double* __restrict__ z) {
+__device__ __forceinline__ void recover_z(int nnodes, const double* __restrict__ ztil, double* __restrict__ z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnodes) return;
   const bool last = i == nnodes - 1;
@@ -684,7 +752,7 @@ __global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const 
   colloc::condense<Cfg::CW>(nint, t, jac, fa, fb, rows, rec, status);
 }
 
-__global__ __launch_bounds__(512) void colloc_cr_pair(int nrows, const double* rows_in, double* rows_out,
+__global__ __launch_bounds__(colloc::CR_THREADS) void colloc_cr_pair(int nrows, const double* rows_in, double* rows_out,
     double* recs) {
   colloc::cr_pair(nrows, rows_in, rows_out, recs);
 }

@@ -403,22 +403,22 @@ def _carray(name, vals):
 
 
 def choose_waves(ny, nz, np_, m):
-    """Waves per workgroup of the register-resident eliminations.
+    """Waves per workgroup of the register-resident Gauss-Jordan eliminations.
 
-    The condensed stage system is (m-1) ny x ((m-1) ny + ny + np + 1); the
-    wave count is picked so that one thread holds at most ~48 of its doubles
-    (4 waves/SIMD, 128 VGPRs), capped at 16 waves."""
-    def pick(n, ncol, cap):
+    The condensed stage system is (m-1) ny x ((m-1) ny + ny + np + 1) and the
+    boundary system (2 ny + np) x (2 ny + np + 1).  Eight waves (two per SIMD,
+    256 VGPRs each) are used when a thread's share is at most 96 doubles,
+    otherwise sixteen waves (128 VGPRs) with at most 48 doubles."""
+    def pick(n, ncol):
         rq = (n + 63) // 64
-        for w in range(1, cap + 1):
-            if rq * ((ncol + w - 1) // w) <= 48:
-                return w
-        return cap
+        if rq * ((ncol + 7) // 8) <= 96:
+            return 8
+        if rq * ((ncol + 15) // 16) <= 48:
+            return 16
+        raise ValueError("a %d x %d elimination does not fit in one workgroup's registers" % (n, ncol))
     ns = (m - 1) * ny
-    cw = pick(ns, ns + ny + np_ + 1, 16)
     nf = 2 * ny + np_
-    fw = pick(nf, nf + 1, 16)
-    return cw, fw
+    return pick(ns, ns + ny + np_ + 1), pick(nf, nf + 1)
 
 
 def build_unit(sym, m, tables):
