@@ -351,6 +351,9 @@ int cs_create(const cs_dims* d, cs_handle** out) {
   if (!d || !out) return fail(CS_E_ARG, "null argument");
   if (d->ny < 1 || d->nz < 0 || d->np < 0 || d->m < 2 || d->n_max < 2 || d->nc < 0)
     return fail(CS_E_SHAPE, "invalid dims ny=%d nz=%d np=%d m=%d n_max=%d", d->ny, d->nz, d->np, d->m, d->n_max);
+  // the cyclic-reduction kernels hold one BABD row per lane of a wave
+  if (d->ny > 64)
+    return fail(CS_E_UNSUPPORTED, "ny=%d: the register-resident kernels support ny <= 64", d->ny);
   cs_handle* h = new cs_handle();
   h->ny = d->ny; h->nz = d->nz; h->np = d->np; h->m = d->m; h->n_max = d->n_max; h->nc = d->nc;
   const int ny = h->ny, nz = h->nz, np = h->np, m = h->m;

@@ -278,6 +278,14 @@ class _Printer(C99CodePrinter):
         a = self._print(e.args[0])
         return "((%s) > 0.0 ? 1.0 : ((%s) < 0.0 ? -1.0 : 0.0))" % (a, a)
 
+    def _print_DiracDelta(self, e):
+        # derivative of sign/abs: zero wherever it is defined
+        return "0.0"
+
+    def _print_Heaviside(self, e):
+        a = self._print(e.args[0])
+        return "((%s) > 0.0 ? 1.0 : ((%s) < 0.0 ? 0.0 : 0.5))" % (a, a)
+
 
 def _cname(sym):
     """C name of a symbol: generated symbols (leading '_') get prefix 'w', user
