@@ -105,3 +105,179 @@ struct GJ {
 };
 
 }  // namespace colloc
+
+namespace colloc {
+
+// Blocked Gauss-Jordan elimination with partial pivoting.
+//
+// Same register layout as GJ (lane l of wave w owns rows l + 64 q and
+// columns w + W cq of the N x NCOL matrix [S | R]).  The matrix columns are
+// eliminated B at a time:
+//   1. the owners copy the B panel columns to LDS (pbuf, row-major, one row
+//      of LD doubles per matrix row);
+//   2. wave 0 applies B Gauss-Jordan sweeps to the panel in LDS: at step kk
+//      it picks the pivot row p among the rows not yet used (largest |a|,
+//      ties to the lower row), scales row p by 1/a_p,kk, subtracts multiples
+//      of it from every other row and replaces column kk by the exchange
+//      column (-a_r,kk / a_p,kk for r != p, 1 / a_p,kk for r = p).  After the
+//      B sweeps the panel holds M = E[:, P]: the columns of the accumulated
+//      row transformation E that belong to the B pivot rows P;
+//   3. every wave applies E to its trailing columns: the pivot rows' values
+//      Tp (B x trailing) go to LDS, the pivot rows are cleared, and
+//      a += M Tp is accumulated in registers from LDS broadcasts, four panel
+//      columns at a time.
+// The pivot rows end up scaled: row pivrow[k] holds unknown k's solution in
+// the right-hand-side columns, so pivval[k] is 1.  Rows never move.
+template <int N, int NCOL, int W, int B>
+struct GJB {
+  static constexpr int RQ = (N + WAVE - 1) / WAVE;
+  static constexpr int CQ = (NCOL + W - 1) / W;
+  static constexpr int NR = RQ * WAVE;            // rows held (N plus padding)
+  static constexpr int LD = B + 2;                // panel row stride (even: 16-byte aligned rows)
+  static constexpr int PBUF = NR * LD;            // doubles of LDS for the panel
+  static constexpr int TP = NCOL * B;             // doubles of LDS for the pivot rows
+  static constexpr int LDS_DOUBLES = PBUF + TP;   // plus NR ints for the row flags
+  static_assert(N <= 256, "row indices are packed into 8 bits of the pivot key");
+  static_assert(B % 4 == 0, "panel width must be a multiple of 4");
+
+  double a[RQ][CQ];
+
+  __device__ __forceinline__ static int col(int cq) { return wave_id() + W * cq; }
+  __device__ __forceinline__ static int row(int q) { return lane_id() + WAVE * q; }
+
+  // Copy matrix columns [k0, k0 + B) (those < N) to the panel buffer; panel
+  // columns at or beyond N are written as zeros by wave 0.
+  __device__ __forceinline__ void put_panel(int k0, double* pbuf) {
+    const int wave = wave_id();
+#pragma unroll
+    for (int cq = 0; cq < CQ; ++cq) {
+      const int c = wave + W * cq;
+      if (c < k0 || c >= k0 + B || c >= N) continue;
+#pragma unroll
+      for (int q = 0; q < RQ; ++q) pbuf[row(q) * LD + (c - k0)] = a[q][cq];
+    }
+    if (wave == 0) {
+      for (int c = (N - k0 > 0 ? N - k0 : 0); c < B; ++c)
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) pbuf[row(q) * LD + c] = 0.0;
+    }
+  }
+
+  // Wave 0: nb Gauss-Jordan sweeps of the panel in LDS.  Returns nonzero on
+  // a zero or non-finite pivot.
+  __device__ __forceinline__ int sweep(int k0, int nb, unsigned& used, double* pbuf, int* flag, int* pivrow,
+                                       double* pivval) {
+    const int lane = lane_id();
+    int bad = 0;
+    for (int kk = 0; kk < nb; ++kk) {
+      double cv[RQ];
+      unsigned long long key = 0;
+#pragma unroll
+      for (int q = 0; q < RQ; ++q) {
+        cv[q] = pbuf[row(q) * LD + kk];
+        const unsigned long long kq =
+            (((unsigned long long)__double_as_longlong(fabs(cv[q]))) & ~0xFFull) |
+            (unsigned long long)(255 - row(q));
+        if (!((used >> q) & 1u) && kq > key) key = kq;
+      }
+      key = wave_max_u64(key);
+      const int p = key ? 255 - (int)(key & 0xFF) : 0;
+      const double piv = pbuf[p * LD + kk];
+      if (!(fabs(piv) > 0.0) || !(fabs(piv) < 1.0e308)) bad = 1;
+      const double inv = 1.0 / piv;
+      const int pl = p & (WAVE - 1), pq = p >> 6;
+      double mlt[RQ];
+#pragma unroll
+      for (int q = 0; q < RQ; ++q) mlt[q] = (q == pq && lane == pl) ? 1.0 - inv : cv[q] * inv;
+      // the pivot row is read (broadcast) four columns at a time, each group
+      // before any lane rewrites it
+#pragma unroll
+      for (int c4 = 0; c4 < B; c4 += 4) {
+        double prow[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) prow[e] = pbuf[p * LD + c4 + e];
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+          double* rp = pbuf + row(q) * LD + c4;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) rp[e] = fma(-mlt[q], prow[e], rp[e]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < RQ; ++q) pbuf[row(q) * LD + kk] = (q == pq && lane == pl) ? inv : -cv[q] * inv;
+      if (lane == pl) used |= 1u << pq;
+      if (lane == 0) {
+        pivrow[k0 + kk] = p;
+        pivval[k0 + kk] = 1.0;
+        flag[p] = kk + 1;
+      }
+    }
+    return bad;
+  }
+
+  // LDS: pbuf (PBUF doubles), tp (TP doubles), flag (NR ints, zero on entry),
+  // pivrow (N ints), pivval (N doubles).
+  __device__ int eliminate(double* pbuf, double* tp, int* flag, int* pivrow, double* pivval) {
+    const int wave = wave_id();
+    unsigned used = 0;
+#pragma unroll
+    for (int q = 0; q < RQ; ++q)
+      if (row(q) >= N) used |= 1u << q;
+    int bad = 0;
+    put_panel(0, pbuf);
+    __syncthreads();
+    for (int k0 = 0; k0 < N; k0 += B) {
+      const int nb = N - k0 < B ? N - k0 : B;
+      if (wave == 0) bad |= sweep(k0, nb, used, pbuf, flag, pivrow, pivval);
+      __syncthreads();
+      // pivot rows: trailing values to tp, then cleared (their new value is M Tp)
+      int fl[RQ];
+#pragma unroll
+      for (int q = 0; q < RQ; ++q) fl[q] = flag[row(q)];
+#pragma unroll
+      for (int cq = 0; cq < CQ; ++cq) {
+        const int c = wave + W * cq;
+        if (c < k0 + nb || c >= NCOL) continue;
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+          if (fl[q]) {
+            tp[c * B + fl[q] - 1] = a[q][cq];
+            a[q][cq] = 0.0;
+          }
+        }
+        if (nb < B && lane_id() < B - nb) tp[c * B + nb + lane_id()] = 0.0;
+      }
+      __syncthreads();
+      // a += M Tp, four panel columns at a time
+      static_for<0, B / 4>([&](auto kc_) {
+        constexpr int kc = decltype(kc_)::value;
+        if (4 * kc < nb) {
+          double L[RQ][4];
+#pragma unroll
+          for (int q = 0; q < RQ; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) L[q][e] = pbuf[row(q) * LD + 4 * kc + e];
+#pragma unroll
+          for (int cq = 0; cq < CQ; ++cq) {
+            const int c = wave + W * cq;
+            if (c < k0 + nb || c >= NCOL) continue;
+            double t4[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) t4[e] = tp[c * B + 4 * kc + e];
+#pragma unroll
+            for (int q = 0; q < RQ; ++q)
+#pragma unroll
+              for (int e = 0; e < 4; ++e) a[q][cq] = fma(L[q][e], t4[e], a[q][cq]);
+          }
+        }
+      });
+      __syncthreads();
+      if (wave == 0 && lane_id() < nb) flag[pivrow[k0 + lane_id()]] = 0;
+      if (k0 + B < N) put_panel(k0 + B, pbuf);
+      __syncthreads();
+    }
+    return bad;
+  }
+};
+
+}  // namespace colloc

@@ -161,10 +161,14 @@ template <int CW>
 __device__ __forceinline__ void condense(int nint, const double* __restrict__ t, const double* __restrict__ jac,
                          const double* __restrict__ fa, const double* __restrict__ fb,
                          double* __restrict__ rows, double* __restrict__ rec, int* __restrict__ status) {
-  using Solver = GJ<NS, NS + NV, CW>;
+  using Solver = GJB<NS, NS + NV, CW, 16>;
   constexpr int NT = CW * WAVE;
-  __shared__ double sT[M * NY * NV];
-  __shared__ double colbuf[Solver::COLBUF];
+  // sT (phases 1-2) and the elimination's panel / pivot-row buffers (phase 3)
+  // share one LDS area
+  constexpr int SMEM = M * NY * NV > Solver::LDS_DOUBLES ? M * NY * NV : Solver::LDS_DOUBLES;
+  __shared__ double smem[SMEM];
+  __shared__ int sflag[Solver::NR];
+  double* sT = smem;
   __shared__ int pivrow[NS];
   __shared__ double pivval[NS];
   __shared__ int invperm[Solver::RQ * WAVE];
@@ -293,8 +297,11 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
   __syncthreads();
 
   // ---- phase 3: Gauss-Jordan
-  for (int k = tid; k < Solver::RQ * WAVE; k += NT) invperm[k] = -1;
-  if (gj.eliminate(colbuf, pivrow, pivval)) bad |= 2;
+  for (int k = tid; k < Solver::RQ * WAVE; k += NT) {
+    invperm[k] = -1;
+    sflag[k] = 0;
+  }
+  if (gj.eliminate(smem, smem + Solver::PBUF, sflag, pivrow, pivval)) bad |= 2;
   __syncthreads();
   for (int k = tid; k < NS; k += NT) invperm[pivrow[k]] = k;
   if (bad) atomicOr(&sbad, bad);
