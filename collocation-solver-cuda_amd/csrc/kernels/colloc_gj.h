@@ -36,7 +36,7 @@ struct GJ {
 
   // Returns nonzero if a zero or non-finite pivot was met.
   // LDS: colbuf (COLBUF doubles), pivrow (N ints), pivval (N doubles).
-  __device__ int eliminate(double* colbuf, int* pivrow, double* pivval) {
+  __device__ __forceinline__ int eliminate(double* colbuf, int* pivrow, double* pivval) {
     const int lane = lane_id();
     const int wave = wave_id();
     unsigned used = 0;
@@ -113,15 +113,19 @@ namespace colloc {
 // Same register layout as GJ (lane l of wave w owns rows l + 64 q and
 // columns w + W cq of the N x NCOL matrix [S | R]).  The matrix columns are
 // eliminated B at a time:
-//   1. the owners copy the B panel columns to LDS (pbuf, row-major, one row
-//      of LD doubles per matrix row);
-//   2. wave 0 applies B Gauss-Jordan sweeps to the panel in LDS: at step kk
-//      it picks the pivot row p among the rows not yet used (largest |a|,
-//      ties to the lower row), scales row p by 1/a_p,kk, subtracts multiples
-//      of it from every other row and replaces column kk by the exchange
-//      column (-a_r,kk / a_p,kk for r != p, 1 / a_p,kk for r = p).  After the
-//      B sweeps the panel holds M = E[:, P]: the columns of the accumulated
-//      row transformation E that belong to the B pivot rows P;
+//   1. the owners copy the B panel columns to LDS (row-major, one row of LD
+//      doubles per matrix row);
+//   2. all waves apply B Gauss-Jordan exchange steps to the panel in LDS,
+//      wave w updating panel columns w, w + W, ...: at step kk every wave
+//      finds the pivot row p among the rows not yet used (largest |a|, ties
+//      to the lower row; the search is repeated in every wave so that no
+//      broadcast is needed), row p is scaled by 1/a_p,kk, multiples of it are
+//      subtracted from the other rows and column kk becomes the exchange
+//      column (-a_r,kk / a_p,kk for r != p, 1 / a_p,kk for r = p).  The panel
+//      is double-buffered by step parity, so a step reads one copy and writes
+//      the other and needs one barrier.  After the B steps the panel holds
+//      M = E[:, P], the columns of the accumulated row transformation E that
+//      belong to the B pivot rows P;
 //   3. every wave applies E to its trailing columns: the pivot rows' values
 //      Tp (B x trailing) go to LDS, the pivot rows are cleared, and
 //      a += M Tp is accumulated in registers from LDS broadcasts, four panel
@@ -134,9 +138,10 @@ struct GJB {
   static constexpr int CQ = (NCOL + W - 1) / W;
   static constexpr int NR = RQ * WAVE;            // rows held (N plus padding)
   static constexpr int LD = B + 2;                // panel row stride (even: 16-byte aligned rows)
-  static constexpr int PBUF = NR * LD;            // doubles of LDS for the panel
+  static constexpr int PBUF = NR * LD;            // doubles of LDS for one copy of the panel
   static constexpr int TP = NCOL * B;             // doubles of LDS for the pivot rows
-  static constexpr int LDS_DOUBLES = PBUF + TP;   // plus NR ints for the row flags
+  static constexpr int LDS_DOUBLES = 2 * PBUF + TP;   // plus NR ints for the row flags
+  static constexpr int PCW = (B + W - 1) / W;     // panel columns per wave in the sweep
   static_assert(N <= 256, "row indices are packed into 8 bits of the pivot key");
   static_assert(B % 4 == 0, "panel width must be a multiple of 4");
 
@@ -163,18 +168,22 @@ struct GJB {
     }
   }
 
-  // Wave 0: nb Gauss-Jordan sweeps of the panel in LDS.  Returns nonzero on
-  // a zero or non-finite pivot.
-  __device__ __forceinline__ int sweep(int k0, int nb, unsigned& used, double* pbuf, int* flag, int* pivrow,
-                                       double* pivval) {
+  // All waves: nb Gauss-Jordan exchange steps on the panel, which starts in
+  // pb0 and ends in pb0 (nb even) or pb1 (nb odd).  Returns nonzero on a zero
+  // or non-finite pivot.
+  __device__ __forceinline__ int sweep(int k0, int nb, unsigned& used, double* pb0, double* pb1, int* flag,
+                                       int* pivrow, double* pivval) {
     const int lane = lane_id();
+    const int wave = wave_id();
     int bad = 0;
     for (int kk = 0; kk < nb; ++kk) {
+      const double* src = (kk & 1) ? pb1 : pb0;
+      double* dst = (kk & 1) ? pb0 : pb1;
       double cv[RQ];
       unsigned long long key = 0;
 #pragma unroll
       for (int q = 0; q < RQ; ++q) {
-        cv[q] = pbuf[row(q) * LD + kk];
+        cv[q] = src[row(q) * LD + kk];
         const unsigned long long kq =
             (((unsigned long long)__double_as_longlong(fabs(cv[q]))) & ~0xFFull) |
             (unsigned long long)(255 - row(q));
@@ -182,54 +191,54 @@ struct GJB {
       }
       key = wave_max_u64(key);
       const int p = key ? 255 - (int)(key & 0xFF) : 0;
-      const double piv = pbuf[p * LD + kk];
+      const double piv = src[p * LD + kk];
       if (!(fabs(piv) > 0.0) || !(fabs(piv) < 1.0e308)) bad = 1;
       const double inv = 1.0 / piv;
       const int pl = p & (WAVE - 1), pq = p >> 6;
       double mlt[RQ];
 #pragma unroll
       for (int q = 0; q < RQ; ++q) mlt[q] = (q == pq && lane == pl) ? 1.0 - inv : cv[q] * inv;
-      // the pivot row is read (broadcast) four columns at a time, each group
-      // before any lane rewrites it
 #pragma unroll
-      for (int c4 = 0; c4 < B; c4 += 4) {
-        double prow[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) prow[e] = pbuf[p * LD + c4 + e];
+      for (int t = 0; t < PCW; ++t) {
+        const int c = wave + W * t;
+        if (c >= B) continue;
+        const double pv = src[p * LD + c];
 #pragma unroll
         for (int q = 0; q < RQ; ++q) {
-          double* rp = pbuf + row(q) * LD + c4;
-#pragma unroll
-          for (int e = 0; e < 4; ++e) rp[e] = fma(-mlt[q], prow[e], rp[e]);
+          double v = fma(-mlt[q], pv, src[row(q) * LD + c]);
+          if (c == kk) v = (q == pq && lane == pl) ? inv : -cv[q] * inv;
+          dst[row(q) * LD + c] = v;
         }
       }
-#pragma unroll
-      for (int q = 0; q < RQ; ++q) pbuf[row(q) * LD + kk] = (q == pq && lane == pl) ? inv : -cv[q] * inv;
       if (lane == pl) used |= 1u << pq;
-      if (lane == 0) {
+      if (wave == 0 && lane == 0) {
         pivrow[k0 + kk] = p;
         pivval[k0 + kk] = 1.0;
         flag[p] = kk + 1;
       }
+      __syncthreads();
     }
     return bad;
   }
 
-  // LDS: pbuf (PBUF doubles), tp (TP doubles), flag (NR ints, zero on entry),
-  // pivrow (N ints), pivval (N doubles).
-  __device__ int eliminate(double* pbuf, double* tp, int* flag, int* pivrow, double* pivval) {
+  // LDS: lds (LDS_DOUBLES doubles), flag (NR ints, zero on entry), pivrow
+  // (N ints), pivval (N doubles).
+  __device__ __forceinline__ int eliminate(double* lds, int* flag, int* pivrow, double* pivval) {
+    double* pb0 = lds;
+    double* pb1 = lds + PBUF;
+    double* tp = lds + 2 * PBUF;
     const int wave = wave_id();
     unsigned used = 0;
 #pragma unroll
     for (int q = 0; q < RQ; ++q)
       if (row(q) >= N) used |= 1u << q;
     int bad = 0;
-    put_panel(0, pbuf);
+    put_panel(0, pb0);
     __syncthreads();
     for (int k0 = 0; k0 < N; k0 += B) {
       const int nb = N - k0 < B ? N - k0 : B;
-      if (wave == 0) bad |= sweep(k0, nb, used, pbuf, flag, pivrow, pivval);
-      __syncthreads();
+      bad |= sweep(k0, nb, used, pb0, pb1, flag, pivrow, pivval);
+      const double* mb = (nb & 1) ? pb1 : pb0;
       // pivot rows: trailing values to tp, then cleared (their new value is M Tp)
       int fl[RQ];
 #pragma unroll
@@ -256,7 +265,7 @@ struct GJB {
 #pragma unroll
           for (int q = 0; q < RQ; ++q)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) L[q][e] = pbuf[row(q) * LD + 4 * kc + e];
+            for (int e = 0; e < 4; ++e) L[q][e] = mb[row(q) * LD + 4 * kc + e];
 #pragma unroll
           for (int cq = 0; cq < CQ; ++cq) {
             const int c = wave + W * cq;
@@ -273,7 +282,7 @@ struct GJB {
       });
       __syncthreads();
       if (wave == 0 && lane_id() < nb) flag[pivrow[k0 + lane_id()]] = 0;
-      if (k0 + B < N) put_panel(k0 + B, pbuf);
+      if (k0 + B < N) put_panel(k0 + B, pb0);
       __syncthreads();
     }
     return bad;
@@ -281,3 +290,4 @@ struct GJB {
 };
 
 }  // namespace colloc
+

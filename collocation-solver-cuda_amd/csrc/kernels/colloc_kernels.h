@@ -301,7 +301,7 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
     invperm[k] = -1;
     sflag[k] = 0;
   }
-  if (gj.eliminate(smem, smem + Solver::PBUF, sflag, pivrow, pivval)) bad |= 2;
+  if (gj.eliminate(smem, sflag, pivrow, pivval)) bad |= 2;
   __syncthreads();
   for (int k = tid; k < NS; k += NT) invperm[pivrow[k]] = k;
   if (bad) atomicOr(&sbad, bad);

@@ -411,22 +411,33 @@ def _carray(name, vals):
 
 
 def choose_waves(ny, nz, np_, m):
-    """Waves per workgroup of the register-resident Gauss-Jordan eliminations.
+    """Waves per workgroup of the register-resident eliminations.
 
-    The condensed stage system is (m-1) ny x ((m-1) ny + ny + np + 1) and the
-    boundary system (2 ny + np) x (2 ny + np + 1).  Eight waves (two per SIMD,
-    256 VGPRs each) are used when a thread's share is at most 96 doubles,
-    otherwise sixteen waves (128 VGPRs) with at most 48 doubles."""
-    def pick(n, ncol):
-        rq = (n + 63) // 64
-        if rq * ((ncol + 7) // 8) <= 96:
-            return 8
-        if rq * ((ncol + 15) // 16) <= 48:
-            return 16
-        raise ValueError("a %d x %d elimination does not fit in one workgroup's registers" % (n, ncol))
+    Condensation (stage system (m-1) ny x ((m-1) ny + ny + np + 1)): the
+    pipelined blocked solver keeps the matrix in W - 1 waves and the panel in
+    wave 0; eight waves (256 VGPRs each) when a matrix wave's share is at
+    most 105 doubles, otherwise sixteen (128 VGPRs) with at most 51.
+    Boundary system ((2 ny + np) x (2 ny + np + 1)), one column per wave
+    slot: eight waves up to 96 doubles per thread, sixteen up to 48."""
+    def rq(n):
+        return (n + 63) // 64
+
     ns = (m - 1) * ny
+    ncol = ns + ny + np_ + 1
+    if rq(ns) * -(-ncol // 7) <= 105:
+        cw = 8
+    elif rq(ns) * -(-ncol // 15) <= 51:
+        cw = 16
+    else:
+        raise ValueError("a %d x %d condensation does not fit in one workgroup's registers" % (ns, ncol))
     nf = 2 * ny + np_
-    return pick(ns, ns + ny + np_ + 1), pick(nf, nf + 1)
+    if rq(nf) * -(-(nf + 1) // 8) <= 96:
+        fw = 8
+    elif rq(nf) * -(-(nf + 1) // 16) <= 48:
+        fw = 16
+    else:
+        raise ValueError("a %d x %d boundary system does not fit in one workgroup's registers" % (nf, nf + 1))
+    return cw, fw
 
 
 def build_unit(sym, m, tables):
