@@ -62,6 +62,31 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
   return m;
 }
 
+// Wave-wide max of a u32 (result uniform): DPP butterflies within rows of
+// 16 lanes, then the four row results through v_readlane.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  unsigned o;
+  o = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xf, 0xf, false);  v = o > v ? o : v;
+  o = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xf, 0xf, false);  v = o > v ? o : v;
+  o = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xf, 0xf, false); v = o > v ? o : v;
+  o = (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xf, 0xf, false); v = o > v ? o : v;
+  const unsigned r0 = __builtin_amdgcn_readlane((int)v, 0), r1 = __builtin_amdgcn_readlane((int)v, 16);
+  const unsigned r2 = __builtin_amdgcn_readlane((int)v, 32), r3 = __builtin_amdgcn_readlane((int)v, 48);
+  const unsigned m01 = r0 > r1 ? r0 : r1, m23 = r2 > r3 ? r2 : r3;
+  return m01 > m23 ? m01 : m23;
+}
+
+// 1/x in FP64 from the hardware reciprocal estimate and two Newton steps
+// (error within a few ulp; used where a pivot's inverse is on a latency
+// chain).
+__device__ __forceinline__ double fast_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
 // NaN-propagating max (fmax would drop a NaN and hide a diverged residual).
 __device__ __forceinline__ double nmax(double a, double b) { return (a != a || a > b) ? a : b; }
 

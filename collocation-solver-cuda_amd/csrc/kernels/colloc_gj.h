@@ -113,15 +113,17 @@ namespace colloc {
 // Same register layout as GJ (lane l of wave w owns rows l + 64 q and
 // columns w + W cq of the N x NCOL matrix [S | R]).  The matrix columns are
 // eliminated B at a time:
-//   1. the owners copy the B panel columns to LDS (row-major, one row of LD
-//      doubles per matrix row);
+//   1. the owners copy the B panel columns to LDS (column-major, so that
+//      lane-per-row accesses are free of bank conflicts);
 //   2. all waves apply B Gauss-Jordan exchange steps to the panel in LDS,
 //      wave w updating panel columns w, w + W, ...: at step kk every wave
 //      finds the pivot row p among the rows not yet used (largest |a|, ties
 //      to the lower row; the search is repeated in every wave so that no
 //      broadcast is needed), row p is scaled by 1/a_p,kk, multiples of it are
 //      subtracted from the other rows and column kk becomes the exchange
-//      column (-a_r,kk / a_p,kk for r != p, 1 / a_p,kk for r = p).  The panel
+//      column (-a_r,kk / a_p,kk for r != p, 1 / a_p,kk for r = p).  The
+//      pivot key keeps 12 mantissa bits of |a|: candidates within a relative
+//      2^-12 of the largest are ties and go to the lower row.  The panel
 //      is double-buffered by step parity, so a step reads one copy and writes
 //      the other and needs one barrier.  After the B steps the panel holds
 //      M = E[:, P], the columns of the accumulated row transformation E that
@@ -137,8 +139,7 @@ struct GJB {
   static constexpr int RQ = (N + WAVE - 1) / WAVE;
   static constexpr int CQ = (NCOL + W - 1) / W;
   static constexpr int NR = RQ * WAVE;            // rows held (N plus padding)
-  static constexpr int LD = B + 2;                // panel row stride (even: 16-byte aligned rows)
-  static constexpr int PBUF = NR * LD;            // doubles of LDS for one copy of the panel
+  static constexpr int PBUF = NR * B;             // doubles of LDS for one copy of the panel (column-major)
   static constexpr int TP = NCOL * B;             // doubles of LDS for the pivot rows
   static constexpr int LDS_DOUBLES = 2 * PBUF + TP;   // plus NR ints for the row flags
   static constexpr int PCW = (B + W - 1) / W;     // panel columns per wave in the sweep
@@ -159,12 +160,12 @@ struct GJB {
       const int c = wave + W * cq;
       if (c < k0 || c >= k0 + B || c >= N) continue;
 #pragma unroll
-      for (int q = 0; q < RQ; ++q) pbuf[row(q) * LD + (c - k0)] = a[q][cq];
+      for (int q = 0; q < RQ; ++q) pbuf[(c - k0) * NR + row(q)] = a[q][cq];
     }
     if (wave == 0) {
       for (int c = (N - k0 > 0 ? N - k0 : 0); c < B; ++c)
 #pragma unroll
-        for (int q = 0; q < RQ; ++q) pbuf[row(q) * LD + c] = 0.0;
+        for (int q = 0; q < RQ; ++q) pbuf[c * NR + row(q)] = 0.0;
     }
   }
 
@@ -179,22 +180,36 @@ struct GJB {
     for (int kk = 0; kk < nb; ++kk) {
       const double* src = (kk & 1) ? pb1 : pb0;
       double* dst = (kk & 1) ? pb0 : pb1;
-      double cv[RQ];
-      unsigned long long key = 0;
+      // every LDS read of the step is issued first: column kk and this wave's
+      // own panel columns (the pivot row's values are then taken from
+      // registers with v_readlane, so nothing waits on LDS after the search)
+      double cv[RQ], own[PCW][RQ];
+#pragma unroll
+      for (int q = 0; q < RQ; ++q) cv[q] = src[kk * NR + row(q)];
+#pragma unroll
+      for (int t = 0; t < PCW; ++t) {
+        const int c = wave + W * t;
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) own[t][q] = c < B ? src[c * NR + row(q)] : 0.0;
+      }
+      // pivot key: the high word of |a| (exponent and 20 mantissa bits) with
+      // its low 8 bits replaced by 255 - row, so candidates within a relative
+      // 2^-12 of the largest count as ties and go to the lower row
+      unsigned key = 0;
 #pragma unroll
       for (int q = 0; q < RQ; ++q) {
-        cv[q] = src[row(q) * LD + kk];
-        const unsigned long long kq =
-            (((unsigned long long)__double_as_longlong(fabs(cv[q]))) & ~0xFFull) |
-            (unsigned long long)(255 - row(q));
+        const unsigned kq = ((unsigned)__double2hiint(fabs(cv[q])) & ~0xFFu) | (unsigned)(255 - row(q));
         if (!((used >> q) & 1u) && kq > key) key = kq;
       }
-      key = wave_max_u64(key);
-      const int p = key ? 255 - (int)(key & 0xFF) : 0;
-      const double piv = src[p * LD + kk];
-      if (!(fabs(piv) > 0.0) || !(fabs(piv) < 1.0e308)) bad = 1;
-      const double inv = 1.0 / piv;
+      key = wave_max_u32(key);
+      const int p = 255 - (int)(key & 0xFF);
       const int pl = p & (WAVE - 1), pq = p >> 6;
+      double sel = cv[0];
+#pragma unroll
+      for (int q = 1; q < RQ; ++q) sel = (pq == q) ? cv[q] : sel;
+      const double piv = readlane_d(sel, pl);
+      if (!(fabs(piv) > 0.0) || !(fabs(piv) < 1.0e308)) bad = 1;
+      const double inv = fast_rcp(piv);
       double mlt[RQ];
 #pragma unroll
       for (int q = 0; q < RQ; ++q) mlt[q] = (q == pq && lane == pl) ? 1.0 - inv : cv[q] * inv;
@@ -202,12 +217,15 @@ struct GJB {
       for (int t = 0; t < PCW; ++t) {
         const int c = wave + W * t;
         if (c >= B) continue;
-        const double pv = src[p * LD + c];
+        double ps = own[t][0];
+#pragma unroll
+        for (int q = 1; q < RQ; ++q) ps = (pq == q) ? own[t][q] : ps;
+        const double pv = readlane_d(ps, pl);
 #pragma unroll
         for (int q = 0; q < RQ; ++q) {
-          double v = fma(-mlt[q], pv, src[row(q) * LD + c]);
+          double v = fma(-mlt[q], pv, own[t][q]);
           if (c == kk) v = (q == pq && lane == pl) ? inv : -cv[q] * inv;
-          dst[row(q) * LD + c] = v;
+          dst[c * NR + row(q)] = v;
         }
       }
       if (lane == pl) used |= 1u << pq;
@@ -240,19 +258,18 @@ struct GJB {
       bad |= sweep(k0, nb, used, pb0, pb1, flag, pivrow, pivval);
       const double* mb = (nb & 1) ? pb1 : pb0;
       // pivot rows: trailing values to tp, then cleared (their new value is M Tp)
+      const int lo = k0 + nb;
       int fl[RQ];
 #pragma unroll
       for (int q = 0; q < RQ; ++q) fl[q] = flag[row(q)];
 #pragma unroll
       for (int cq = 0; cq < CQ; ++cq) {
         const int c = wave + W * cq;
-        if (c < k0 + nb || c >= NCOL) continue;
+        if (c < lo || c >= NCOL) continue;
 #pragma unroll
         for (int q = 0; q < RQ; ++q) {
-          if (fl[q]) {
-            tp[c * B + fl[q] - 1] = a[q][cq];
-            a[q][cq] = 0.0;
-          }
+          if (fl[q]) tp[c * B + fl[q] - 1] = a[q][cq];
+          a[q][cq] = fl[q] ? 0.0 : a[q][cq];
         }
         if (nb < B && lane_id() < B - nb) tp[c * B + nb + lane_id()] = 0.0;
       }
@@ -265,11 +282,11 @@ struct GJB {
 #pragma unroll
           for (int q = 0; q < RQ; ++q)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) L[q][e] = mb[row(q) * LD + 4 * kc + e];
+            for (int e = 0; e < 4; ++e) L[q][e] = mb[(4 * kc + e) * NR + row(q)];
 #pragma unroll
           for (int cq = 0; cq < CQ; ++cq) {
             const int c = wave + W * cq;
-            if (c < k0 + nb || c >= NCOL) continue;
+            if (c < lo || c >= NCOL) continue;
             double t4[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) t4[e] = tp[c * B + 4 * kc + e];

@@ -247,18 +247,43 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
     // gc now holds [I | G_j]; write G_j and form T_j column by column
 #pragma unroll
     for (int sl = 0; sl < SL; ++sl) {
-      const int cc = lane + WAVE * sl;
-      if (cc < NZ || cc >= NCZ) continue;
-      const int q = cc - NZ;
+      const int q = lane + WAVE * sl - NZ;
+      if (q < 0 || q >= NV) continue;
 #pragma unroll
       for (int r = 0; r < NZ; ++r) recG[(j * NZ + r) * NV + q] = gc[sl][r];
-      for (int r = 0; r < NY; ++r) {
-        const double* hrow = Dfj + r * NJC;
-        double v = q < NY ? hrow[q] : (q < NY + NP ? hrow[NZ + q] : faj[r]);
+    }
+    // T_j = [h_y | h_p | f_a,h] - h_z G_j, rows in batches of TB whose
+    // Jacobian loads are all issued before any of them is used
+    constexpr int TB = 5;
+    for (int r0 = 0; r0 < NY; r0 += TB) {
+      double hz[TB][NZ1], hb[TB][SL];
 #pragma unroll
-        for (int z = 0; z < NZ; ++z) v = fma(-hrow[NY + z], gc[sl][z], v);
-        sT[(j * NY + r) * NV + q] = v;
-        recT[(j * NY + r) * NV + q] = v;
+      for (int rr = 0; rr < TB; ++rr) {
+        const int r = r0 + rr < NY ? r0 + rr : NY - 1;
+        const double* hrow = Dfj + r * NJC;
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) hz[rr][z] = hrow[NY + z];
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+          const int q = lane + WAVE * sl - NZ;
+          const int src = q < NY ? q : q + NZ;       // h_y, then h_p after the NZ z-columns
+          hb[rr][sl] = (q >= 0 && q < NY + NP) ? hrow[src] : (q == NY + NP ? faj[r] : 0.0);
+        }
+      }
+#pragma unroll
+      for (int rr = 0; rr < TB; ++rr) {
+        const int r = r0 + rr;
+        if (r >= NY) continue;
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+          const int q = lane + WAVE * sl - NZ;
+          if (q < 0 || q >= NV) continue;
+          double v = hb[rr][sl];
+#pragma unroll
+          for (int z = 0; z < NZ; ++z) v = fma(-hz[rr][z], gc[sl][z], v);
+          sT[(j * NY + r) * NV + q] = v;
+          recT[(j * NY + r) * NV + q] = v;
+        }
       }
     }
   }
