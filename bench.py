@@ -57,6 +57,19 @@ def dense_condense_flops(ny, nz, np_, m):
     return (2.0 / 3.0) * K ** 3 + 2 * K * K * (ny + np_ + 1) + 2 * m * ny * (ny + np_ + 1)
 
 
+def measured_traffic(nint):
+    """HBM bytes per colloc_condense launch at this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy kernel);
+    None when no measurement exists for this mesh size."""
+    path = os.path.join(ROOT, "profiles", "traffic_tanh20_nint%d_m5.md" % nint)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch colloc_condense:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(inst, m, nint_sample):
     """Oracle (NumPy/LAPACK) plain solve of the same family at a reduced mesh;
     work per solve is linear in Nint at equal Newton counts, so the rate is
@@ -165,7 +178,7 @@ def main():
                    "nint": nint, "m": m, "ny": ny, "nz": nz, "np": np_, "tolerance": tol,
                    "converged_steps": converged, "newton_iterations": results[-1][1]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(nint),
                      "kernel": "colloc_condense", "flops_per_launch": fl, "avg_launch_ms": avg_s * 1e3,
                      "launches": cond_n,
                      "dense_equivalent_tflops": dense_condense_flops(ny, nz, np_, m) * nint / avg_s / 1e12
