@@ -631,8 +631,10 @@ int cs_newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_ti
   return sync(h);
 }
 
-int cs_segment_residual(cs_handle* h, double alpha, double tol, double* residual_out, double* max_residual) {
+int cs_segment_residual_ex(cs_handle* h, double alpha, double tol, int residual_type, int scale_by_time,
+                           int relative, double* residual_out, double* max_residual) {
   if (!h) return fail(CS_E_ARG, "null handle");
+  if (residual_type < 0 || residual_type > 2) return fail(CS_E_ARG, "residual_type %d (0, 1 or 2)", residual_type);
   TRY(need_mesh(h));
   TRY(fill_scalars(h));
   int nint = h->N - 1;
@@ -641,13 +643,18 @@ int cs_segment_residual(cs_handle* h, double alpha, double tol, double* residual
   double* zt = h->zt();
   double* p = h->p();
   unsigned long long* slot = h->scal + 1;
-  void* args[] = {&nint, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &tol, &h->segres, &slot};
+  int st = scale_by_time ? 1 : 0, rel = relative ? 1 : 0;
+  void* args[] = {&nint, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &tol, &residual_type, &st, &rel, &h->segres, &slot};
   TRY(launch(h, F_SEGRES, (unsigned)((nint + 1 + 255) / 256), 256, args));
   double mr = 0.0;
   TRY(read_u64_double(h, 1, &mr));
   if (max_residual) *max_residual = mr;
   TRY(copy_out(h, residual_out, h->segres, (size_t)h->N));
   return sync(h);
+}
+
+int cs_segment_residual(cs_handle* h, double alpha, double tol, double* residual_out, double* max_residual) {
+  return cs_segment_residual_ex(h, alpha, tol, 1, 1, 0, residual_out, max_residual);
 }
 
 int cs_recover_solution(cs_handle* h, double* z) {

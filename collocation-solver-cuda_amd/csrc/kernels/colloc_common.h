@@ -76,6 +76,34 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
   return m01 > m23 ? m01 : m23;
 }
 
+// Cross-row lane exchanges for a wave seen as 4 rows of 16 lanes (gfx950
+// v_permlane16_swap / v_permlane32_swap; VALU, no LDS round trip).
+// sum_rows(v): in every lane, the sum of v over the 4 lanes with the same
+// lane index modulo 16.
+__device__ __forceinline__ double sum_rows(double v) {
+  auto l16 = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+  auto h16 = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+  // rows (0,1,2,3): [0] holds rows (0,0,2,2), [1] rows (1,1,3,3)
+  v = __hiloint2double((int)h16[0], (int)l16[0]) + __hiloint2double((int)h16[1], (int)l16[1]);
+  auto l32 = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false, false);
+  auto h32 = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false, false);
+  // halves (lo, hi): [0] holds (lo, lo), [1] holds (hi, hi)
+  return __hiloint2double((int)h32[0], (int)l32[0]) + __hiloint2double((int)h32[1], (int)l32[1]);
+}
+
+// bcast_row<R>(v): in every lane, v of the lane in row R with the same lane
+// index modulo 16.
+template <int R>
+__device__ __forceinline__ double bcast_row(double v) {
+  static_assert(R >= 0 && R < 4, "four rows of 16 lanes");
+  auto l16 = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+  auto h16 = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+  const int lo = (int)l16[R & 1], hi = (int)h16[R & 1];     // rows (R', R', R'+2, R'+2), R' = R & 1
+  auto l32 = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  auto h32 = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double((int)h32[R >> 1], (int)l32[R >> 1]);
+}
+
 // 1/x in FP64 from the hardware reciprocal estimate and two Newton steps
 // (error within a few ulp; used where a pivot's inverse is on a latency
 // chain).

@@ -32,12 +32,13 @@ constexpr int REC = (NS + M * NY + M * NZ) * NV;
 constexpr int CRW = 2 * NY + NP + 1;       // BABD row width
 constexpr int RECW = 3 * NY + NP + 1;      // cyclic-reduction record width
 constexpr int NFIN = 2 * NY + NP;          // boundary (final) system size
-constexpr int CR_PARTS = 4;                          // lanes per record column in cr_pair
+constexpr int CR_PARTS = 4;                          // lanes per record column in cr_pair (one per 16-lane row)
 constexpr int CR_COLS = WAVE / CR_PARTS;              // columns per wave
 constexpr int CR_ROWS = (2 * NY + CR_PARTS - 1) / CR_PARTS;   // rows per lane
 constexpr int CR_THREADS = (RECW + CR_COLS - 1) / CR_COLS * WAVE;
 
 static_assert(M >= 2, "need at least two collocation stages");
+static_assert(CR_PARTS == 4, "cr_pair sums over the four 16-lane rows of a wave");
 static_assert(NY <= 64, "cyclic-reduction kernels hold one row per lane");
 
 // -------------------------------------------------------------------- fill
@@ -371,8 +372,9 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
 // the top NY rows [R | Gp | Ep | Jp | d] are the record used to recover x_M,
 // the bottom NY rows are the combined row [G | C | H | b] of level l+1.
 // Each column of the 2NY x RECW block is held by CR_PARTS lanes of one wave
-// (CR_ROWS rows each), so a pair has many waves and short per-lane chains;
-// column sums over the parts are two lane shuffles.  The Householder vector
+// (CR_ROWS rows each), one lane in each 16-lane row of the wave, so a pair
+// has many waves and short per-lane chains; column sums over the parts and
+// the pivot broadcast are permlane16/32 swaps.  The Householder vector
 // is broadcast through LDS, where rows above the current pivot are kept zero
 // so every loop has a fixed trip count.  An odd last row is carried.
 __device__ __forceinline__ void cr_pair(int nrows, const double* __restrict__ rows_in,
@@ -424,9 +426,8 @@ __device__ __forceinline__ void cr_pair(int nrows, const double* __restrict__ ro
     constexpr int buf = kk & 1;
     constexpr int pk = kk / CR_ROWS, ik = kk % CR_ROWS;
     // column norm below the pivot and the pivot entry, for every column
-    double tfull = tail + __shfl_xor(tail, CR_COLS);
-    if (CR_PARTS > 2) tfull += __shfl_xor(tfull, 2 * CR_COLS);
-    const double akk = __shfl(a[ik], pk * CR_COLS + (lane % CR_COLS));
+    const double tfull = sum_rows(tail);
+    const double akk = bcast_row<pk>(a[ik]);
     if (cidx == kk) {
       const double nrm = sqrt(tfull);
       const double g = akk >= 0.0 ? -nrm : nrm;
@@ -445,41 +446,34 @@ __device__ __forceinline__ void cr_pair(int nrows, const double* __restrict__ ro
           else if (r > kk) a[i] = 0.0;
         }
       }
-      if (part == 0) sbeta[buf] = reflect ? -2.0 / vs : 0.0;
+      if (part == 0) sbeta[buf] = reflect ? -2.0 * fast_rcp(vs) : 0.0;
     }
     __syncthreads();
     const double beta = sbeta[buf];
     if (beta == 0.0) stopped = true;   // the reference stops reflecting at |v|^2 < eps
+    // dot products and squared tails in four independent chains (FP64 FMA
+    // latency is long)
     double w = 0.0;
     if (!stopped && cidx > kk && live) {
-      double w0 = 0.0, w1 = 0.0;
+      double wc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int i = 0; i < CR_ROWS; i += 2) {
-        w0 = fma(sv[buf][part * CR_ROWS + i], a[i], w0);
-        if (i + 1 < CR_ROWS) w1 = fma(sv[buf][part * CR_ROWS + i + 1], a[i + 1], w1);
-      }
-      w = w0 + w1;
+      for (int i = 0; i < CR_ROWS; ++i) wc[i & 3] = fma(sv[buf][part * CR_ROWS + i], a[i], wc[i & 3]);
+      w = (wc[0] + wc[1]) + (wc[2] + wc[3]);
     }
-    w += __shfl_xor(w, CR_COLS);
-    if (CR_PARTS > 2) w += __shfl_xor(w, 2 * CR_COLS);
+    w = sum_rows(w);
     if (!stopped && cidx > kk && live) {
       w *= beta;
-      double t0 = 0.0;
+#pragma unroll
+      for (int i = 0; i < CR_ROWS; ++i) a[i] = fma(w, sv[buf][part * CR_ROWS + i], a[i]);
+    }
+    {
+      double tc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int i = 0; i < CR_ROWS; ++i) {
         const int r = part * CR_ROWS + i;
-        a[i] = fma(w, sv[buf][r], a[i]);
-        t0 = (r > kk) ? fma(a[i], a[i], t0) : t0;
+        tc[i & 3] = (r > kk) ? fma(a[i], a[i], tc[i & 3]) : tc[i & 3];
       }
-      tail = t0;
-    } else {
-      double t0 = 0.0;
-#pragma unroll
-      for (int i = 0; i < CR_ROWS; ++i) {
-        const int r = part * CR_ROWS + i;
-        t0 = (r > kk) ? fma(a[i], a[i], t0) : t0;
-      }
-      tail = t0;
+      tail = (tc[0] + tc[1]) + (tc[2] + tc[3]);
     }
   });
   if (live) {
@@ -675,58 +669,110 @@ __device__ __forceinline__ void init_input(int nint, const double* __restrict__ 
 }
 
 // ------------------------------------------------------- segment residual
-// compute_segment_residual_kernel (solver:1833-1892), residual_type 1,
-// scale_by_time: note the scaling rho *= dt inside the component loop.
+// Per-interval acceptance residual at the m+1 Gauss points.
+//  relative == 0: compute_segment_residual_kernel (solver:1833-1892) with
+//    residual_type 0 (max), 1 (sum of |.|) or 2 (sum of squares), and with
+//    scale_by_time the scaling rho *= dt applied inside the component loop,
+//    as the reference does; for residual_type 0 the g part takes |h| of the
+//    same component index (solver:1880), components beyond ny contribute 0.
+//  relative == 1: compute_segment_residual_kernel2 (solver:1959-2013): the
+//    largest |h_k| / (1 + max_gp |ydot_k|) and |g_k| / (1 + max_gp |z_k|).
+// Entry nint is ||r||_inf / tol (solver:1811-1817).
+__device__ __forceinline__ void gauss_point(int gp, double h, const double* __restrict__ yi,
+                                            const double* __restrict__ yd, const double* __restrict__ zz,
+                                            double* ydt, double* yt, double* zt) {
+#pragma unroll
+  for (int l = 0; l < NY; ++l) {
+    double s = 0.0;
+    double yy = yi[l];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      s += c_gL[gp * M + k] * yd[k * NY + l];
+      yy += h * c_gI[gp * M + k] * yd[k * NY + l];
+    }
+    ydt[l] = s;
+    yt[l] = yy;
+  }
+#pragma unroll
+  for (int l = 0; l < NZ; ++l) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) s += c_gL[gp * M + k] * zz[k * NZ + l];
+    zt[l] = s;
+  }
+}
+
 __device__ __forceinline__ void segres(int nint, const double* __restrict__ t, const double* __restrict__ y,
                        const double* __restrict__ ydot, const double* __restrict__ ztil,
                        const double* __restrict__ p, const double* __restrict__ c, double alpha, double tol,
+                       int rtype, int scale_time, int relative,
                        double* __restrict__ residual, unsigned long long* __restrict__ maxres) {
   __shared__ double sred[16];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double val = 0.0;
   if (i < nint) {
     const double h = t[i + 1] - t[i];
+    const double* yi = y + (size_t)i * NY;
     const double* yd = ydot + (size_t)i * M * NY;
     const double* zz = ztil + (size_t)i * M * NZ;
     double rho_h = 0.0, rho_g = 0.0;
-    for (int gp = 0; gp < M + 1; ++gp) {
-      double ydt[NY], yt[NY], zt[NZ1], fo[NY], go[NZ1];
+    if (!relative) {
+      for (int gp = 0; gp < M + 1; ++gp) {
+        double ydt[NY], yt[NY], zt[NZ1], fo[NY], go[NZ1];
+        gauss_point(gp, h, yi, yd, zz, ydt, yt, zt);
+        prob::f(yt, zt, p, c, fo);
+        const double w = c_gw[gp];
 #pragma unroll
-      for (int l = 0; l < NY; ++l) {
-        double s = 0.0;
-        double yy = y[(size_t)i * NY + l];
-#pragma unroll
-        for (int k = 0; k < M; ++k) {
-          s += c_gL[gp * M + k] * yd[k * NY + l];
-          yy += h * c_gI[gp * M + k] * yd[k * NY + l];
+        for (int l = 0; l < NY; ++l) {
+          fo[l] -= ydt[l];
+          const double e = fabs(fo[l]);
+          if (rtype == 2) rho_h += w * fo[l] * fo[l];
+          else if (rtype == 1) rho_h += w * e;
+          else rho_h = fmax(rho_h, w * e);
+          if (scale_time) rho_h *= h;
         }
-        ydt[l] = s;
-        yt[l] = yy;
-      }
+        if (NZ > 0) {
+          prob::g(yt, zt, p, c, alpha, go);
 #pragma unroll
-      for (int l = 0; l < NZ; ++l) {
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < M; ++k) s += c_gL[gp * M + k] * zz[k * NZ + l];
-        zt[l] = s;
-      }
-      prob::f(yt, zt, p, c, fo);
-      const double w = c_gw[gp];
-#pragma unroll
-      for (int l = 0; l < NY; ++l) {
-        rho_h += w * fabs(fo[l] - ydt[l]);
-        rho_h *= h;
-      }
-      if (NZ > 0) {
-        prob::g(yt, zt, p, c, alpha, go);
-#pragma unroll
-        for (int l = 0; l < NZ; ++l) {
-          rho_g += w * fabs(go[l]);
-          rho_g *= h;
+          for (int l = 0; l < NZ; ++l) {
+            if (rtype == 2) rho_g += w * go[l] * go[l];
+            else if (rtype == 1) rho_g += w * fabs(go[l]);
+            else rho_g = fmax(rho_g, l < NY ? w * fabs(fo[l < NY ? l : 0]) : 0.0);
+            if (scale_time) rho_g *= h;
+          }
         }
       }
+      if (rtype == 2) val = sqrt(rho_h + rho_g) / tol;
+      else if (rtype == 1) val = (fabs(rho_h) + fabs(rho_g)) / tol;
+      else val = fmax(rho_h, rho_g) / tol;
+    } else {
+      double ydm[NY], zm[NZ1];
+#pragma unroll
+      for (int l = 0; l < NY; ++l) ydm[l] = 0.0;
+#pragma unroll
+      for (int l = 0; l < NZ; ++l) zm[l] = 0.0;
+      for (int gp = 0; gp < M + 1; ++gp) {
+        double ydt[NY], yt[NY], zt[NZ1];
+        gauss_point(gp, h, yi, yd, zz, ydt, yt, zt);
+#pragma unroll
+        for (int l = 0; l < NY; ++l) ydm[l] = fmax(ydm[l], fabs(ydt[l]));
+#pragma unroll
+        for (int l = 0; l < NZ; ++l) zm[l] = fmax(zm[l], fabs(zt[l]));
+      }
+      for (int gp = 0; gp < M + 1; ++gp) {
+        double ydt[NY], yt[NY], zt[NZ1], fo[NY], go[NZ1];
+        gauss_point(gp, h, yi, yd, zz, ydt, yt, zt);
+        prob::f(yt, zt, p, c, fo);
+#pragma unroll
+        for (int l = 0; l < NY; ++l) rho_h = fmax(rho_h, fabs(fo[l] - ydt[l]) / (1.0 + ydm[l]));
+        if (NZ > 0) {
+          prob::g(yt, zt, p, c, alpha, go);
+#pragma unroll
+          for (int l = 0; l < NZ; ++l) rho_g = fmax(rho_g, fabs(go[l]) / (1.0 + zm[l]));
+        }
+      }
+      val = fmax(rho_h, rho_g) / tol;
     }
-    val = (fabs(rho_h) + fabs(rho_g)) / tol;
     residual[i] = val;
   } else if (i == nint) {
     double ro[NY + NP];
@@ -816,8 +862,8 @@ __global__ __launch_bounds__(256) void colloc_init_input(int nint, const double*
 
 __global__ __launch_bounds__(256) void colloc_segres(int nint, const double* t, const double* y,
     const double* ydot, const double* ztil, const double* p, const double* c, double alpha, double tol,
-    double* residual, unsigned long long* maxres) {
-  colloc::segres(nint, t, y, ydot, ztil, p, c, alpha, tol, residual, maxres);
+    int rtype, int scale_time, int relative, double* residual, unsigned long long* maxres) {
+  colloc::segres(nint, t, y, ydot, ztil, p, c, alpha, tol, rtype, scale_time, relative, residual, maxres);
 }
 
 __global__ __launch_bounds__(256) void colloc_recover_z(int nnodes, const double* ztil, double* z) {

@@ -40,6 +40,7 @@ SIGNATURES = {
     "cs_recover_delta_k": (c_int, [c_void_p, P_D, P_D]),
     "cs_newton": (c_int, [c_void_p, c_double, c_double, c_int, P_I, P_D, P_D, c_int]),
     "cs_segment_residual": (c_int, [c_void_p, c_double, c_double, P_D, P_D]),
+    "cs_segment_residual_ex": (c_int, [c_void_p, c_double, c_double, c_int, c_int, c_int, P_D, P_D]),
     "cs_recover_solution": (c_int, [c_void_p, P_D]),
     "cs_profile": (c_int, [c_void_p, c_int]),
     "cs_profile_get": (c_int, [c_void_p, c_int, P_D, P_I]),

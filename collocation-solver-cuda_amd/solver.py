@@ -221,11 +221,14 @@ class CollocationSolver:
                               ptr(hist), max_iter))
         return it.value, nrm.value, hist[:min(it.value + 1, max_iter)].copy()
 
-    def segment_residual(self, alpha, tol):
-        """compute_segment_residual_parallel (solver:1770): (residual, max_residual)."""
+    def segment_residual(self, alpha, tol, residual_type=1, scale_by_time=True, residual2=False):
+        """compute_segment_residual_parallel (solver:1770), or with residual2
+        compute_segment_residual_parallel2 (solver:1895): (residual, max_residual).
+        residual_type / scale_by_time are the reference's module flags (solver:30-31)."""
         res = np.empty(self.N)
         mx = native.c_double()
-        check(lib().cs_segment_residual(self.h, alpha, tol, ptr(res), native.ctypes.byref(mx)))
+        check(lib().cs_segment_residual_ex(self.h, alpha, tol, int(residual_type), int(bool(scale_by_time)),
+                                           int(bool(residual2)), ptr(res), native.ctypes.byref(mx)))
         return res, mx.value
 
     def recover_solution(self):
@@ -249,7 +252,8 @@ class CollocationSolver:
         return ms.value, n.value
 
 
-def collocation_solver_parallel(bvp_dae, m=4, solver=None, write_output=False, log=None, homotopy=None):
+def collocation_solver_parallel(bvp_dae, m=4, solver=None, write_output=False, log=None, homotopy=None,
+                                residual_type=1, scale_by_time=True, residual2=False):
     """Continuation + Newton + mesh loop (solver:36-237).
 
     Same control flow as the reference: alpha starts at 0.1 (alpha_m = 1e-6
@@ -257,6 +261,9 @@ def collocation_solver_parallel(bvp_dae, m=4, solver=None, write_output=False, l
     damped Newton loop, checks the limits, computes the segment residual and
     either remeshes (residual > 1) or accepts.  Returns a dict with the
     solution on the final mesh and the per-step history.
+
+    residual_type, scale_by_time, residual2: the reference's module flags
+    (solver:30-33) selecting the mesh-acceptance residual.
 
     homotopy (an extension, not in the reference): a list of instance-constant
     vectors to pass through before the problem's own constants.  Newton is run
@@ -303,7 +310,7 @@ def collocation_solver_parallel(bvp_dae, m=4, solver=None, write_output=False, l
                 or N < min_nodes:
             success = 0
             break
-        residual, max_residual = solver.segment_residual(alpha, tol)
+        residual, max_residual = solver.segment_residual(alpha, tol, residual_type, scale_by_time, residual2)
         history[-1]["max_residual"] = max_residual
         if max_residual > 1:
             y, _, _, para = solver.get_state()

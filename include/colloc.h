@@ -139,6 +139,16 @@ int cs_newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_ti
 int cs_segment_residual(cs_handle* h, double alpha, double tol, double* residual,
                         double* max_residual);
 
+/* The other acceptance residuals of the reference, selected as its module
+ * flags select them (solver:30-33): relative = 0 is
+ * compute_segment_residual_parallel (solver:1770-1892) with residual_type
+ * 0 (max), 1 (sum of |.|) or 2 (sum of squares, then sqrt) and the
+ * scale_by_time flag; relative = 1 is compute_segment_residual_parallel2
+ * (solver:1895-2013, the residual2 flag), where residual_type and
+ * scale_by_time are ignored.  cs_segment_residual is (1, 1, 0). */
+int cs_segment_residual_ex(cs_handle* h, double alpha, double tol, int residual_type, int scale_by_time,
+                           int relative, double* residual, double* max_residual);
+
 /* recover_solution_parallel (solver:2034-2075): z at the nodes, z[N][nz]. */
 int cs_recover_solution(cs_handle* h, double* z);
 

@@ -165,38 +165,92 @@ def newton_solve(prob, m, rk, t_span, y, y_dot, z_tilde, p, alpha, tol, max_iter
     return y, y_dot, z_tilde, p, iter_time, norms
 
 
-def compute_segment_residual(prob, m, N, t_span, y, y_dot, z_tilde, p, alpha, tol):
-    """Per-interval acceptance residual (solver:1770-1892), residual_type = 1,
-    scale_by_time = True.  Quirk kept: the time scaling rho *= dt sits inside
-    the component loop (solver:1868-1869, 1883-1884).  The last entry is the
-    boundary norm ||r||_inf / tol (solver:1811-1817)."""
+def _gauss_point_values(prob, m, N, t_span, y, y_dot, z_tilde, i_gp, gc):
+    """ydot, y and z at Gauss point i_gp of every interval (get_y_dot/get_y/get_z, solver:1671-1742)."""
+    ny, nz, _ = _dims(prob)
+    nint = N - 1
+    dt = np.diff(t_span)
+    Yd = y_dot.reshape(nint, m, ny)
+    Zt = z_tilde.reshape(nint, m, nz)
+    L = compute_L(m, gc.t[i_gp])
+    I = compute_I(m, gc.t[i_gp])
+    yd_tau = np.einsum("k,ikl->il", L, Yd)
+    y_tau = y[:-1].copy()
+    for k in range(m):                       # get_y accumulates dt * I_k * ydot_k term by term
+        y_tau += dt[:, None] * I[k] * Yd[:, k, :]
+    z_tau = np.einsum("k,ikl->il", L, Zt)
+    return yd_tau, y_tau, z_tau
+
+
+def compute_segment_residual(prob, m, N, t_span, y, y_dot, z_tilde, p, alpha, tol, residual_type=1,
+                             scale_by_time=True):
+    """Per-interval acceptance residual (solver:1770-1892).  residual_type 0
+    (max), 1 (sum of |.|), 2 (sum of squares, sqrt at the end).  Quirks kept:
+    the time scaling rho *= dt sits inside the component loop
+    (solver:1868-1869, 1883-1884); for residual_type 0 the g part takes |h|
+    of the same component index (solver:1880; components beyond ny count 0).
+    The last entry is the boundary norm ||r||_inf / tol (solver:1811-1817)."""
     ny, nz, np_ = _dims(prob)
     nint = N - 1
     gc = gauss(m + 1)
     dt = np.diff(t_span)
-    Yd = y_dot.reshape(nint, m, ny)
-    Zt = z_tilde.reshape(nint, m, nz)
     rho_h = np.zeros(nint)
     rho_g = np.zeros(nint)
     for i in range(m + 1):
-        L = compute_L(m, gc.t[i])
-        I = compute_I(m, gc.t[i])
-        yd_tau = np.einsum("k,ikl->il", L, Yd)
-        y_tau = y[:-1].copy()
-        for k in range(m):                       # get_y accumulates dt * I_k * ydot_k term by term
-            y_tau += dt[:, None] * I[k] * Yd[:, k, :]
-        z_tau = np.einsum("k,ikl->il", L, Zt)
+        yd_tau, y_tau, z_tau = _gauss_point_values(prob, m, N, t_span, y, y_dot, z_tilde, i, gc)
         h_res = prob.f(y_tau, z_tau, p) - yd_tau
         for k in range(ny):
-            rho_h += gc.w[i] * np.abs(h_res[:, k])
-            rho_h *= dt
+            if residual_type == 2:
+                rho_h += gc.w[i] * h_res[:, k] * h_res[:, k]
+            elif residual_type == 1:
+                rho_h += gc.w[i] * np.abs(h_res[:, k])
+            else:
+                rho_h = np.maximum(rho_h, gc.w[i] * np.abs(h_res[:, k]))
+            if scale_by_time:
+                rho_h *= dt
         if nz > 0:
             g_res = prob.g(y_tau, z_tau, p, alpha)
             for k in range(nz):
-                rho_g += gc.w[i] * np.abs(g_res[:, k])
-                rho_g *= dt
+                if residual_type == 2:
+                    rho_g += gc.w[i] * g_res[:, k] * g_res[:, k]
+                elif residual_type == 1:
+                    rho_g += gc.w[i] * np.abs(g_res[:, k])
+                else:
+                    rho_g = np.maximum(rho_g, gc.w[i] * np.abs(h_res[:, k]) if k < ny else 0.0)
+                if scale_by_time:
+                    rho_g *= dt
     residual = np.zeros(N)
-    residual[:nint] = (np.abs(rho_h) + np.abs(rho_g)) / tol
+    if residual_type == 2:
+        residual[:nint] = np.sqrt(rho_h + rho_g) / tol
+    elif residual_type == 1:
+        residual[:nint] = (np.abs(rho_h) + np.abs(rho_g)) / tol
+    else:
+        residual[:nint] = np.maximum(rho_h, rho_g) / tol
+    if ny + np_ > 0:
+        residual[N - 1] = np.max(np.abs(prob.r(y[0], y[N - 1], p))) / tol
+    return residual, np.max(residual)
+
+
+def compute_segment_residual2(prob, m, N, t_span, y, y_dot, z_tilde, p, alpha, tol):
+    """compute_segment_residual_parallel2 (solver:1895-2013): per interval the
+    largest |h_k| / (1 + max over Gauss points |ydot_k|) and
+    |g_k| / (1 + max |z_k|), over / tol; last entry ||r||_inf / tol."""
+    ny, nz, np_ = _dims(prob)
+    nint = N - 1
+    gc = gauss(m + 1)
+    vals = [_gauss_point_values(prob, m, N, t_span, y, y_dot, z_tilde, i, gc) for i in range(m + 1)]
+    ydm = np.max([np.abs(v[0]) for v in vals], axis=0)
+    zm = np.max([np.abs(v[2]) for v in vals], axis=0) if nz > 0 else None
+    rho_h = np.zeros(nint)
+    rho_g = np.zeros(nint)
+    for yd_tau, y_tau, z_tau in vals:
+        h_res = prob.f(y_tau, z_tau, p) - yd_tau
+        rho_h = np.maximum(rho_h, np.max(np.abs(h_res) / (1.0 + ydm), axis=1))
+        if nz > 0:
+            g_res = prob.g(y_tau, z_tau, p, alpha)
+            rho_g = np.maximum(rho_g, np.max(np.abs(g_res) / (1.0 + zm), axis=1))
+    residual = np.zeros(N)
+    residual[:nint] = np.maximum(rho_h, rho_g) / tol
     if ny + np_ > 0:
         residual[N - 1] = np.max(np.abs(prob.r(y[0], y[N - 1], p))) / tol
     return residual, np.max(residual)

@@ -126,6 +126,30 @@ def test_initial_input_and_segment_residual(small):
     assert rel_err(z, osolver.recover_solution(m, N, zt)) < 1e-15
 
 
+@pytest.mark.parametrize("rtype, scale, relative", [(0, True, False), (2, True, False), (1, False, False),
+                                                    (1, True, True)])
+def test_segment_residual_variants(small, rtype, scale, relative):
+    """residual_type 0 / 2, unscaled, and the relative residual of
+    compute_segment_residual_parallel2, on a seeded state."""
+    inst, m, bvp, S = small
+    prob = family_of(inst)
+    N = bvp.N
+    rng = np.random.default_rng(21)
+    t = np.sort(np.concatenate([[0.0, inst.T], rng.uniform(0.0, inst.T, N - 2)]))
+    y, yd, zt, p = seeded_state(prob, m, N, 7, scale=0.3)
+    S.set_mesh(t)
+    S.set_state(y, yd, zt, p)
+    say("segment residual type=%d scale=%d relative=%d" % (rtype, scale, relative))
+    res, mx = S.segment_residual(0.1, 1e-8, residual_type=rtype, scale_by_time=scale, residual2=relative)
+    if relative:
+        res_o, mx_o = osolver.compute_segment_residual2(prob, m, N, t, y, yd, zt, p, 0.1, 1e-8)
+    else:
+        res_o, mx_o = osolver.compute_segment_residual(prob, m, N, t, y, yd, zt, p, 0.1, 1e-8,
+                                                       residual_type=rtype, scale_by_time=scale)
+    assert rel_err(res, res_o) < 1e-12
+    assert abs(mx - mx_o) <= 1e-12 * mx_o
+
+
 def compare_solves(res, ref, tol=1e-9):
     assert res["success"] == ref["success"] == 1
     assert [h["iterations"] for h in res["history"]] == [h["iterations"] for h in ref["history"]]

@@ -196,3 +196,71 @@ def test_remesh_by_hand():
     np.testing.assert_allclose(z2[:, 0], kept)
     np.testing.assert_allclose(t2, np.array(kept) / (N - 1))
     assert N2 == len(kept)
+
+
+class _TwoComp:
+    """ny = 2, nz = 1 test system for the acceptance residual variants."""
+    size_y, size_z, size_p = 2, 1, 1
+    f = staticmethod(lambda Y, Z, P: np.stack([Y[:, 1] * Z[:, 0], -Y[:, 0]], axis=1))
+    g = staticmethod(lambda Y, Z, P, alpha=0.0: (Z[:, 0] - Y[:, 0] ** 2)[:, None])
+    r = staticmethod(lambda y0, y1, p: np.array([y0[0] - 1.0, y1[1], p[0]]))
+
+
+def _residual_by_loops(m, t, y, yd, zt, p, tol, rtype, scale, relative):
+    """The acceptance residual written out with scalar loops; the stage
+    polynomials are rebuilt with numpy.polynomial (fit through the Lobatto
+    nodes, integrated with polyint), independent of the oracle's tables."""
+    c = tables.lobatto(m).c
+    gc = tables.gauss(m + 1)
+    N = len(t)
+    out = np.zeros(N)
+    for i in range(N - 1):
+        h = t[i + 1] - t[i]
+        st = slice(i * m, (i + 1) * m)
+        pts = []
+        for gi in range(m + 1):
+            tau = gc.t[gi]
+            ydt = np.array([np.polyval(np.polyfit(c, yd[st, k], m - 1), tau) for k in range(2)])
+            yt = np.array([y[i, k] + h * np.polyval(np.polyint(np.polyfit(c, yd[st, k], m - 1)), tau) for k in range(2)])
+            z = np.array([np.polyval(np.polyfit(c, zt[st, 0], m - 1), tau)])
+            hr = _TwoComp.f(yt[None], z[None], p)[0] - ydt
+            gr = _TwoComp.g(yt[None], z[None], p)[0]
+            pts.append((ydt, z, hr, gr, gc.w[gi]))
+        if relative:
+            ydm = np.max([np.abs(q[0]) for q in pts], axis=0)
+            zm = np.max([np.abs(q[1]) for q in pts], axis=0)
+            rh = max(abs(q[2][k]) / (1 + ydm[k]) for q in pts for k in range(2))
+            rg = max(abs(q[3][0]) / (1 + zm[0]) for q in pts)
+            out[i] = max(rh, rg) / tol
+            continue
+        rh = rg = 0.0
+        for ydt, z, hr, gr, w in pts:
+            for k in range(2):
+                rh = rh + w * hr[k] ** 2 if rtype == 2 else (rh + w * abs(hr[k]) if rtype == 1 else max(rh, w * abs(hr[k])))
+                rh = rh * h if scale else rh
+            for k in range(1):
+                rg = rg + w * gr[k] ** 2 if rtype == 2 else (rg + w * abs(gr[k]) if rtype == 1 else max(rg, w * abs(hr[k])))
+                rg = rg * h if scale else rg
+        out[i] = (np.sqrt(rh + rg) if rtype == 2 else (abs(rh) + abs(rg) if rtype == 1 else max(rh, rg))) / tol
+    out[N - 1] = np.max(np.abs(_TwoComp.r(y[0], y[-1], p))) / tol
+    return out
+
+
+@pytest.mark.parametrize("rtype, scale, relative", [(0, True, False), (1, True, False), (2, True, False),
+                                                    (0, False, False), (2, False, False), (1, True, True)])
+def test_segment_residual_variants_by_loops(rtype, scale, relative):
+    rng = np.random.default_rng(11 + rtype)
+    m, N = 4, 5
+    t = np.sort(np.concatenate([[0.0, 1.3], rng.uniform(0, 1.3, N - 2)]))
+    y = rng.normal(size=(N, 2))
+    yd = rng.normal(size=((N - 1) * m, 2))
+    zt = rng.normal(size=((N - 1) * m, 1))
+    p = np.array([0.3])
+    if relative:
+        res, mx = solver.compute_segment_residual2(_TwoComp, m, N, t, y, yd, zt, p, 0.0, 1e-6)
+    else:
+        res, mx = solver.compute_segment_residual(_TwoComp, m, N, t, y, yd, zt, p, 0.0, 1e-6,
+                                                  residual_type=rtype, scale_by_time=scale)
+    ref = _residual_by_loops(m, t, y, yd, zt, p, 1e-6, rtype, scale, relative)
+    np.testing.assert_allclose(res, ref, rtol=1e-11)
+    assert mx == np.max(res)
