@@ -38,14 +38,16 @@ def load_package():
     return mod
 
 
-def condense_flops(ny, nz, np_, m):
+def condense_flops(ny, nz, npa, m):
     """Executed FP64 flops of one interval of the condensation kernel.
 
     per stage: Gauss-Jordan on [g_z | g_y g_p f_a,g]  2 nz^2 (nz + nv)
                T_j = [h_y h_p f_a,h] - h_z G_j        2 ny nv nz
     stage system: Gauss-Jordan on ns x (ns + nv)      ns^2 (ns - 1 + 2 nv)
-    with nv = ny + np + 1, ns = (m - 1) ny."""
-    nv = ny + np_ + 1
+    with nv = ny + npa + 1 (npa: parameters the stage functions depend on;
+    the other parameter columns are structurally zero and not formed) and
+    ns = (m - 1) ny."""
+    nv = ny + npa + 1
     ns = (m - 1) * ny
     return m * (2 * nz * nz * (nz + nv) + 2 * ny * nv * nz) + ns * ns * (ns - 1 + 2 * nv)
 
@@ -154,7 +156,9 @@ def main():
     value = converged / elapsed
     nint = bvp.N - 1
     ny, nz, np_ = bvp.size_y, bvp.size_z, bvp.size_p
-    fl = condense_flops(ny, nz, np_, m) * nint
+    from colloc_amd import jit
+    npa = len(jit.unit_info(bvp.ocp_text, m)["active_params"])
+    fl = condense_flops(ny, nz, npa, m) * nint
     avg_s = cond_ms / max(cond_n, 1) / 1e3
     achieved = fl / avg_s / 1e12 if avg_s > 0 else 0.0
     print("per-solve Newton iterations %s, final residual %.2e, max segment residual %.3g"
@@ -175,7 +179,8 @@ def main():
         "data": "synthetic (seeded OCP family, generated in-process)",
         "config": {"workload": "plain FP64 single-instance solve, tanh OCP n_x=20 n_u=10 (ny=40 nz=10 np=40), "
                                "Nint=%d, m=%d, all-ones guess + homotopy S: 0->0.5" % (nint, m),
-                   "nint": nint, "m": m, "ny": ny, "nz": nz, "np": np_, "tolerance": tol,
+                   "nint": nint, "m": m, "ny": ny, "nz": nz, "np": np_, "np_in_stage_functions": npa,
+                   "tolerance": tol,
                    "converged_steps": converged, "newton_iterations": results[-1][1]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(nint),

@@ -11,7 +11,7 @@
 //   ytil[Nint*M][NY]; fa[Nint][M*(NY+NZ)]; fb[Nint][NY]; rbc[NY+NP]
 //   jac[Nint*M][JSTAGE]      stage block = Df (NY x NJC) then Dg (NZ x NJC)
 //   rows[Nint][NY][CRW]      BABD row  [G | C | H | b]  (G x_left + C x_right + H p = b)
-//   rec[Nint][REC]           condensation record [U (NS x NV) | T (M NY x NV) | G (M NZ x NV)]
+//   rec[Nint][REC]           condensation record [U (NS x NVA) | T (M NY x NVA) | G (M NZ x NVA)]
 //   crrec[Nint-1][NY][RECW]  cyclic-reduction record [R | Gp | Ep | Jp | d]
 #pragma once
 #include "colloc_common.h"
@@ -26,9 +26,10 @@ constexpr int NC1 = NC > 0 ? NC : 1;
 constexpr int NYZ = NY + NZ;
 constexpr int NJC = NY + NZ + NP;          // columns of Df / Dg
 constexpr int JSTAGE = (NY + NZ) * NJC;    // doubles per stage Jacobian block
-constexpr int NV = NY + NP + 1;            // unknown-coefficient columns [dy_i | dp | 1]
+constexpr int NPA = Cfg::NPA;              // parameters the stage functions depend on
+constexpr int NVA = NY + NPA + 1;          // condensed columns [dy_i | dp (active) | 1]
 constexpr int NS = (M - 1) * NY;           // size of the condensed stage system
-constexpr int REC = (NS + M * NY + M * NZ) * NV;
+constexpr int REC = (NS + M * NY + M * NZ) * NVA;
 constexpr int CRW = 2 * NY + NP + 1;       // BABD row width
 constexpr int RECW = 3 * NY + NP + 1;      // cyclic-reduction record width
 constexpr int NFIN = 2 * NY + NP;          // boundary (final) system size
@@ -38,6 +39,14 @@ constexpr int CR_ROWS = (2 * NY + CR_PARTS - 1) / CR_PARTS;   // rows per lane
 constexpr int CR_THREADS = (RECW + CR_COLS - 1) / CR_COLS * WAVE;
 
 static_assert(M >= 2, "need at least two collocation stages");
+
+// Parameter j of the condensed column q = NY + j (j < NPA) is Cfg::PACT[j].
+__device__ __forceinline__ int pact(int j) { return Cfg::PACT[j]; }
+__host__ __device__ constexpr bool p_active(int j) {
+  for (int a = 0; a < NPA; ++a)
+    if (Cfg::PACT[a] == j) return true;
+  return false;
+}
 static_assert(CR_PARTS == 4, "cr_pair sums over the four 16-lane rows of a wave");
 static_assert(NY <= 64, "cyclic-reduction kernels hold one row per lane");
 
@@ -147,26 +156,28 @@ __device__ __forceinline__ void jacobian(int nint, const double* __restrict__ y,
 // interval's linearised collocation equations (the role of
 // reduce_jacobian_parallel_kernel, solver:887-1001) using the stage
 // structure of W instead of a dense K x K LU:
-//   per stage j:  G_j = g_z^-1 [g_y | g_p | f_a,g]        (NZ x NV)
-//                 T_j = [h_y | h_p | f_a,h] - h_z G_j      (NY x NV)
+//   per stage j:  G_j = g_z^-1 [g_y | g_p | f_a,g]        (NZ x NVA)
+//                 T_j = [h_y | h_p | f_a,h] - h_z G_j      (NY x NVA)
 //   so that  dz_j = -G_j [u_j; dp; 1],  dydot_j = T_j [u_j; dp; 1]
 //   with u_j the stage-state increment.  Lobatto IIIA has a_0k = 0, so
 //   u_0 = dy_i and the unknowns u_1..u_{M-1} satisfy
 //     u_j - h sum_{k>=1} a_jk M_k u_k = (E_y + h a_j0 T_0 + h sum_{k>=1} a_jk [0|N_k|s_k]) v
 //   with M_k = T_k[:, :NY], [N_k|s_k] = T_k[:, NY:], v = [dy_i; dp; 1].  That
-//   NS x NS system with NV right-hand sides is solved by register-resident
+//   NS x NS system with NVA right-hand sides is solved by register-resident
 //   Gauss-Jordan.  Because b = A[M-1, :] (c_{M-1} = 1), the continuity row is
 //   dy_{i+1} - u_{M-1} = -f_b, giving the BABD row
-//   G = -U_{M-1}[:, :NY], C = I, H = -U_{M-1}[:, NY:NY+NP], b = -f_b + U_{M-1}[:, NV-1].
+//   G = -U_{M-1}[:, :NY], C = I, H = -U_{M-1}[:, NY:NY+NPA] (zero in the other
+//   parameter columns), b = -f_b + U_{M-1}[:, NVA-1].  Only the NPA parameters
+//   the stage functions depend on get columns (g_p = f_p = 0 for the rest).
 template <int CW>
 __device__ __forceinline__ void condense(int nint, const double* __restrict__ t, const double* __restrict__ jac,
                          const double* __restrict__ fa, const double* __restrict__ fb,
                          double* __restrict__ rows, double* __restrict__ rec, int* __restrict__ status) {
-  using Solver = GJB<NS, NS + NV, CW, 16>;
+  using Solver = GJB<NS, NS + NVA, CW, 16>;
   constexpr int NT = CW * WAVE;
   // sT (phases 1-2) and the elimination's panel / pivot-row buffers (phase 3)
   // share one LDS area
-  constexpr int SMEM = M * NY * NV > Solver::LDS_DOUBLES ? M * NY * NV : Solver::LDS_DOUBLES;
+  constexpr int SMEM = M * NY * NVA > Solver::LDS_DOUBLES ? M * NY * NVA : Solver::LDS_DOUBLES;
   __shared__ double smem[SMEM];
   __shared__ int sflag[Solver::NR];
   double* sT = smem;
@@ -181,8 +192,8 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
   if (tid == 0) sbad = 0;
   const double h = t[i + 1] - t[i];
   double* recU = rec + (size_t)i * REC;
-  double* recT = recU + NS * NV;
-  double* recG = recT + M * NY * NV;
+  double* recT = recU + NS * NVA;
+  double* recG = recT + M * NY * NVA;
   int bad = 0;
 
   // ---- phase 1: wave j eliminates the algebraic unknowns of stage j.
@@ -190,7 +201,7 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
   // Gauss-Jordan with partial pivoting and physical row swaps, the pivot
   // column broadcast by v_readlane, so the wave needs no barrier.  The same
   // wave then forms T_j = [h_y | h_p | f_a,h] - h_z G_j for its columns.
-  constexpr int NCZ = NZ + NV;
+  constexpr int NCZ = NZ + NVA;
   constexpr int SL = (NCZ + WAVE - 1) / WAVE;
   for (int j = wave; j < M; j += CW) {
     const double* Dfj = jac + ((size_t)i * M + j) * JSTAGE;
@@ -206,7 +217,7 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
         if (cc < NZ) v = Dgj[r * NJC + NY + cc];
         else if (cc < NCZ) {
           const int q = cc - NZ;
-          v = q < NY ? Dgj[r * NJC + q] : (q < NY + NP ? Dgj[r * NJC + NZ + q] : faj[NY + r]);
+          v = q < NY ? Dgj[r * NJC + q] : (q < NY + NPA ? Dgj[r * NJC + NY + NZ + pact(q - NY)] : faj[NY + r]);
         }
         gc[sl][r] = v;
       }
@@ -249,9 +260,9 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
 #pragma unroll
     for (int sl = 0; sl < SL; ++sl) {
       const int q = lane + WAVE * sl - NZ;
-      if (q < 0 || q >= NV) continue;
+      if (q < 0 || q >= NVA) continue;
 #pragma unroll
-      for (int r = 0; r < NZ; ++r) recG[(j * NZ + r) * NV + q] = gc[sl][r];
+      for (int r = 0; r < NZ; ++r) recG[(j * NZ + r) * NVA + q] = gc[sl][r];
     }
     // T_j = [h_y | h_p | f_a,h] - h_z G_j, rows in batches of TB whose
     // Jacobian loads are all issued before any of them is used
@@ -267,8 +278,9 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
 #pragma unroll
         for (int sl = 0; sl < SL; ++sl) {
           const int q = lane + WAVE * sl - NZ;
-          const int src = q < NY ? q : q + NZ;       // h_y, then h_p after the NZ z-columns
-          hb[rr][sl] = (q >= 0 && q < NY + NP) ? hrow[src] : (q == NY + NP ? faj[r] : 0.0);
+          // h_y, then the active h_p columns after the NZ z-columns
+          const int src = q < NY ? q : NY + NZ + pact(q - NY < NPA ? q - NY : 0);
+          hb[rr][sl] = (q >= 0 && q < NY + NPA) ? hrow[src] : (q == NY + NPA ? faj[r] : 0.0);
         }
       }
 #pragma unroll
@@ -278,12 +290,12 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
 #pragma unroll
         for (int sl = 0; sl < SL; ++sl) {
           const int q = lane + WAVE * sl - NZ;
-          if (q < 0 || q >= NV) continue;
+          if (q < 0 || q >= NVA) continue;
           double v = hb[rr][sl];
 #pragma unroll
           for (int z = 0; z < NZ; ++z) v = fma(-hz[rr][z], gc[sl][z], v);
-          sT[(j * NY + r) * NV + q] = v;
-          recT[(j * NY + r) * NV + q] = v;
+          sT[(j * NY + r) * NVA + q] = v;
+          recT[(j * NY + r) * NVA + q] = v;
         }
       }
     }
@@ -301,19 +313,19 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
     for (int cq = 0; cq < Solver::CQ; ++cq) {
       const int cc = Solver::col(cq);
       double v = 0.0;
-      if (r < NS && cc < NS + NV) {
+      if (r < NS && cc < NS + NVA) {
         if (cc < NS) {
           const int kc = cc / NY + 1;
           const int bc = cc - (kc - 1) * NY;
-          v = -h * c_rkA[jr * M + kc] * sT[(kc * NY + ar) * NV + bc];
+          v = -h * c_rkA[jr * M + kc] * sT[(kc * NY + ar) * NVA + bc];
           if (jr == kc && ar == bc) v += 1.0;
         } else {
           const int qc = cc - NS;
-          v = h * c_rkA[jr * M] * sT[ar * NV + qc];
+          v = h * c_rkA[jr * M] * sT[ar * NVA + qc];
           if (qc < NY) {
             if (qc == ar) v += 1.0;
           } else {
-            for (int k = 1; k < M; ++k) v += h * c_rkA[jr * M + k] * sT[(k * NY + ar) * NV + qc];
+            for (int k = 1; k < M; ++k) v += h * c_rkA[jr * M + k] * sT[(k * NY + ar) * NVA + qc];
           }
         }
       }
@@ -346,13 +358,13 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
 #pragma unroll
     for (int cq = 0; cq < Solver::CQ; ++cq) {
       const int cc = Solver::col(cq);
-      if (cc < NS || cc >= NS + NV) continue;
+      if (cc < NS || cc >= NS + NVA) continue;
       const int qc = cc - NS;
       const double v = gj.a[q][cq] / pivval[kk];
-      recU[kk * NV + qc] = v;
+      recU[kk * NVA + qc] = v;
       if (last) {
         if (qc < NY) rowi[a * CRW + qc] = -v;
-        else if (qc < NY + NP) rowi[a * CRW + NY + qc] = -v;
+        else if (qc < NY + NPA) rowi[a * CRW + 2 * NY + pact(qc - NY)] = -v;
         else rowi[a * CRW + CRW - 1] = v - fb[(size_t)i * NY + a];
       }
     }
@@ -360,6 +372,12 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
   for (int e = tid; e < NY * NY; e += NT) {
     const int r = e / NY, q = e - (e / NY) * NY;
     rowi[r * CRW + NY + q] = (r == q) ? 1.0 : 0.0;
+  }
+  if (NPA < NP) {   // H columns of parameters the stage functions do not depend on
+    for (int e = tid; e < NY * NP; e += NT) {
+      const int r = e / NP, j = e - (e / NP) * NP;
+      if (!p_active(j)) rowi[r * CRW + 2 * NY + j] = 0.0;
+    }
   }
   if (tid == 0 && sbad) atomicOr(status, sbad);
 }
@@ -586,27 +604,27 @@ __device__ __forceinline__ void cr_back(int level, int nrows, int nint, const do
 //   u_0 = dy_i, u_j = U_j v,  dydot_j = T_j [u_j; dp; 1],  dz_j = -G_j [u_j; dp; 1].
 __device__ __forceinline__ void recover(int nint, const double* __restrict__ rec, const double* __restrict__ dy,
                         const double* __restrict__ dp, double* __restrict__ dydot, double* __restrict__ dz) {
-  __shared__ double sw[M][NV];
+  __shared__ double sw[M][NVA];
   const int i = blockIdx.x;
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = wave_id();
   const int nw = blockDim.x / WAVE;
   const double* recU = rec + (size_t)i * REC;
-  const double* recT = recU + NS * NV;
-  const double* recG = recT + M * NY * NV;
-  for (int e = tid; e < M * NV; e += blockDim.x) {
-    const int j = e / NV, q = e - (e / NV) * NV;
+  const double* recT = recU + NS * NVA;
+  const double* recG = recT + M * NY * NVA;
+  for (int e = tid; e < M * NVA; e += blockDim.x) {
+    const int j = e / NVA, q = e - (e / NVA) * NVA;
     double v = 0.0;
     if (q < NY) v = (j == 0) ? dy[(size_t)i * NY + q] : 0.0;
-    else if (q < NY + NP) v = dp[q - NY];
+    else if (q < NY + NPA) v = dp[pact(q - NY)];
     else v = 1.0;
     sw[j][q] = v;
   }
   __syncthreads();
-  // u_j rows: one wave per row, lanes across the NV columns
+  // u_j rows: one wave per row, lanes across the NVA columns
   for (int rr = wave; rr < NS; rr += nw) {
     double s = 0.0;
-    for (int q = lane; q < NV; q += WAVE) s = fma(recU[rr * NV + q], sw[0][q], s);
+    for (int q = lane; q < NVA; q += WAVE) s = fma(recU[rr * NVA + q], sw[0][q], s);
     s = wave_sum(s);
     if (lane == 0) sw[rr / NY + 1][rr % NY] = s;
   }
@@ -614,9 +632,9 @@ __device__ __forceinline__ void recover(int nint, const double* __restrict__ rec
   for (int rr = wave; rr < M * (NY + NZ); rr += nw) {
     const bool isz = rr >= M * NY;
     const int j = isz ? (rr - M * NY) / NZ1 : rr / NY;
-    const double* rowp = isz ? recG + (size_t)(rr - M * NY) * NV : recT + (size_t)rr * NV;
+    const double* rowp = isz ? recG + (size_t)(rr - M * NY) * NVA : recT + (size_t)rr * NVA;
     double s = 0.0;
-    for (int q = lane; q < NV; q += WAVE) s = fma(rowp[q], sw[j][q], s);
+    for (int q = lane; q < NVA; q += WAVE) s = fma(rowp[q], sw[j][q], s);
     s = wave_sum(s);
     if (lane == 0) {
       if (isz) {

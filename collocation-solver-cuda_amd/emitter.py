@@ -410,23 +410,24 @@ def _carray(name, vals):
     return "__constant__ double %s[%d] = {%s};" % (name, len(vals), ", ".join(repr(float(v)) for v in vals))
 
 
-def choose_waves(ny, nz, np_, m):
+def choose_waves(ny, nz, np_, m, npa=None):
     """Waves per workgroup of the register-resident eliminations.
 
-    Condensation (stage system (m-1) ny x ((m-1) ny + ny + np + 1)): the
-    pipelined blocked solver keeps the matrix in W - 1 waves and the panel in
-    wave 0; eight waves (256 VGPRs each) when a matrix wave's share is at
-    most 105 doubles, otherwise sixteen (128 VGPRs) with at most 51.
-    Boundary system ((2 ny + np) x (2 ny + np + 1)), one column per wave
-    slot: eight waves up to 96 doubles per thread, sixteen up to 48."""
+    Condensation: the stage system (m-1) ny x ((m-1) ny + ny + npa + 1), npa
+    being the parameters the stage functions depend on, held by the blocked
+    Gauss-Jordan solver (lane = row, columns spread over the waves): eight
+    waves (256 VGPRs each) when a thread's share is at most 96 doubles,
+    otherwise sixteen (128 VGPRs) with at most 42.  Boundary system
+    ((2 ny + np) x (2 ny + np + 1)): eight waves up to 96 doubles per
+    thread, sixteen up to 48."""
     def rq(n):
         return (n + 63) // 64
 
     ns = (m - 1) * ny
-    ncol = ns + ny + np_ + 1
-    if rq(ns) * -(-ncol // 7) <= 105:
+    ncol = ns + ny + (np_ if npa is None else npa) + 1
+    if rq(ns) * -(-ncol // 8) <= 96:
         cw = 8
-    elif rq(ns) * -(-ncol // 15) <= 51:
+    elif rq(ns) * -(-ncol // 16) <= 42:
         cw = 16
     else:
         raise ValueError("a %d x %d condensation does not fit in one workgroup's registers" % (ns, ncol))
@@ -440,6 +441,22 @@ def choose_waves(ny, nz, np_, m):
     return cw, fw
 
 
+def active_p_columns(sym):
+    """Parameters that the stage functions f, g depend on.
+
+    A parameter column of [f_p; g_p] that is identically zero contributes an
+    identically zero column to the condensed system (its H block is zero);
+    the condensation skips such columns.  Returns the sorted list of the
+    parameter indices whose Jacobian column is not structurally zero."""
+    base = sym.size_y + sym.size_z
+    out = []
+    for j in range(sym.size_p):
+        col = [row[base + j] for row in sym.Df_exprs] + [row[base + j] for row in sym.Dg_exprs]
+        if any(sp.sympify(e) != 0 for e in col):
+            out.append(j)
+    return out
+
+
 def build_unit(sym, m, tables):
     """Complete hiprtc translation unit: callbacks, sizes, tables, kernels.
 
@@ -447,10 +464,14 @@ def build_unit(sym, m, tables):
     L(tau) and its integral I(tau) at the Gauss points and L(0), L(1)."""
     ny, nz, np_ = sym.size_y, sym.size_z, sym.size_p
     nc = len(sym.instance_constants)
-    cw, fw = choose_waves(ny, nz, np_, m)
+    pact = active_p_columns(sym)
+    cw, fw = choose_waves(ny, nz, np_, m, npa=len(pact))
     parts = ['#include "colloc_common.h"', emit_problem(sym)]
-    parts.append("struct Cfg { static constexpr int NY = %d, NZ = %d, NP = %d, NC = %d, M = %d, CW = %d, FW = %d; };"
-                 % (ny, nz, np_, nc, m, cw, fw))
+    parts.append("struct Cfg {\n  static constexpr int NY = %d, NZ = %d, NP = %d, NC = %d, M = %d, CW = %d, FW = %d;\n"
+                 "  // parameters the stage functions depend on (the other H columns are zero)\n"
+                 "  static constexpr int NPA = %d;\n  static constexpr int PACT[%d] = {%s};\n};"
+                 % (ny, nz, np_, nc, m, cw, fw, len(pact), max(len(pact), 1),
+                    ", ".join(str(j) for j in pact) if pact else "0"))
     parts.append(_carray("c_rkA", tables.A.ravel()))
     parts.append(_carray("c_rkb", tables.b))
     parts.append(_carray("c_rkc", tables.c))

@@ -82,5 +82,16 @@ def code_object(ocp_text, m, verbose=False):
         f.write(code)
     os.replace(tmp, path)
     with open(os.path.join(CACHE_DIR, key + ".json"), "w") as f:
-        json.dump({"m": m, "sizes": sizes_of(ocp), "bytes": len(code)}, f)
+        json.dump({"m": m, "sizes": sizes_of(ocp), "bytes": len(code),
+                   "active_params": emitter.active_p_columns(sym)}, f)
     return code
+
+
+def unit_info(ocp_text, m):
+    """Metadata of the problem's code object: sizes and the parameters whose
+    condensed columns are kept (compiles the code object on a cache miss)."""
+    path = os.path.join(CACHE_DIR, cache_key(ocp_text, m) + ".json")
+    if not os.path.exists(path):
+        code_object(ocp_text, m)
+    with open(path) as f:
+        return json.load(f)

@@ -164,6 +164,35 @@ def batch_family(count, seed=1, nx=20, nu=10, nint_range=(256, 2048)):
     return out
 
 
+# A problem whose dynamics depend on a parameter (so its condensed system has
+# an active parameter column): a forced oscillator whose frequency pp is a
+# free parameter with a terminal penalty pulling it towards 1.2.
+PARAM_OSC_OCP = """
+StateVariables = [x1, x2];
+ControlVariables = [u1];
+ParameterVariables = [pp];
+CostFunctional = x1*x1 + x2*x2 + u1*u1;
+TerminalPenalty = 5*(pp - 1.2)^2;
+DifferentialEquations = [pp*x2, -pp*x1 + u1];
+InitialConstraints = [x1 - 1, x2];
+TerminalConstraints = [x1];
+InitialTime = 0;
+FinalTime = 1;
+Nodes = 65;
+Tolerance = 1e-8;
+"""
+
+
+class TextProblem:
+    """An OCP given as text (no generated family data)."""
+
+    def __init__(self, text):
+        self.text = text
+
+    def ocp_text(self):
+        return self.text
+
+
 def standard_units():
     """Problems whose device code the build step compiles ahead of time:
     (label, FamilyInstance, m).  Mesh size does not change the generated
@@ -172,3 +201,4 @@ def standard_units():
     small, _ = config(1, nx=4, nu=2, nint=32)
     yield "tanh4_m5", small, 5
     yield "tanh20_m5", config(1)[0], 5
+    yield "param_osc_m4", TextProblem(PARAM_OSC_OCP), 4

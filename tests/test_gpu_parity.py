@@ -241,3 +241,51 @@ def test_tanh20_full_size_converges():
     np.testing.assert_allclose(res["p"][nx:], res["y"][-1, nx:], atol=1e-8)
     # optimality: u = -B' lambda / (2 c_u) at the nodes
     np.testing.assert_allclose(res["z"], -(res["y"][:, nx:] @ inst.B) / (2 * inst.c_u), atol=1e-7)
+
+
+@pytest.fixture(scope="module")
+def param_osc():
+    from symbolic_problem import SymbolicProblem
+    prob = SymbolicProblem(problems.PARAM_OSC_OCP)
+    bvp = solver.BvpDae(problems.PARAM_OSC_OCP)
+    return prob, bvp
+
+
+def test_active_parameter_column_condensation(param_osc):
+    """Dynamics that depend on a parameter: the condensed rows' H has a
+    nonzero column for it and zeros for the boundary multipliers."""
+    prob, bvp = param_osc
+    m = 4
+    S = solver.CollocationSolver(bvp, m, n_max=bvp.N)
+    N = bvp.N
+    rng = np.random.default_rng(3)
+    t = np.sort(np.concatenate([[0.0, 1.0], rng.uniform(0.0, 1.0, N - 2)]))
+    y, yd, zt, p = seeded_state(prob, m, N, 4, scale=0.5)
+    S.set_mesh(t)
+    S.set_state(y, yd, zt, p)
+    say("param_osc stage parity")
+    _, yt, fa, fb, rbc = S.compute_f_q(0.1)
+    S.construct_jacobian(0.1)
+    A, C, H, b = S.reduce_jacobian()
+    rk = otables.lobatto(m)
+    f_q, yt_o, fa_o, fb_o, rbc_o = osolver.compute_f_q(prob, m, N, rk, t, y, yd, zt, p, 0.1)
+    J, V, D, W, B0, Bn, Vn = osolver.construct_jacobian(prob, m, N, rk, t, y, yt_o, zt, p, 0.1)
+    A_o, C_o, H_o, b_o = osolver.reduce_jacobian(prob, m, N, W, D, J, V, fa_o, fb_o)
+    assert np.max(np.abs(H_o[:, :, 0])) > 1e-3          # the parameter column is active
+    assert rel_err(A, A_o) < 1e-11 and rel_err(H, H_o) < 1e-11 and rel_err(b, b_o) < 1e-11
+    assert np.all(H[:, :, 1:] == 0.0)
+    dy, dp = S.solve_babd()
+    dy_o, dp_o = obabd.solve_babd(prob.size_y, prob.size_p, N, A_o, C_o, H_o, b_o, B0, Bn, Vn, rbc_o, M=8)
+    assert rel_err(dy, dy_o) < 1e-10 and rel_err(dp, dp_o) < 1e-10
+    dyd, dz = S.recover_delta_k()
+    _, dyd_o, dz_o = osolver.recover_delta_k(prob, m, N, dy_o, dp_o, fa_o, J, V, W)
+    assert rel_err(dyd, dyd_o) < 1e-10 and rel_err(dz, dz_o) < 1e-10
+    S.close()
+
+
+def test_active_parameter_solve(param_osc):
+    prob, bvp = param_osc
+    say("param_osc solve")
+    res = solver.collocation_solver_parallel(bvp, 4)
+    ref = osolver.collocation_solver(prob, 4, bvp.T0, bvp.Y0, bvp.Z0, bvp.P0, tol=bvp.tolerance)
+    compare_solves(res, ref)
