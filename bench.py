@@ -38,18 +38,29 @@ def load_package():
     return mod
 
 
-def condense_flops(ny, nz, npa, m):
-    """Executed FP64 flops of one interval of the condensation kernel.
+def stage_elim_flops(ny, nz, npa, m):
+    """Executed FP64 flops of the per-stage eliminations of one interval:
+    Gauss-Jordan on [g_z | g_y g_p f_a,g] (2 nz^2 (nz + nv)) and
+    T_j = [h_y h_p f_a,h] - h_z G_j (2 ny nv nz) for each of the m stages,
+    nv = ny + npa + 1 (npa: parameters the stage functions depend on; the
+    other parameter columns are structurally zero and not formed)."""
+    nv = ny + npa + 1
+    return m * (2 * nz * nz * (nz + nv) + 2 * ny * nv * nz)
 
-    per stage: Gauss-Jordan on [g_z | g_y g_p f_a,g]  2 nz^2 (nz + nv)
-               T_j = [h_y h_p f_a,h] - h_z G_j        2 ny nv nz
-    stage system: Gauss-Jordan on ns x (ns + nv)      ns^2 (ns - 1 + 2 nv)
-    with nv = ny + npa + 1 (npa: parameters the stage functions depend on;
-    the other parameter columns are structurally zero and not formed) and
-    ns = (m - 1) ny."""
+
+def series_term_flops(ny, npa, m):
+    """One term of the series solve of the stage system of one interval:
+    Y_k = M_k U_k for the m - 1 stage blocks (2 (m-1) ny^2 nv) and the
+    mixing U = R + h (A^ x I) Y (2 (m-1)^2 ny nv)."""
+    nv = ny + npa + 1
+    return 2 * (m - 1) * ny * ny * nv + 2 * (m - 1) ** 2 * ny * nv
+
+
+def gauss_jordan_flops(ny, npa, m):
+    """Gauss-Jordan on the ns x (ns + nv) stage system: ns^2 (ns - 1 + 2 nv)."""
     nv = ny + npa + 1
     ns = (m - 1) * ny
-    return m * (2 * nz * nz * (nz + nv) + 2 * ny * nv * nz) + ns * ns * (ns - 1 + 2 * nv)
+    return ns * ns * (ns - 1 + 2 * nv)
 
 
 def dense_condense_flops(ny, nz, np_, m):
@@ -146,9 +157,11 @@ def main():
     for _ in range(args.warmup):
         one_solve()
     S.profile(True)
+    fb0, terms0 = S.condense_stats()
     t0 = time.perf_counter()
     results = [one_solve() for _ in range(args.steps)]
     elapsed = time.perf_counter() - t0
+    fb1, terms1 = S.condense_stats()
     cond_ms, cond_n = S.profile_get("condense")
     prof = {k: S.profile_get(k) for k in ("residual", "jacobian", "condense", "babd", "recover")}
     S.profile(False)
@@ -158,7 +171,13 @@ def main():
     ny, nz, np_ = bvp.size_y, bvp.size_z, bvp.size_p
     from colloc_amd import jit
     npa = len(jit.unit_info(bvp.ocp_text, m)["active_params"])
-    fl = condense_flops(ny, nz, npa, m) * nint
+    # executed flops of all timed condensation launches: stage eliminations of
+    # every interval, the series terms actually run, Gauss-Jordan for the
+    # intervals the series left
+    fl_total = (cond_n * nint * stage_elim_flops(ny, nz, npa, m) + (terms1 - terms0) * series_term_flops(ny, npa, m)
+                + (fb1 - fb0) * gauss_jordan_flops(ny, npa, m))
+    fl = fl_total / max(cond_n, 1)
+    terms_per_interval = (terms1 - terms0) / max(cond_n * nint, 1)
     avg_s = cond_ms / max(cond_n, 1) / 1e3
     achieved = fl / avg_s / 1e12 if avg_s > 0 else 0.0
     print("per-solve Newton iterations %s, final residual %.2e, max segment residual %.3g"
@@ -184,8 +203,9 @@ def main():
                    "converged_steps": converged, "newton_iterations": results[-1][1]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(nint),
-                     "kernel": "colloc_condense", "flops_per_launch": fl, "avg_launch_ms": avg_s * 1e3,
-                     "launches": cond_n,
+                     "kernel": "colloc_condense_ns (+ colloc_condense for intervals the series leaves)",
+                     "flops_per_launch": fl, "avg_launch_ms": avg_s * 1e3, "launches": cond_n,
+                     "series_terms_per_interval": terms_per_interval, "gauss_jordan_intervals": fb1 - fb0,
                      "dense_equivalent_tflops": dense_condense_flops(ny, nz, np_, m) * nint / avg_s / 1e12
                      if avg_s > 0 else 0.0},
     }

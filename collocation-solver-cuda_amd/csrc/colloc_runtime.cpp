@@ -43,13 +43,14 @@ struct Seg4 {
 };
 
 enum Fn {
-  F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_COND, F_CR_PAIR, F_CR_FINAL, F_CR_BACK, F_RECOVER,
+  F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_COND, F_COND_NS, F_CR_PAIR, F_CR_FINAL, F_CR_BACK, F_RECOVER,
   F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
   "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian",
-  "colloc_condense", "colloc_cr_pair", "colloc_cr_final", "colloc_cr_back", "colloc_recover",
-  "colloc_axpy4", "colloc_init_input", "colloc_segres", "colloc_recover_z"};
+  "colloc_condense", "colloc_condense_ns", "colloc_cr_pair", "colloc_cr_final", "colloc_cr_back",
+  "colloc_recover", "colloc_axpy4", "colloc_init_input", "colloc_segres", "colloc_recover_z"};
+constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
 
 }  // namespace
 
@@ -84,6 +85,9 @@ struct cs_handle {
   double* segres = nullptr;
   double* zbuf = nullptr;
   double* y0buf = nullptr;
+  int* fbk = nullptr;        // per interval: 1 if the series condensation left it to Gauss-Jordan
+  unsigned long long cond_fallback = 0;   // intervals condensed by Gauss-Jordan since creation
+  unsigned long long cond_terms = 0;      // series terms summed over intervals since creation
   unsigned long long* scal = nullptr;  // [0]: norm, [1]: max residual
   int* status = nullptr;
   // profiling
@@ -213,10 +217,31 @@ int jacobian(cs_handle* h, double alpha) {
   return launch(h, F_JAC, (nthreads + 255) / 256, 256, args);
 }
 
+// Condensation: the series kernel handles every interval whose stage system
+// converges as a Neumann series (fine meshes) and counts the others; when
+// there are any, the Gauss-Jordan kernel redoes them (a block per interval,
+// the converged ones exit at once).
 int condense(cs_handle* h) {
   ProfScope ps(h, CS_KERNEL_CONDENSE);
   int nint = h->N - 1;
-  void* args[] = {&nint, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status};
+  unsigned long long* counters = h->scal + 2;   // [intervals left to Gauss-Jordan, series terms]
+  {
+    unsigned long long zero = 0ull;
+    int n = 2;
+    void* args[] = {&counters, &zero, &n};
+    TRY(launch(h, F_FILL, 1, 64, args));
+  }
+  void* args_ns[] = {&nint, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &h->fbk, &counters};
+  TRY(launch(h, F_COND_NS, (unsigned)nint, kNsThreads, args_ns));
+  unsigned long long cnt[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->cond_fallback += cnt[0];
+  h->cond_terms += cnt[1];
+  const unsigned long long nleft = cnt[0];
+  if (nleft == 0) return CS_OK;
+  const int* only = h->fbk;
+  void* args[] = {&nint, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &only};
   return launch(h, F_COND, (unsigned)nint, (unsigned)(h->cw * 64), args);
 }
 
@@ -390,7 +415,8 @@ int cs_create(const cs_dims* d, cs_handle** out) {
   A(&h->zbuf, nmax * std::max(nz, 1));
   A(&h->y0buf, nmax * (ny + std::max(nz, 1)) + np);
   if (r == CS_OK) {
-    hipError_t e = hipMalloc((void**)&h->scal, 4 * sizeof(unsigned long long));
+    hipError_t e = hipMalloc((void**)&h->fbk, std::max<size_t>(nint, 1) * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&h->scal, 4 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&h->status, sizeof(int));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(h->status, 0, sizeof(int));
@@ -414,6 +440,7 @@ void cs_destroy(cs_handle* h) {
                     h->rows0, h->rows1, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf};
   for (double* b : bufs)
     if (b) hipFree(b);
+  if (h->fbk) hipFree(h->fbk);
   if (h->scal) hipFree(h->scal);
   if (h->status) hipFree(h->status);
   for (int k = 0; k < CS_KERNEL_COUNT; ++k)
@@ -684,6 +711,13 @@ int cs_profile_get(cs_handle* h, int kernel, double* total_ms, int* launches) {
   drain_prof(h);
   if (total_ms) *total_ms = h->prof_ms[kernel];
   if (launches) *launches = h->prof_n[kernel];
+  return CS_OK;
+}
+
+int cs_condense_stats(cs_handle* h, long long* fallback_intervals, long long* series_terms) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (fallback_intervals) *fallback_intervals = (long long)h->cond_fallback;
+  if (series_terms) *series_terms = (long long)h->cond_terms;
   return CS_OK;
 }
 

@@ -151,59 +151,22 @@ __device__ __forceinline__ void jacobian(int nint, const double* __restrict__ y,
   }
 }
 
-// --------------------------------------------------------------- condense
-// One workgroup per interval.  Condenses the stage unknowns out of the
-// interval's linearised collocation equations (the role of
-// reduce_jacobian_parallel_kernel, solver:887-1001) using the stage
-// structure of W instead of a dense K x K LU:
-//   per stage j:  G_j = g_z^-1 [g_y | g_p | f_a,g]        (NZ x NVA)
-//                 T_j = [h_y | h_p | f_a,h] - h_z G_j      (NY x NVA)
-//   so that  dz_j = -G_j [u_j; dp; 1],  dydot_j = T_j [u_j; dp; 1]
-//   with u_j the stage-state increment.  Lobatto IIIA has a_0k = 0, so
-//   u_0 = dy_i and the unknowns u_1..u_{M-1} satisfy
-//     u_j - h sum_{k>=1} a_jk M_k u_k = (E_y + h a_j0 T_0 + h sum_{k>=1} a_jk [0|N_k|s_k]) v
-//   with M_k = T_k[:, :NY], [N_k|s_k] = T_k[:, NY:], v = [dy_i; dp; 1].  That
-//   NS x NS system with NVA right-hand sides is solved by register-resident
-//   Gauss-Jordan.  Because b = A[M-1, :] (c_{M-1} = 1), the continuity row is
-//   dy_{i+1} - u_{M-1} = -f_b, giving the BABD row
-//   G = -U_{M-1}[:, :NY], C = I, H = -U_{M-1}[:, NY:NY+NPA] (zero in the other
-//   parameter columns), b = -f_b + U_{M-1}[:, NVA-1].  Only the NPA parameters
-//   the stage functions depend on get columns (g_p = f_p = 0 for the rest).
-template <int CW>
-__device__ __forceinline__ void condense(int nint, const double* __restrict__ t, const double* __restrict__ jac,
-                         const double* __restrict__ fa, const double* __restrict__ fb,
-                         double* __restrict__ rows, double* __restrict__ rec, int* __restrict__ status) {
-  using Solver = GJB<NS, NS + NVA, CW, 16>;
-  constexpr int NT = CW * WAVE;
-  // sT (phases 1-2) and the elimination's panel / pivot-row buffers (phase 3)
-  // share one LDS area
-  constexpr int SMEM = M * NY * NVA > Solver::LDS_DOUBLES ? M * NY * NVA : Solver::LDS_DOUBLES;
-  __shared__ double smem[SMEM];
-  __shared__ int sflag[Solver::NR];
-  double* sT = smem;
-  __shared__ int pivrow[NS];
-  __shared__ double pivval[NS];
-  __shared__ int invperm[Solver::RQ * WAVE];
-  __shared__ int sbad;
-
-  const int i = blockIdx.x;
-  const int tid = threadIdx.x;
+// Stage eliminations of one interval (the first phase of both condensation
+// kernels): wave j eliminates the algebraic unknowns of stage j.  Writes T_j
+// to sT (LDS, rows of NVA) and recT, G_j to recG; returns nonzero on a zero
+// pivot in some g_z.
+template <int NW>
+__device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__ jac, const double* __restrict__ fa,
+                                               double* sT, double* __restrict__ recT, double* __restrict__ recG) {
   const int lane = lane_id(), wave = wave_id();
-  if (tid == 0) sbad = 0;
-  const double h = t[i + 1] - t[i];
-  double* recU = rec + (size_t)i * REC;
-  double* recT = recU + NS * NVA;
-  double* recG = recT + M * NY * NVA;
   int bad = 0;
-
-  // ---- phase 1: wave j eliminates the algebraic unknowns of stage j.
   // Lane l holds columns l + 64 s of [g_z | g_y g_p f_a,g] (NZ rows each);
   // Gauss-Jordan with partial pivoting and physical row swaps, the pivot
   // column broadcast by v_readlane, so the wave needs no barrier.  The same
   // wave then forms T_j = [h_y | h_p | f_a,h] - h_z G_j for its columns.
   constexpr int NCZ = NZ + NVA;
   constexpr int SL = (NCZ + WAVE - 1) / WAVE;
-  for (int j = wave; j < M; j += CW) {
+  for (int j = wave; j < M; j += NW) {
     const double* Dfj = jac + ((size_t)i * M + j) * JSTAGE;
     const double* Dgj = Dfj + NY * NJC;
     const double* faj = fa + (size_t)i * M * NYZ + j * NYZ;
@@ -300,6 +263,59 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
       }
     }
   }
+  return bad;
+}
+
+// --------------------------------------------------------------- condense
+// One workgroup per interval.  Condenses the stage unknowns out of the
+// interval's linearised collocation equations (the role of
+// reduce_jacobian_parallel_kernel, solver:887-1001) using the stage
+// structure of W instead of a dense K x K LU:
+//   per stage j:  G_j = g_z^-1 [g_y | g_p | f_a,g]        (NZ x NVA)
+//                 T_j = [h_y | h_p | f_a,h] - h_z G_j      (NY x NVA)
+//   so that  dz_j = -G_j [u_j; dp; 1],  dydot_j = T_j [u_j; dp; 1]
+//   with u_j the stage-state increment.  Lobatto IIIA has a_0k = 0, so
+//   u_0 = dy_i and the unknowns u_1..u_{M-1} satisfy
+//     u_j - h sum_{k>=1} a_jk M_k u_k = (E_y + h a_j0 T_0 + h sum_{k>=1} a_jk [0|N_k|s_k]) v
+//   with M_k = T_k[:, :NY], [N_k|s_k] = T_k[:, NY:], v = [dy_i; dp; 1].  That
+//   NS x NS system with NVA right-hand sides is solved by register-resident
+//   Gauss-Jordan.  Because b = A[M-1, :] (c_{M-1} = 1), the continuity row is
+//   dy_{i+1} - u_{M-1} = -f_b, giving the BABD row
+//   G = -U_{M-1}[:, :NY], C = I, H = -U_{M-1}[:, NY:NY+NPA] (zero in the other
+//   parameter columns), b = -f_b + U_{M-1}[:, NVA-1].  Only the NPA parameters
+//   the stage functions depend on get columns (g_p = f_p = 0 for the rest).
+template <int CW>
+__device__ __forceinline__ void condense(int nint, const double* __restrict__ t, const double* __restrict__ jac,
+                         const double* __restrict__ fa, const double* __restrict__ fb,
+                         double* __restrict__ rows, double* __restrict__ rec, int* __restrict__ status,
+                         const int* __restrict__ only) {
+  // only != nullptr: redo just the intervals the series kernel left (only[i] != 0)
+  if (only && !only[blockIdx.x]) return;
+  using Solver = GJB<NS, NS + NVA, CW, 16>;
+  constexpr int NT = CW * WAVE;
+  // sT (phases 1-2) and the elimination's panel / pivot-row buffers (phase 3)
+  // share one LDS area
+  constexpr int SMEM = M * NY * NVA > Solver::LDS_DOUBLES ? M * NY * NVA : Solver::LDS_DOUBLES;
+  __shared__ double smem[SMEM];
+  __shared__ int sflag[Solver::NR];
+  double* sT = smem;
+  __shared__ int pivrow[NS];
+  __shared__ double pivval[NS];
+  __shared__ int invperm[Solver::RQ * WAVE];
+  __shared__ int sbad;
+
+  const int i = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = lane_id(), wave = wave_id();
+  if (tid == 0) sbad = 0;
+  const double h = t[i + 1] - t[i];
+  double* recU = rec + (size_t)i * REC;
+  double* recT = recU + NS * NVA;
+  double* recG = recT + M * NY * NVA;
+  int bad = 0;
+
+  // ---- phase 1: wave j eliminates the algebraic unknowns of stage j.
+  bad |= stage_eliminate<CW>(i, jac, fa, sT, recT, recG);
   __syncthreads();
 
   // ---- phase 2: assemble [S | RHS] into registers
@@ -380,6 +396,187 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
     }
   }
   if (tid == 0 && sbad) atomicOr(status, sbad);
+}
+
+// --------------------------------------------------- series condensation
+// The stage system S U = R of condense, S = I - h (A^ x I) diag(M_k) with
+// A^ = A[1:, 1:] and M_k = T_k[:, :NY], solved by the fixed-point iteration
+//   U <- R + h (A^ x I) diag(M_k) U,
+// the Neumann series of S^-1 applied to R.  ||h (A^ x I) diag(M_k)|| is O(h):
+// on fine meshes a few terms reach FP64 accuracy.  The iteration stops when a
+// term changes U by at most 2^-50 of its largest entry (a few units of FP64
+// rounding: each term recomputes U from R, so the change does not go to 0);
+// an interval that has not converged after NS_MAX_TERMS terms (or produced a
+// non-finite value) is marked in fallback[] and re-done by condense.
+// Work item (k, r, h) of the workgroup owns row r of stage block k and the
+// columns [h NS_CPT, (h + 1) NS_CPT) of U.  U and the products Y_k = M_k U_k
+// pass through one LDS buffer whose rows are padded to NVAP columns, so every
+// work item reads whole, aligned column groups.
+constexpr int NS_MAX_TERMS = 40;
+constexpr int NS_THREADS = 512;
+constexpr int NSK = M - 1;                                 // stage blocks
+constexpr int NS_CPT = 24;                                 // columns per work item
+constexpr int NS_NH = (NVA + NS_CPT - 1) / NS_CPT;
+constexpr int NVAP = NS_NH * NS_CPT;                       // padded row of the U buffer
+constexpr int NS_ITEMS = NSK * NY * NS_NH;
+constexpr bool NS_OK = NY <= 48 && NS_ITEMS <= NS_THREADS &&
+                       (M * NY * NVA + NSK * NY * NVAP) * 8 <= 150 * 1024;
+
+__device__ __forceinline__ void condense_ns(int nint, const double* __restrict__ t, const double* __restrict__ jac,
+                                            const double* __restrict__ fa, const double* __restrict__ fb,
+                                            double* __restrict__ rows, double* __restrict__ rec,
+                                            int* __restrict__ status, int* __restrict__ fallback,
+                                            unsigned long long* __restrict__ counters) {
+  constexpr int NWV = NS_THREADS / WAVE;
+  // one LDS array with buf at its start, so that the U-buffer addresses fit
+  // the 16-bit offsets of the LDS instructions
+  __shared__ double smem_ns[NSK * NY * NVAP + M * NY * NVA];
+  double* buf = smem_ns;
+  double* sT = smem_ns + NSK * NY * NVAP;
+  __shared__ double sred[2 * NWV];
+  __shared__ int sbad;
+  const int i = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = lane_id(), wave = wave_id();
+  if (tid == 0) sbad = 0;
+  const double h = t[i + 1] - t[i];
+  double* recU = rec + (size_t)i * REC;
+  double* recT = recU + NS * NVA;
+  double* recG = recT + M * NY * NVA;
+  const int bad = stage_eliminate<NWV>(i, jac, fa, sT, recT, recG);
+  __syncthreads();
+
+  const bool act = tid < NS_ITEMS;
+  const int it = act ? tid : 0;
+  const int k = it / (NY * NS_NH);
+  const int rem = it - k * NY * NS_NH;
+  const int r = rem / NS_NH;
+  const int c0 = (rem - r * NS_NH) * NS_CPT;
+  const int js = k + 1;
+  double wk[NSK];                      // h A[k+1][k'+1]
+#pragma unroll
+  for (int q = 0; q < NSK; ++q) wk[q] = h * c_rkA[js * M + q + 1];
+  // R = h A[k+1][0] T_0 + (c < NY ? I : h sum_j A[k+1][j] T_j), kept in
+  // registers; it is also the first iterate
+  double rv[NS_CPT], xo[NS_CPT];
+  double* mine = buf + (k * NY + r) * NVAP;
+#pragma unroll
+  for (int cc = 0; cc < NS_CPT; ++cc) {
+    const int c = c0 + cc < NVA ? c0 + cc : NVA - 1;
+    double v = h * c_rkA[js * M] * sT[r * NVA + c];
+    if (c < NY) {
+      if (c == r) v += 1.0;
+    } else {
+#pragma unroll
+      for (int j = 1; j < M; ++j) v += h * c_rkA[js * M + j] * sT[(j * NY + r) * NVA + c];
+    }
+    if (c0 + cc >= NVA) v = 0.0;       // padding columns stay zero
+    rv[cc] = v;
+    xo[cc] = v;
+    if (act) mine[c0 + cc] = v;
+  }
+  __syncthreads();
+  bool converged = false;
+  int terms = 0;
+  for (int term = 0; term < NS_MAX_TERMS; ++term) {
+    ++terms;
+    // Y = M_{k+1} U_k on this item's columns.  The row of M is re-read from
+    // LDS every term through an opaque offset (not hoisted into registers).
+    int moff = (js * NY + r) * NVA;
+    asm volatile("" : "+v"(moff));
+    double acc[NS_CPT];
+#pragma unroll
+    for (int cc = 0; cc < NS_CPT; ++cc) acc[cc] = 0.0;
+    const double* Uk = buf + k * NY * NVAP + c0;
+    // a runtime loop over the rows of U_k, two rows per step: each step's
+    // loads and FMAs are scheduled together (a fully unrolled product keeps
+    // every load live)
+#pragma nounroll
+    for (int q = 0; q < NY; ++q) {
+      const double mq = sT[moff + q];
+      double u[NS_CPT];
+#pragma unroll
+      for (int cc = 0; cc < NS_CPT; ++cc) u[cc] = Uk[q * NVAP + cc];
+#pragma unroll
+      for (int cc = 0; cc < NS_CPT; ++cc) acc[cc] = fma(mq, u[cc], acc[cc]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int cc = 0; cc < NS_CPT; ++cc)
+      if (act) mine[c0 + cc] = acc[cc];
+    __syncthreads();
+    // U_new = R + h (A^ x I) Y
+    double dmax = 0.0, xmax = 0.0;
+#pragma unroll
+    for (int cc = 0; cc < NS_CPT; ++cc) {
+      double v = rv[cc];
+#pragma unroll
+      for (int q = 0; q < NSK; ++q) v = fma(wk[q], buf[(q * NY + r) * NVAP + c0 + cc], v);
+      acc[cc] = v;
+      if (act && c0 + cc < NVA) {
+        dmax = nmax(fabs(v - xo[cc]), dmax);
+        xmax = nmax(fabs(v), xmax);
+      }
+      xo[cc] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int cc = 0; cc < NS_CPT; ++cc)
+      if (act) mine[c0 + cc] = acc[cc];
+    dmax = wave_max(dmax);
+    xmax = wave_max(xmax);
+    if (lane == 0) {
+      sred[wave] = dmax;
+      sred[NWV + wave] = xmax;
+    }
+    __syncthreads();
+    double dm = sred[0], xm = sred[NWV];
+#pragma unroll
+    for (int w = 1; w < NWV; ++w) {
+      dm = nmax(sred[w], dm);
+      xm = nmax(sred[NWV + w], xm);
+    }
+    if (dm <= 0x1p-50 * xm) {
+      converged = true;
+      break;
+    }
+    __syncthreads();   // sred is rewritten by the next term
+  }
+  if (tid == 0) {
+    fallback[i] = converged ? 0 : 1;
+    if (!converged) atomicAdd(counters, 1ull);    // counters[0]: intervals left to Gauss-Jordan
+    atomicAdd(counters + 1, (unsigned long long)terms);   // counters[1]: series terms
+  }
+  if (bad) atomicOr(&sbad, bad);
+  __syncthreads();
+  if (tid == 0 && sbad) atomicOr(status, sbad);
+  if (!converged) return;
+  // outputs: U, the BABD row from the last stage block, C = I, zero H columns
+  double* rowi = rows + (size_t)i * NY * CRW;
+  if (act) {
+#pragma unroll
+    for (int cc = 0; cc < NS_CPT; ++cc) {
+      const int c = c0 + cc;
+      if (c >= NVA) continue;
+      const double v = xo[cc];
+      recU[(k * NY + r) * NVA + c] = v;
+      if (k == NSK - 1) {
+        if (c < NY) rowi[r * CRW + c] = -v;
+        else if (c < NY + NPA) rowi[r * CRW + 2 * NY + pact(c - NY)] = -v;
+        else rowi[r * CRW + CRW - 1] = v - fb[(size_t)i * NY + r];
+      }
+    }
+  }
+  for (int e = tid; e < NY * NY; e += NS_THREADS) {
+    const int a = e / NY, q = e - (e / NY) * NY;
+    rowi[a * CRW + NY + q] = (a == q) ? 1.0 : 0.0;
+  }
+  if (NPA < NP) {
+    for (int e = tid; e < NY * NP; e += NS_THREADS) {
+      const int a = e / NP, j = e - (e / NP) * NP;
+      if (!p_active(j)) rowi[a * CRW + 2 * NY + j] = 0.0;
+    }
+  }
 }
 
 // ------------------------------------------------------ cyclic reduction
@@ -844,8 +1041,18 @@ __global__ __launch_bounds__(256) void colloc_jacobian(int nint, const double* y
 }
 
 __global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const double* t, const double* jac,
-    const double* fa, const double* fb, double* rows, double* rec, int* status) {
-  colloc::condense<Cfg::CW>(nint, t, jac, fa, fb, rows, rec, status);
+    const double* fa, const double* fb, double* rows, double* rec, int* status, const int* only) {
+  colloc::condense<Cfg::CW>(nint, t, jac, fa, fb, rows, rec, status, only);
+}
+__global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nint, const double* t,
+    const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status, int* fallback,
+    unsigned long long* counters) {
+  if constexpr (colloc::NS_OK) {
+    colloc::condense_ns(nint, t, jac, fa, fb, rows, rec, status, fallback, counters);
+  } else if (threadIdx.x == 0) {
+    fallback[blockIdx.x] = 1;
+    atomicAdd(counters, 1ull);
+  }
 }
 
 __global__ __launch_bounds__(colloc::CR_THREADS) void colloc_cr_pair(int nrows, const double* rows_in, double* rows_out,

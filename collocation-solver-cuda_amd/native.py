@@ -45,6 +45,7 @@ SIGNATURES = {
     "cs_profile": (c_int, [c_void_p, c_int]),
     "cs_profile_get": (c_int, [c_void_p, c_int, P_D, P_I]),
     "cs_status": (c_int, [c_void_p, P_I]),
+    "cs_condense_stats": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 

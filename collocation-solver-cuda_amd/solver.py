@@ -237,6 +237,18 @@ class CollocationSolver:
         check(lib().cs_recover_solution(self.h, ptr(z)))
         return z
 
+    def condense_stats(self):
+        """(intervals condensed by Gauss-Jordan because the series solve did
+        not converge for them, series terms summed over intervals), since the
+        solver was created."""
+        n = native.ctypes.c_longlong()
+        k = native.ctypes.c_longlong()
+        check(lib().cs_condense_stats(self.h, native.ctypes.byref(n), native.ctypes.byref(k)))
+        return n.value, k.value
+
+    def condense_fallbacks(self):
+        return self.condense_stats()[0]
+
     def status(self):
         s = native.c_int()
         check(lib().cs_status(self.h, native.ctypes.byref(s)))

@@ -164,6 +164,14 @@ int cs_recover_solution(cs_handle* h, double* z);
 int cs_profile(cs_handle* h, int enable);
 int cs_profile_get(cs_handle* h, int kernel, double* total_ms, int* launches);
 
+/* Condensation (reduce_jacobian) runs a series solve of each interval's
+ * stage system and falls back to Gauss-Jordan elimination for intervals
+ * where the series does not converge (coarse meshes).  Since the handle was
+ * created: *fallback_intervals = intervals condensed by the fallback,
+ * *series_terms = series terms summed over all intervals (either may be
+ * NULL).  No counterpart in the reference (it always factors W). */
+int cs_condense_stats(cs_handle* h, long long* fallback_intervals, long long* series_terms);
+
 /* Device status word (bit 0: singular stage block, bit 1: singular stage
  * system, bit 2: singular boundary system); cleared by reading. */
 int cs_status(cs_handle* h, int* status);

@@ -289,3 +289,36 @@ def test_active_parameter_solve(param_osc):
     res = solver.collocation_solver_parallel(bvp, 4)
     ref = osolver.collocation_solver(prob, 4, bvp.T0, bvp.Y0, bvp.Z0, bvp.P0, tol=bvp.tolerance)
     compare_solves(res, ref)
+
+
+@pytest.mark.parametrize("nint, series", [(512, True), (2, False)])
+def test_condensation_paths(nint, series):
+    """The series solve of the stage systems takes every interval of a fine
+    mesh (h = 1/256); on a very coarse one (h = 1) it does not converge and
+    the Gauss-Jordan fallback condenses; both agree with the oracle."""
+    inst, m = problems.config(1, nint=nint)
+    bvp = solver.BvpDae(inst.ocp_text())
+    prob = family_of(inst)
+    N = bvp.N
+    S = solver.CollocationSolver(bvp, m, n_max=N)
+    t = np.linspace(0.0, inst.T, N)
+    y, yd, zt, p = seeded_state(prob, m, N, 9, scale=0.3)
+    S.set_mesh(t)
+    S.set_state(y, yd, zt, p)
+    say("condensation paths nint=%d" % nint)
+    S.compute_f_q(0.1, outputs=False)
+    S.construct_jacobian(0.1, outputs=False)
+    A, C, H, b = S.reduce_jacobian()
+    n_fb = S.condense_fallbacks()
+    S.close()
+    if series:
+        assert n_fb == 0
+    else:
+        assert n_fb > 0
+    rk = otables.lobatto(m)
+    f_q, yt_o, fa_o, fb_o, rbc_o = osolver.compute_f_q(prob, m, N, rk, t, y, yd, zt, p, 0.1)
+    J, V, D, W, B0, Bn, Vn = osolver.construct_jacobian(prob, m, N, rk, t, y, yt_o, zt, p, 0.1)
+    A_o, C_o, H_o, b_o = osolver.reduce_jacobian(prob, m, N, W, D, J, V, fa_o, fb_o)
+    assert rel_err(A, A_o) < 1e-11
+    assert rel_err(b, b_o) < 1e-11
+    assert np.all(H == 0.0)
