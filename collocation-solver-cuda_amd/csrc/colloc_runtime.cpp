@@ -42,6 +42,18 @@ struct Seg4 {
   long long len[4];
 };
 
+// Kernel-side descriptors (layouts match struct Batch / CRSeg of the kernels).
+struct Batch {
+  const int* iinst;
+  const int* ninst;
+  const int* noff;
+  const int* act;
+  int B;
+};
+struct CRSeg {
+  int in_off, out_off, rec_off, nrows;
+};
+
 enum Fn {
   F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_COND, F_COND_NS, F_CR_PAIR, F_CR_FINAL, F_CR_BACK, F_RECOVER,
   F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
@@ -55,17 +67,21 @@ constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation 
 }  // namespace
 
 struct cs_handle {
-  int ny, nz, np, m, n_max, nc;
-  int N = 0;
+  int ny, nz, np, m, n_max, nc, bmax;
+  int N = 0;        // nodes of the current mesh, over all instances
+  int B = 0;        // instances of the current mesh
+  int nint = 0;     // intervals of the current mesh (N - B)
   int init_N = 0;   // mesh size of the saved initial input (y0buf)
+  std::vector<int> hnoff;   // node offsets of the instances (B + 1)
   // launch shapes read from the module
-  int cw = 0, fw = 0;
+  int cw = 0, fw = 0, npa = -1;
   hipStream_t stream = nullptr;
   hipModule_t module = nullptr;
   hipFunction_t fn[F_COUNT] = {};
   // sizes derived from dims
-  int nj, nv, ns, recn, crw, recw, jstage;
-  size_t st_y, st_yd, st_z, st_len;   // packed state offsets: [y | ydot | ztil | p]
+  int nj, ns, crw, recw, jstage, drsz, nrb;
+  size_t recn = 0;                    // doubles per interval of the condensation record
+  size_t st_y, st_yd, st_z, st_p, st_len;   // packed state offsets: [y | ydot | ztil | p]
   // device buffers
   double* t = nullptr;
   double* state = nullptr;   // current state
@@ -85,14 +101,30 @@ struct cs_handle {
   double* segres = nullptr;
   double* zbuf = nullptr;
   double* y0buf = nullptr;
+  double* steps = nullptr;   // per-instance step of the state update
+  int* iinst = nullptr;      // instance of each interval
+  int* ninst = nullptr;      // instance of each node
+  int* noff = nullptr;       // node offsets (B + 1)
+  int* act = nullptr;        // instances still iterating
+  int* srch = nullptr;       // instances still line-searching
   int* fbk = nullptr;        // per interval: 1 if the series condensation left it to Gauss-Jordan
+  // cyclic-reduction schedule of the current mesh: per level, the instance
+  // of each output row and each instance's CRSeg
+  int* cr_oinst = nullptr;
+  size_t cr_oinst_cap = 0;
+  CRSeg* cr_seg = nullptr;
+  size_t cr_seg_cap = 0;
+  std::vector<int> lev_nout;          // output rows per level
+  std::vector<size_t> lev_obase;      // offset of the level in cr_oinst
+  std::vector<size_t> lev_sbase;      // offset of the level in cr_seg
   unsigned long long cond_fallback = 0;   // intervals condensed by Gauss-Jordan since creation
   unsigned long long cond_terms = 0;      // series terms summed over intervals since creation
-  unsigned long long* scal = nullptr;  // [0]: norm, [1]: max residual
+  // [0, bmax): residual norms; [bmax, 2 bmax): segment-residual maxima;
+  // [2 bmax, 2 bmax + 2): condensation counters
+  unsigned long long* scal = nullptr;
   int* status = nullptr;
   // profiling
   bool prof = false;
-  hipEvent_t ev0[CS_KERNEL_COUNT][2] = {};
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[CS_KERNEL_COUNT];
   double prof_ms[CS_KERNEL_COUNT] = {};
   int prof_n[CS_KERNEL_COUNT] = {};
@@ -101,7 +133,8 @@ struct cs_handle {
   double* y() const { return state + st_y; }
   double* yd() const { return state + st_yd; }
   double* zt() const { return state + st_z; }
-  double* p() const { return state + st_len - np; }
+  double* p() const { return state + st_p; }
+  Batch batch(const int* mask) const { return Batch{iinst, ninst, noff, mask, B}; }
 };
 
 namespace {
@@ -169,51 +202,69 @@ int dmalloc(double** p, size_t n) {
   return CS_OK;
 }
 
-int fill_scalars(cs_handle* h) {
-  unsigned long long zero = 0ull;
-  int n = 2;
-  void* args[] = {&h->scal, &zero, &n};
-  return launch(h, F_FILL, 1, 64, args);
+int imalloc(int** p, size_t n) {
+  HIPCHK(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(int)));
+  return CS_OK;
 }
 
-int read_u64_double(cs_handle* h, int slot, double* out) {
-  unsigned long long v = 0;
-  HIPCHK(hipMemcpyAsync(&v, h->scal + slot, sizeof v, hipMemcpyDeviceToHost, h->stream));
+int fill_u64(cs_handle* h, unsigned long long* p, int n) {
+  unsigned long long zero = 0ull;
+  void* args[] = {&p, &zero, &n};
+  return launch(h, F_FILL, (unsigned)((n + 255) / 256), 256, args);
+}
+
+// B per-instance maxima (u64 slots holding non-negative doubles) to the host.
+int read_slots(cs_handle* h, const unsigned long long* slots, double* out) {
+  std::vector<unsigned long long> v(h->B);
+  HIPCHK(hipMemcpyAsync(v.data(), slots, h->B * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
-  std::memcpy(out, &v, sizeof v);
+  std::memcpy(out, v.data(), h->B * sizeof(double));
   return CS_OK;
 }
 
 int need_mesh(cs_handle* h) {
-  if (h->N < 2) return fail(CS_E_ARG, "mesh not set");
+  if (h->N < 2 || h->B < 1) return fail(CS_E_ARG, "mesh not set");
   return CS_OK;
 }
 
-// Residual of a packed state; writes f_a/f_b/y_tilde/r_bc when full.
-int residual(cs_handle* h, const double* st, double alpha, bool full, double* norm) {
+int upload_ints(cs_handle* h, int* dev, const std::vector<int>& v) {
+  HIPCHK(hipMemcpyAsync(dev, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  return CS_OK;
+}
+
+int upload_doubles(cs_handle* h, double* dev, const std::vector<double>& v) {
+  HIPCHK(hipMemcpyAsync(dev, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  return CS_OK;
+}
+
+// Residual of a packed state for the instances in mask (null: all); writes
+// f_a/f_b/y_tilde/r_bc when full; per-instance norms to norms[B].
+int residual(cs_handle* h, const double* st, double alpha, bool full, const int* mask, double* norms) {
   TRY(need_mesh(h));
   ProfScope ps(h, CS_KERNEL_RESIDUAL);
-  TRY(fill_scalars(h));
-  int nint = h->N - 1;
+  TRY(fill_u64(h, h->scal, h->B));
+  int nint = h->nint;
   const double* y = st + h->st_y;
   const double* yd = st + h->st_yd;
   const double* zt = st + h->st_z;
-  const double* p = st + h->st_len - h->np;
+  const double* p = st + h->st_p;
   unsigned long long* slot = h->scal;
-  void* args[] = {&nint, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &h->ytil, &h->fa, &h->fb, &h->rbc, &slot};
-  unsigned nthreads = (unsigned)(nint * h->m + 1);
+  Batch bt = h->batch(mask);
+  void* args[] = {&nint, &bt, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &h->ytil, &h->fa, &h->fb, &h->rbc, &slot};
+  unsigned nthreads = (unsigned)(nint * h->m + h->B);
   TRY(launch(h, full ? F_RES_FULL : F_RES_NORM, (nthreads + 255) / 256, 256, args));
-  return read_u64_double(h, 0, norm);
+  return read_slots(h, h->scal, norms);
 }
 
-int jacobian(cs_handle* h, double alpha) {
+int jacobian(cs_handle* h, double alpha, const int* mask) {
   ProfScope ps(h, CS_KERNEL_JACOBIAN);
-  int nint = h->N - 1;
+  int nint = h->nint;
   double* y = h->y();
   double* p = h->p();
   double* zt = h->zt();
-  void* args[] = {&nint, &y, &h->ytil, &zt, &p, &h->c, &alpha, &h->jac, &h->Dr};
-  unsigned nthreads = (unsigned)(nint * h->m + 1);
+  Batch bt = h->batch(mask);
+  void* args[] = {&nint, &bt, &y, &h->ytil, &zt, &p, &h->c, &alpha, &h->jac, &h->Dr};
+  unsigned nthreads = (unsigned)(nint * h->m + h->B);
   return launch(h, F_JAC, (nthreads + 255) / 256, 256, args);
 }
 
@@ -221,17 +272,13 @@ int jacobian(cs_handle* h, double alpha) {
 // converges as a Neumann series (fine meshes) and counts the others; when
 // there are any, the Gauss-Jordan kernel redoes them (a block per interval,
 // the converged ones exit at once).
-int condense(cs_handle* h) {
+int condense(cs_handle* h, const int* mask) {
   ProfScope ps(h, CS_KERNEL_CONDENSE);
-  int nint = h->N - 1;
-  unsigned long long* counters = h->scal + 2;   // [intervals left to Gauss-Jordan, series terms]
-  {
-    unsigned long long zero = 0ull;
-    int n = 2;
-    void* args[] = {&counters, &zero, &n};
-    TRY(launch(h, F_FILL, 1, 64, args));
-  }
-  void* args_ns[] = {&nint, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &h->fbk, &counters};
+  int nint = h->nint;
+  unsigned long long* counters = h->scal + 2 * h->bmax;   // [intervals left to Gauss-Jordan, series terms]
+  TRY(fill_u64(h, counters, 2));
+  Batch bt = h->batch(mask);
+  void* args_ns[] = {&nint, &bt, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &h->fbk, &counters};
   TRY(launch(h, F_COND_NS, (unsigned)nint, kNsThreads, args_ns));
   unsigned long long cnt[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
@@ -241,72 +288,123 @@ int condense(cs_handle* h) {
   const unsigned long long nleft = cnt[0];
   if (nleft == 0) return CS_OK;
   const int* only = h->fbk;
-  void* args[] = {&nint, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &only};
+  void* args[] = {&nint, &h->iinst, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &only};
   return launch(h, F_COND, (unsigned)nint, (unsigned)(h->cw * 64), args);
 }
 
-int babd(cs_handle* h) {
+// Cyclic-reduction schedule of the current mesh (host side, once per mesh):
+// at each level every instance pairs its rows, carrying an odd last one,
+// until each has a single row; the level's output rows are numbered
+// instance by instance, so after the last level instance b's row is row b.
+int build_cr_schedule(cs_handle* h) {
+  const int B = h->B;
+  std::vector<int> n(B), in_off(B);
+  for (int b = 0; b < B; ++b) {
+    n[b] = h->hnoff[b + 1] - h->hnoff[b] - 1;
+    in_off[b] = h->hnoff[b] - b;
+  }
+  std::vector<int> oinst;
+  std::vector<CRSeg> segs;
+  h->lev_nout.clear();
+  h->lev_obase.clear();
+  h->lev_sbase.clear();
+  size_t rec_base = 0;
+  for (;;) {
+    bool more = false;
+    for (int b = 0; b < B; ++b) more |= n[b] > 1;
+    if (!more) break;
+    h->lev_obase.push_back(oinst.size());
+    h->lev_sbase.push_back(segs.size());
+    int out = 0;
+    for (int b = 0; b < B; ++b) {
+      const int nout = n[b] / 2 + (n[b] & 1);
+      segs.push_back(CRSeg{in_off[b], out, (int)rec_base, n[b]});
+      rec_base += n[b] / 2;
+      for (int k = 0; k < nout; ++k) oinst.push_back(b);
+      in_off[b] = out;
+      out += nout;
+      n[b] = nout;
+    }
+    h->lev_nout.push_back(out);
+  }
+  if (oinst.size() > h->cr_oinst_cap) {
+    if (h->cr_oinst) hipFree(h->cr_oinst);
+    h->cr_oinst = nullptr;
+    h->cr_oinst_cap = 0;
+    TRY(imalloc(&h->cr_oinst, oinst.size()));
+    h->cr_oinst_cap = oinst.size();
+  }
+  if (segs.size() > h->cr_seg_cap) {
+    if (h->cr_seg) hipFree(h->cr_seg);
+    h->cr_seg = nullptr;
+    h->cr_seg_cap = 0;
+    HIPCHK(hipMalloc((void**)&h->cr_seg, segs.size() * sizeof(CRSeg)));
+    h->cr_seg_cap = segs.size();
+  }
+  if (!oinst.empty()) TRY(upload_ints(h, h->cr_oinst, oinst));
+  if (!segs.empty())
+    HIPCHK(hipMemcpyAsync(h->cr_seg, segs.data(), segs.size() * sizeof(CRSeg), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+int babd(cs_handle* h, const int* mask) {
   ProfScope ps(h, CS_KERNEL_BABD);
-  int nint = h->N - 1;
   double* dy = h->dir + h->st_y;
-  double* dp = h->dir + h->st_len - h->np;
-  // forward: cyclic reduction levels
-  std::vector<int> lev_rows;
-  std::vector<size_t> lev_base;
-  int nrows = nint;
-  size_t base = 0;
+  double* dp = h->dir + h->st_p;
+  Batch bt = h->batch(mask);
   double* rin = h->rows0;
   double* rout = h->rows1;
   unsigned pair_block = (unsigned)((h->recw + 15) / 16 * 64);   // 4 lanes per record column
-  while (nrows > 1) {
-    int npairs = nrows / 2;
-    double* recs = h->crrec + base * (size_t)h->ny * h->recw;
-    void* args[] = {&nrows, &rin, &rout, &recs};
-    TRY(launch(h, F_CR_PAIR, (unsigned)(npairs + (nrows & 1)), pair_block, args));
-    lev_rows.push_back(nrows);
-    lev_base.push_back(base);
-    base += npairs;
-    nrows = npairs + (nrows & 1);
+  const int nlev = (int)h->lev_nout.size();
+  for (int l = 0; l < nlev; ++l) {
+    const int* oinst = h->cr_oinst + h->lev_obase[l];
+    const CRSeg* seg = h->cr_seg + h->lev_sbase[l];
+    const int* act = mask;
+    void* args[] = {&oinst, &seg, &act, &rin, &rout, &h->crrec};
+    TRY(launch(h, F_CR_PAIR, (unsigned)h->lev_nout[l], pair_block, args));
     std::swap(rin, rout);
   }
   {
-    void* args[] = {&nint, &rin, &h->Dr, &h->rbc, &dy, &dp, &h->status};
-    TRY(launch(h, F_CR_FINAL, 1, (unsigned)(h->fw * 64), args));
+    void* args[] = {&bt, &rin, &h->Dr, &h->rbc, &dy, &dp, &h->status};
+    TRY(launch(h, F_CR_FINAL, (unsigned)h->B, (unsigned)(h->fw * 64), args));
   }
-  for (int l = (int)lev_rows.size() - 1; l >= 0; --l) {
-    int nr = lev_rows[l];
-    double* recs = h->crrec + lev_base[l] * (size_t)h->ny * h->recw;
-    void* args[] = {&l, &nr, &nint, &recs, &dy, &dp};
-    TRY(launch(h, F_CR_BACK, (unsigned)(nr / 2), 64, args));
+  for (int l = nlev - 1; l >= 0; --l) {
+    const int* oinst = h->cr_oinst + h->lev_obase[l];
+    const CRSeg* seg = h->cr_seg + h->lev_sbase[l];
+    void* args[] = {&l, &bt, &oinst, &seg, &h->crrec, &dy, &dp};
+    TRY(launch(h, F_CR_BACK, (unsigned)h->lev_nout[l], 64, args));
   }
   return CS_OK;
 }
 
-int recover(cs_handle* h) {
+int recover(cs_handle* h, const int* mask) {
   ProfScope ps(h, CS_KERNEL_RECOVER);
-  int nint = h->N - 1;
+  int nint = h->nint;
   double* dy = h->dir + h->st_y;
-  double* dp = h->dir + h->st_len - h->np;
+  double* dp = h->dir + h->st_p;
   double* dyd = h->dir + h->st_yd;
   double* dz = h->dir + h->st_z;
-  void* args[] = {&nint, &h->rec, &dy, &dp, &dyd, &dz};
+  Batch bt = h->batch(mask);
+  void* args[] = {&nint, &bt, &h->rec, &dy, &dp, &dyd, &dz};
   return launch(h, F_RECOVER, (unsigned)nint, 256, args);
 }
 
 Seg4 state_segs(cs_handle* h) {
   Seg4 s;
-  int nint = h->N - 1;
   s.off[0] = (long long)h->st_y;  s.len[0] = (long long)h->N * h->ny;
-  s.off[1] = (long long)h->st_yd; s.len[1] = (long long)nint * h->m * h->ny;
-  s.off[2] = (long long)h->st_z;  s.len[2] = (long long)nint * h->m * h->nz;
-  s.off[3] = (long long)(h->st_len - h->np); s.len[3] = h->np;
+  s.off[1] = (long long)h->st_yd; s.len[1] = (long long)h->nint * h->m * h->ny;
+  s.off[2] = (long long)h->st_z;  s.len[2] = (long long)h->nint * h->m * h->nz;
+  s.off[3] = (long long)h->st_p;  s.len[3] = (long long)h->B * h->np;
   return s;
 }
 
-int axpy_state(cs_handle* h, const double* x, const double* d, double a, double* out) {
+// out = x + steps[b] d per instance b (steps on the device).
+int axpy_state(cs_handle* h, const double* x, const double* d, double* out) {
   Seg4 s = state_segs(h);
   long long total = s.len[0] + s.len[1] + s.len[2] + s.len[3];
-  void* args[] = {&s, &x, &d, &a, &out};
+  Batch bt = h->batch(nullptr);
+  void* args[] = {&s, &bt, &x, &d, &h->steps, &out};
   return launch(h, F_AXPY, (unsigned)((total + 255) / 256), 256, args);
 }
 
@@ -328,6 +426,7 @@ int sync(cs_handle* h) {
 }
 
 int init_from_saved(cs_handle* h);
+int set_batch_mesh(cs_handle* h, int B, const int* nodes, const double* t);
 
 }  // namespace
 
@@ -374,53 +473,63 @@ int cs_jit_compile(const char* source, const char* name, const char* include_dir
 
 int cs_create(const cs_dims* d, cs_handle** out) {
   if (!d || !out) return fail(CS_E_ARG, "null argument");
-  if (d->ny < 1 || d->nz < 0 || d->np < 0 || d->m < 2 || d->n_max < 2 || d->nc < 0)
-    return fail(CS_E_SHAPE, "invalid dims ny=%d nz=%d np=%d m=%d n_max=%d", d->ny, d->nz, d->np, d->m, d->n_max);
+  const int bmax = d->batch > 0 ? d->batch : 1;
+  if (d->ny < 1 || d->nz < 0 || d->np < 0 || d->m < 2 || d->n_max < 2 || d->nc < 0 || d->batch < 0 ||
+      d->n_max < 2 * bmax)
+    return fail(CS_E_SHAPE, "invalid dims ny=%d nz=%d np=%d m=%d n_max=%d nc=%d batch=%d", d->ny, d->nz, d->np,
+                d->m, d->n_max, d->nc, d->batch);
   // the cyclic-reduction kernels hold one BABD row per lane of a wave
   if (d->ny > 64)
     return fail(CS_E_UNSUPPORTED, "ny=%d: the register-resident kernels support ny <= 64", d->ny);
   cs_handle* h = new cs_handle();
-  h->ny = d->ny; h->nz = d->nz; h->np = d->np; h->m = d->m; h->n_max = d->n_max; h->nc = d->nc;
+  h->ny = d->ny; h->nz = d->nz; h->np = d->np; h->m = d->m; h->n_max = d->n_max; h->nc = d->nc; h->bmax = bmax;
   const int ny = h->ny, nz = h->nz, np = h->np, m = h->m;
   h->nj = ny + nz + np;
-  h->nv = ny + np + 1;
   h->ns = (m - 1) * ny;
-  h->recn = (h->ns + m * ny + m * nz) * h->nv;
   h->crw = 2 * ny + np + 1;
   h->recw = 3 * ny + np + 1;
   h->jstage = (ny + nz) * h->nj;
-  const size_t nmax = (size_t)h->n_max, nint = nmax - 1;
+  h->drsz = (ny + np) * (2 * ny + np);
+  h->nrb = ny + np;
+  const size_t nmax = (size_t)h->n_max, nint = nmax - 1, bm = (size_t)bmax;
   h->st_y = 0;
   h->st_yd = nmax * ny;
   h->st_z = h->st_yd + nint * m * ny;
-  h->st_len = h->st_z + nint * m * nz + np;
+  h->st_p = h->st_z + nint * m * nz;
+  h->st_len = h->st_p + bm * np;
   int r = CS_OK;
   auto A = [&](double** p, size_t n) { if (r == CS_OK) r = dmalloc(p, n); };
+  auto I = [&](int** p, size_t n) { if (r == CS_OK) r = imalloc(p, n); };
   A(&h->t, nmax);
   A(&h->state, h->st_len);
   A(&h->trial, h->st_len);
   A(&h->dir, h->st_len);
-  A(&h->c, (size_t)std::max(h->nc, 1));
+  A(&h->c, bm * std::max(h->nc, 1));
   A(&h->ytil, nint * m * ny);
   A(&h->fa, nint * m * (ny + nz));
   A(&h->fb, nint * ny);
-  A(&h->rbc, (size_t)(ny + np));
+  A(&h->rbc, bm * h->nrb);
   A(&h->jac, nint * m * (size_t)h->jstage);
-  A(&h->Dr, (size_t)(ny + np) * (2 * ny + np));
+  A(&h->Dr, bm * h->drsz);
   A(&h->rows0, nint * ny * (size_t)h->crw);
-  A(&h->rows1, (nint / 2 + 1) * ny * (size_t)h->crw);
-  A(&h->rec, nint * (size_t)h->recn);
+  A(&h->rows1, (nint / 2 + bm) * ny * (size_t)h->crw);   // a level's output: <= nint/2 + B rows
   A(&h->crrec, nint * ny * (size_t)h->recw);
   A(&h->segres, nmax);
   A(&h->zbuf, nmax * std::max(nz, 1));
-  A(&h->y0buf, nmax * (ny + std::max(nz, 1)) + np);
+  A(&h->y0buf, nmax * (ny + std::max(nz, 1)) + bm * np);
+  A(&h->steps, bm);
+  I(&h->iinst, nint);
+  I(&h->ninst, nmax);
+  I(&h->noff, bm + 1);
+  I(&h->act, bm);
+  I(&h->srch, bm);
+  I(&h->fbk, nint);
   if (r == CS_OK) {
-    hipError_t e = hipMalloc((void**)&h->fbk, std::max<size_t>(nint, 1) * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc((void**)&h->scal, 4 * sizeof(unsigned long long));
+    hipError_t e = hipMalloc((void**)&h->scal, (2 * bm + 2) * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&h->status, sizeof(int));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(h->status, 0, sizeof(int));
-    if (e == hipSuccess) e = hipMemset(h->c, 0, sizeof(double) * std::max(h->nc, 1));
+    if (e == hipSuccess) e = hipMemset(h->c, 0, sizeof(double) * bm * std::max(h->nc, 1));
     if (e != hipSuccess) r = fail(CS_E_HIP, "allocation failed: %s", hipGetErrorString(e));
   }
   if (r != CS_OK) {
@@ -437,10 +546,13 @@ void cs_destroy(cs_handle* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
   double* bufs[] = {h->t, h->state, h->trial, h->dir, h->c, h->ytil, h->fa, h->fb, h->rbc, h->jac, h->Dr,
-                    h->rows0, h->rows1, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf};
+                    h->rows0, h->rows1, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps};
   for (double* b : bufs)
     if (b) hipFree(b);
-  if (h->fbk) hipFree(h->fbk);
+  int* ibufs[] = {h->iinst, h->ninst, h->noff, h->act, h->srch, h->fbk, h->cr_oinst};
+  for (int* b : ibufs)
+    if (b) hipFree(b);
+  if (h->cr_seg) hipFree(h->cr_seg);
   if (h->scal) hipFree(h->scal);
   if (h->status) hipFree(h->status);
   for (int k = 0; k < CS_KERNEL_COUNT; ++k)
@@ -473,43 +585,69 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
                 meta[2], meta[3], meta[4], h->ny, h->nz, h->np, h->m, h->nc);
   h->cw = meta[5];
   h->fw = meta[6];
+  // the condensation record keeps the columns [dy_i | active dp | 1]
+  const int npa = meta[7];
+  if (npa < 0 || npa > h->np) return fail(CS_E_SHAPE, "module has %d active parameters of %d", npa, h->np);
+  const size_t recn = (size_t)(h->ns + h->m * h->ny + h->m * h->nz) * (h->ny + npa + 1);
+  if (recn != h->recn) {
+    if (h->rec) hipFree(h->rec);
+    h->rec = nullptr;
+    h->recn = 0;
+    TRY(dmalloc(&h->rec, (size_t)(h->n_max - 1) * recn));
+    h->recn = recn;
+  }
+  h->npa = npa;
   return CS_OK;
 }
 
 int cs_set_mesh(cs_handle* h, const double* t, int N) {
   if (!h || !t) return fail(CS_E_ARG, "null argument");
-  if (N < 2 || N > h->n_max) return fail(CS_E_SHAPE, "N=%d outside [2, %d]", N, h->n_max);
-  h->N = N;
-  TRY(copy_in(h, h->t, t, (size_t)N));
-  return sync(h);
+  return set_batch_mesh(h, 1, &N, t);
+}
+
+int cs_set_batch_mesh(cs_handle* h, int B, const int* nodes, const double* t) {
+  if (!h || !t || !nodes) return fail(CS_E_ARG, "null argument");
+  return set_batch_mesh(h, B, nodes, t);
 }
 
 int cs_set_constants(cs_handle* h, const double* c, int nc) {
   if (!h) return fail(CS_E_ARG, "null handle");
   if (nc != h->nc) return fail(CS_E_SHAPE, "nc=%d, handle has %d", nc, h->nc);
-  TRY(copy_in(h, h->c, c, (size_t)nc));
+  if (nc == 0) return CS_OK;
+  if (!c) return fail(CS_E_ARG, "null argument");
+  for (int b = 0; b < h->bmax; ++b) TRY(copy_in(h, h->c + (size_t)b * nc, c, (size_t)nc));
+  return sync(h);
+}
+
+int cs_set_batch_constants(cs_handle* h, const double* c, int B, int nc) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (nc != h->nc) return fail(CS_E_SHAPE, "nc=%d, handle has %d", nc, h->nc);
+  if (B < 1 || B > h->bmax) return fail(CS_E_SHAPE, "B=%d outside [1, %d]", B, h->bmax);
+  if (nc == 0) return CS_OK;
+  if (!c) return fail(CS_E_ARG, "null argument");
+  TRY(copy_in(h, h->c, c, (size_t)B * nc));
   return sync(h);
 }
 
 int cs_set_state(cs_handle* h, const double* y, const double* y_dot, const double* z_tilde, const double* p) {
   if (!h) return fail(CS_E_ARG, "null handle");
   TRY(need_mesh(h));
-  const size_t nint = h->N - 1;
+  const size_t nint = h->nint;
   TRY(copy_in(h, h->y(), y, (size_t)h->N * h->ny));
   TRY(copy_in(h, h->yd(), y_dot, nint * h->m * h->ny));
   TRY(copy_in(h, h->zt(), z_tilde, nint * h->m * h->nz));
-  TRY(copy_in(h, h->p(), p, (size_t)h->np));
+  TRY(copy_in(h, h->p(), p, (size_t)h->B * h->np));
   return sync(h);
 }
 
 int cs_get_state(cs_handle* h, double* y, double* y_dot, double* z_tilde, double* p) {
   if (!h) return fail(CS_E_ARG, "null handle");
   TRY(need_mesh(h));
-  const size_t nint = h->N - 1;
+  const size_t nint = h->nint;
   TRY(copy_out(h, y, h->y(), (size_t)h->N * h->ny));
   TRY(copy_out(h, y_dot, h->yd(), nint * h->m * h->ny));
   TRY(copy_out(h, z_tilde, h->zt(), nint * h->m * h->nz));
-  TRY(copy_out(h, p, h->p(), (size_t)h->np));
+  TRY(copy_out(h, p, h->p(), (size_t)h->B * h->np));
   return sync(h);
 }
 
@@ -519,7 +657,7 @@ int cs_form_initial_input(cs_handle* h, const double* y0, const double* z0, cons
   const size_t N = h->N;
   TRY(copy_in(h, h->y0buf, y0, N * h->ny));
   TRY(copy_in(h, h->y0buf + N * h->ny, z0, N * h->nz));
-  TRY(copy_in(h, h->y0buf + N * (h->ny + h->nz), p0, (size_t)h->np));
+  TRY(copy_in(h, h->y0buf + N * (h->ny + h->nz), p0, (size_t)h->B * h->np));
   h->init_N = h->N;
   return init_from_saved(h);
 }
@@ -533,20 +671,122 @@ int cs_restart(cs_handle* h) {
 }  // extern "C"
 
 namespace {
+int set_batch_mesh(cs_handle* h, int B, const int* nodes, const double* t) {
+  if (B < 1 || B > h->bmax) return fail(CS_E_SHAPE, "B=%d outside [1, %d]", B, h->bmax);
+  std::vector<int> noff(B + 1, 0);
+  long long total = 0;
+  for (int b = 0; b < B; ++b) {
+    if (nodes[b] < 2) return fail(CS_E_SHAPE, "instance %d has %d nodes (at least 2)", b, nodes[b]);
+    total += nodes[b];
+    if (total > h->n_max) return fail(CS_E_SHAPE, "N=%lld nodes in total, handle holds %d", total, h->n_max);
+    noff[b + 1] = (int)total;
+  }
+  const int N = (int)total, nint = N - B;
+  std::vector<int> iinst(nint), ninst(N);
+  for (int b = 0; b < B; ++b) {
+    for (int n = noff[b]; n < noff[b + 1]; ++n) ninst[n] = b;
+    for (int i = noff[b] - b; i < noff[b + 1] - b - 1; ++i) iinst[i] = b;
+  }
+  h->N = N;
+  h->B = B;
+  h->nint = nint;
+  h->hnoff = noff;
+  h->init_N = 0;
+  TRY(copy_in(h, h->t, t, (size_t)N));
+  TRY(upload_ints(h, h->noff, noff));
+  TRY(upload_ints(h, h->iinst, iinst));
+  TRY(upload_ints(h, h->ninst, ninst));
+  TRY(sync(h));
+  return build_cr_schedule(h);
+}
+
 int init_from_saved(cs_handle* h) {
   const size_t N = h->N;
   double* zb = h->y0buf + N * h->ny;
   HIPCHK(hipMemcpyAsync(h->y(), h->y0buf, N * h->ny * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
   if (h->np)
-    HIPCHK(hipMemcpyAsync(h->p(), h->y0buf + N * (h->ny + h->nz), h->np * sizeof(double),
+    HIPCHK(hipMemcpyAsync(h->p(), h->y0buf + N * (h->ny + h->nz), (size_t)h->B * h->np * sizeof(double),
                           hipMemcpyDeviceToDevice, h->stream));
-  int nint = h->N - 1;
+  int nint = h->nint;
   double* y = h->y();
   double* p = h->p();
   double* yd = h->yd();
   double* zt = h->zt();
-  void* args[] = {&nint, &y, &zb, &p, &h->c, &yd, &zt};
+  Batch bt = h->batch(nullptr);
+  void* args[] = {&nint, &bt, &y, &zb, &p, &h->c, &yd, &zt};
   TRY(launch(h, F_INIT, (unsigned)((nint * h->m + 255) / 256), 256, args));
+  return sync(h);
+}
+
+// The damped Newton loop of every instance of the batch (solver:109-174):
+// each instance stops when its own ||f_q||_inf < tol, and its line search
+// takes the first step 2^-k (k < 20) that strictly decreases its own norm.
+// The kernels of an iteration run over the instances still iterating (act),
+// the line-search residuals over those still searching (srch); a step is
+// accepted by adding it to the state of its instance only.
+int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time, double* norm, double* hist,
+           int hist_len) {
+  const int B = h->B;
+  std::vector<double> nrm(B, 0.0), nn(B, 0.0), step(B, 0.0), taken(B, 0.0);
+  std::vector<int> active(B, 1), search(B, 0), done(B, -1);
+  for (int it = 0; it < max_iter; ++it) {
+    TRY(upload_ints(h, h->act, active));
+    std::vector<double> cur(B, 0.0);
+    TRY(residual(h, h->state, alpha, true, h->act, cur.data()));
+    int nact = 0;
+    for (int b = 0; b < B; ++b) {
+      if (!active[b]) continue;
+      nrm[b] = cur[b];
+      if (hist && it < hist_len) hist[(size_t)b * hist_len + it] = cur[b];
+      if (cur[b] < tol) {
+        active[b] = 0;
+        done[b] = it;
+      } else {
+        ++nact;
+      }
+    }
+    if (nact == 0) break;
+    TRY(upload_ints(h, h->act, active));
+    TRY(jacobian(h, alpha, h->act));
+    TRY(condense(h, h->act));
+    TRY(babd(h, h->act));
+    TRY(recover(h, h->act));
+    for (int b = 0; b < B; ++b) {
+      search[b] = active[b];
+      step[b] = active[b] ? 1.0 : 0.0;
+      taken[b] = 0.0;
+    }
+    bool any_taken = false;
+    for (int ls = 0; ls < 20; ++ls) {
+      TRY(upload_ints(h, h->srch, search));
+      TRY(upload_doubles(h, h->steps, step));
+      TRY(axpy_state(h, h->state, h->dir, h->trial));
+      TRY(residual(h, h->trial, alpha, false, h->srch, nn.data()));
+      int left = 0;
+      for (int b = 0; b < B; ++b) {
+        if (!search[b]) continue;
+        if (nn[b] < nrm[b]) {
+          taken[b] = step[b];
+          any_taken = true;
+          search[b] = 0;
+          step[b] = 0.0;
+        } else {
+          step[b] *= 0.5;
+          ++left;
+        }
+      }
+      if (left == 0) break;
+    }
+    if (any_taken) {
+      TRY(upload_doubles(h, h->steps, taken));
+      TRY(axpy_state(h, h->state, h->dir, h->state));
+    }
+  }
+  for (int b = 0; b < B; ++b) {
+    // the reference's loop variable stops at max_iter - 1
+    if (iter_time) iter_time[b] = done[b] >= 0 ? done[b] : max_iter - 1;
+    if (norm) norm[b] = nrm[b];
+  }
   return sync(h);
 }
 }  // namespace
@@ -556,22 +796,23 @@ extern "C" {
 int cs_compute_f_q(cs_handle* h, double alpha, double* norm_inf, double* y_tilde, double* f_a, double* f_b,
                    double* r_bc) {
   if (!h) return fail(CS_E_ARG, "null handle");
-  double nrm = 0.0;
-  TRY(residual(h, h->state, alpha, true, &nrm));
-  if (norm_inf) *norm_inf = nrm;
-  const size_t nint = h->N - 1;
+  TRY(need_mesh(h));
+  std::vector<double> nrm(h->B);
+  TRY(residual(h, h->state, alpha, true, nullptr, nrm.data()));
+  if (norm_inf) std::memcpy(norm_inf, nrm.data(), nrm.size() * sizeof(double));
+  const size_t nint = h->nint;
   TRY(copy_out(h, y_tilde, h->ytil, nint * h->m * h->ny));
   TRY(copy_out(h, f_a, h->fa, nint * h->m * (h->ny + h->nz)));
   TRY(copy_out(h, f_b, h->fb, nint * h->ny));
-  TRY(copy_out(h, r_bc, h->rbc, (size_t)(h->ny + h->np)));
+  TRY(copy_out(h, r_bc, h->rbc, (size_t)h->B * h->nrb));
   return sync(h);
 }
 
 int cs_construct_jacobian(cs_handle* h, double alpha, double* Df, double* Dg, double* Dr) {
   if (!h) return fail(CS_E_ARG, "null handle");
   TRY(need_mesh(h));
-  TRY(jacobian(h, alpha));
-  const size_t nst = (size_t)(h->N - 1) * h->m;
+  TRY(jacobian(h, alpha, nullptr));
+  const size_t nst = (size_t)h->nint * h->m;
   const size_t nf = (size_t)h->ny * h->nj, ng = (size_t)h->nz * h->nj;
   if (Df || Dg) {
     std::vector<double> buf(nst * h->jstage);
@@ -582,16 +823,16 @@ int cs_construct_jacobian(cs_handle* h, double alpha, double* Df, double* Dg, do
       if (Dg && ng) std::memcpy(Dg + s * ng, buf.data() + s * h->jstage + nf, ng * sizeof(double));
     }
   }
-  TRY(copy_out(h, Dr, h->Dr, (size_t)(h->ny + h->np) * (2 * h->ny + h->np)));
+  TRY(copy_out(h, Dr, h->Dr, (size_t)h->B * h->drsz));
   return sync(h);
 }
 
 int cs_reduce_jacobian(cs_handle* h, double* A, double* H, double* b) {
   if (!h) return fail(CS_E_ARG, "null handle");
   TRY(need_mesh(h));
-  TRY(condense(h));
+  TRY(condense(h, nullptr));
   if (A || H || b) {
-    const size_t nint = h->N - 1;
+    const size_t nint = h->nint;
     std::vector<double> buf(nint * h->ny * h->crw);
     TRY(copy_out(h, buf.data(), h->rows0, buf.size()));
     TRY(sync(h));
@@ -609,17 +850,17 @@ int cs_reduce_jacobian(cs_handle* h, double* A, double* H, double* b) {
 int cs_solve_babd(cs_handle* h, double* delta_y, double* delta_p) {
   if (!h) return fail(CS_E_ARG, "null handle");
   TRY(need_mesh(h));
-  TRY(babd(h));
+  TRY(babd(h, nullptr));
   TRY(copy_out(h, delta_y, h->dir + h->st_y, (size_t)h->N * h->ny));
-  TRY(copy_out(h, delta_p, h->dir + h->st_len - h->np, (size_t)h->np));
+  TRY(copy_out(h, delta_p, h->dir + h->st_p, (size_t)h->B * h->np));
   return sync(h);
 }
 
 int cs_recover_delta_k(cs_handle* h, double* delta_y_dot, double* delta_z_tilde) {
   if (!h) return fail(CS_E_ARG, "null handle");
   TRY(need_mesh(h));
-  TRY(recover(h));
-  const size_t nint = h->N - 1;
+  TRY(recover(h, nullptr));
+  const size_t nint = h->nint;
   TRY(copy_out(h, delta_y_dot, h->dir + h->st_yd, nint * h->m * h->ny));
   TRY(copy_out(h, delta_z_tilde, h->dir + h->st_z, nint * h->m * h->nz));
   return sync(h);
@@ -630,32 +871,8 @@ int cs_newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_ti
   if (!h) return fail(CS_E_ARG, "null handle");
   TRY(need_mesh(h));
   if (max_iter < 1) return fail(CS_E_ARG, "max_iter must be >= 1");
-  int it = 0;
-  double nrm = 0.0;
-  for (it = 0; it < max_iter; ++it) {
-    TRY(residual(h, h->state, alpha, true, &nrm));
-    if (norms && it < hist_len) norms[it] = nrm;
-    if (nrm < tol) break;
-    TRY(jacobian(h, alpha));
-    TRY(condense(h));
-    TRY(babd(h));
-    TRY(recover(h));
-    double a0 = 1.0;
-    for (int ls = 0; ls < 20; ++ls) {
-      TRY(axpy_state(h, h->state, h->dir, a0, h->trial));
-      double nn = 0.0;
-      TRY(residual(h, h->trial, alpha, false, &nn));
-      if (nn < nrm) {
-        std::swap(h->state, h->trial);
-        break;
-      }
-      a0 *= 0.5;
-    }
-  }
-  if (it == max_iter) it = max_iter - 1;   // the reference's loop variable stops at max_iter - 1
-  if (iter_time) *iter_time = it;
-  if (norm) *norm = nrm;
-  return sync(h);
+  if (hist_len < 0) return fail(CS_E_ARG, "hist_len must be >= 0");
+  return newton(h, alpha, tol, max_iter, iter_time, norm, norms, hist_len);
 }
 
 int cs_segment_residual_ex(cs_handle* h, double alpha, double tol, int residual_type, int scale_by_time,
@@ -663,19 +880,21 @@ int cs_segment_residual_ex(cs_handle* h, double alpha, double tol, int residual_
   if (!h) return fail(CS_E_ARG, "null handle");
   if (residual_type < 0 || residual_type > 2) return fail(CS_E_ARG, "residual_type %d (0, 1 or 2)", residual_type);
   TRY(need_mesh(h));
-  TRY(fill_scalars(h));
-  int nint = h->N - 1;
+  unsigned long long* slot = h->scal + h->bmax;
+  TRY(fill_u64(h, slot, h->B));
+  int nint = h->nint;
   double* y = h->y();
   double* yd = h->yd();
   double* zt = h->zt();
   double* p = h->p();
-  unsigned long long* slot = h->scal + 1;
   int st = scale_by_time ? 1 : 0, rel = relative ? 1 : 0;
-  void* args[] = {&nint, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &tol, &residual_type, &st, &rel, &h->segres, &slot};
-  TRY(launch(h, F_SEGRES, (unsigned)((nint + 1 + 255) / 256), 256, args));
-  double mr = 0.0;
-  TRY(read_u64_double(h, 1, &mr));
-  if (max_residual) *max_residual = mr;
+  Batch bt = h->batch(nullptr);
+  void* args[] = {&nint, &bt, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &tol, &residual_type, &st, &rel,
+                  &h->segres, &slot};
+  TRY(launch(h, F_SEGRES, (unsigned)((nint + h->B + 255) / 256), 256, args));
+  std::vector<double> mr(h->B);
+  TRY(read_slots(h, slot, mr.data()));
+  if (max_residual) std::memcpy(max_residual, mr.data(), mr.size() * sizeof(double));
   TRY(copy_out(h, residual_out, h->segres, (size_t)h->N));
   return sync(h);
 }
@@ -689,10 +908,18 @@ int cs_recover_solution(cs_handle* h, double* z) {
   TRY(need_mesh(h));
   int n = h->N;
   double* zt = h->zt();
-  void* args[] = {&n, &zt, &h->zbuf};
+  Batch bt = h->batch(nullptr);
+  void* args[] = {&n, &bt, &zt, &h->zbuf};
   TRY(launch(h, F_RECZ, (unsigned)((n + 255) / 256), 256, args));
   TRY(copy_out(h, z, h->zbuf, (size_t)n * h->nz));
   return sync(h);
+}
+
+int cs_batch_info(cs_handle* h, int* B, int* N) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (B) *B = h->B;
+  if (N) *N = h->N;
+  return CS_OK;
 }
 
 int cs_profile(cs_handle* h, int enable) {

@@ -6,9 +6,10 @@
 //   __constant__ tables  c_rkA[M*M], c_rkb[M], c_rkc[M],
 //                        c_gL[(M+1)*M], c_gI[(M+1)*M], c_gw[M+1], c_L0[M], c_L1[M]
 //
-// Device memory layouts (row-major, float64):
-//   t[N]; y[N][NY]; ydot[Nint*M][NY]; ztil[Nint*M][NZ]; p[NP]
-//   ytil[Nint*M][NY]; fa[Nint][M*(NY+NZ)]; fb[Nint][NY]; rbc[NY+NP]
+// Device memory layouts (row-major, float64) for a batch of B instances laid
+// end to end (N nodes and Nint = N - B intervals in total, see Batch):
+//   t[N]; y[N][NY]; ydot[Nint*M][NY]; ztil[Nint*M][NZ]; p[B][NP]
+//   ytil[Nint*M][NY]; fa[Nint][M*(NY+NZ)]; fb[Nint][NY]; rbc[B][NY+NP]
 //   jac[Nint*M][JSTAGE]      stage block = Df (NY x NJC) then Dg (NZ x NJC)
 //   rows[Nint][NY][CRW]      BABD row  [G | C | H | b]  (G x_left + C x_right + H p = b)
 //   rec[Nint][REC]           condensation record [U (NS x NVA) | T (M NY x NVA) | G (M NZ x NVA)]
@@ -50,32 +51,83 @@ __host__ __device__ constexpr bool p_active(int j) {
 static_assert(CR_PARTS == 4, "cr_pair sums over the four 16-lane rows of a wave");
 static_assert(NY <= 64, "cyclic-reduction kernels hold one row per lane");
 
+constexpr int DRSZ = (NY + NP) * (2 * NY + NP);   // one instance's Dr
+constexpr int NRB = NY + NP;                       // one instance's r_bc
+
+// A batch of B >= 1 instances of the problem laid end to end.  Instance b
+// owns nodes [noff[b], noff[b+1]) of the node arrays (t, y, the residual of
+// segres, z) and the intervals [noff[b] - b, noff[b+1] - b - 1) of the
+// interval / stage arrays; iinst[i] is the instance of interval i (its left
+// node is i + iinst[i]) and ninst[n] the instance of node n.  p, c, r_bc and
+// Dr hold one block per instance.  act (may be null) marks the instances a
+// kernel works on; the others are skipped and their outputs left as they are.
+struct Batch {
+  const int* iinst;
+  const int* ninst;
+  const int* noff;
+  const int* act;
+  int B;
+};
+__device__ __forceinline__ bool skipped(const Batch& bt, int b) { return bt.act && !bt.act[b]; }
+
+// One CR level of one instance: its rows start at in_off of the input
+// buffer, its ceil(nrows / 2) output rows at out_off, its nrows / 2 records
+// at rec_off.
+struct CRSeg {
+  int in_off, out_off, rec_off, nrows;
+};
+
 // -------------------------------------------------------------------- fill
 __device__ __forceinline__ void fill_u64(unsigned long long* p, unsigned long long v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
+// max|v| per instance folded into slot[b] (u64 slots, as block_max_to).  b < 0:
+// the lane has nothing to add.  The lanes that share the instance of the
+// wave's first contributing lane reduce together; any other lane (a wave
+// spanning an instance boundary) adds its own value.  Every lane of the wave
+// must call this.
+__device__ __forceinline__ void seg_max_to(double v, int b, unsigned long long* __restrict__ slot) {
+  const unsigned long long m = __ballot(b >= 0);
+  if (m == 0) return;
+  const int first = __ffsll((long long)m) - 1;
+  const int b0 = __shfl(b, first);
+  double w = (b == b0) ? v : 0.0;
+  w = wave_max(w);
+  if (lane_id() == first) atomicMax(slot + b0, (unsigned long long)__double_as_longlong(fabs(w)));
+  if (b >= 0 && b != b0) atomicMax(slot + b, (unsigned long long)__double_as_longlong(fabs(v)));
+}
+
 // ---------------------------------------------------------------- residual
-// One thread per (interval, stage); thread Nint*M evaluates r.  Restates
-// collocation_update_kernel + compute_f_q_kernel1/2 (solver:478-559) and the
-// host max-norm of f_q (solver:116) as a block max folded into `norm`.
+// One thread per (interval, stage); thread Nint*M + b evaluates r of
+// instance b.  Restates collocation_update_kernel + compute_f_q_kernel1/2
+// (solver:478-559) and the host max-norm of f_q (solver:116), one norm per
+// instance folded into norm[b].
 template <bool FULL>
-__device__ __forceinline__ void residual(int nint, const double* __restrict__ t, const double* __restrict__ y,
-                         const double* __restrict__ ydot, const double* __restrict__ ztil,
-                         const double* __restrict__ p, const double* __restrict__ c, double alpha,
+__device__ __forceinline__ void residual(int nint, Batch bt, const double* __restrict__ t,
+                         const double* __restrict__ y, const double* __restrict__ ydot,
+                         const double* __restrict__ ztil, const double* __restrict__ p,
+                         const double* __restrict__ c, double alpha,
                          double* __restrict__ ytil, double* __restrict__ fa, double* __restrict__ fb,
                          double* __restrict__ rbc, unsigned long long* __restrict__ norm) {
-  __shared__ double sred[16];
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nst = nint * M;
   double amax = 0.0;
+  int b = -1;
   if (gid < nst) {
+    b = bt.iinst[gid / M];
+    if (skipped(bt, b)) b = -1;
+  }
+  if (b >= 0) {
     const int i = gid / M;
     const int j = gid - i * M;
-    const double h = t[i + 1] - t[i];
-    const double* yi = y + (size_t)i * NY;
+    const int nd = i + b;
+    const double h = t[nd + 1] - t[nd];
+    const double* yi = y + (size_t)nd * NY;
     const double* yd = ydot + (size_t)i * M * NY;
+    const double* pb = p + (size_t)b * NP;
+    const double* cb = c + (size_t)b * NC;
     double yt[NY], zt[NZ1], fo[NY], go[NZ1];
 #pragma unroll
     for (int l = 0; l < NY; ++l) {
@@ -86,8 +138,8 @@ __device__ __forceinline__ void residual(int nint, const double* __restrict__ t,
     }
 #pragma unroll
     for (int l = 0; l < NZ; ++l) zt[l] = ztil[(size_t)gid * NZ + l];
-    prob::f(yt, zt, p, c, fo);
-    prob::g(yt, zt, p, c, alpha, go);
+    prob::f(yt, zt, pb, cb, fo);
+    prob::g(yt, zt, pb, cb, alpha, go);
     double* fai = fa + (size_t)gid * NYZ;
 #pragma unroll
     for (int l = 0; l < NY; ++l) {
@@ -110,44 +162,53 @@ __device__ __forceinline__ void residual(int nint, const double* __restrict__ t,
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k < M; ++k) s += c_rkb[k] * yd[k * NY + l];
-        const double v = y[(size_t)(i + 1) * NY + l] - yi[l] - h * s;
+        const double v = y[(size_t)(nd + 1) * NY + l] - yi[l] - h * s;
         amax = nmax(fabs(v), amax);
         if (FULL) fb[(size_t)i * NY + l] = v;
       }
     }
-  } else if (gid == nst) {
+  } else if (gid >= nst && gid < nst + bt.B && !skipped(bt, gid - nst)) {
+    b = gid - nst;
     double ro[NY + NP];
-    prob::r(y, y + (size_t)nint * NY, p, c, ro);
+    prob::r(y + (size_t)bt.noff[b] * NY, y + (size_t)(bt.noff[b + 1] - 1) * NY, p + (size_t)b * NP,
+            c + (size_t)b * NC, ro);
 #pragma unroll
     for (int l = 0; l < NY + NP; ++l) {
       amax = nmax(fabs(ro[l]), amax);
-      if (FULL) rbc[l] = ro[l];
+      if (FULL) rbc[(size_t)b * NRB + l] = ro[l];
     }
   }
-  block_max_to(amax, norm, sred);
+  seg_max_to(amax, b, norm);
 }
 
 // ---------------------------------------------------------------- jacobian
-// One thread per stage writes Df, Dg densely; thread Nint*M writes Dr.
-// Restates construct_jacobian_kernel (solver:680-797) without materialising
-// J, V, D, W (the condensation kernel assembles what it needs).
-__device__ __forceinline__ void jacobian(int nint, const double* __restrict__ y, const double* __restrict__ ytil,
-                         const double* __restrict__ ztil, const double* __restrict__ p,
-                         const double* __restrict__ c, double alpha, double* __restrict__ jac,
-                         double* __restrict__ Dr) {
+// One thread per stage writes Df, Dg densely; thread Nint*M + b writes Dr of
+// instance b.  Restates construct_jacobian_kernel (solver:680-797) without
+// materialising J, V, D, W (the condensation kernel assembles what it needs).
+__device__ __forceinline__ void jacobian(int nint, Batch bt, const double* __restrict__ y,
+                         const double* __restrict__ ytil, const double* __restrict__ ztil,
+                         const double* __restrict__ p, const double* __restrict__ c, double alpha,
+                         double* __restrict__ jac, double* __restrict__ Dr) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nst = nint * M;
   if (gid < nst) {
+    const int b = bt.iinst[gid / M];
+    if (skipped(bt, b)) return;
     double yt[NY], zt[NZ1];
 #pragma unroll
     for (int l = 0; l < NY; ++l) yt[l] = ytil[(size_t)gid * NY + l];
 #pragma unroll
     for (int l = 0; l < NZ; ++l) zt[l] = ztil[(size_t)gid * NZ + l];
     double* out = jac + (size_t)gid * JSTAGE;
-    prob::Df(yt, zt, p, c, out);
-    prob::Dg(yt, zt, p, c, alpha, out + NY * NJC);
-  } else if (gid == nst) {
-    prob::Dr(y, y + (size_t)nint * NY, p, c, Dr);
+    const double* pb = p + (size_t)b * NP;
+    const double* cb = c + (size_t)b * NC;
+    prob::Df(yt, zt, pb, cb, out);
+    prob::Dg(yt, zt, pb, cb, alpha, out + NY * NJC);
+  } else if (gid < nst + bt.B) {
+    const int b = gid - nst;
+    if (skipped(bt, b)) return;
+    prob::Dr(y + (size_t)bt.noff[b] * NY, y + (size_t)(bt.noff[b + 1] - 1) * NY, p + (size_t)b * NP,
+             c + (size_t)b * NC, Dr + (size_t)b * DRSZ);
   }
 }
 
@@ -285,7 +346,8 @@ __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__
 //   parameter columns), b = -f_b + U_{M-1}[:, NVA-1].  Only the NPA parameters
 //   the stage functions depend on get columns (g_p = f_p = 0 for the rest).
 template <int CW>
-__device__ __forceinline__ void condense(int nint, const double* __restrict__ t, const double* __restrict__ jac,
+__device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst, const double* __restrict__ t,
+                         const double* __restrict__ jac,
                          const double* __restrict__ fa, const double* __restrict__ fb,
                          double* __restrict__ rows, double* __restrict__ rec, int* __restrict__ status,
                          const int* __restrict__ only) {
@@ -308,7 +370,8 @@ __device__ __forceinline__ void condense(int nint, const double* __restrict__ t,
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = wave_id();
   if (tid == 0) sbad = 0;
-  const double h = t[i + 1] - t[i];
+  const int nd = i + iinst[i];
+  const double h = t[nd + 1] - t[nd];
   double* recU = rec + (size_t)i * REC;
   double* recT = recU + NS * NVA;
   double* recG = recT + M * NY * NVA;
@@ -422,7 +485,8 @@ constexpr int NS_ITEMS = NSK * NY * NS_NH;
 constexpr bool NS_OK = NY <= 48 && NS_ITEMS <= NS_THREADS &&
                        (M * NY * NVA + NSK * NY * NVAP) * 8 <= 150 * 1024;
 
-__device__ __forceinline__ void condense_ns(int nint, const double* __restrict__ t, const double* __restrict__ jac,
+__device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __restrict__ t,
+                                            const double* __restrict__ jac,
                                             const double* __restrict__ fa, const double* __restrict__ fb,
                                             double* __restrict__ rows, double* __restrict__ rec,
                                             int* __restrict__ status, int* __restrict__ fallback,
@@ -438,8 +502,13 @@ __device__ __forceinline__ void condense_ns(int nint, const double* __restrict__
   const int i = blockIdx.x;
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = wave_id();
+  const int ib = bt.iinst[i];
+  if (skipped(bt, ib)) {   // inactive instance: nothing to condense, nothing for Gauss-Jordan
+    if (tid == 0) fallback[i] = 0;
+    return;
+  }
   if (tid == 0) sbad = 0;
-  const double h = t[i + 1] - t[i];
+  const double h = t[i + ib + 1] - t[i + ib];
   double* recU = rec + (size_t)i * REC;
   double* recT = recU + NS * NVA;
   double* recG = recT + M * NY * NVA;
@@ -592,12 +661,22 @@ __device__ __forceinline__ void condense_ns(int nint, const double* __restrict__
 // the pivot broadcast are permlane16/32 swaps.  The Householder vector
 // is broadcast through LDS, where rows above the current pivot are kept zero
 // so every loop has a fixed trip count.  An odd last row is carried.
-__device__ __forceinline__ void cr_pair(int nrows, const double* __restrict__ rows_in,
+// Workgroup g makes output row g of the level: oinst[g] is its instance,
+// seg[oinst[g]] the instance's place in this level's buffers.
+__device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
+                                        const int* __restrict__ act, const double* __restrict__ rows_in,
                                         double* __restrict__ rows_out, double* __restrict__ recs) {
   constexpr int R2 = 2 * NY;
   __shared__ double sv[2][CR_PARTS * CR_ROWS];
   __shared__ double sbeta[2];
-  const int k = blockIdx.x;
+  const int ib = oinst[blockIdx.x];
+  if (act && !act[ib]) return;
+  const CRSeg sg = seg[ib];
+  const int k = blockIdx.x - sg.out_off;
+  const int nrows = sg.nrows;
+  rows_in += (size_t)sg.in_off * NY * CRW;
+  rows_out += (size_t)sg.out_off * NY * CRW;
+  recs += (size_t)sg.rec_off * NY * RECW;
   const int npairs = nrows >> 1;
   const int tid = threadIdx.x;
   if (k == npairs) {  // carry the odd last row
@@ -706,8 +785,9 @@ __device__ __forceinline__ void cr_pair(int nrows, const double* __restrict__ ro
 // Boundary system on the last remaining row G x_0 + C x_N + H p = b and the
 // boundary conditions B_0 x_0 + B_n x_N + V_n p = -r_bc: unknowns
 // [x_N | x_0 | p] as in qr_decomposition's final block (solve_babd_system.py:364-381).
+// Workgroup b solves instance b, whose last row is row b of `rows`.
 template <int FW>
-__device__ __forceinline__ void cr_final(int nint, const double* __restrict__ row, const double* __restrict__ Dr,
+__device__ __forceinline__ void cr_final(Batch bt, const double* __restrict__ rows, const double* __restrict__ Dr,
                          const double* __restrict__ rbc, double* __restrict__ dy, double* __restrict__ dp,
                          int* __restrict__ status) {
   using Solver = GJ<NFIN, NFIN + 1, FW>;
@@ -715,6 +795,14 @@ __device__ __forceinline__ void cr_final(int nint, const double* __restrict__ ro
   __shared__ int pivrow[NFIN];
   __shared__ double pivval[NFIN];
   __shared__ int invperm[Solver::RQ * WAVE];
+  const int ib = blockIdx.x;
+  if (skipped(bt, ib)) return;
+  const double* row = rows + (size_t)ib * NY * CRW;
+  Dr += (size_t)ib * DRSZ;
+  rbc += (size_t)ib * NRB;
+  double* x0 = dy + (size_t)bt.noff[ib] * NY;
+  double* xn = dy + (size_t)(bt.noff[ib + 1] - 1) * NY;
+  dp += (size_t)ib * NP;
   Solver gj;
   constexpr int DRW = 2 * NY + NP;
   for (int k = threadIdx.x; k < Solver::RQ * WAVE; k += blockDim.x) invperm[k] = -1;
@@ -753,8 +841,8 @@ __device__ __forceinline__ void cr_final(int nint, const double* __restrict__ ro
       const int kk = invperm[r];
       if (kk < 0 || kk >= NFIN) continue;
       const double v = gj.a[q][cq] / pivval[kk];
-      if (kk < NY) dy[(size_t)nint * NY + kk] = v;
-      else if (kk < 2 * NY) dy[kk - NY] = v;
+      if (kk < NY) xn[kk] = v;
+      else if (kk < 2 * NY) x0[kk - NY] = v;
       else dp[kk - 2 * NY] = v;
     }
   }
@@ -764,16 +852,26 @@ __device__ __forceinline__ void cr_final(int nint, const double* __restrict__ ro
 // Back substitution of one cyclic-reduction level: one wave per record,
 //   x_M = R^-1 (d - Gp x_L - Ep x_R - Jp p)   (as partition_backward_substitution_kernel,
 // solve_babd_system.py:537-572, with the triangular solve done directly).
-__device__ __forceinline__ void cr_back(int level, int nrows, int nint, const double* __restrict__ recs,
+// Launched with the level's output-row grid of cr_pair; carried rows have
+// no record and return.
+__device__ __forceinline__ void cr_back(int level, Batch bt, const int* __restrict__ oinst,
+                        const CRSeg* __restrict__ seg, const double* __restrict__ recs,
                         double* __restrict__ dy, const double* __restrict__ dp) {
-  const int k = blockIdx.x;
+  const int ib = oinst[blockIdx.x];
+  if (skipped(bt, ib)) return;
+  const CRSeg sg = seg[ib];
+  const int k = blockIdx.x - sg.out_off;
+  if (k >= (sg.nrows >> 1)) return;
   const int lane = threadIdx.x;
-  const int mid = (2 * k + 1) << level;
-  const int left = (2 * k) << level;
-  const int right = min((2 * k + 2) << level, nint);
-  const double* rec = recs + (size_t)k * NY * RECW;
+  const int base = bt.noff[ib];
+  const int nint = bt.noff[ib + 1] - base - 1;
+  const int mid = base + ((2 * k + 1) << level);
+  const int left = base + ((2 * k) << level);
+  const int right = base + min((2 * k + 2) << level, nint);
+  const double* rec = recs + (size_t)(sg.rec_off + k) * NY * RECW;
   const double* xl = dy + (size_t)left * NY;
   const double* xr = dy + (size_t)right * NY;
+  dp += (size_t)ib * NP;
   // lane r holds row r of R and the right-hand side d - Gp x_L - Ep x_R - Jp p
   const double* rw = rec + (lane < NY ? lane : 0) * RECW;
   double Rr[NY];
@@ -799,10 +897,15 @@ __device__ __forceinline__ void cr_back(int level, int nrows, int nint, const do
 // ---------------------------------------------------------------- recover
 // One workgroup per interval (recover_delta_k_kernel, solver:1619-1650):
 //   u_0 = dy_i, u_j = U_j v,  dydot_j = T_j [u_j; dp; 1],  dz_j = -G_j [u_j; dp; 1].
-__device__ __forceinline__ void recover(int nint, const double* __restrict__ rec, const double* __restrict__ dy,
-                        const double* __restrict__ dp, double* __restrict__ dydot, double* __restrict__ dz) {
+__device__ __forceinline__ void recover(int nint, Batch bt, const double* __restrict__ rec,
+                        const double* __restrict__ dy, const double* __restrict__ dp,
+                        double* __restrict__ dydot, double* __restrict__ dz) {
   __shared__ double sw[M][NVA];
   const int i = blockIdx.x;
+  const int ib = bt.iinst[i];
+  if (skipped(bt, ib)) return;
+  dy += (size_t)ib * NY;    // left node of interval i is i + ib
+  dp += (size_t)ib * NP;
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = wave_id();
   const int nw = blockDim.x / WAVE;
@@ -846,15 +949,24 @@ __device__ __forceinline__ void recover(int nint, const double* __restrict__ rec
 }
 
 // ------------------------------------------------------------------- axpy
+// out = x + a_b d over the packed state [y | ydot | ztil | p], with a_b =
+// steps[b] the step of the element's instance; a_b = 0 copies x (so that a
+// non-finite direction of an instance that takes no step cannot leak).
 struct Seg4 { long long off[4]; long long len[4]; };
-__device__ __forceinline__ void axpy4(Seg4 s, const double* __restrict__ x, const double* __restrict__ d, double a,
-                      double* __restrict__ out) {
+__device__ __forceinline__ void axpy4(Seg4 s, Batch bt, const double* x, const double* __restrict__ d,
+                                      const double* __restrict__ steps, double* out) {
   long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if (gid < s.len[q]) {
+      int b;
+      if (q == 0) b = bt.ninst[gid / NY];
+      else if (q == 1) b = bt.iinst[gid / (M * NY)];
+      else if (q == 2) b = bt.iinst[gid / (M * NZ1)];
+      else b = (int)(gid / NP1);
+      const double a = steps[b];
       const long long e = s.off[q] + gid;
-      out[e] = fma(a, d[e], x[e]);
+      out[e] = (a != 0.0) ? fma(a, d[e], x[e]) : x[e];
       return;
     }
     gid -= s.len[q];
@@ -863,22 +975,24 @@ __device__ __forceinline__ void axpy4(Seg4 s, const double* __restrict__ x, cons
 
 // -------------------------------------------------------- initial input
 // form_initial_input_kernel (solver:316-356).
-__device__ __forceinline__ void init_input(int nint, const double* __restrict__ y0, const double* __restrict__ z0,
-                           const double* __restrict__ p, const double* __restrict__ c,
-                           double* __restrict__ ydot, double* __restrict__ ztil) {
+__device__ __forceinline__ void init_input(int nint, Batch bt, const double* __restrict__ y0,
+                           const double* __restrict__ z0, const double* __restrict__ p,
+                           const double* __restrict__ c, double* __restrict__ ydot, double* __restrict__ ztil) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= nint * M) return;
   const int i = gid / M, j = gid - (gid / M) * M;
+  const int ib = bt.iinst[i];
+  const int nd = i + ib;
   const double cj = c_rkc[j];
   double yt[NY], zt[NZ1], fo[NY];
 #pragma unroll
-  for (int l = 0; l < NY; ++l) yt[l] = (1.0 - cj) * y0[(size_t)i * NY + l] + cj * y0[(size_t)(i + 1) * NY + l];
+  for (int l = 0; l < NY; ++l) yt[l] = (1.0 - cj) * y0[(size_t)nd * NY + l] + cj * y0[(size_t)(nd + 1) * NY + l];
 #pragma unroll
   for (int l = 0; l < NZ; ++l) {
-    zt[l] = (1.0 - cj) * z0[(size_t)i * NZ + l] + cj * z0[(size_t)(i + 1) * NZ + l];
+    zt[l] = (1.0 - cj) * z0[(size_t)nd * NZ + l] + cj * z0[(size_t)(nd + 1) * NZ + l];
     ztil[(size_t)gid * NZ + l] = zt[l];
   }
-  prob::f(yt, zt, p, c, fo);
+  prob::f(yt, zt, p + (size_t)ib * NP, c + (size_t)ib * NC, fo);
 #pragma unroll
   for (int l = 0; l < NY; ++l) ydot[(size_t)gid * NY + l] = fo[l];
 }
@@ -917,17 +1031,22 @@ __device__ __forceinline__ void gauss_point(int gp, double h, const double* __re
   }
 }
 
-__device__ __forceinline__ void segres(int nint, const double* __restrict__ t, const double* __restrict__ y,
-                       const double* __restrict__ ydot, const double* __restrict__ ztil,
-                       const double* __restrict__ p, const double* __restrict__ c, double alpha, double tol,
+__device__ __forceinline__ void segres(int nint, Batch bt, const double* __restrict__ t,
+                       const double* __restrict__ y, const double* __restrict__ ydot,
+                       const double* __restrict__ ztil, const double* __restrict__ p,
+                       const double* __restrict__ c, double alpha, double tol,
                        int rtype, int scale_time, int relative,
                        double* __restrict__ residual, unsigned long long* __restrict__ maxres) {
-  __shared__ double sred[16];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double val = 0.0;
+  int ib = -1;
   if (i < nint) {
-    const double h = t[i + 1] - t[i];
-    const double* yi = y + (size_t)i * NY;
+    ib = bt.iinst[i];
+    const int nd = i + ib;
+    p += (size_t)ib * NP;
+    c += (size_t)ib * NC;
+    const double h = t[nd + 1] - t[nd];
+    const double* yi = y + (size_t)nd * NY;
     const double* yd = ydot + (size_t)i * M * NY;
     const double* zz = ztil + (size_t)i * M * NZ;
     double rho_h = 0.0, rho_g = 0.0;
@@ -988,26 +1107,30 @@ __device__ __forceinline__ void segres(int nint, const double* __restrict__ t, c
       }
       val = fmax(rho_h, rho_g) / tol;
     }
-    residual[i] = val;
-  } else if (i == nint) {
+    residual[nd] = val;
+  } else if (i < nint + bt.B) {
+    ib = i - nint;
+    const int last = bt.noff[ib + 1] - 1;
     double ro[NY + NP];
-    prob::r(y, y + (size_t)nint * NY, p, c, ro);
+    prob::r(y + (size_t)bt.noff[ib] * NY, y + (size_t)last * NY, p + (size_t)ib * NP, c + (size_t)ib * NC, ro);
     double m = 0.0;
 #pragma unroll
     for (int l = 0; l < NY + NP; ++l) m = nmax(fabs(ro[l]), m);
     val = m / tol;
-    residual[i] = val;
+    residual[last] = val;
   }
-  block_max_to(val, maxres, sred);
+  seg_max_to(val, ib, maxres);
 }
 
 // ---------------------------------------------------------- recover z
 // recover_solution_kernel (solver:2049-2075).
-__device__ __forceinline__ void recover_z(int nnodes, const double* __restrict__ ztil, double* __restrict__ z) {
+__device__ __forceinline__ void recover_z(int nnodes, Batch bt, const double* __restrict__ ztil,
+                                          double* __restrict__ z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnodes) return;
-  const bool last = i == nnodes - 1;
-  const int iv = last ? i - 1 : i;
+  const int ib = bt.ninst[i];
+  const bool last = i == bt.noff[ib + 1] - 1;
+  const int iv = (last ? i - 1 : i) - ib;   // interval whose stages give node i
 #pragma unroll
   for (int l = 0; l < NZ; ++l) {
     double s = 0.0;
@@ -1023,75 +1146,76 @@ __device__ __forceinline__ void recover_z(int nnodes, const double* __restrict__
 extern "C" {
 __global__ void colloc_fill_u64(unsigned long long* p, unsigned long long v, int n) { colloc::fill_u64(p, v, n); }
 
-__global__ __launch_bounds__(256) void colloc_residual_full(int nint, const double* t, const double* y,
-    const double* ydot, const double* ztil, const double* p, const double* c, double alpha, double* ytil,
-    double* fa, double* fb, double* rbc, unsigned long long* norm) {
-  colloc::residual<true>(nint, t, y, ydot, ztil, p, c, alpha, ytil, fa, fb, rbc, norm);
+__global__ __launch_bounds__(256) void colloc_residual_full(int nint, colloc::Batch bt, const double* t,
+    const double* y, const double* ydot, const double* ztil, const double* p, const double* c, double alpha,
+    double* ytil, double* fa, double* fb, double* rbc, unsigned long long* norm) {
+  colloc::residual<true>(nint, bt, t, y, ydot, ztil, p, c, alpha, ytil, fa, fb, rbc, norm);
 }
 
-__global__ __launch_bounds__(256) void colloc_residual_norm(int nint, const double* t, const double* y,
-    const double* ydot, const double* ztil, const double* p, const double* c, double alpha, double* ytil,
-    double* fa, double* fb, double* rbc, unsigned long long* norm) {
-  colloc::residual<false>(nint, t, y, ydot, ztil, p, c, alpha, ytil, fa, fb, rbc, norm);
+__global__ __launch_bounds__(256) void colloc_residual_norm(int nint, colloc::Batch bt, const double* t,
+    const double* y, const double* ydot, const double* ztil, const double* p, const double* c, double alpha,
+    double* ytil, double* fa, double* fb, double* rbc, unsigned long long* norm) {
+  colloc::residual<false>(nint, bt, t, y, ydot, ztil, p, c, alpha, ytil, fa, fb, rbc, norm);
 }
 
-__global__ __launch_bounds__(256) void colloc_jacobian(int nint, const double* y, const double* ytil,
-    const double* ztil, const double* p, const double* c, double alpha, double* jac, double* Dr) {
-  colloc::jacobian(nint, y, ytil, ztil, p, c, alpha, jac, Dr);
+__global__ __launch_bounds__(256) void colloc_jacobian(int nint, colloc::Batch bt, const double* y,
+    const double* ytil, const double* ztil, const double* p, const double* c, double alpha, double* jac, double* Dr) {
+  colloc::jacobian(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
 }
 
-__global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const double* t, const double* jac,
-    const double* fa, const double* fb, double* rows, double* rec, int* status, const int* only) {
-  colloc::condense<Cfg::CW>(nint, t, jac, fa, fb, rows, rec, status, only);
+__global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const int* iinst, const double* t,
+    const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status, const int* only) {
+  colloc::condense<Cfg::CW>(nint, iinst, t, jac, fa, fb, rows, rec, status, only);
 }
-__global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nint, const double* t,
-    const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status, int* fallback,
-    unsigned long long* counters) {
+__global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nint, colloc::Batch bt,
+    const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
+    int* fallback, unsigned long long* counters) {
   if constexpr (colloc::NS_OK) {
-    colloc::condense_ns(nint, t, jac, fa, fb, rows, rec, status, fallback, counters);
+    colloc::condense_ns(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
   } else if (threadIdx.x == 0) {
-    fallback[blockIdx.x] = 1;
-    atomicAdd(counters, 1ull);
+    const bool skip = bt.act && !bt.act[bt.iinst[blockIdx.x]];
+    fallback[blockIdx.x] = skip ? 0 : 1;
+    if (!skip) atomicAdd(counters, 1ull);
   }
 }
 
-__global__ __launch_bounds__(colloc::CR_THREADS) void colloc_cr_pair(int nrows, const double* rows_in, double* rows_out,
-    double* recs) {
-  colloc::cr_pair(nrows, rows_in, rows_out, recs);
+__global__ __launch_bounds__(colloc::CR_THREADS) void colloc_cr_pair(const int* oinst, const colloc::CRSeg* seg,
+    const int* act, const double* rows_in, double* rows_out, double* recs) {
+  colloc::cr_pair(oinst, seg, act, rows_in, rows_out, recs);
 }
 
-__global__ __launch_bounds__(Cfg::FW * 64) void colloc_cr_final(int nint, const double* row, const double* Dr,
-    const double* rbc, double* dy, double* dp, int* status) {
-  colloc::cr_final<Cfg::FW>(nint, row, Dr, rbc, dy, dp, status);
+__global__ __launch_bounds__(Cfg::FW * 64) void colloc_cr_final(colloc::Batch bt, const double* rows,
+    const double* Dr, const double* rbc, double* dy, double* dp, int* status) {
+  colloc::cr_final<Cfg::FW>(bt, rows, Dr, rbc, dy, dp, status);
 }
 
-__global__ __launch_bounds__(64) void colloc_cr_back(int level, int nrows, int nint, const double* recs,
-    double* dy, const double* dp) {
-  colloc::cr_back(level, nrows, nint, recs, dy, dp);
+__global__ __launch_bounds__(64) void colloc_cr_back(int level, colloc::Batch bt, const int* oinst,
+    const colloc::CRSeg* seg, const double* recs, double* dy, const double* dp) {
+  colloc::cr_back(level, bt, oinst, seg, recs, dy, dp);
 }
 
-__global__ __launch_bounds__(256) void colloc_recover(int nint, const double* rec, const double* dy,
-    const double* dp, double* dydot, double* dz) {
-  colloc::recover(nint, rec, dy, dp, dydot, dz);
+__global__ __launch_bounds__(256) void colloc_recover(int nint, colloc::Batch bt, const double* rec,
+    const double* dy, const double* dp, double* dydot, double* dz) {
+  colloc::recover(nint, bt, rec, dy, dp, dydot, dz);
 }
 
-__global__ __launch_bounds__(256) void colloc_axpy4(colloc::Seg4 s, const double* x, const double* d, double a,
-    double* out) {
-  colloc::axpy4(s, x, d, a, out);
+__global__ __launch_bounds__(256) void colloc_axpy4(colloc::Seg4 s, colloc::Batch bt, const double* x,
+    const double* d, const double* steps, double* out) {
+  colloc::axpy4(s, bt, x, d, steps, out);
 }
 
-__global__ __launch_bounds__(256) void colloc_init_input(int nint, const double* y0, const double* z0,
-    const double* p, const double* c, double* ydot, double* ztil) {
-  colloc::init_input(nint, y0, z0, p, c, ydot, ztil);
+__global__ __launch_bounds__(256) void colloc_init_input(int nint, colloc::Batch bt, const double* y0,
+    const double* z0, const double* p, const double* c, double* ydot, double* ztil) {
+  colloc::init_input(nint, bt, y0, z0, p, c, ydot, ztil);
 }
 
-__global__ __launch_bounds__(256) void colloc_segres(int nint, const double* t, const double* y,
+__global__ __launch_bounds__(256) void colloc_segres(int nint, colloc::Batch bt, const double* t, const double* y,
     const double* ydot, const double* ztil, const double* p, const double* c, double alpha, double tol,
     int rtype, int scale_time, int relative, double* residual, unsigned long long* maxres) {
-  colloc::segres(nint, t, y, ydot, ztil, p, c, alpha, tol, rtype, scale_time, relative, residual, maxres);
+  colloc::segres(nint, bt, t, y, ydot, ztil, p, c, alpha, tol, rtype, scale_time, relative, residual, maxres);
 }
 
-__global__ __launch_bounds__(256) void colloc_recover_z(int nnodes, const double* ztil, double* z) {
-  colloc::recover_z(nnodes, ztil, z);
+__global__ __launch_bounds__(256) void colloc_recover_z(int nnodes, colloc::Batch bt, const double* ztil, double* z) {
+  colloc::recover_z(nnodes, bt, ztil, z);
 }
 }

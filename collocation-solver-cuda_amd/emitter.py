@@ -480,8 +480,8 @@ def build_unit(sym, m, tables):
     parts.append(_carray("c_gw", tables.gw))
     parts.append(_carray("c_L0", tables.L0))
     parts.append(_carray("c_L1", tables.L1))
-    parts.append('extern "C" __device__ int colloc_meta[8] = {%d, %d, %d, %d, %d, %d, %d, 0};'
-                 % (ny, nz, np_, m, nc, cw, fw))
+    parts.append('extern "C" __device__ int colloc_meta[8] = {%d, %d, %d, %d, %d, %d, %d, %d};'
+                 % (ny, nz, np_, m, nc, cw, fw, len(pact)))
     parts.append('#include "colloc_kernels.h"')
     return "\n".join(parts) + "\n"
 

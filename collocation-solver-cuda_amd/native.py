@@ -28,7 +28,10 @@ SIGNATURES = {
     "cs_destroy": (None, [c_void_p]),
     "cs_load_module": (c_int, [c_void_p, c_void_p, c_size_t]),
     "cs_set_mesh": (c_int, [c_void_p, P_D, c_int]),
+    "cs_set_batch_mesh": (c_int, [c_void_p, c_int, P_I, P_D]),
     "cs_set_constants": (c_int, [c_void_p, P_D, c_int]),
+    "cs_set_batch_constants": (c_int, [c_void_p, P_D, c_int, c_int]),
+    "cs_batch_info": (c_int, [c_void_p, P_I, P_I]),
     "cs_set_state": (c_int, [c_void_p, P_D, P_D, P_D, P_D]),
     "cs_get_state": (c_int, [c_void_p, P_D, P_D, P_D, P_D]),
     "cs_form_initial_input": (c_int, [c_void_p, P_D, P_D, P_D]),
@@ -50,7 +53,8 @@ SIGNATURES = {
 
 
 class CSDims(ctypes.Structure):
-    _fields_ = [("ny", c_int), ("nz", c_int), ("np", c_int), ("m", c_int), ("n_max", c_int), ("nc", c_int)]
+    _fields_ = [("ny", c_int), ("nz", c_int), ("np", c_int), ("m", c_int), ("n_max", c_int), ("nc", c_int),
+                ("batch", c_int)]
 
 
 class ColloError(RuntimeError):
@@ -81,6 +85,12 @@ def lib():
 def check(rc):
     if rc != CS_OK:
         raise ColloError(rc, lib().cs_last_error().decode(errors="replace"))
+
+
+def iptr(a):
+    """Pointer to a C-contiguous int32 array."""
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(P_I)
 
 
 def ptr(a):

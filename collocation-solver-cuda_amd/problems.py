@@ -202,3 +202,7 @@ def standard_units():
     yield "tanh4_m5", small, 5
     yield "tanh20_m5", config(1)[0], 5
     yield "param_osc_m4", TextProblem(PARAM_OSC_OCP), 4
+    # batched families: x0 (and the tanh scale) are instance constants, so
+    # instance 0's text stands for every member
+    yield "tanh4_batch_m5", batch_family(1, nx=4, nu=2, nint_range=(16, 64))[0], 5
+    yield "tanh20_batch_m5", batch_family(1)[0], 5

@@ -94,16 +94,25 @@ class BvpDae:
 
 
 class CollocationSolver:
-    """Device-resident solver state for one BVP-DAE and stage count m."""
+    """Device-resident solver state for one BVP-DAE and stage count m.
 
-    def __init__(self, bvp, m, n_max=None):
+    The handle holds a batch of up to `batch` instances of the problem (same
+    generated code, each with its own mesh, constants, state and parameters);
+    set_mesh makes a batch of one.  For a batch of one every method returns
+    what the reference's stage function returns; for B > 1 the per-solve
+    scalars (norms, iteration counts, maximum residuals) are arrays of B
+    values and the arrays are the instances' arrays concatenated.
+    """
+
+    def __init__(self, bvp, m, n_max=None, batch=1):
         self.bvp = bvp
         self.m = m
         self.ny, self.nz, self.np = bvp.size_y, bvp.size_z, bvp.size_p
         self.n_max = int(n_max or max(bvp.maximum_nodes, bvp.N))
+        self.batch = int(batch)
         code = jit.code_object(bvp.ocp_text, m)
         L = lib()
-        dims = CSDims(self.ny, self.nz, self.np, m, self.n_max, bvp.size_c)
+        dims = CSDims(self.ny, self.nz, self.np, m, self.n_max, bvp.size_c, self.batch)
         h = native.c_void_p()
         check(L.cs_create(native.ctypes.byref(dims), native.ctypes.byref(h)))
         self.h = h
@@ -112,11 +121,19 @@ class CollocationSolver:
         if bvp.size_c:
             self.set_constants(bvp.constants)
         self.N = 0
+        self.B = 1
+        self.nodes = np.zeros(1, dtype=np.int32)
 
     def set_constants(self, values):
-        """Per-instance constants of the problem (InstanceConstants order)."""
+        """Per-instance constants of the problem (InstanceConstants order),
+        given to every instance of the batch."""
         c = np.ascontiguousarray(values, dtype=np.float64)
         check(lib().cs_set_constants(self.h, ptr(c), c.shape[0]))
+
+    def set_batch_constants(self, values):
+        """Constants of each instance: values[B][nc]."""
+        c = np.ascontiguousarray(values, dtype=np.float64).reshape(self.B, -1)
+        check(lib().cs_set_batch_constants(self.h, ptr(c), c.shape[0], c.shape[1]))
 
     def close(self):
         if getattr(self, "h", None):
@@ -134,11 +151,33 @@ class CollocationSolver:
         t = np.ascontiguousarray(t_span, dtype=np.float64)
         check(lib().cs_set_mesh(self.h, ptr(t), t.shape[0]))
         self.N = t.shape[0]
+        self.B = 1
+        self.nodes = np.array([self.N], dtype=np.int32)
         self.t_span = t
 
+    def set_batch_mesh(self, t_spans):
+        """Meshes of a batch: a list of B node arrays."""
+        nodes = np.ascontiguousarray([len(t) for t in t_spans], dtype=np.int32)
+        t = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.float64) for t in t_spans]))
+        check(lib().cs_set_batch_mesh(self.h, len(t_spans), native.iptr(nodes), ptr(t)))
+        self.N = int(nodes.sum())
+        self.B = len(t_spans)
+        self.nodes = nodes
+        self.t_span = t
+
+    def node_offsets(self):
+        return np.concatenate([[0], np.cumsum(self.nodes)])
+
+    def interval_offsets(self):
+        return np.concatenate([[0], np.cumsum(self.nodes - 1)])
+
+    def _per_solve(self, v):
+        return v[0] if self.B == 1 else v
+
     def _shapes(self):
-        nst = (self.N - 1) * self.m
-        return (self.N, self.ny), (nst, self.ny), (nst, self.nz), (self.np,)
+        nst = (self.N - self.B) * self.m
+        ps = (self.B, self.np) if self.B > 1 else (self.np,)
+        return (self.N, self.ny), (nst, self.ny), (nst, self.nz), ps
 
     def set_state(self, y, y_dot, z_tilde, p):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (y, y_dot, z_tilde, p)]
@@ -166,23 +205,26 @@ class CollocationSolver:
 
     def compute_f_q(self, alpha, outputs=True):
         """compute_f_q_parallel (solver:399): (norm, y_tilde, f_a, f_b, r_bc)."""
-        nrm = native.c_double()
-        nst = (self.N - 1) * self.m
+        nrm = np.zeros(self.B)
+        nint = self.N - self.B
+        nst = nint * self.m
         if outputs:
-            yt = np.empty((nst, self.ny)); fa = np.empty((self.N - 1, self.m * (self.ny + self.nz)))
-            fb = np.empty((self.N - 1, self.ny)); rbc = np.empty(self.ny + self.np)
+            yt = np.empty((nst, self.ny)); fa = np.empty((nint, self.m * (self.ny + self.nz)))
+            fb = np.empty((nint, self.ny))
+            rbc = np.empty(self.ny + self.np) if self.B == 1 else np.empty((self.B, self.ny + self.np))
         else:
             yt = fa = fb = rbc = None
-        check(lib().cs_compute_f_q(self.h, alpha, native.ctypes.byref(nrm), ptr(yt), ptr(fa), ptr(fb), ptr(rbc)))
-        return nrm.value, yt, fa, fb, rbc
+        check(lib().cs_compute_f_q(self.h, alpha, ptr(nrm), ptr(yt), ptr(fa), ptr(fb), ptr(rbc)))
+        return self._per_solve(nrm), yt, fa, fb, rbc
 
     def construct_jacobian(self, alpha, outputs=True):
         """construct_jacobian_parallel (solver:611): per-stage Df, Dg and Dr."""
-        nst = (self.N - 1) * self.m
+        nst = (self.N - self.B) * self.m
         nj = self.ny + self.nz + self.np
         if outputs:
             Df = np.empty((nst, self.ny, nj)); Dg = np.empty((nst, self.nz, nj))
-            Dr = np.empty((self.ny + self.np, 2 * self.ny + self.np))
+            shp = (self.ny + self.np, 2 * self.ny + self.np)
+            Dr = np.empty(shp if self.B == 1 else (self.B,) + shp)
         else:
             Df = Dg = Dr = None
         check(lib().cs_construct_jacobian(self.h, alpha, ptr(Df), ptr(Dg), ptr(Dr)))
@@ -190,7 +232,7 @@ class CollocationSolver:
 
     def reduce_jacobian(self, outputs=True):
         """reduce_jacobian_parallel (solver:840): A, C (= I), H, b."""
-        nint = self.N - 1
+        nint = self.N - self.B
         if outputs:
             A = np.empty((nint, self.ny, self.ny)); H = np.empty((nint, self.ny, self.np)); b = np.empty((nint, self.ny))
         else:
@@ -201,35 +243,40 @@ class CollocationSolver:
 
     def solve_babd(self):
         """BABD solve (solve_babd_system.py:93-572): delta_y (N, ny), delta_p."""
-        dy = np.empty((self.N, self.ny)); dp = np.empty(self.np)
+        dy = np.empty((self.N, self.ny))
+        dp = np.empty(self.np) if self.B == 1 else np.empty((self.B, self.np))
         check(lib().cs_solve_babd(self.h, ptr(dy), ptr(dp)))
         return dy, dp
 
     def recover_delta_k(self):
         """recover_delta_k_parallel (solver:1580): delta_y_dot, delta_z_tilde."""
-        nst = (self.N - 1) * self.m
+        nst = (self.N - self.B) * self.m
         dyd = np.empty((nst, self.ny)); dz = np.empty((nst, self.nz))
         check(lib().cs_recover_delta_k(self.h, ptr(dyd), ptr(dz)))
         return dyd, dz
 
     def newton(self, alpha, tol, max_iter):
-        """Damped Newton loop (solver:109-174): (iter_time, norm, norm history)."""
-        it = native.c_int()
-        nrm = native.c_double()
-        hist = np.zeros(max_iter)
-        check(lib().cs_newton(self.h, alpha, tol, max_iter, native.ctypes.byref(it), native.ctypes.byref(nrm),
-                              ptr(hist), max_iter))
-        return it.value, nrm.value, hist[:min(it.value + 1, max_iter)].copy()
+        """Damped Newton loop (solver:109-174): (iter_time, norm, norm history).
+        For a batch: arrays of B iteration counts and norms, and a list of B
+        histories."""
+        it = np.zeros(self.B, dtype=np.int32)
+        nrm = np.zeros(self.B)
+        hist = np.zeros((self.B, max_iter))
+        check(lib().cs_newton(self.h, alpha, tol, max_iter, native.iptr(it), ptr(nrm), ptr(hist), max_iter))
+        hists = [hist[b, :min(int(it[b]) + 1, max_iter)].copy() for b in range(self.B)]
+        if self.B == 1:
+            return int(it[0]), float(nrm[0]), hists[0]
+        return it.astype(int), nrm, hists
 
     def segment_residual(self, alpha, tol, residual_type=1, scale_by_time=True, residual2=False):
         """compute_segment_residual_parallel (solver:1770), or with residual2
         compute_segment_residual_parallel2 (solver:1895): (residual, max_residual).
         residual_type / scale_by_time are the reference's module flags (solver:30-31)."""
         res = np.empty(self.N)
-        mx = native.c_double()
+        mx = np.zeros(self.B)
         check(lib().cs_segment_residual_ex(self.h, alpha, tol, int(residual_type), int(bool(scale_by_time)),
-                                           int(bool(residual2)), ptr(res), native.ctypes.byref(mx)))
-        return res, mx.value
+                                           int(bool(residual2)), ptr(res), ptr(mx)))
+        return res, self._per_solve(mx)
 
     def recover_solution(self):
         """recover_solution_parallel (solver:2034): z at the nodes."""
@@ -350,3 +397,119 @@ def collocation_solver_parallel(bvp_dae, m=4, solver=None, write_output=False, l
     if own:
         solver.close()
     return result
+
+
+def solve_batch(solver, t_spans, Y0s, Z0s, P0s, constants=None, homotopy=(), alpha=0.1, tol=1e-8, max_iter=200,
+                residual_type=1, scale_by_time=True, residual2=False):
+    """One damped-Newton solve (solver:109-174) of every instance of a batch
+    in one device-resident sequence of launches, followed by the acceptance
+    residual (solver:1770-1892) of each.
+
+    t_spans, Y0s, Z0s, P0s: per instance mesh and initial guess (the
+    reference's T0, Y0, Z0, P0).  constants: [B][nc] instance constants of
+    the final solve; homotopy: a list of [B][nc] constant arrays solved first,
+    each warm-starting the next (as the homotopy of
+    collocation_solver_parallel).  Returns a dict with per-instance arrays
+    "iterations" ([stages][B]), "norm", "max_residual" and "converged"
+    (norm < tol, max residual <= 1 and fewer than max_iter - 1 iterations).
+    """
+    B = len(t_spans)
+    solver.set_batch_mesh(t_spans)
+    solver.form_initial_input(np.concatenate(Y0s), np.concatenate(Z0s),
+                              np.stack([np.asarray(p, dtype=np.float64) for p in P0s]) if B > 1 else P0s[0])
+    return _newton_stages(solver, constants, homotopy, alpha, tol, max_iter, residual_type, scale_by_time,
+                          residual2)
+
+
+def _newton_stages(solver, constants, homotopy, alpha, tol, max_iter, residual_type=1, scale_by_time=True,
+                   residual2=False):
+    B = solver.B
+    its = []
+    nrm = None
+
+    def run():
+        it, n, _ = solver.newton(alpha, tol, max_iter)
+        return np.atleast_1d(it), np.atleast_1d(n)
+    for cst in homotopy:
+        solver.set_batch_constants(cst)
+        it, nrm = run()
+        its.append(it)
+    if constants is not None:
+        solver.set_batch_constants(constants)
+    it, nrm = run()
+    its.append(it)
+    _, mx = solver.segment_residual(alpha, tol, residual_type, scale_by_time, residual2)
+    mx = np.atleast_1d(mx)
+    ok = (nrm < tol) & (mx <= 1.0) & (it < max_iter - 1)
+    return {"iterations": np.array(its), "norm": nrm, "max_residual": mx, "converged": ok, "B": B}
+
+
+def batch_solutions(solver):
+    """Per-instance solutions of the solver's current batch: a list of dicts
+    with t, y, z (at the nodes, recover_solution_parallel) and p."""
+    y, _, _, p = solver.get_state()
+    z = solver.recover_solution()
+    p = np.atleast_2d(p)
+    off = solver.node_offsets()
+    return [{"t": solver.t_span[off[b]:off[b + 1]], "y": y[off[b]:off[b + 1]], "z": z[off[b]:off[b + 1]],
+             "p": p[b]} for b in range(solver.B)]
+
+
+def plan_chunks(nodes, max_nodes, max_batch):
+    """Split instances (node counts, in order) into consecutive chunks of at
+    most max_batch instances and max_nodes nodes."""
+    chunks, cur, tot = [], [], 0
+    for i, n in enumerate(nodes):
+        if n > max_nodes:
+            raise ValueError("instance %d has %d nodes, more than a chunk holds (%d)" % (i, n, max_nodes))
+        if cur and (tot + n > max_nodes or len(cur) == max_batch):
+            chunks.append(cur)
+            cur, tot = [], 0
+        cur.append(i)
+        tot += n
+    if cur:
+        chunks.append(cur)
+    return chunks
+
+
+def collocation_solver_batch(bvp_dae, m, t_spans, Y0s, Z0s, P0s, constants=None, homotopy=(), solver=None,
+                             max_nodes=None, max_batch=None, solutions=True, log=None):
+    """Solve many instances of one problem with batched launches.
+
+    The instances are split into chunks that fit the solver's handle; each
+    chunk is solved by solve_batch.  constants / homotopy stages are [B][nc]
+    arrays over all instances.  Returns (stats, solutions) with stats the
+    per-instance arrays of solve_batch concatenated over chunks and
+    solutions a list of per-instance dicts (None unless solutions=True).
+    """
+    B = len(t_spans)
+    nodes = [len(t) for t in t_spans]
+    if solver is None:
+        max_nodes = max_nodes or sum(nodes)
+        max_batch = max_batch or B
+        solver = CollocationSolver(bvp_dae, m, n_max=max_nodes, batch=max_batch)
+    max_nodes = min(max_nodes or solver.n_max, solver.n_max)
+    max_batch = min(max_batch or solver.batch, solver.batch)
+    chunks = plan_chunks(nodes, max_nodes, max_batch)
+    tol = bvp_dae.tolerance
+    max_iter = bvp_dae.maximum_newton_iterations
+    nc = bvp_dae.size_c
+    cons = None if constants is None else np.asarray(constants, dtype=np.float64).reshape(B, nc)
+    homs = [np.asarray(h, dtype=np.float64).reshape(B, nc) for h in homotopy]
+    stats = {"iterations": np.zeros((len(homs) + 1, B), dtype=int), "norm": np.zeros(B),
+             "max_residual": np.zeros(B), "converged": np.zeros(B, dtype=bool)}
+    sols = [None] * B if solutions else None
+    for ch in chunks:
+        r = solve_batch(solver, [t_spans[i] for i in ch], [Y0s[i] for i in ch], [Z0s[i] for i in ch],
+                        [P0s[i] for i in ch], None if cons is None else cons[ch], [h[ch] for h in homs],
+                        tol=tol, max_iter=max_iter)
+        stats["iterations"][:, ch] = r["iterations"]
+        for k in ("norm", "max_residual", "converged"):
+            stats[k][ch] = r[k]
+        if solutions:
+            for i, s in zip(ch, batch_solutions(solver)):
+                sols[i] = s
+        if log:
+            log("chunk of %d instances (%d nodes): %d converged" % (len(ch), sum(nodes[i] for i in ch),
+                                                                     int(r["converged"].sum())))
+    return stats, sols

@@ -13,6 +13,15 @@
  *  - Plain C types only; host arrays are row-major float64 in the reference's
  *    layouts: t[N]; y[N][ny]; y_dot[(N-1)*m][ny] and z_tilde[(N-1)*m][nz]
  *    with row i*m+j for stage j of interval i (solver:278-283); p[np].
+ *  - Batches: a handle holds B >= 1 instances of one problem (same callbacks,
+ *    own mesh, constants, state and parameters), laid end to end.  With
+ *    N_b nodes per instance, the node arrays (t, y, z, the segment residual)
+ *    are the instances' arrays concatenated (N = sum N_b rows), the stage
+ *    arrays likewise (sum (N_b - 1) * m rows), and p, r_bc, Dr and the
+ *    constants hold one block per instance.  Scalars that the reference
+ *    returns per solve (residual norm, iteration count, maximum segment
+ *    residual) are arrays of B values.  cs_set_mesh makes a batch of one, in
+ *    which every layout is exactly the reference's.
  *  - All solver state lives in device memory owned by the handle; host
  *    arrays are copied in and out on the handle's stream, and every call
  *    returns after the stream has drained.
@@ -49,8 +58,10 @@ typedef struct {
   int nz;      /* size_z                                            */
   int np;      /* size_p                                            */
   int m;       /* collocation stages (solver -m flag, solver:2381)   */
-  int n_max;   /* largest number of time nodes N the handle holds     */
+  int n_max;   /* largest number of time nodes N the handle holds
+                  (summed over the instances of a batch)              */
   int nc;      /* number of per-instance constants (0 if none)        */
+  int batch;   /* most instances of a batch (0 or 1: single instance) */
 } cs_dims;
 
 /* Error text of the last failed call on this thread. */
@@ -77,13 +88,26 @@ void cs_destroy(cs_handle* h);
 /* Load a code object produced by cs_jit_compile for this handle's sizes. */
 int cs_load_module(cs_handle* h, const void* code, size_t size);
 
-/* Time mesh (solver:53-55 t_span).  2 <= N <= n_max. */
+/* Time mesh (solver:53-55 t_span) of a single instance.  2 <= N <= n_max. */
 int cs_set_mesh(cs_handle* h, const double* t, int N);
 
-/* Per-instance constants of the generated problem (nc values). */
+/* Meshes of a batch of B instances (1 <= B <= dims.batch): instance b has
+ * nodes[b] >= 2 time nodes; t holds the meshes concatenated.  The batched
+ * counterpart of the per-solve t_span (solver:53-55); the reference solves
+ * one instance per process.  Invalidates the saved initial input. */
+int cs_set_batch_mesh(cs_handle* h, int B, const int* nodes, const double* t);
+
+/* Per-instance constants of the generated problem (nc values), given to
+ * every instance of the batch. */
 int cs_set_constants(cs_handle* h, const double* c, int nc);
 
-/* Collocation state: y[N][ny], y_dot, z_tilde, p (solver:101 outputs). */
+/* Constants of each instance of a batch: c[B][nc]. */
+int cs_set_batch_constants(cs_handle* h, const double* c, int B, int nc);
+
+/* Instances and total nodes of the current mesh. */
+int cs_batch_info(cs_handle* h, int* B, int* N);
+
+/* Collocation state: y[N][ny], y_dot, z_tilde, p[B][np] (solver:101 outputs). */
 int cs_set_state(cs_handle* h, const double* y, const double* y_dot,
                  const double* z_tilde, const double* p);
 int cs_get_state(cs_handle* h, double* y, double* y_dot, double* z_tilde, double* p);
@@ -99,16 +123,16 @@ int cs_form_initial_input(cs_handle* h, const double* y0, const double* z0, cons
 int cs_restart(cs_handle* h);
 
 /* compute_f_q_parallel (solver:399-466): residual of the current state.
- * *norm_inf = ||f_q||_inf (the host reduction of solver:116).  Any of the
- * output arrays may be NULL; otherwise y_tilde[(N-1)*m][ny],
- * f_a[N-1][m*(ny+nz)], f_b[N-1][ny], r_bc[ny+np]. */
+ * norm_inf[b] = ||f_q||_inf of instance b (the host reduction of solver:116).
+ * Any of the output arrays may be NULL; otherwise y_tilde[(N-B)*m][ny],
+ * f_a[N-B][m*(ny+nz)], f_b[N-B][ny], r_bc[B][ny+np]. */
 int cs_compute_f_q(cs_handle* h, double alpha, double* norm_inf, double* y_tilde,
                    double* f_a, double* f_b, double* r_bc);
 
 /* construct_jacobian_parallel (solver:611-797): stage Jacobians at the
  * stage values of the last cs_compute_f_q call.  Kept on the device; when
- * non-NULL, Df[(N-1)*m][ny][ny+nz+np], Dg[(N-1)*m][nz][ny+nz+np] and
- * Dr[ny+np][2*ny+np] (= [B_0 | B_n | V_n], solver:655-665) are copied out. */
+ * non-NULL, Df[(N-B)*m][ny][ny+nz+np], Dg[(N-B)*m][nz][ny+nz+np] and
+ * Dr[B][ny+np][2*ny+np] (= [B_0 | B_n | V_n], solver:655-665) are copied out. */
 int cs_construct_jacobian(cs_handle* h, double alpha, double* Df, double* Dg, double* Dr);
 
 /* reduce_jacobian_parallel (solver:840-1001): per-interval condensation to
@@ -120,22 +144,25 @@ int cs_reduce_jacobian(cs_handle* h, double* A, double* H, double* b);
  * + backward_substitution + recover_babd_solution +
  * partition_backward_substitution_parallel (solve_babd_system.py:93-572,
  * driven at solver:134-147).  Outputs (may be NULL): delta_y[N][ny],
- * delta_p[np]. */
+ * delta_p[B][np]. */
 int cs_solve_babd(cs_handle* h, double* delta_y, double* delta_p);
 
 /* recover_delta_k_parallel (solver:1580-1650).  Outputs (may be NULL):
  * delta_y_dot[(N-1)*m][ny], delta_z_tilde[(N-1)*m][nz]. */
 int cs_recover_delta_k(cs_handle* h, double* delta_y_dot, double* delta_z_tilde);
 
-/* The damped Newton loop with line search (solver:109-174).  Runs until
- * ||f_q||_inf < tol or max_iter iterations; *iter_time receives the loop
- * counter at exit (as the reference's iter_time), *norm the last residual
- * norm, norms[0..hist_len) the norm history (may be NULL). */
+/* The damped Newton loop with line search (solver:109-174), for every
+ * instance of the batch at once: instance b iterates until its own
+ * ||f_q||_inf < tol or max_iter iterations, with its own line search.
+ * iter_time[b] receives its loop counter at exit (as the reference's
+ * iter_time), norm[b] its last residual norm, norms[b][0..hist_len) its norm
+ * history (may be NULL; entries past its last iteration are not written). */
 int cs_newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
               double* norm, double* norms, int hist_len);
 
 /* compute_segment_residual_parallel (solver:1770-1892) with residual_type 1
- * and scale_by_time.  residual[N] (may be NULL), *max_residual. */
+ * and scale_by_time.  residual[N] (may be NULL; the last node of each
+ * instance holds its ||r||_inf / tol), max_residual[B]. */
 int cs_segment_residual(cs_handle* h, double alpha, double tol, double* residual,
                         double* max_residual);
 
