@@ -1,12 +1,18 @@
 """Throughput benchmark: converged BVP solves per second on one MI355X.
 
-Workload (the single-GPU configuration the metric is quoted on): the
-n_x=20, n_u=10 tanh OCP (seed 1, T=2), Nint=4096 uniform intervals, m=5
-Lobatto IIIA stages, FP64, from the plain all-ones initial guess with the
-two-step homotopy in the tanh scale (0 -> 0.5).  One step = one complete
-solve: restart from the device-resident initial guess, Newton at each
-homotopy stage to ||f_q||_inf < 1e-8, segment-residual acceptance.  A step
-counts only if it converged (residual <= 1e-8 and max segment residual <= 1).
+Default workload (--workload batch): the batched family, 1024 instances of
+the n_x=20, n_u=10 tanh OCP sharing A, B, C (seed 1), each with its own
+x0 ~ U[-1,1]^20, T ~ U[0.5, 2] and uniform mesh of Nint ~ U{256..2048}
+intervals, m=5 Lobatto IIIA stages, FP64.  One step = every instance solved
+once from the plain all-ones guess with the two-step homotopy in the tanh
+scale (0 -> 0.5): the instances are split into chunks that fit one handle,
+and each chunk runs as one batch (every launch covers all its instances).
+A solve counts only if it converged (residual < 1e-8, max segment
+residual <= 1).  Beside it, the plain single-instance solve looped over a
+sample of the same instances.
+
+--workload single: one instance, Nint=4096 (the single-instance
+configuration), one complete solve per step.
 
 Printed: one JSON line with the metric, the roofline of the dominant kernel
 (condensation, timed live with HIP events on the solver's stream) and the CPU
@@ -70,11 +76,11 @@ def dense_condense_flops(ny, nz, np_, m):
     return (2.0 / 3.0) * K ** 3 + 2 * K * K * (ny + np_ + 1) + 2 * m * ny * (ny + np_ + 1)
 
 
-def measured_traffic(nint):
+def measured_traffic(workload):
     """HBM bytes per colloc_condense launch at this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy kernel);
-    None when no measurement exists for this mesh size."""
-    path = os.path.join(ROOT, "profiles", "traffic_tanh20_nint%d_m5.md" % nint)
+    None when no measurement exists for it."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s_m5.md" % workload)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -83,54 +89,195 @@ def measured_traffic(nint):
     return None
 
 
-def cpu_baseline(inst, m, nint_sample):
-    """Oracle (NumPy/LAPACK) plain solve of the same family at a reduced mesh;
+def cpu_baseline(inst, m, nint_sample, nint_target):
+    """Oracle (NumPy/LAPACK) plain solve of a family member at a reduced mesh;
     work per solve is linear in Nint at equal Newton counts, so the rate is
-    scaled by nint_sample / Nint."""
+    scaled by nint_sample / nint_target."""
     if ROOT not in sys.path:
         sys.path.insert(0, ROOT)
     from oracle import families
     from oracle import solver as osolver
-    from colloc_amd import problems
-    small, _ = problems.config(1, nint=nint_sample)
-    prob = families.TanhFamily(small.A, small.B, small.x0, C=small.C, s=small.s, q=small.q, c_u=small.c_u)
-    hom = [families.TanhFamily(small.A, small.B, small.x0, C=small.C, s=0.0, q=small.q, c_u=small.c_u)]
-    N = small.nodes
-    t = np.linspace(0.0, small.T, N)
+    prob = families.TanhFamily(inst.A, inst.B, inst.x0, C=inst.C, s=inst.s, q=inst.q, c_u=inst.c_u)
+    hom = [families.TanhFamily(inst.A, inst.B, inst.x0, C=inst.C, s=0.0, q=inst.q, c_u=inst.c_u)]
+    N = nint_sample + 1
+    t = np.linspace(0.0, inst.T, N)
     ny, nz, np_ = prob.size_y, prob.size_z, prob.size_p
     t0 = time.perf_counter()
     res = osolver.collocation_solver(prob, m, t, np.ones((N, ny)), np.ones((N, nz)), np.ones(np_),
-                                     tol=small.tolerance, homotopy=hom)
+                                     tol=inst.tolerance, homotopy=hom)
     dt = time.perf_counter() - t0
     ok = res["success"] == 1
-    scale = (inst.nodes - 1) / float(nint_sample)
+    scale = nint_target / float(nint_sample)
     return {"value": (1.0 / (dt * scale)) if ok else 0.0, "unit": "solves/s",
             "cores": len(os.sched_getaffinity(0)), "kind": "port",
-            "sample": "NumPy oracle, full plain solve (homotopy 0->0.5) of the same family at Nint=%d "
-                      "(%.1f s), rate scaled by %d/%d to Nint=%d; LAPACK threads = all cores"
-                      % (nint_sample, dt, nint_sample, inst.nodes - 1, inst.nodes - 1),
+            "sample": "NumPy oracle, full plain solve (homotopy 0->0.5) of one instance of the same family at "
+                      "Nint=%d, T=%.3g (%.1f s), rate scaled by %d/%.0f to Nint=%.0f; LAPACK threads = all cores"
+                      % (nint_sample, inst.T, dt, nint_sample, nint_target, nint_target),
             "seconds": dt, "converged": ok}
+
+
+def condense_roofline(S, before, after, cond_ms, cond_n, ny, nz, np_, npa, m, traffic):
+    """Roofline entry of the condensation from the executed-flop counters."""
+    fb = after[0] - before[0]
+    terms = after[1] - before[1]
+    ncond = after[2] - before[2]
+    fl_total = ncond * stage_elim_flops(ny, nz, npa, m) + terms * series_term_flops(ny, npa, m) \
+        + fb * gauss_jordan_flops(ny, npa, m)
+    fl = fl_total / max(cond_n, 1)
+    avg_s = cond_ms / max(cond_n, 1) / 1e3
+    achieved = fl / avg_s / 1e12 if avg_s > 0 else 0.0
+    dense = dense_condense_flops(ny, nz, np_, m) * ncond / (cond_ms / 1e3) / 1e12 if cond_ms > 0 else 0.0
+    return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "kernel": "colloc_condense_ns (+ colloc_condense for intervals the series leaves)",
+            "flops_per_launch": fl, "avg_launch_ms": avg_s * 1e3, "launches": cond_n,
+            "intervals_per_launch": ncond / max(cond_n, 1),
+            "series_terms_per_interval": terms / max(ncond, 1), "gauss_jordan_intervals": fb,
+            "dense_equivalent_tflops": dense}
+
+
+def check_gpus(args):
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the solve shards by "
+              "interval ranges or by instance; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: 2 batch passes / 5 solves)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nint", type=int, default=None, help="override Nint (default: the configuration's 4096)")
+    ap.add_argument("--workload", choices=("batch", "single"), default="batch")
+    ap.add_argument("--nint", type=int, default=None, help="single: override Nint (default 4096)")
+    ap.add_argument("--instances", type=int, default=1024, help="batch: instances of the family")
+    ap.add_argument("--chunk-nodes", type=int, default=320000, help="batch: most nodes solved in one chunk")
+    ap.add_argument("--loop-sample", type=int, default=64,
+                    help="batch: instances solved one by one for the looped plain rate")
     ap.add_argument("--cpu-sample", type=int, default=128, help="Nint of the CPU baseline sample (0: skip)")
     args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (the solve shards by "
-              "interval ranges or by instance; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
+    check_gpus(args)
     load_package()
     from colloc_amd import build as b
     if not os.path.exists(b.LIB_PATH):
         import __graft_entry__
         __graft_entry__.build()
-    from colloc_amd import problems, solver
+    if args.workload == "batch":
+        line = batch_workload(args)
+    else:
+        line = single_workload(args)
+    print(json.dumps(line), flush=True)
 
+
+def batch_workload(args):
+    from colloc_amd import jit, problems, solver
+    steps = args.steps if args.steps is not None else 2
+    insts = problems.batch_family(args.instances)
+    m = 5
+    bvp = solver.BvpDae(insts[0].ocp_text())
+    ny, nz, np_ = bvp.size_y, bvp.size_z, bvp.size_p
+    tol, max_iter = bvp.tolerance, bvp.maximum_newton_iterations
+    nodes = [inst.nodes for inst in insts]
+    chunks = solver.plan_chunks(nodes, args.chunk_nodes, len(insts))
+    S = solver.CollocationSolver(bvp, m, n_max=max(sum(nodes[i] for i in ch) for ch in chunks),
+                                 batch=max(len(ch) for ch in chunks))
+    prep = []
+    for ch in chunks:
+        ts = [np.linspace(0.0, insts[i].T, insts[i].nodes) for i in ch]
+        N = sum(len(t) for t in ts)
+        prep.append((ch, ts, np.ones((N, ny)), np.ones((N, nz)), np.ones((len(ch), np_)),
+                     np.array([insts[i].constant_values() for i in ch]),
+                     [np.array([insts[i].constant_values(s=0.0) for i in ch])]))
+
+    def batch_pass():
+        conv = np.zeros(len(insts), dtype=bool)
+        its = np.zeros((2, len(insts)), dtype=int)
+        for ch, ts, Y, Z, P, C, H in prep:
+            S.set_batch_mesh(ts)
+            S.form_initial_input(Y, Z, P)
+            r = solver.newton_stages(S, C, H, 0.1, tol, max_iter)
+            conv[ch] = r["converged"]
+            its[:, ch] = r["iterations"]
+        return conv, its
+
+    for _ in range(args.warmup):
+        batch_pass()
+    S.profile(True)
+    st0 = S.condense_stats()
+    t0 = time.perf_counter()
+    results = [batch_pass() for _ in range(steps)]
+    elapsed = time.perf_counter() - t0
+    st1 = S.condense_stats()
+    prof = {k: S.profile_get(k) for k in ("residual", "jacobian", "condense", "babd", "recover")}
+    S.profile(False)
+    converged = int(sum(r[0].sum() for r in results))
+    value = converged / elapsed
+    conv, its = results[-1]
+    npa = len(jit.unit_info(bvp.ocp_text, m)["active_params"])
+    roof = condense_roofline(S, st0, st1, prof["condense"][0], prof["condense"][1], ny, nz, np_, npa, m,
+                             measured_traffic("batch1024"))
+    S.close()
+
+    # the plain single-instance solve looped over a sample of the same instances
+    S1 = solver.CollocationSolver(bvp, m, n_max=max(nodes), batch=1)
+    sample = list(range(min(args.loop_sample, len(insts))))
+
+    def loop_pass():
+        ok = 0
+        for i in sample:
+            inst = insts[i]
+            t = np.linspace(0.0, inst.T, inst.nodes)
+            r = solver.solve_batch(S1, [t], [np.ones((len(t), ny))], [np.ones((len(t), nz))], [np.ones(np_)],
+                                   np.array([inst.constant_values()]),
+                                   [np.array([inst.constant_values(s=0.0)])], tol=tol, max_iter=max_iter)
+            ok += int(r["converged"][0])
+        return ok
+    t0 = time.perf_counter()
+    loop_ok = loop_pass()
+    loop_s = time.perf_counter() - t0
+    S1.close()
+    loop_rate = loop_ok / loop_s
+    sample_conv = int(conv[sample].sum())
+    mean_nint = float(np.mean([n - 1 for n in nodes]))
+    print("batch: %d chunks, converged %d/%d per pass, Newton iterations (homotopy, final) mean %s max %s"
+          % (len(chunks), conv.sum(), len(insts), its.mean(axis=1).round(2).tolist(), its.max(axis=1).tolist()),
+          file=sys.stderr)
+    print("kernel totals over %d passes (ms, launches): %s" % (steps, prof), file=sys.stderr)
+    print("looped plain solve: %d instances in %.2f s (%d converged; batch converged %d of them)"
+          % (len(sample), loop_s, loop_ok, sample_conv), file=sys.stderr)
+    line = {
+        "metric": "converged BVP solves/s @ residual <=1e-8, batched vs plain FP64, 1 GPU",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded OCP family, generated in-process)",
+        "config": {"workload": "batched family: %d instances of the tanh OCP n_x=20 n_u=10 (ny=40 nz=10 np=40), "
+                               "shared A,B,C (seed 1), x0~U[-1,1]^20, T~U[0.5,2], Nint~U{256..2048}, m=%d; "
+                               "plain all-ones guess + homotopy S: 0->0.5; %d chunk(s) of <= %d nodes"
+                               % (len(insts), m, len(chunks), args.chunk_nodes),
+                   "instances": len(insts), "mean_nint": mean_nint, "m": m, "tolerance": tol,
+                   "converged_per_pass": int(conv.sum()), "chunks": len(chunks),
+                   "newton_iterations_mean": its.mean(axis=1).tolist(),
+                   "looped_plain_solves_per_s": loop_rate, "looped_sample": len(sample),
+                   "batched_over_looped": value / loop_rate if loop_rate > 0 else None,
+                   "pcie_included": "per chunk: mesh and all-ones initial guess copied in (inside the timed step)"},
+        "roofline": roof,
+    }
+    if args.cpu_sample > 0:
+        line["cpu_baseline"] = cpu_baseline(insts[0], m, args.cpu_sample, mean_nint)
+    return line
+
+
+def single_workload(args):
+    from colloc_amd import jit, problems, solver
+    steps = args.steps if args.steps is not None else 5
     inst, m = problems.config(1, nint=args.nint)
     bvp = solver.BvpDae(inst.ocp_text())
     S = solver.CollocationSolver(bvp, m, n_max=bvp.N)
@@ -157,40 +304,31 @@ def main():
     for _ in range(args.warmup):
         one_solve()
     S.profile(True)
-    fb0, terms0 = S.condense_stats()
+    st0 = S.condense_stats()
     t0 = time.perf_counter()
-    results = [one_solve() for _ in range(args.steps)]
+    results = [one_solve() for _ in range(steps)]
     elapsed = time.perf_counter() - t0
-    fb1, terms1 = S.condense_stats()
-    cond_ms, cond_n = S.profile_get("condense")
+    st1 = S.condense_stats()
     prof = {k: S.profile_get(k) for k in ("residual", "jacobian", "condense", "babd", "recover")}
     S.profile(False)
     converged = sum(1 for r in results if r[0])
     value = converged / elapsed
     nint = bvp.N - 1
     ny, nz, np_ = bvp.size_y, bvp.size_z, bvp.size_p
-    from colloc_amd import jit
     npa = len(jit.unit_info(bvp.ocp_text, m)["active_params"])
-    # executed flops of all timed condensation launches: stage eliminations of
-    # every interval, the series terms actually run, Gauss-Jordan for the
-    # intervals the series left
-    fl_total = (cond_n * nint * stage_elim_flops(ny, nz, npa, m) + (terms1 - terms0) * series_term_flops(ny, npa, m)
-                + (fb1 - fb0) * gauss_jordan_flops(ny, npa, m))
-    fl = fl_total / max(cond_n, 1)
-    terms_per_interval = (terms1 - terms0) / max(cond_n * nint, 1)
-    avg_s = cond_ms / max(cond_n, 1) / 1e3
-    achieved = fl / avg_s / 1e12 if avg_s > 0 else 0.0
+    roof = condense_roofline(S, st0, st1, prof["condense"][0], prof["condense"][1], ny, nz, np_, npa, m,
+                             measured_traffic("tanh20_nint%d" % nint))
     print("per-solve Newton iterations %s, final residual %.2e, max segment residual %.3g"
           % (results[-1][1], results[-1][2], results[-1][3]), file=sys.stderr)
-    print("kernel totals over %d solves (ms, launches): %s" % (args.steps, prof), file=sys.stderr)
+    print("kernel totals over %d solves (ms, launches): %s" % (steps, prof), file=sys.stderr)
     line = {
         "metric": "converged BVP solves/s @ residual <=1e-8 (plain FP64 path, 1 GPU)",
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": elapsed / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -201,18 +339,12 @@ def main():
                    "nint": nint, "m": m, "ny": ny, "nz": nz, "np": np_, "np_in_stage_functions": npa,
                    "tolerance": tol,
                    "converged_steps": converged, "newton_iterations": results[-1][1]},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(nint),
-                     "kernel": "colloc_condense_ns (+ colloc_condense for intervals the series leaves)",
-                     "flops_per_launch": fl, "avg_launch_ms": avg_s * 1e3, "launches": cond_n,
-                     "series_terms_per_interval": terms_per_interval, "gauss_jordan_intervals": fb1 - fb0,
-                     "dense_equivalent_tflops": dense_condense_flops(ny, nz, np_, m) * nint / avg_s / 1e12
-                     if avg_s > 0 else 0.0},
+        "roofline": roof,
     }
     if args.cpu_sample > 0:
-        line["cpu_baseline"] = cpu_baseline(inst, m, args.cpu_sample)
+        line["cpu_baseline"] = cpu_baseline(inst, m, args.cpu_sample, nint)
     S.close()
-    print(json.dumps(line), flush=True)
+    return line
 
 
 if __name__ == "__main__":

@@ -119,6 +119,7 @@ struct cs_handle {
   std::vector<size_t> lev_sbase;      // offset of the level in cr_seg
   unsigned long long cond_fallback = 0;   // intervals condensed by Gauss-Jordan since creation
   unsigned long long cond_terms = 0;      // series terms summed over intervals since creation
+  unsigned long long cond_intervals = 0;  // intervals condensed since creation
   // [0, bmax): residual norms; [bmax, 2 bmax): segment-residual maxima;
   // [2 bmax, 2 bmax + 2): condensation counters
   unsigned long long* scal = nullptr;
@@ -272,8 +273,9 @@ int jacobian(cs_handle* h, double alpha, const int* mask) {
 // converges as a Neumann series (fine meshes) and counts the others; when
 // there are any, the Gauss-Jordan kernel redoes them (a block per interval,
 // the converged ones exit at once).
-int condense(cs_handle* h, const int* mask) {
+int condense(cs_handle* h, const int* mask, long long ncond) {
   ProfScope ps(h, CS_KERNEL_CONDENSE);
+  h->cond_intervals += (unsigned long long)ncond;
   int nint = h->nint;
   unsigned long long* counters = h->scal + 2 * h->bmax;   // [intervals left to Gauss-Jordan, series terms]
   TRY(fill_u64(h, counters, 2));
@@ -662,6 +664,20 @@ int cs_form_initial_input(cs_handle* h, const double* y0, const double* z0, cons
   return init_from_saved(h);
 }
 
+int cs_form_initial_input_device(cs_handle* h, const double* y0, const double* z0, const double* p0) {
+  if (!h || !y0 || (!z0 && h->nz) || (!p0 && h->np)) return fail(CS_E_ARG, "null argument");
+  TRY(need_mesh(h));
+  const size_t N = h->N;
+  HIPCHK(hipMemcpyAsync(h->y0buf, y0, N * h->ny * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  if (h->nz)
+    HIPCHK(hipMemcpyAsync(h->y0buf + N * h->ny, z0, N * h->nz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  if (h->np)
+    HIPCHK(hipMemcpyAsync(h->y0buf + N * (h->ny + h->nz), p0, (size_t)h->B * h->np * sizeof(double),
+                          hipMemcpyDeviceToDevice, h->stream));
+  h->init_N = h->N;
+  return init_from_saved(h);
+}
+
 int cs_restart(cs_handle* h) {
   if (!h) return fail(CS_E_ARG, "null handle");
   if (h->init_N != h->N || h->N < 2) return fail(CS_E_ARG, "no initial input for the current mesh");
@@ -723,7 +739,12 @@ int init_from_saved(cs_handle* h) {
 // takes the first step 2^-k (k < 20) that strictly decreases its own norm.
 // The kernels of an iteration run over the instances still iterating (act),
 // the line-search residuals over those still searching (srch); a step is
-// accepted by adding it to the state of its instance only.
+// accepted by adding it to the state of its instance only.  A line search
+// that accepts no step leaves the instance's state as it was, and nothing
+// else changes between iterations (alpha, mesh and constants are fixed), so
+// every later iteration of that instance would repeat this one exactly: the
+// instance stops here with the outcome those iterations would give (loop
+// counter max_iter - 1, the same norm, the norm history continued).
 int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time, double* norm, double* hist,
            int hist_len) {
   const int B = h->B;
@@ -748,7 +769,10 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
     if (nact == 0) break;
     TRY(upload_ints(h, h->act, active));
     TRY(jacobian(h, alpha, h->act));
-    TRY(condense(h, h->act));
+    long long ncond = 0;
+    for (int b = 0; b < B; ++b)
+      if (active[b]) ncond += h->hnoff[b + 1] - h->hnoff[b] - 1;
+    TRY(condense(h, h->act, ncond));
     TRY(babd(h, h->act));
     TRY(recover(h, h->act));
     for (int b = 0; b < B; ++b) {
@@ -780,6 +804,12 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
     if (any_taken) {
       TRY(upload_doubles(h, h->steps, taken));
       TRY(axpy_state(h, h->state, h->dir, h->state));
+    }
+    for (int b = 0; b < B; ++b) {
+      if (!active[b] || taken[b] != 0.0) continue;
+      active[b] = 0;   // stalled: done[b] stays -1
+      if (hist)
+        for (int k = it + 1; k < std::min(max_iter, hist_len); ++k) hist[(size_t)b * hist_len + k] = nrm[b];
     }
   }
   for (int b = 0; b < B; ++b) {
@@ -830,7 +860,7 @@ int cs_construct_jacobian(cs_handle* h, double alpha, double* Df, double* Dg, do
 int cs_reduce_jacobian(cs_handle* h, double* A, double* H, double* b) {
   if (!h) return fail(CS_E_ARG, "null handle");
   TRY(need_mesh(h));
-  TRY(condense(h, nullptr));
+  TRY(condense(h, nullptr, h->nint));
   if (A || H || b) {
     const size_t nint = h->nint;
     std::vector<double> buf(nint * h->ny * h->crw);
@@ -941,8 +971,10 @@ int cs_profile_get(cs_handle* h, int kernel, double* total_ms, int* launches) {
   return CS_OK;
 }
 
-int cs_condense_stats(cs_handle* h, long long* fallback_intervals, long long* series_terms) {
+int cs_condense_stats(cs_handle* h, long long* fallback_intervals, long long* series_terms,
+                      long long* intervals) {
   if (!h) return fail(CS_E_ARG, "null handle");
+  if (intervals) *intervals = (long long)h->cond_intervals;
   if (fallback_intervals) *fallback_intervals = (long long)h->cond_fallback;
   if (series_terms) *series_terms = (long long)h->cond_terms;
   return CS_OK;

@@ -1,0 +1,398 @@
+"""Learned warm start: a network maps instance parameters to the solution.
+
+The reference solver starts Newton from the problem's initial guess (all
+ones, OCP2ABVP.py:392-398) and, for the families here, needs a
+continuation in the tanh scale to converge at all.  This module trains a
+multilayer perceptron on solutions the solver itself produced:
+
+  input   (x0, T) of an instance, scaled to [-1, 1]   (21 values)
+  hidden  4 layers of width 1024, GELU
+  output  y and z at 65 uniform normalised times      (65 x 50 values),
+          standardised per output by the training set's mean and spread
+
+Weights and activations are bf16 and every product accumulates in fp32 on
+the matrix cores; the forward, backward and Adam kernels are this
+project's own (libcolloc_mlp.so, include/colloc_mlp.h).  PyTorch is used
+only to hold device memory and to order work on its current stream.
+
+A warm-started solve interpolates the network's grid solution onto each
+instance's mesh on the GPU (cubic Lagrange), sets the boundary multipliers
+from it (kappa_i = -lambda(0), kappa_f = lambda(T), from r = 0 of the
+family's boundary conditions) and runs Newton at the instance's own
+constants.  Instances whose Newton solve does not converge fall back to the
+plain guess with continuation (solver.solve_batch), which is also the judge
+of the network's error: both must reach the same solution.
+"""
+import ctypes
+import os
+import time
+
+import numpy as np
+
+from . import build, native, solver
+
+GRID = 65   # normalised-time grid of the network's output
+
+
+# ------------------------------------------------------------- native layer
+_mlp = None
+P = ctypes.c_void_p
+I = ctypes.c_int
+F = ctypes.c_float
+LL = ctypes.c_longlong
+MLP_SIGNATURES = {
+    "cm_last_error": (ctypes.c_char_p, []),
+    "cm_gemm_nt": (I, [I, I, I, P, I, P, I, P, P, I, I, P, I, P, I, P]),
+    "cm_transpose_bf16": (I, [I, I, P, I, P, I, P]),
+    "cm_col_sum": (I, [I, I, P, I, P, P]),
+    "cm_mse_grad": (I, [I, I, I, P, I, P, I, F, P, P, I, P, P]),
+    "cm_adam": (I, [LL, P, P, P, P, F, F, F, F, I, P, P, I, I, P]),
+    "cm_cast_bf16": (I, [LL, P, P, P]),
+    "cm_grid_to_mesh": (I, [I, P, I, P, I, I, I, P, I, P, P, P, P, P]),
+}
+ACT_NONE, ACT_GELU, ACT_GELU_BWD = 0, 1, 2
+
+
+def mlp_lib():
+    """libcolloc_mlp.so with its ctypes signatures; raises if it is not built."""
+    global _mlp
+    if _mlp is None:
+        if not os.path.exists(build.MLP_LIB_PATH):
+            raise ImportError("%s is missing: build it with __graft_entry__.build()" % build.MLP_LIB_PATH)
+        L = ctypes.CDLL(build.MLP_LIB_PATH)
+        for name, (res, args) in MLP_SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _mlp = L
+    return _mlp
+
+
+def _chk(rc):
+    if rc != 0:
+        raise RuntimeError("colloc_mlp error %d: %s" % (rc, mlp_lib().cm_last_error().decode()))
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("the learned warm start runs on the GPU (no device visible)")
+    return torch
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _up(n, k):
+    return (n + k - 1) // k * k
+
+
+def gemm(A, B, M, N, K, bias=None, aux=None, act=ACT_NONE, Cf=None, Cb=None):
+    """acc = A[M,K] B[N,K]^T (+bias) with the epilogue act (see colloc_mlp.h)."""
+    _chk(mlp_lib().cm_gemm_nt(M, N, K, _p(A), A.stride(0), _p(B), B.stride(0), _p(bias), _p(aux),
+                              aux.stride(0) if aux is not None else 0, act, _p(Cf),
+                              Cf.stride(0) if Cf is not None else 0, _p(Cb), Cb.stride(0) if Cb is not None else 0,
+                              _stream()))
+
+
+def transpose(x, out):
+    R, C = x.shape
+    _chk(mlp_lib().cm_transpose_bf16(R, C, _p(x), x.stride(0), _p(out), out.stride(0), _stream()))
+
+
+# ------------------------------------------------------------------ network
+class WarmStartNet:
+    """The perceptron, its fp32 master weights, bf16 copies and Adam state.
+
+    sizes: [n_in, hidden..., n_out]; the input is zero-padded to a multiple
+    of 32 features and the output to a multiple of 64 (the padding weights
+    stay zero: their gradients are zero)."""
+
+    def __init__(self, n_in, n_out, width=1024, depth=4, seed=0):
+        torch = _torch()
+        self.n_in, self.n_out = n_in, n_out
+        self.k_in = _up(n_in, 32)
+        self.n_outp = _up(n_out, 64)
+        dims = [self.k_in] + [width] * depth + [self.n_outp]
+        real_in = [n_in] + [width] * depth
+        real_out = [width] * depth + [n_out]
+        rng = np.random.default_rng(seed)
+        dev = "cuda"
+        self.W, self.b, self.Wb, self.WTb, self.m, self.v = [], [], [], [], [], []
+        for l in range(len(dims) - 1):
+            fi, fo = dims[l], dims[l + 1]
+            w = np.zeros((fo, fi), dtype=np.float32)
+            gain = np.sqrt(2.0 / real_in[l]) if l < len(dims) - 2 else np.sqrt(1.0 / real_in[l])
+            w[:real_out[l], :real_in[l]] = rng.normal(0.0, gain, (real_out[l], real_in[l]))
+            W = torch.tensor(w, device=dev)
+            self.W.append(W)
+            self.b.append(torch.zeros(fo, device=dev))
+            self.Wb.append(torch.empty((fo, fi), dtype=torch.bfloat16, device=dev))
+            self.WTb.append(torch.empty((fi, fo), dtype=torch.bfloat16, device=dev))
+            self.m.append([torch.zeros_like(W), torch.zeros(fo, device=dev)])
+            self.v.append([torch.zeros_like(W), torch.zeros(fo, device=dev)])
+        self.dims = dims
+        self.step = 0
+        self._bufs = {}
+        self.refresh_copies()
+
+    def refresh_copies(self):
+        for W, Wb, WTb in zip(self.W, self.Wb, self.WTb):
+            _chk(mlp_lib().cm_cast_bf16(W.numel(), _p(W), _p(Wb), _stream()))
+            transpose(Wb, WTb)
+
+    def _buffers(self, bt):
+        if bt in self._bufs:
+            return self._bufs[bt]
+        torch = _torch()
+        dev = "cuda"
+        L = len(self.dims) - 1
+        bf = torch.bfloat16
+        b = {"X": torch.zeros((bt, self.k_in), dtype=bf, device=dev),
+             "Z": [torch.empty((bt, self.dims[l + 1]), device=dev) for l in range(L)],
+             "H": [torch.empty((bt, self.dims[l + 1]), dtype=bf, device=dev) for l in range(L - 1)],
+             "dZf": [torch.empty((bt, self.dims[l + 1]), device=dev) for l in range(L)],
+             "dZb": [torch.empty((bt, self.dims[l + 1]), dtype=bf, device=dev) for l in range(L)],
+             "dZT": [torch.empty((self.dims[l + 1], bt), dtype=bf, device=dev) for l in range(L)],
+             "inT": [torch.empty((self.dims[l], bt), dtype=bf, device=dev) for l in range(L)],
+             "gW": [torch.empty_like(W) for W in self.W],
+             "gb": [torch.empty_like(bb) for bb in self.b],
+             "T": torch.zeros((bt, self.n_outp), device=dev),
+             "loss": torch.zeros(1, dtype=torch.float64, device=dev)}
+        self._bufs[bt] = b
+        return b
+
+    def forward(self, X):
+        """X: (bt, n_in) float32 device tensor -> (bt, n_outp) float32 output
+        (pre-activations of every layer kept for backward)."""
+        bt = X.shape[0]
+        if bt % 32:
+            raise ValueError("batch of %d rows: a multiple of 32 is required" % bt)
+        B = self._buffers(bt)
+        B["X"][:, :self.n_in].copy_(X)
+        h = B["X"]
+        L = len(self.dims) - 1
+        for l in range(L):
+            last = l == L - 1
+            gemm(h, self.Wb[l], bt, self.dims[l + 1], self.dims[l], bias=self.b[l],
+                 act=ACT_NONE if last else ACT_GELU, Cf=B["Z"][l], Cb=None if last else B["H"][l])
+            if not last:
+                h = B["H"][l]
+        return B["Z"][L - 1]
+
+    def gradients(self, X, T, scale=None):
+        """Forward + backward on one batch: fills the gradient buffers and
+        returns (loss tensor (device), buffers)."""
+        bt = X.shape[0]
+        B = self._buffers(bt)
+        Y = self.forward(X)
+        B["T"][:, :self.n_out].copy_(T)
+        B["loss"].zero_()
+        L = len(self.dims) - 1
+        scale = 1.0 / (bt * self.n_out) if scale is None else scale
+        _chk(mlp_lib().cm_mse_grad(bt, self.n_outp, self.n_out, _p(Y), Y.stride(0), _p(B["T"]), B["T"].stride(0),
+                                   scale, _p(B["dZf"][L - 1]), _p(B["dZb"][L - 1]), self.n_outp, _p(B["loss"]),
+                                   _stream()))
+        for l in range(L - 1, -1, -1):
+            fi, fo = self.dims[l], self.dims[l + 1]
+            xin = B["X"] if l == 0 else B["H"][l - 1]
+            transpose(B["dZb"][l], B["dZT"][l])
+            transpose(xin, B["inT"][l])
+            gemm(B["dZT"][l], B["inT"][l], fo, fi, bt, Cf=B["gW"][l])
+            _chk(mlp_lib().cm_col_sum(bt, fo, _p(B["dZf"][l]), fo, _p(B["gb"][l]), _stream()))
+            if l > 0:
+                gemm(B["dZb"][l], self.WTb[l], bt, fi, fo, aux=B["Z"][l - 1], act=ACT_GELU_BWD,
+                     Cf=B["dZf"][l - 1], Cb=B["dZb"][l - 1])
+        self._grads = B
+        return B["loss"], B
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8):
+        """One Adam step with the gradients of the last gradients() call."""
+        self.step += 1
+        B = self._grads
+        for l in range(len(self.W)):
+            W, Wb, WTb = self.W[l], self.Wb[l], self.WTb[l]
+            _chk(mlp_lib().cm_adam(W.numel(), _p(W), _p(B["gW"][l]), _p(self.m[l][0]), _p(self.v[l][0]), lr, b1, b2,
+                                   eps, self.step, _p(Wb), _p(WTb), W.shape[0], W.shape[1], _stream()))
+            bb = self.b[l]
+            _chk(mlp_lib().cm_adam(bb.numel(), _p(bb), _p(B["gb"][l]), _p(self.m[l][1]), _p(self.v[l][1]), lr, b1,
+                                   b2, eps, self.step, None, None, 1, bb.numel(), _stream()))
+
+    def train_step(self, X, T, lr):
+        loss, _ = self.gradients(X, T)
+        self.adam(lr)
+        return loss
+
+    # -- persistence -------------------------------------------------------
+    def state(self):
+        return {"dims": np.array(self.dims), "n_in": self.n_in, "n_out": self.n_out, "step": self.step,
+                **{"W%d" % l: W.cpu().numpy() for l, W in enumerate(self.W)},
+                **{"b%d" % l: b.cpu().numpy() for l, b in enumerate(self.b)}}
+
+    @classmethod
+    def from_state(cls, st):
+        dims = [int(d) for d in st["dims"]]
+        net = cls.__new__(cls)
+        torch = _torch()
+        net.n_in, net.n_out = int(st["n_in"]), int(st["n_out"])
+        net.k_in, net.n_outp = dims[0], dims[-1]
+        net.dims = dims
+        net.step = int(st["step"])
+        net._bufs = {}
+        net.W = [torch.tensor(st["W%d" % l], device="cuda") for l in range(len(dims) - 1)]
+        net.b = [torch.tensor(st["b%d" % l], device="cuda") for l in range(len(dims) - 1)]
+        net.Wb = [torch.empty(W.shape, dtype=torch.bfloat16, device="cuda") for W in net.W]
+        net.WTb = [torch.empty((W.shape[1], W.shape[0]), dtype=torch.bfloat16, device="cuda") for W in net.W]
+        net.m = [[torch.zeros_like(W), torch.zeros_like(b)] for W, b in zip(net.W, net.b)]
+        net.v = [[torch.zeros_like(W), torch.zeros_like(b)] for W, b in zip(net.W, net.b)]
+        net.refresh_copies()
+        return net
+
+
+# ------------------------------------------------------- family interface
+def features(x0, T, T_range=(0.5, 2.0)):
+    """Network input of instances with initial states x0 (B, nx) and
+    horizons T (B,): x0 as is (already in [-1, 1]) and T scaled to [-1, 1]."""
+    x0 = np.atleast_2d(np.asarray(x0, dtype=np.float64))
+    T = np.asarray(T, dtype=np.float64).reshape(-1, 1)
+    t = 2.0 * (T - T_range[0]) / (T_range[1] - T_range[0]) - 1.0
+    return np.hstack([x0, t]).astype(np.float32)
+
+
+def grid_targets(sol, ny, nz):
+    """(GRID * (ny + nz),) target row of a solution on a uniform mesh whose
+    interval count is a multiple of GRID - 1: y and z at the grid nodes,
+    node-major."""
+    y, z = sol["y"], sol["z"]
+    n = y.shape[0] - 1
+    if n % (GRID - 1):
+        raise ValueError("mesh of %d intervals: a multiple of %d is needed" % (n, GRID - 1))
+    step = n // (GRID - 1)
+    return np.hstack([y[::step], z[::step]]).reshape(-1)
+
+
+class WarmStart:
+    """A trained network with its output standardisation, for one family."""
+
+    def __init__(self, net, mean, sd, ny, nz, nx):
+        torch = _torch()
+        self.net = net
+        self.ny, self.nz, self.nx = ny, nz, nx
+        self.mean = torch.tensor(np.asarray(mean, dtype=np.float32), device="cuda")
+        self.sd = torch.tensor(np.asarray(sd, dtype=np.float32), device="cuda")
+
+    def save(self, path):
+        st = self.net.state()
+        np.savez(path, mean=self.mean.cpu().numpy(), sd=self.sd.cpu().numpy(), ny=self.ny, nz=self.nz, nx=self.nx,
+                 **st)
+
+    @classmethod
+    def load(cls, path):
+        st = dict(np.load(path))
+        net = WarmStartNet.from_state(st)
+        return cls(net, st["mean"], st["sd"], int(st["ny"]), int(st["nz"]), int(st["nx"]))
+
+    def predict_grid(self, feats):
+        """Standardised network output (B, n_outp) for a feature matrix; the
+        batch is padded to a multiple of 32 rows."""
+        torch = _torch()
+        B = feats.shape[0]
+        bt = _up(B, 32)
+        X = torch.zeros((bt, feats.shape[1]), device="cuda")
+        X[:B] = torch.from_numpy(feats).cuda()
+        return self.net.forward(X)
+
+    def guess_on_meshes(self, solv, feats):
+        """Initial guess of every instance of solv's current batch (its meshes
+        set) from the network, written with cs_form_initial_input_device."""
+        torch = _torch()
+        B = solv.B
+        G = self.predict_grid(feats)
+        t = torch.from_numpy(np.ascontiguousarray(solv.t_span)).cuda()
+        noff = torch.from_numpy(solv.node_offsets().astype(np.int32)).cuda()
+        N = solv.N
+        Y = torch.empty((N, self.ny), dtype=torch.float64, device="cuda")
+        Z = torch.empty((N, max(self.nz, 1)), dtype=torch.float64, device="cuda")
+        nc = self.ny + self.nz
+        _chk(mlp_lib().cm_grid_to_mesh(B, _p(noff), N, _p(t), GRID, nc, self.ny, _p(G), G.stride(0), _p(self.mean),
+                                       _p(self.sd), _p(Y), _p(Z), _stream()))
+        # boundary multipliers from r = 0: kappa_i = -lambda(0), kappa_f = lambda(T)
+        g = G[:B, :GRID * nc].double() * self.sd.double() + self.mean.double()
+        g = g.view(B, GRID, nc)
+        nx = self.nx
+        Pm = torch.cat([-g[:, 0, nx:2 * nx], g[:, GRID - 1, nx:2 * nx]], dim=1).contiguous()
+        torch.cuda.current_stream().synchronize()
+        native.check(native.lib().cs_form_initial_input_device(solv.h, _p(Y), _p(Z), _p(Pm)))
+        return Y, Z, Pm
+
+
+# ------------------------------------------------------ data and training
+def generate_solutions(bvp, m, insts, nint, schedule=(0.0, 0.25), chunk_nodes=320000, log=None):
+    """Solve instances of a batched family on uniform meshes of nint
+    intervals (plain guess + continuation in the tanh scale), batched.
+    Returns (features, targets, mask of the instances whose Newton solve
+    converged).  The mesh-acceptance test is not applied: a training target
+    only needs the collocation solution at the grid nodes, which a coarse
+    mesh already gives far below the network's own error."""
+    ny, nz, np_ = bvp.size_y, bvp.size_z, bvp.size_p
+    per = max(1, min(len(insts), chunk_nodes // (nint + 1)))
+    S = solver.CollocationSolver(bvp, m, n_max=per * (nint + 1), batch=per)
+    feats = features([i.x0 for i in insts], [i.T for i in insts])
+    targets = np.zeros((len(insts), GRID * (ny + nz)), dtype=np.float32)
+    ok = np.zeros(len(insts), dtype=bool)
+    t0 = time.time()
+    for c0 in range(0, len(insts), per):
+        ch = insts[c0:c0 + per]
+        ts = [np.linspace(0.0, i.T, nint + 1) for i in ch]
+        Nt = len(ch) * (nint + 1)
+        S.set_batch_mesh(ts)
+        S.form_initial_input(np.ones((Nt, ny)), np.ones((Nt, nz)), np.ones((len(ch), np_)))
+        cons = np.array([i.constant_values() for i in ch])
+        hom = [np.array([i.constant_values(s=s) for i in ch]) for s in schedule]
+        r = solver.newton_stages(S, cons, hom, 0.1, bvp.tolerance, bvp.maximum_newton_iterations)
+        sols = solver.batch_solutions(S)
+        conv = r["norm"] < bvp.tolerance
+        for k, s in enumerate(sols):
+            if conv[k]:
+                targets[c0 + k] = grid_targets(s, ny, nz)
+        ok[c0:c0 + per] = conv
+        if log:
+            log("solutions %d/%d: %d converged (%.1f s)" % (c0 + len(ch), len(insts), int(ok[:c0 + per].sum()),
+                                                           time.time() - t0))
+    S.close()
+    return feats, targets, ok
+
+
+def train(feats, targets, epochs=60, batch=512, lr=1e-3, seed=0, width=1024, depth=4, log=None):
+    """Train a WarmStartNet on standardised targets with Adam (cosine-decayed
+    learning rate).  Returns (net, mean, sd, the last epoch's mean squared
+    error of the standardised targets)."""
+    torch = _torch()
+    mean = targets.mean(axis=0)
+    sd = targets.std(axis=0) + 1e-6
+    Tn = ((targets - mean) / sd).astype(np.float32)
+    net = WarmStartNet(feats.shape[1], targets.shape[1], width=width, depth=depth, seed=seed)
+    Xd = torch.from_numpy(feats).cuda()
+    Td = torch.from_numpy(Tn).cuda()
+    n = (feats.shape[0] // batch) * batch
+    rng = np.random.default_rng(seed)
+    steps = epochs * (n // batch)
+    k = 0
+    ep_loss = 0.0
+    for ep in range(epochs):
+        perm = torch.from_numpy(rng.permutation(feats.shape[0])[:n]).cuda()
+        tot = torch.zeros(1, dtype=torch.float64, device="cuda")
+        for s in range(0, n, batch):
+            idx = perm[s:s + batch]
+            lr_k = lr * 0.5 * (1.0 + np.cos(np.pi * k / steps))
+            tot += net.train_step(Xd[idx], Td[idx], lr_k)
+            k += 1
+        ep_loss = float(tot.item()) / (n // batch) / (batch * targets.shape[1])   # mean squared error
+        if log and (ep % 10 == 0 or ep == epochs - 1):
+            log("epoch %d: mean loss %.3e" % (ep, ep_loss))
+    return net, mean, sd, ep_loss

@@ -35,6 +35,7 @@ SIGNATURES = {
     "cs_set_state": (c_int, [c_void_p, P_D, P_D, P_D, P_D]),
     "cs_get_state": (c_int, [c_void_p, P_D, P_D, P_D, P_D]),
     "cs_form_initial_input": (c_int, [c_void_p, P_D, P_D, P_D]),
+    "cs_form_initial_input_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "cs_restart": (c_int, [c_void_p]),
     "cs_compute_f_q": (c_int, [c_void_p, c_double, P_D, P_D, P_D, P_D, P_D]),
     "cs_construct_jacobian": (c_int, [c_void_p, c_double, P_D, P_D, P_D]),
@@ -48,7 +49,8 @@ SIGNATURES = {
     "cs_profile": (c_int, [c_void_p, c_int]),
     "cs_profile_get": (c_int, [c_void_p, c_int, P_D, P_I]),
     "cs_status": (c_int, [c_void_p, P_I]),
-    "cs_condense_stats": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+    "cs_condense_stats": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                                  ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 

@@ -286,12 +286,13 @@ class CollocationSolver:
 
     def condense_stats(self):
         """(intervals condensed by Gauss-Jordan because the series solve did
-        not converge for them, series terms summed over intervals), since the
-        solver was created."""
+        not converge for them, series terms summed over intervals, intervals
+        condensed), since the solver was created."""
         n = native.ctypes.c_longlong()
         k = native.ctypes.c_longlong()
-        check(lib().cs_condense_stats(self.h, native.ctypes.byref(n), native.ctypes.byref(k)))
-        return n.value, k.value
+        c = native.ctypes.c_longlong()
+        check(lib().cs_condense_stats(self.h, native.ctypes.byref(n), native.ctypes.byref(k), native.ctypes.byref(c)))
+        return n.value, k.value, c.value
 
     def condense_fallbacks(self):
         return self.condense_stats()[0]
@@ -417,11 +418,11 @@ def solve_batch(solver, t_spans, Y0s, Z0s, P0s, constants=None, homotopy=(), alp
     solver.set_batch_mesh(t_spans)
     solver.form_initial_input(np.concatenate(Y0s), np.concatenate(Z0s),
                               np.stack([np.asarray(p, dtype=np.float64) for p in P0s]) if B > 1 else P0s[0])
-    return _newton_stages(solver, constants, homotopy, alpha, tol, max_iter, residual_type, scale_by_time,
+    return newton_stages(solver, constants, homotopy, alpha, tol, max_iter, residual_type, scale_by_time,
                           residual2)
 
 
-def _newton_stages(solver, constants, homotopy, alpha, tol, max_iter, residual_type=1, scale_by_time=True,
+def newton_stages(solver, constants, homotopy, alpha, tol, max_iter, residual_type=1, scale_by_time=True,
                    residual2=False):
     B = solver.B
     its = []

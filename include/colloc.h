@@ -116,6 +116,12 @@ int cs_get_state(cs_handle* h, double* y, double* y_dot, double* z_tilde, double
  * interpolated linearly from node values and y_dot = f(y_stage, z_tilde, p). */
 int cs_form_initial_input(cs_handle* h, const double* y0, const double* z0, const double* p0);
 
+/* cs_form_initial_input with y0, z0 and p0 in device memory (for example
+ * an initial guess computed on the GPU, colloc_mlp.h).  The handle's
+ * stream waits for nothing else: the caller makes sure the producer of the
+ * arrays has finished (e.g. hipStreamSynchronize / hipDeviceSynchronize). */
+int cs_form_initial_input_device(cs_handle* h, const double* y0, const double* z0, const double* p0);
+
 /* Re-form the collocation state from the node values, mesh and parameters
  * last passed to cs_form_initial_input (kept in device memory), without any
  * host transfer.  Same result as repeating that call; used to restart a
@@ -156,7 +162,11 @@ int cs_recover_delta_k(cs_handle* h, double* delta_y_dot, double* delta_z_tilde)
  * ||f_q||_inf < tol or max_iter iterations, with its own line search.
  * iter_time[b] receives its loop counter at exit (as the reference's
  * iter_time), norm[b] its last residual norm, norms[b][0..hist_len) its norm
- * history (may be NULL; entries past its last iteration are not written). */
+ * history (may be NULL; entries past its last iteration are not written).
+ * An instance whose line search accepts no step keeps its state, so each of
+ * its later iterations would repeat that one: it stops at once with the
+ * result they would give (iter_time = max_iter - 1, the same norm, the
+ * history continued at that norm). */
 int cs_newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
               double* norm, double* norms, int hist_len);
 
@@ -195,9 +205,12 @@ int cs_profile_get(cs_handle* h, int kernel, double* total_ms, int* launches);
  * stage system and falls back to Gauss-Jordan elimination for intervals
  * where the series does not converge (coarse meshes).  Since the handle was
  * created: *fallback_intervals = intervals condensed by the fallback,
- * *series_terms = series terms summed over all intervals (either may be
- * NULL).  No counterpart in the reference (it always factors W). */
-int cs_condense_stats(cs_handle* h, long long* fallback_intervals, long long* series_terms);
+ * *series_terms = series terms summed over all intervals, *intervals =
+ * intervals condensed (instances that had already converged are skipped).
+ * Any pointer may be NULL.  No counterpart in the reference (it always
+ * factors W). */
+int cs_condense_stats(cs_handle* h, long long* fallback_intervals, long long* series_terms,
+                      long long* intervals);
 
 /* Device status word (bit 0: singular stage block, bit 1: singular stage
  * system, bit 2: singular boundary system); cleared by reading. */
