@@ -154,6 +154,12 @@ def main():
     ap.add_argument("--chunk-nodes", type=int, default=320000, help="batch: most nodes solved in one chunk")
     ap.add_argument("--loop-sample", type=int, default=64,
                     help="batch: instances solved one by one for the looped plain rate")
+    ap.add_argument("--form", choices=("learned", "plain"), default="learned",
+                    help="batch: network warm start (default) or the plain guess with continuation")
+    ap.add_argument("--model", default=None, help="batch, learned: trained model (default: the family's)")
+    ap.add_argument("--plain-pass", type=int, default=1, help="batch, learned: also time one plain batched pass")
+    ap.add_argument("--warm-cap", type=int, default=12,
+                    help="batch, learned: Newton iterations from the network's guess before the fallback")
     ap.add_argument("--cpu-sample", type=int, default=128, help="Nint of the CPU baseline sample (0: skip)")
     args = ap.parse_args()
     check_gpus(args)
@@ -170,6 +176,8 @@ def main():
 
 
 def batch_workload(args):
+    if args.form == "learned":
+        from colloc_amd import learned   # imports torch before libcolloc.so is loaded (shared HIP runtime)
     from colloc_amd import jit, problems, solver
     steps = args.steps if args.steps is not None else 2
     insts = problems.batch_family(args.instances)
@@ -181,72 +189,111 @@ def batch_workload(args):
     chunks = solver.plan_chunks(nodes, args.chunk_nodes, len(insts))
     S = solver.CollocationSolver(bvp, m, n_max=max(sum(nodes[i] for i in ch) for ch in chunks),
                                  batch=max(len(ch) for ch in chunks))
-    prep = []
-    for ch in chunks:
-        ts = [np.linspace(0.0, insts[i].T, insts[i].nodes) for i in ch]
-        N = sum(len(t) for t in ts)
-        prep.append((ch, ts, np.ones((N, ny)), np.ones((N, nz)), np.ones((len(ch), np_)),
-                     np.array([insts[i].constant_values() for i in ch]),
-                     [np.array([insts[i].constant_values(s=0.0) for i in ch])]))
+    ts_all = [np.linspace(0.0, inst.T, inst.nodes) for inst in insts]
+    cons_all = np.array([inst.constant_values() for inst in insts])
+    hom_all = [np.array([inst.constant_values(s=0.0) for inst in insts])]
 
-    def batch_pass():
-        conv = np.zeros(len(insts), dtype=bool)
-        its = np.zeros((2, len(insts)), dtype=int)
-        for ch, ts, Y, Z, P, C, H in prep:
-            S.set_batch_mesh(ts)
-            S.form_initial_input(Y, Z, P)
-            r = solver.newton_stages(S, C, H, 0.1, tol, max_iter)
+    def plain_batch(idx):
+        """Plain all-ones guess + continuation for the instances idx, chunked."""
+        conv = np.zeros(len(idx), dtype=bool)
+        its = np.zeros((2, len(idx)), dtype=int)
+        for ch in solver.plan_chunks([nodes[i] for i in idx], S.n_max, S.batch):
+            sel = [idx[k] for k in ch]
+            Nt = sum(nodes[i] for i in sel)
+            S.set_batch_mesh([ts_all[i] for i in sel])
+            S.form_initial_input(np.ones((Nt, ny)), np.ones((Nt, nz)), np.ones((len(sel), np_)))
+            r = solver.newton_stages(S, cons_all[sel], [h[sel] for h in hom_all], 0.1, tol, max_iter)
             conv[ch] = r["converged"]
             its[:, ch] = r["iterations"]
         return conv, its
 
+    ws = None
+    model_info = None
+    if args.form == "learned":
+        model = args.model or learned.FAMILY_MODEL
+        if not os.path.exists(model):
+            print("no trained model at %s: training one now (see train_warmstart.py)" % model, file=sys.stderr)
+            learned.fit_family(path=model, log=lambda s: print(s, file=sys.stderr, flush=True))
+        ws = learned.WarmStart.load(model)
+        if os.path.exists(model + ".json"):
+            model_info = json.load(open(model + ".json"))
+        feats_all = learned.features([inst.x0 for inst in insts], [inst.T for inst in insts])
+
+    def learned_pass():
+        """Network guess + Newton at the final constants, batched; the
+        instances whose Newton solve does not converge go to plain_batch."""
+        conv = np.zeros(len(insts), dtype=bool)
+        its = np.zeros(len(insts), dtype=int)
+        newton_ok = np.zeros(len(insts), dtype=bool)
+        for ch in chunks:
+            r = learned.warm_solve(ws, S, [ts_all[i] for i in ch], feats_all[ch], cons_all[ch], hom_all[0][ch],
+                                   tol=tol, max_iter=max_iter, max_iter_warm=args.warm_cap)
+            conv[ch] = r["converged"]
+            its[ch] = r["iterations"][0] + r["lq_iterations"]
+            newton_ok[ch] = r["norm"] < tol
+        back = [i for i in range(len(insts)) if not newton_ok[i]]
+        if back:
+            c2, _ = plain_batch(back)
+            conv[back] = c2
+        return conv, its, len(back)
+
+    def plain_pass():
+        conv, its = plain_batch(list(range(len(insts))))
+        return conv, its, 0
+
+    one_pass = learned_pass if args.form == "learned" else plain_pass
     for _ in range(args.warmup):
-        batch_pass()
+        one_pass()
     S.profile(True)
     st0 = S.condense_stats()
     t0 = time.perf_counter()
-    results = [batch_pass() for _ in range(steps)]
+    results = [one_pass() for _ in range(steps)]
     elapsed = time.perf_counter() - t0
     st1 = S.condense_stats()
     prof = {k: S.profile_get(k) for k in ("residual", "jacobian", "condense", "babd", "recover")}
     S.profile(False)
     converged = int(sum(r[0].sum() for r in results))
     value = converged / elapsed
-    conv, its = results[-1]
+    conv, its, n_back = results[-1]
     npa = len(jit.unit_info(bvp.ocp_text, m)["active_params"])
     roof = condense_roofline(S, st0, st1, prof["condense"][0], prof["condense"][1], ny, nz, np_, npa, m,
-                             measured_traffic("batch1024"))
+                             measured_traffic("batch%d_%s" % (len(insts), args.form)))
+    extra = {}
+    if args.form == "learned" and args.plain_pass:
+        # the plain batched form on the same instances, for the split of the gain
+        t0 = time.perf_counter()
+        pconv, pits, _ = plain_pass()
+        pt = time.perf_counter() - t0
+        extra = {"plain_batched_solves_per_s": pconv.sum() / pt, "plain_batched_converged": int(pconv.sum()),
+                 "plain_newton_iterations_mean": pits.sum(axis=0).mean()}
     S.close()
 
     # the plain single-instance solve looped over a sample of the same instances
     S1 = solver.CollocationSolver(bvp, m, n_max=max(nodes), batch=1)
     sample = list(range(min(args.loop_sample, len(insts))))
-
-    def loop_pass():
-        ok = 0
-        for i in sample:
-            inst = insts[i]
-            t = np.linspace(0.0, inst.T, inst.nodes)
-            r = solver.solve_batch(S1, [t], [np.ones((len(t), ny))], [np.ones((len(t), nz))], [np.ones(np_)],
-                                   np.array([inst.constant_values()]),
-                                   [np.array([inst.constant_values(s=0.0)])], tol=tol, max_iter=max_iter)
-            ok += int(r["converged"][0])
-        return ok
     t0 = time.perf_counter()
-    loop_ok = loop_pass()
+    loop_ok = 0
+    for i in sample:
+        r = solver.solve_batch(S1, [ts_all[i]], [np.ones((nodes[i], ny))], [np.ones((nodes[i], nz))], [np.ones(np_)],
+                               cons_all[i:i + 1], [h[i:i + 1] for h in hom_all], tol=tol, max_iter=max_iter)
+        loop_ok += int(r["converged"][0])
     loop_s = time.perf_counter() - t0
     S1.close()
     loop_rate = loop_ok / loop_s
-    sample_conv = int(conv[sample].sum())
     mean_nint = float(np.mean([n - 1 for n in nodes]))
-    print("batch: %d chunks, converged %d/%d per pass, Newton iterations (homotopy, final) mean %s max %s"
-          % (len(chunks), conv.sum(), len(insts), its.mean(axis=1).round(2).tolist(), its.max(axis=1).tolist()),
-          file=sys.stderr)
+    it_mean = np.asarray(its).reshape(-1, len(insts)).sum(axis=0).mean()
+    print("batch (%s): %d chunks, converged %d/%d per pass, Newton iterations mean %.2f, fallback to the plain "
+          "path %d" % (args.form, len(chunks), conv.sum(), len(insts), it_mean, n_back), file=sys.stderr)
     print("kernel totals over %d passes (ms, launches): %s" % (steps, prof), file=sys.stderr)
-    print("looped plain solve: %d instances in %.2f s (%d converged; batch converged %d of them)"
-          % (len(sample), loop_s, loop_ok, sample_conv), file=sys.stderr)
+    print("looped plain solve: %d instances in %.2f s (%d converged)" % (len(sample), loop_s, loop_ok),
+          file=sys.stderr)
+    form_txt = ("LQ solve (1 Newton iteration) + network correction (MLP 21->4x1024->3250, bf16, trained on "
+                "solver solutions) + at most %d Newton iterations at the final constants, plain guess + homotopy as "
+                "the fallback" % args.warm_cap if args.form == "learned"
+                else "plain all-ones guess + homotopy S: 0->0.5")
     line = {
-        "metric": "converged BVP solves/s @ residual <=1e-8, batched vs plain FP64, 1 GPU",
+        "metric": "converged BVP solves/s @ residual <=1e-8, %s vs plain FP64, 1 GPU"
+                  % ("learned+batched" if args.form == "learned" else "batched"),
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -259,15 +306,17 @@ def batch_workload(args):
         "dtype": "f64",
         "data": "synthetic (seeded OCP family, generated in-process)",
         "config": {"workload": "batched family: %d instances of the tanh OCP n_x=20 n_u=10 (ny=40 nz=10 np=40), "
-                               "shared A,B,C (seed 1), x0~U[-1,1]^20, T~U[0.5,2], Nint~U{256..2048}, m=%d; "
-                               "plain all-ones guess + homotopy S: 0->0.5; %d chunk(s) of <= %d nodes"
-                               % (len(insts), m, len(chunks), args.chunk_nodes),
-                   "instances": len(insts), "mean_nint": mean_nint, "m": m, "tolerance": tol,
-                   "converged_per_pass": int(conv.sum()), "chunks": len(chunks),
-                   "newton_iterations_mean": its.mean(axis=1).tolist(),
+                               "shared A,B,C (seed 1), x0~U[-1,1]^20, T~U[0.5,2], Nint~U{256..2048}, m=%d; %s; "
+                               "%d chunk(s) of <= %d nodes" % (len(insts), m, form_txt, len(chunks), args.chunk_nodes),
+                   "form": args.form, "instances": len(insts), "mean_nint": mean_nint, "m": m, "tolerance": tol,
+                   "converged_per_pass": int(conv.sum()), "fallback_per_pass": n_back, "chunks": len(chunks),
+                   "newton_iterations_mean": float(it_mean),
                    "looped_plain_solves_per_s": loop_rate, "looped_sample": len(sample),
-                   "batched_over_looped": value / loop_rate if loop_rate > 0 else None,
-                   "pcie_included": "per chunk: mesh and all-ones initial guess copied in (inside the timed step)"},
+                   "looped_sample_converged": loop_ok,
+                   "vs_looped_plain": value / loop_rate if loop_rate > 0 else None,
+                   "model": model_info,
+                   "pcie_included": "per chunk: mesh (and for the plain form the all-ones guess) copied in, "
+                                    "network input copied in (inside the timed step)", **extra},
         "roofline": roof,
     }
     if args.cpu_sample > 0:

@@ -945,6 +945,21 @@ int cs_recover_solution(cs_handle* h, double* z) {
   return sync(h);
 }
 
+int cs_node_solution_device(cs_handle* h, const double** y, const double** z, const double** p) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  int n = h->N;
+  double* zt = h->zt();
+  Batch bt = h->batch(nullptr);
+  void* args[] = {&n, &bt, &zt, &h->zbuf};
+  TRY(launch(h, F_RECZ, (unsigned)((n + 255) / 256), 256, args));
+  TRY(sync(h));
+  if (y) *y = h->y();
+  if (z) *z = h->zbuf;
+  if (p) *p = h->p();
+  return CS_OK;
+}
+
 int cs_batch_info(cs_handle* h, int* B, int* N) {
   if (!h) return fail(CS_E_ARG, "null handle");
   if (B) *B = h->B;

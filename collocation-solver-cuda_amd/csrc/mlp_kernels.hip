@@ -218,12 +218,14 @@ __global__ __launch_bounds__(256) void mlp_cast_bf16(long long n, const float* _
 // G[b][k * nc + c] (row stride ldg); value = mean[k nc + c] + sd[k nc + c] G.
 // Node n of instance b (nodes noff[b] .. noff[b+1]-1, times t) gets the
 // cubic Lagrange interpolant through the four grid nodes around its
-// normalised time; components c < ny go to Y[n][c], the rest to Z[n][c-ny].
+// normalised time; components c < ny go to Y[n][c], the rest to Z[n][c-ny],
+// each added to the base value Yb / Zb of the node when those are given.
 __global__ __launch_bounds__(256) void mlp_grid_to_mesh(int B, const int* __restrict__ noff,
                                                         const double* __restrict__ t, int ng, int nc, int ny,
                                                         const float* __restrict__ G, int ldg,
                                                         const float* __restrict__ mean, const float* __restrict__ sd,
-                                                        double* __restrict__ Y, double* __restrict__ Z) {
+                                                        const double* Yb, const double* Zb, double* Y,
+                                                        double* Z) {   // Yb may alias Y, Zb Z
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= noff[B]) return;
   int lo = 0, hi = B - 1;   // instance of node n: noff[b] <= n < noff[b+1]
@@ -255,8 +257,8 @@ __global__ __launch_bounds__(256) void mlp_grid_to_mesh(int B, const int* __rest
       const int idx = (k0 + a) * nc + c;
       v += w[a] * ((double)mean[idx] + (double)sd[idx] * (double)g[idx]);
     }
-    if (c < ny) Y[(size_t)n * ny + c] = v;
-    else Z[(size_t)n * nz + (c - ny)] = v;
+    if (c < ny) Y[(size_t)n * ny + c] = v + (Yb ? Yb[(size_t)n * ny + c] : 0.0);
+    else Z[(size_t)n * nz + (c - ny)] = v + (Zb ? Zb[(size_t)n * nz + (c - ny)] : 0.0);
   }
 }
 
@@ -328,12 +330,13 @@ int cm_cast_bf16(long long n, const float* x, uint16_t* y, void* stream) {
 }
 
 int cm_grid_to_mesh(int B, const int* noff, int n_nodes, const double* t, int ng, int nc, int ny, const float* G,
-                    int ldg, const float* mean, const float* sd, double* Y, double* Z, void* stream) {
+                    int ldg, const float* mean, const float* sd, const double* Yb, const double* Zb, double* Y,
+                    double* Z, void* stream) {
   if (B <= 0 || n_nodes <= 0 || ng < 4 || nc <= 0 || ny < 0 || ny > nc || ldg < ng * nc || !noff || !t || !G ||
       !mean || !sd || !Y || (ny < nc && !Z))
     return fail(CM_E_ARG, "bad grid_to_mesh arguments");
   hipLaunchKernelGGL(mlp_grid_to_mesh, dim3(blocks(n_nodes, 256)), dim3(256), 0, (hipStream_t)stream, B, noff, t, ng,
-                     nc, ny, G, ldg, mean, sd, Y, Z);
+                     nc, ny, G, ldg, mean, sd, Yb, Zb, Y, Z);
   LAUNCHCHK();
   return CM_OK;
 }

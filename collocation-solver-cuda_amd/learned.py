@@ -15,19 +15,33 @@ the matrix cores; the forward, backward and Adam kernels are this
 project's own (libcolloc_mlp.so, include/colloc_mlp.h).  PyTorch is used
 only to hold device memory and to order work on its current stream.
 
-A warm-started solve interpolates the network's grid solution onto each
-instance's mesh on the GPU (cubic Lagrange), sets the boundary multipliers
-from it (kappa_i = -lambda(0), kappa_f = lambda(T), from r = 0 of the
-family's boundary conditions) and runs Newton at the instance's own
-constants.  Instances whose Newton solve does not converge fall back to the
-plain guess with continuation (solver.solve_batch), which is also the judge
-of the network's error: both must reach the same solution.
+The network predicts the solution relative to a base: with base "lq" it
+learns the difference between the instance's solution and the solution of
+its linear-quadratic problem (tanh scale 0), which one Newton iteration
+finds exactly from any guess (the first stage of the plain continuation).
+The base carries the large multipliers of short horizons exactly, so the
+network only models the nonlinear part.
+
+A warm-started solve (warm_solve) runs that one LQ iteration, interpolates
+the network's grid output onto each instance's mesh on the GPU (cubic
+Lagrange) and adds it to the LQ solution at the nodes, sets the boundary
+multipliers from r = 0 of the family's boundary conditions (kappa_i =
+-lambda(0), kappa_f = lambda(T)), and runs Newton at the instance's own
+constants, for at most a few iterations.  Instances whose Newton solve
+does not converge fall back to the plain guess with continuation
+(solver.solve_batch), which is also the judge of the network's error: both
+must reach the same solution.
 """
 import ctypes
 import os
 import time
 
 import numpy as np
+# PyTorch-ROCm ships its own HIP runtime under the same soname as the
+# system's (libamdhip64.so.7): the first one loaded serves the process, and
+# torch only works with its own.  Importing torch here, before this
+# package's libraries are loaded, lets both share torch's runtime.
+import torch
 
 from . import build, native, solver
 
@@ -48,7 +62,7 @@ MLP_SIGNATURES = {
     "cm_mse_grad": (I, [I, I, I, P, I, P, I, F, P, P, I, P, P]),
     "cm_adam": (I, [LL, P, P, P, P, F, F, F, F, I, P, P, I, I, P]),
     "cm_cast_bf16": (I, [LL, P, P, P]),
-    "cm_grid_to_mesh": (I, [I, P, I, P, I, I, I, P, I, P, P, P, P, P]),
+    "cm_grid_to_mesh": (I, [I, P, I, P, I, I, I, P, I, P, P, P, P, P, P, P]),
 }
 ACT_NONE, ACT_GELU, ACT_GELU_BWD = 0, 1, 2
 
@@ -74,8 +88,11 @@ def _chk(rc):
 
 
 def _torch():
-    import torch
     if not torch.cuda.is_available():
+        if native._lib is not None:
+            raise RuntimeError("torch sees no GPU: libcolloc.so was loaded before torch, so the process runs the "
+                               "system's HIP runtime, which this torch build cannot use; import the learned module "
+                               "(or torch) before the solver's first library call")
         raise RuntimeError("the learned warm start runs on the GPU (no device visible)")
     return torch
 
@@ -277,25 +294,29 @@ def grid_targets(sol, ny, nz):
 
 
 class WarmStart:
-    """A trained network with its output standardisation, for one family."""
+    """A trained network with its output standardisation, for one family.
+    base: "none" (the network predicts the solution) or "lq" (the
+    difference from the instance's linear-quadratic solution)."""
 
-    def __init__(self, net, mean, sd, ny, nz, nx):
+    def __init__(self, net, mean, sd, ny, nz, nx, base="none"):
         torch = _torch()
         self.net = net
         self.ny, self.nz, self.nx = ny, nz, nx
+        self.base = base
         self.mean = torch.tensor(np.asarray(mean, dtype=np.float32), device="cuda")
         self.sd = torch.tensor(np.asarray(sd, dtype=np.float32), device="cuda")
 
     def save(self, path):
         st = self.net.state()
         np.savez(path, mean=self.mean.cpu().numpy(), sd=self.sd.cpu().numpy(), ny=self.ny, nz=self.nz, nx=self.nx,
-                 **st)
+                 base=self.base, **st)
 
     @classmethod
     def load(cls, path):
         st = dict(np.load(path))
         net = WarmStartNet.from_state(st)
-        return cls(net, st["mean"], st["sd"], int(st["ny"]), int(st["nz"]), int(st["nx"]))
+        base = str(st["base"]) if "base" in st else "none"
+        return cls(net, st["mean"], st["sd"], int(st["ny"]), int(st["nz"]), int(st["nx"]), base)
 
     def predict_grid(self, feats):
         """Standardised network output (B, n_outp) for a feature matrix; the
@@ -309,7 +330,9 @@ class WarmStart:
 
     def guess_on_meshes(self, solv, feats):
         """Initial guess of every instance of solv's current batch (its meshes
-        set) from the network, written with cs_form_initial_input_device."""
+        set) from the network, written with cs_form_initial_input_device.
+        With base "lq" the solver's current state must be the instances' LQ
+        solution on these meshes: the network's output is added to it."""
         torch = _torch()
         B = solv.B
         G = self.predict_grid(feats)
@@ -319,26 +342,68 @@ class WarmStart:
         Y = torch.empty((N, self.ny), dtype=torch.float64, device="cuda")
         Z = torch.empty((N, max(self.nz, 1)), dtype=torch.float64, device="cuda")
         nc = self.ny + self.nz
+        yb = zb = None
+        p_base = np.zeros((B, solv.np))
+        if self.base == "lq":
+            yp, zp, pp = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+            native.check(native.lib().cs_node_solution_device(solv.h, ctypes.byref(yp), ctypes.byref(zp),
+                                                              ctypes.byref(pp)))
+            yb, zb = yp, zp
+            _, _, _, p_host = solv.get_state_parts(p=True)
+            p_base = np.asarray(p_host).reshape(B, solv.np)
         _chk(mlp_lib().cm_grid_to_mesh(B, _p(noff), N, _p(t), GRID, nc, self.ny, _p(G), G.stride(0), _p(self.mean),
-                                       _p(self.sd), _p(Y), _p(Z), _stream()))
+                                       _p(self.sd), yb, zb, _p(Y), _p(Z), _stream()))
         # boundary multipliers from r = 0: kappa_i = -lambda(0), kappa_f = lambda(T)
         g = G[:B, :GRID * nc].double() * self.sd.double() + self.mean.double()
         g = g.view(B, GRID, nc)
         nx = self.nx
-        Pm = torch.cat([-g[:, 0, nx:2 * nx], g[:, GRID - 1, nx:2 * nx]], dim=1).contiguous()
+        Pm = torch.cat([-g[:, 0, nx:2 * nx], g[:, GRID - 1, nx:2 * nx]], dim=1)
+        Pm = (Pm + torch.from_numpy(p_base).cuda()).contiguous()
         torch.cuda.current_stream().synchronize()
         native.check(native.lib().cs_form_initial_input_device(solv.h, _p(Y), _p(Z), _p(Pm)))
         return Y, Z, Pm
 
 
+def warm_solve(ws, solv, t_spans, feats, constants, lq_constants, tol=1e-8, max_iter=200, max_iter_warm=12,
+               alpha=0.1):
+    """Warm-started batched solve of the instances (meshes t_spans): for an
+    "lq" network, one Newton solve of the LQ problems from the all-ones
+    guess (converged after one iteration), then the network's guess and at
+    most max_iter_warm Newton iterations at the instances' constants.
+    Returns the solve_batch-style result dict; "lq_iterations" holds the
+    iterations of the LQ stage."""
+    torch = _torch()
+    B = len(t_spans)
+    solv.set_batch_mesh(t_spans)
+    lq_its = np.zeros(B, dtype=int)
+    if ws.base == "lq":
+        N = solv.N
+        ones_y = torch.ones((N, solv.ny), dtype=torch.float64, device="cuda")
+        ones_z = torch.ones((N, max(solv.nz, 1)), dtype=torch.float64, device="cuda")
+        ones_p = torch.ones((B, max(solv.np, 1)), dtype=torch.float64, device="cuda")
+        torch.cuda.current_stream().synchronize()
+        native.check(native.lib().cs_form_initial_input_device(solv.h, _p(ones_y), _p(ones_z), _p(ones_p)))
+        solv.set_batch_constants(lq_constants)
+        it, _, _ = solv.newton(alpha, tol, max_iter)
+        lq_its = np.atleast_1d(it)
+    ws.guess_on_meshes(solv, feats)
+    r = solver.newton_stages(solv, constants, [], alpha, tol, max_iter_warm)
+    r["lq_iterations"] = lq_its
+    return r
+
+
 # ------------------------------------------------------ data and training
-def generate_solutions(bvp, m, insts, nint, schedule=(0.0, 0.25), chunk_nodes=320000, log=None):
+def generate_solutions(bvp, m, insts, nint, schedule=(0.0, 0.25), chunk_nodes=320000, base="lq", log=None):
     """Solve instances of a batched family on uniform meshes of nint
-    intervals (plain guess + continuation in the tanh scale), batched.
-    Returns (features, targets, mask of the instances whose Newton solve
-    converged).  The mesh-acceptance test is not applied: a training target
-    only needs the collocation solution at the grid nodes, which a coarse
-    mesh already gives far below the network's own error."""
+    intervals (plain guess + continuation in the tanh scale, whose first
+    stage, scale 0, is the LQ problem), batched.  Returns (features, targets,
+    mask of the instances whose Newton solve converged); with base "lq" the
+    targets are the solution minus the LQ solution.  The mesh-acceptance
+    test is not applied: a training target only needs the collocation
+    solution at the grid nodes, which a coarse mesh already gives far below
+    the network's own error."""
+    if base == "lq" and (not schedule or schedule[0] != 0.0):
+        raise ValueError("an LQ base needs a continuation that starts at scale 0")
     ny, nz, np_ = bvp.size_y, bvp.size_z, bvp.size_p
     per = max(1, min(len(insts), chunk_nodes // (nint + 1)))
     S = solver.CollocationSolver(bvp, m, n_max=per * (nint + 1), batch=per)
@@ -354,12 +419,18 @@ def generate_solutions(bvp, m, insts, nint, schedule=(0.0, 0.25), chunk_nodes=32
         S.form_initial_input(np.ones((Nt, ny)), np.ones((Nt, nz)), np.ones((len(ch), np_)))
         cons = np.array([i.constant_values() for i in ch])
         hom = [np.array([i.constant_values(s=s) for i in ch]) for s in schedule]
+        base_rows = np.zeros((len(ch), targets.shape[1]))
+        if base == "lq":
+            S.set_batch_constants(hom[0])
+            S.newton(0.1, bvp.tolerance, bvp.maximum_newton_iterations)
+            base_rows = np.array([grid_targets(s, ny, nz) for s in solver.batch_solutions(S)])
+            hom = hom[1:]
         r = solver.newton_stages(S, cons, hom, 0.1, bvp.tolerance, bvp.maximum_newton_iterations)
         sols = solver.batch_solutions(S)
         conv = r["norm"] < bvp.tolerance
         for k, s in enumerate(sols):
             if conv[k]:
-                targets[c0 + k] = grid_targets(s, ny, nz)
+                targets[c0 + k] = grid_targets(s, ny, nz) - base_rows[k]
         ok[c0:c0 + per] = conv
         if log:
             log("solutions %d/%d: %d converged (%.1f s)" % (c0 + len(ch), len(insts), int(ok[:c0 + per].sum()),
@@ -396,3 +467,55 @@ def train(feats, targets, epochs=60, batch=512, lr=1e-3, seed=0, width=1024, dep
         if log and (ep % 10 == 0 or ep == epochs - 1):
             log("epoch %d: mean loss %.3e" % (ep, ep_loss))
     return net, mean, sd, ep_loss
+
+
+# ------------------------------------------------------- the batched family
+MODEL_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_model")
+FAMILY_MODEL = os.path.join(MODEL_DIR, "warmstart_tanh20_m5.npz")
+TRAIN_OFFSET = 100000   # training instances are family members TRAIN_OFFSET + i (held-out: 0 .. 99999)
+
+
+def family_training_set(n, seed=1, nx=20, nu=10):
+    """Members TRAIN_OFFSET .. TRAIN_OFFSET + n - 1 of the batched family (the
+    benchmark and the evaluation use members 0, 1, ...)."""
+    from . import problems
+    A, B, C, _ = problems.draw_family(nx, nu, seed, True)
+    out = []
+    for i in range(TRAIN_OFFSET, TRAIN_OFFSET + n):
+        rng = np.random.default_rng([seed, i])
+        x0 = rng.uniform(-1.0, 1.0, nx)
+        T = rng.uniform(0.5, 2.0)
+        out.append(problems.FamilyInstance(nx=nx, nu=nu, A=A, B=B, C=C, x0=x0, s=0.5, q=1.0, c_u=0.1, T=T,
+                                           nodes=257, tolerance=1e-8, instance_x0=True, homotopy=True))
+    return out
+
+
+def fit_family(n_train=50000, nint=256, epochs=200, batch=512, lr=1e-3, width=1024, depth=4, path=None, base="lq",
+               log=None):
+    """Generate n_train solutions of the n_x=20 batched family with the
+    plain batched solver, train the network on the converged ones and save
+    it (with the timing and fit figures) to path."""
+    from . import problems
+    m = 5
+    insts = family_training_set(n_train)
+    bvp = solver.BvpDae(problems.batch_family(1)[0].ocp_text())
+    t0 = time.time()
+    feats, targets, ok = generate_solutions(bvp, m, insts, nint, base=base, log=log)
+    t_gen = time.time() - t0
+    t0 = time.time()
+    net, mean, sd, loss = train(feats[ok], targets[ok], epochs=epochs, batch=batch, lr=lr, width=width, depth=depth,
+                                log=log)
+    _torch().cuda.synchronize()
+    t_train = time.time() - t0
+    ws = WarmStart(net, mean, sd, bvp.size_y, bvp.size_z, nx=20, base=base)
+    path = path or FAMILY_MODEL
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    ws.save(path)
+    info = {"n_train": int(n_train), "converged": int(ok.sum()), "nint": nint, "epochs": epochs, "batch": batch,
+            "width": width, "depth": depth, "base": base, "seconds_generate": t_gen, "seconds_train": t_train,
+            "final_mse": loss}
+    with open(path + ".json", "w") as f:
+        import json
+        json.dump(info, f, indent=1)
+    return ws, info
+

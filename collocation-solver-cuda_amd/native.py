@@ -46,6 +46,8 @@ SIGNATURES = {
     "cs_segment_residual": (c_int, [c_void_p, c_double, c_double, P_D, P_D]),
     "cs_segment_residual_ex": (c_int, [c_void_p, c_double, c_double, c_int, c_int, c_int, P_D, P_D]),
     "cs_recover_solution": (c_int, [c_void_p, P_D]),
+    "cs_node_solution_device": (c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                        ctypes.POINTER(c_void_p)]),
     "cs_profile": (c_int, [c_void_p, c_int]),
     "cs_profile_get": (c_int, [c_void_p, c_int, P_D, P_I]),
     "cs_status": (c_int, [c_void_p, P_I]),

@@ -191,6 +191,12 @@ class CollocationSolver:
         check(lib().cs_get_state(self.h, *[ptr(a) for a in out]))
         return tuple(out)
 
+    def get_state_parts(self, y=False, y_dot=False, z_tilde=False, p=False):
+        """get_state copying only the requested arrays (the others are None)."""
+        out = [np.empty(s) if want else None for s, want in zip(self._shapes(), (y, y_dot, z_tilde, p))]
+        check(lib().cs_get_state(self.h, *[ptr(a) for a in out]))
+        return tuple(out)
+
     # -- stages --------------------------------------------------------------
     def form_initial_input(self, y0, z0, p0):
         """form_initial_input_parallel (solver:287): returns y, y_dot, z_tilde."""

@@ -189,6 +189,15 @@ int cs_segment_residual_ex(cs_handle* h, double alpha, double tol, int residual_
 /* recover_solution_parallel (solver:2034-2075): z at the nodes, z[N][nz]. */
 int cs_recover_solution(cs_handle* h, double* z);
 
+/* The current solution at the nodes without a host copy: forms z at the
+ * nodes (as cs_recover_solution) in the handle's buffer and returns device
+ * pointers to y[N][ny], z[N][nz] and p[B][np] (any may be NULL).  The
+ * pointers are borrowed: they stay valid while the handle lives, and the
+ * values until the next call that changes the state or the mesh.  For
+ * device-side producers of initial guesses (colloc_mlp.h); the reference
+ * returns numpy copies only. */
+int cs_node_solution_device(cs_handle* h, const double** y, const double** z, const double** p);
+
 /* Kernel timing with HIP events on the handle's stream.  cs_profile(h, 1)
  * starts recording every launch of the kernels listed in CS_KERNEL_*;
  * cs_profile_get returns the summed milliseconds and launch count. */

@@ -72,10 +72,12 @@ int cm_cast_bf16(long long n, const float* x, uint16_t* y, void* stream);
  * as mean + sd * G.  Node n of instance b (noff[b] <= n < noff[b+1], times
  * t[n], n_nodes = noff[B]) gets the cubic Lagrange interpolant through the
  * four nearest grid nodes: Y[n][c] for c < ny, Z[n][c - ny] otherwise
- * (fp64, the layouts of cs_form_initial_input's y0 and z0). */
+ * (fp64, the layouts of cs_form_initial_input's y0 and z0), added to the
+ * base values Yb[n][c] / Zb[n][c - ny] when those are given (may be NULL;
+ * Yb may be Y and Zb may be Z). */
 int cm_grid_to_mesh(int B, const int* noff, int n_nodes, const double* t, int ng, int nc, int ny,
-                    const float* G, int ldg, const float* mean, const float* sd, double* Y, double* Z,
-                    void* stream);
+                    const float* G, int ldg, const float* mean, const float* sd, const double* Yb,
+                    const double* Zb, double* Y, double* Z, void* stream);
 
 #ifdef __cplusplus
 }

@@ -159,7 +159,7 @@ def test_grid_to_mesh_interpolation():
     md, sdd = torch.from_numpy(mean).cuda(), torch.from_numpy(sd).cuda()   # kept alive until the kernel has run
     learned._chk(learned.mlp_lib().cm_grid_to_mesh(2, learned._p(nd), 53, learned._p(td), ng, nc, ny,
                                                    learned._p(Gd), G.shape[1], learned._p(md), learned._p(sdd),
-                                                   learned._p(Y), learned._p(Z), learned._stream()))
+                                                   None, None, learned._p(Y), learned._p(Z), learned._stream()))
     torch.cuda.synchronize()
     for b, tm in enumerate(meshes):
         vals = (mean.astype(np.float64) + sd.astype(np.float64) * G[b, :ng * nc].astype(np.float64)).reshape(ng, nc)
@@ -172,6 +172,14 @@ def test_grid_to_mesh_interpolation():
             n = noff[b] + j
             assert np.allclose(Y[n].cpu().numpy(), ref[:ny], rtol=1e-12, atol=1e-12)
             assert np.allclose(Z[n].cpu().numpy(), ref[ny:], rtol=1e-12, atol=1e-12)
+    # with base values (here the output itself, in place): the sum
+    Y0, Z0 = Y.clone(), Z.clone()
+    learned._chk(learned.mlp_lib().cm_grid_to_mesh(2, learned._p(nd), 53, learned._p(td), ng, nc, ny,
+                                                   learned._p(Gd), G.shape[1], learned._p(md), learned._p(sdd),
+                                                   learned._p(Y), learned._p(Z), learned._p(Y), learned._p(Z),
+                                                   learned._stream()))
+    torch.cuda.synchronize()
+    assert torch.allclose(Y, 2 * Y0, rtol=1e-14, atol=0) and torch.allclose(Z, 2 * Z0, rtol=1e-14, atol=0)
 
 
 def test_warm_start_gains_over_plain_guess():
@@ -185,11 +193,11 @@ def test_warm_start_gains_over_plain_guess():
     bvp = solver.BvpDae(held[0].ocp_text())
     ny, nz, np_ = bvp.size_y, bvp.size_z, bvp.size_p
     say("generating %d training solutions" % len(train_insts))
-    feats, targets, ok = learned.generate_solutions(bvp, m, train_insts, nint=64, log=say)
+    feats, targets, ok = learned.generate_solutions(bvp, m, train_insts, nint=64, base="lq", log=say)
     say("training on %d converged solutions" % ok.sum())
     net, mean, sd, loss = learned.train(feats[ok], targets[ok], epochs=150, batch=256, lr=2e-3, width=256, depth=4,
                                         log=say)
-    ws = learned.WarmStart(net, mean, sd, ny, nz, nx=4)
+    ws = learned.WarmStart(net, mean, sd, ny, nz, nx=4, base="lq")
     ts = [np.linspace(0.0, i.T, i.nodes) for i in held]
     cons = np.array([i.constant_values() for i in held])
     S = solver.CollocationSolver(bvp, m, n_max=sum(len(t) for t in ts), batch=len(held))
@@ -199,16 +207,17 @@ def test_warm_start_gains_over_plain_guess():
                                [np.array([i.constant_values(s=0.0) for i in held])], tol=bvp.tolerance,
                                max_iter=bvp.maximum_newton_iterations)
     sol_plain = solver.batch_solutions(S)
-    say("network guess")
-    S.set_batch_mesh(ts)
-    ws.guess_on_meshes(S, learned.features([i.x0 for i in held], [i.T for i in held]))
-    warm = solver.newton_stages(S, cons, [], 0.1, bvp.tolerance, bvp.maximum_newton_iterations)
+    say("LQ solve + network correction")
+    warm = learned.warm_solve(ws, S, ts, learned.features([i.x0 for i in held], [i.T for i in held]), cons,
+                              np.array([i.constant_values(s=0.0) for i in held]), tol=bvp.tolerance,
+                              max_iter=bvp.maximum_newton_iterations, max_iter_warm=bvp.maximum_newton_iterations)
     sol_warm = solver.batch_solutions(S)
+    assert np.all(warm["lq_iterations"] == 1)   # the LQ problem is linear: one Newton step
     # Newton convergence (the mesh acceptance test does not depend on the guess)
     cp, cw = plain["norm"] < bvp.tolerance, warm["norm"] < bvp.tolerance
     both = cp & cw
     it_plain = plain["iterations"].sum(axis=0)
-    it_warm = warm["iterations"].sum(axis=0)
+    it_warm = warm["iterations"].sum(axis=0) + warm["lq_iterations"]
     say("Newton converged: plain %d, warm %d of %d; mean iterations (both converged) plain %.2f, warm %.2f"
         % (cp.sum(), cw.sum(), len(held), it_plain[both].mean(), it_warm[both].mean()))
     # this small network (width 256, ~2.5k samples) reaches a standardised MSE
