@@ -77,14 +77,14 @@ def dense_condense_flops(ny, nz, np_, m):
 
 
 def measured_traffic(workload):
-    """HBM bytes per colloc_condense launch at this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy kernel);
-    None when no measurement exists for it."""
+    """HBM bytes per interval condensed by colloc_condense_ns at this workload,
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated
+    against a copy kernel); None when no measurement exists for it."""
     path = os.path.join(ROOT, "profiles", "traffic_%s_m5.md" % workload)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch colloc_condense:"):
+        if line.startswith("traffic_bytes_per_interval colloc_condense_ns:"):
             return float(line.split(":")[1])
     return None
 
@@ -128,7 +128,9 @@ def condense_roofline(S, before, after, cond_ms, cond_n, ny, nz, np_, npa, m, tr
     achieved = fl / avg_s / 1e12 if avg_s > 0 else 0.0
     dense = dense_condense_flops(ny, nz, np_, m) * ncond / (cond_ms / 1e3) / 1e12 if cond_ms > 0 else 0.0
     return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": traffic * ncond / max(cond_n, 1) if traffic else None,
+            "traffic_bytes_per_interval": traffic,
             "kernel": "colloc_condense_ns (+ colloc_condense for intervals the series leaves)",
             "flops_per_launch": fl, "avg_launch_ms": avg_s * 1e3, "launches": cond_n,
             "intervals_per_launch": ncond / max(cond_n, 1),
@@ -257,7 +259,7 @@ def batch_workload(args):
     conv, its, n_back = results[-1]
     npa = len(jit.unit_info(bvp.ocp_text, m)["active_params"])
     roof = condense_roofline(S, st0, st1, prof["condense"][0], prof["condense"][1], ny, nz, np_, npa, m,
-                             measured_traffic("batch%d_%s" % (len(insts), args.form)))
+                             measured_traffic("batch_%s" % args.form))
     extra = {}
     if args.form == "learned" and args.plain_pass:
         # the plain batched form on the same instances, for the split of the gain

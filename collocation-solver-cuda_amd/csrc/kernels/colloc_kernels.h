@@ -950,8 +950,11 @@ __device__ __forceinline__ void recover(int nint, Batch bt, const double* __rest
 
 // ------------------------------------------------------------------- axpy
 // out = x + a_b d over the packed state [y | ydot | ztil | p], with a_b =
-// steps[b] the step of the element's instance; a_b = 0 copies x (so that a
-// non-finite direction of an instance that takes no step cannot leak).
+// steps[b] the step of the element's instance.  Elements of instances with
+// a_b = 0 are not touched: the line search reads a trial state only for the
+// instances it is still searching for, and the accepted step leaves the
+// other instances' state as it is (so a non-finite direction of an instance
+// that takes no step cannot leak).
 struct Seg4 { long long off[4]; long long len[4]; };
 __device__ __forceinline__ void axpy4(Seg4 s, Batch bt, const double* x, const double* __restrict__ d,
                                       const double* __restrict__ steps, double* out) {
@@ -966,7 +969,7 @@ __device__ __forceinline__ void axpy4(Seg4 s, Batch bt, const double* x, const d
       else b = (int)(gid / NP1);
       const double a = steps[b];
       const long long e = s.off[q] + gid;
-      out[e] = (a != 0.0) ? fma(a, d[e], x[e]) : x[e];
+      if (a != 0.0) out[e] = fma(a, d[e], x[e]);
       return;
     }
     gid -= s.len[q];
