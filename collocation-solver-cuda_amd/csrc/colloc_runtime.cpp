@@ -49,19 +49,23 @@ struct Batch {
   const int* noff;
   const int* act;
   int B;
+  const int* ilist;
+  int nlist;
 };
 struct CRSeg {
-  int in_off, out_off, rec_off, nrows;
+  int in_off, out_off, rec_off, nrows, subst;
 };
 
 enum Fn {
-  F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_COND, F_COND_NS, F_CR_PAIR, F_CR_FINAL, F_CR_BACK, F_RECOVER,
-  F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
+  F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_COND, F_COND_NS, F_CR_PAIR, F_CR_SUBST, F_CR_FINAL, F_CR_BACK,
+  F_RECOVER, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
   "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian",
-  "colloc_condense", "colloc_condense_ns", "colloc_cr_pair", "colloc_cr_final", "colloc_cr_back",
-  "colloc_recover", "colloc_axpy4", "colloc_init_input", "colloc_segres", "colloc_recover_z"};
+  "colloc_condense", "colloc_condense_ns", "colloc_cr_pair", "colloc_cr_subst", "colloc_cr_final",
+  "colloc_cr_back", "colloc_recover", "colloc_axpy4", "colloc_init_input", "colloc_segres", "colloc_recover_z"};
+constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
+constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
 
 }  // namespace
@@ -108,6 +112,8 @@ struct cs_handle {
   int* act = nullptr;        // instances still iterating
   int* srch = nullptr;       // instances still line-searching
   int* fbk = nullptr;        // per interval: 1 if the series condensation left it to Gauss-Jordan
+  int* alist = nullptr;      // intervals of the instances still iterating (launch list)
+  int nalist = 0;            // its length; 0: launch over all intervals
   // cyclic-reduction schedule of the current mesh: per level, the instance
   // of each output row and each instance's CRSeg
   int* cr_oinst = nullptr;
@@ -115,6 +121,8 @@ struct cs_handle {
   CRSeg* cr_seg = nullptr;
   size_t cr_seg_cap = 0;
   std::vector<int> lev_nout;          // output rows per level
+  std::vector<char> lev_qr, lev_subst;  // level has instances paired by QR / by substitution
+  int subst_spans = kDefaultSubstSpans; // see cs_set_babd_substitution (0: Householder everywhere)
   std::vector<size_t> lev_obase;      // offset of the level in cr_oinst
   std::vector<size_t> lev_sbase;      // offset of the level in cr_seg
   unsigned long long cond_fallback = 0;   // intervals condensed by Gauss-Jordan since creation
@@ -135,7 +143,12 @@ struct cs_handle {
   double* yd() const { return state + st_yd; }
   double* zt() const { return state + st_z; }
   double* p() const { return state + st_p; }
-  Batch batch(const int* mask) const { return Batch{iinst, ninst, noff, mask, B}; }
+  Batch batch(const int* mask) const { return Batch{iinst, ninst, noff, mask, B, nullptr, 0}; }
+  // with the launch list of the instances still iterating, when one is set
+  Batch batch_listed(const int* mask) const {
+    return Batch{iinst, ninst, noff, mask, B, nalist ? alist : nullptr, nalist};
+  }
+  int launch_intervals() const { return nalist ? nalist : nint; }
 };
 
 namespace {
@@ -250,9 +263,12 @@ int residual(cs_handle* h, const double* st, double alpha, bool full, const int*
   const double* zt = st + h->st_z;
   const double* p = st + h->st_p;
   unsigned long long* slot = h->scal;
-  Batch bt = h->batch(mask);
+  // the full residual (start of an iteration) runs over the listed intervals;
+  // the line-search residual over all, masked by the searching instances
+  Batch bt = full ? h->batch_listed(mask) : h->batch(mask);
   void* args[] = {&nint, &bt, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &h->ytil, &h->fa, &h->fb, &h->rbc, &slot};
-  unsigned nthreads = (unsigned)(nint * h->m + h->B);
+  const int nlaunch = full ? h->launch_intervals() : nint;
+  unsigned nthreads = (unsigned)(nlaunch * h->m + h->B);
   TRY(launch(h, full ? F_RES_FULL : F_RES_NORM, (nthreads + 255) / 256, 256, args));
   return read_slots(h, h->scal, norms);
 }
@@ -263,9 +279,9 @@ int jacobian(cs_handle* h, double alpha, const int* mask) {
   double* y = h->y();
   double* p = h->p();
   double* zt = h->zt();
-  Batch bt = h->batch(mask);
+  Batch bt = h->batch_listed(mask);
   void* args[] = {&nint, &bt, &y, &h->ytil, &zt, &p, &h->c, &alpha, &h->jac, &h->Dr};
-  unsigned nthreads = (unsigned)(nint * h->m + h->B);
+  unsigned nthreads = (unsigned)(h->launch_intervals() * h->m + h->B);
   return launch(h, F_JAC, (nthreads + 255) / 256, 256, args);
 }
 
@@ -279,9 +295,9 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
   int nint = h->nint;
   unsigned long long* counters = h->scal + 2 * h->bmax;   // [intervals left to Gauss-Jordan, series terms]
   TRY(fill_u64(h, counters, 2));
-  Batch bt = h->batch(mask);
+  Batch bt = h->batch_listed(mask);
   void* args_ns[] = {&nint, &bt, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &h->fbk, &counters};
-  TRY(launch(h, F_COND_NS, (unsigned)nint, kNsThreads, args_ns));
+  TRY(launch(h, F_COND_NS, (unsigned)h->launch_intervals(), kNsThreads, args_ns));
   unsigned long long cnt[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -290,8 +306,9 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
   const unsigned long long nleft = cnt[0];
   if (nleft == 0) return CS_OK;
   const int* only = h->fbk;
-  void* args[] = {&nint, &h->iinst, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &only};
-  return launch(h, F_COND, (unsigned)nint, (unsigned)(h->cw * 64), args);
+  const int* ilist = bt.ilist;
+  void* args[] = {&nint, &h->iinst, &ilist, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &only};
+  return launch(h, F_COND, (unsigned)h->launch_intervals(), (unsigned)(h->cw * 64), args);
 }
 
 // Cyclic-reduction schedule of the current mesh (host side, once per mesh):
@@ -305,22 +322,34 @@ int build_cr_schedule(cs_handle* h) {
     n[b] = h->hnoff[b + 1] - h->hnoff[b] - 1;
     in_off[b] = h->hnoff[b] - b;
   }
+  // substitution levels of instance b: the largest L with 2^L pairs-spans,
+  // i.e. 2^L * subst_spans <= its interval count
+  std::vector<int> nsub(B, 0);
+  if (h->subst_spans > 0)
+    for (int b = 0; b < B; ++b)
+      while (((long long)h->subst_spans << (nsub[b] + 1)) <= n[b]) ++nsub[b];
   std::vector<int> oinst;
   std::vector<CRSeg> segs;
   h->lev_nout.clear();
   h->lev_obase.clear();
   h->lev_sbase.clear();
+  h->lev_qr.clear();
+  h->lev_subst.clear();
   size_t rec_base = 0;
-  for (;;) {
+  for (int lev = 0;; ++lev) {
     bool more = false;
     for (int b = 0; b < B; ++b) more |= n[b] > 1;
     if (!more) break;
     h->lev_obase.push_back(oinst.size());
     h->lev_sbase.push_back(segs.size());
     int out = 0;
+    char any_qr = 0, any_sub = 0;
     for (int b = 0; b < B; ++b) {
       const int nout = n[b] / 2 + (n[b] & 1);
-      segs.push_back(CRSeg{in_off[b], out, (int)rec_base, n[b]});
+      const int sub = lev < nsub[b] ? 1 : 0;
+      segs.push_back(CRSeg{in_off[b], out, (int)rec_base, n[b], sub});
+      if (sub) any_sub = 1;
+      else any_qr = 1;
       rec_base += n[b] / 2;
       for (int k = 0; k < nout; ++k) oinst.push_back(b);
       in_off[b] = out;
@@ -328,6 +357,8 @@ int build_cr_schedule(cs_handle* h) {
       n[b] = nout;
     }
     h->lev_nout.push_back(out);
+    h->lev_qr.push_back(any_qr);
+    h->lev_subst.push_back(any_sub);
   }
   if (oinst.size() > h->cr_oinst_cap) {
     if (h->cr_oinst) hipFree(h->cr_oinst);
@@ -364,7 +395,8 @@ int babd(cs_handle* h, const int* mask) {
     const CRSeg* seg = h->cr_seg + h->lev_sbase[l];
     const int* act = mask;
     void* args[] = {&oinst, &seg, &act, &rin, &rout, &h->crrec};
-    TRY(launch(h, F_CR_PAIR, (unsigned)h->lev_nout[l], pair_block, args));
+    if (h->lev_subst[l]) TRY(launch(h, F_CR_SUBST, (unsigned)h->lev_nout[l], kSubstThreads, args));
+    if (h->lev_qr[l]) TRY(launch(h, F_CR_PAIR, (unsigned)h->lev_nout[l], pair_block, args));
     std::swap(rin, rout);
   }
   {
@@ -387,9 +419,9 @@ int recover(cs_handle* h, const int* mask) {
   double* dp = h->dir + h->st_p;
   double* dyd = h->dir + h->st_yd;
   double* dz = h->dir + h->st_z;
-  Batch bt = h->batch(mask);
+  Batch bt = h->batch_listed(mask);
   void* args[] = {&nint, &bt, &h->rec, &dy, &dp, &dyd, &dz};
-  return launch(h, F_RECOVER, (unsigned)nint, 256, args);
+  return launch(h, F_RECOVER, (unsigned)h->launch_intervals(), 256, args);
 }
 
 Seg4 state_segs(cs_handle* h) {
@@ -526,6 +558,7 @@ int cs_create(const cs_dims* d, cs_handle** out) {
   I(&h->act, bm);
   I(&h->srch, bm);
   I(&h->fbk, nint);
+  I(&h->alist, nint);
   if (r == CS_OK) {
     hipError_t e = hipMalloc((void**)&h->scal, (2 * bm + 2) * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&h->status, sizeof(int));
@@ -551,7 +584,7 @@ void cs_destroy(cs_handle* h) {
                     h->rows0, h->rows1, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps};
   for (double* b : bufs)
     if (b) hipFree(b);
-  int* ibufs[] = {h->iinst, h->ninst, h->noff, h->act, h->srch, h->fbk, h->cr_oinst};
+  int* ibufs[] = {h->iinst, h->ninst, h->noff, h->act, h->srch, h->fbk, h->alist, h->cr_oinst};
   for (int* b : ibufs)
     if (b) hipFree(b);
   if (h->cr_seg) hipFree(h->cr_seg);
@@ -745,13 +778,41 @@ int init_from_saved(cs_handle* h) {
 // every later iteration of that instance would repeat this one exactly: the
 // instance stops here with the outcome those iterations would give (loop
 // counter max_iter - 1, the same norm, the norm history continued).
+// Launch list of the intervals of the active instances (none, i.e. every
+// interval, when all are active); re-uploaded only when the set changed.
+struct LaunchList {
+  cs_handle* h;
+  std::vector<int> last;
+  explicit LaunchList(cs_handle* h_) : h(h_) {}
+  ~LaunchList() { h->nalist = 0; }   // other entry points launch over every interval
+  int set(const std::vector<int>& active) {
+    if (active == last) return CS_OK;
+    last = active;
+    int nact = 0;
+    for (int a : active) nact += a;
+    if (nact == h->B || nact == 0) {
+      h->nalist = 0;
+      return CS_OK;
+    }
+    std::vector<int> list;
+    for (int b = 0; b < h->B; ++b)
+      if (active[b])
+        for (int i = h->hnoff[b] - b; i < h->hnoff[b + 1] - b - 1; ++i) list.push_back(i);
+    TRY(upload_ints(h, h->alist, list));
+    h->nalist = (int)list.size();
+    return CS_OK;
+  }
+};
+
 int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time, double* norm, double* hist,
            int hist_len) {
   const int B = h->B;
   std::vector<double> nrm(B, 0.0), nn(B, 0.0), step(B, 0.0), taken(B, 0.0);
   std::vector<int> active(B, 1), search(B, 0), done(B, -1);
+  LaunchList lst(h);
   for (int it = 0; it < max_iter; ++it) {
     TRY(upload_ints(h, h->act, active));
+    TRY(lst.set(active));
     std::vector<double> cur(B, 0.0);
     TRY(residual(h, h->state, alpha, true, h->act, cur.data()));
     int nact = 0;
@@ -768,6 +829,7 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
     }
     if (nact == 0) break;
     TRY(upload_ints(h, h->act, active));
+    TRY(lst.set(active));
     TRY(jacobian(h, alpha, h->act));
     long long ncond = 0;
     for (int b = 0; b < B; ++b)
@@ -957,6 +1019,14 @@ int cs_node_solution_device(cs_handle* h, const double** y, const double** z, co
   if (y) *y = h->y();
   if (z) *z = h->zbuf;
   if (p) *p = h->p();
+  return CS_OK;
+}
+
+int cs_set_babd_substitution(cs_handle* h, int min_spans) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (min_spans < 0) return fail(CS_E_ARG, "min_spans=%d (>= 0)", min_spans);
+  h->subst_spans = min_spans;
+  if (h->B >= 1 && h->N >= 2) return build_cr_schedule(h);
   return CS_OK;
 }
 

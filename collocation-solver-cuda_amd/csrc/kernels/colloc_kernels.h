@@ -61,20 +61,29 @@ constexpr int NRB = NY + NP;                       // one instance's r_bc
 // node is i + iinst[i]) and ninst[n] the instance of node n.  p, c, r_bc and
 // Dr hold one block per instance.  act (may be null) marks the instances a
 // kernel works on; the others are skipped and their outputs left as they are.
+// ilist (may be null) lists the intervals a per-interval or per-stage kernel
+// is launched over (nlist of them, those of the instances still iterating),
+// so that converged instances cost no workgroups; null: all intervals.
 struct Batch {
   const int* iinst;
   const int* ninst;
   const int* noff;
   const int* act;
   int B;
+  const int* ilist;
+  int nlist;
 };
 __device__ __forceinline__ bool skipped(const Batch& bt, int b) { return bt.act && !bt.act[b]; }
+// Intervals a launch covers and the interval of its k-th work unit.
+__device__ __forceinline__ int launch_intervals(const Batch& bt, int nint) { return bt.ilist ? bt.nlist : nint; }
+__device__ __forceinline__ int listed_interval(const Batch& bt, int k) { return bt.ilist ? bt.ilist[k] : k; }
 
 // One CR level of one instance: its rows start at in_off of the input
 // buffer, its ceil(nrows / 2) output rows at out_off, its nrows / 2 records
-// at rec_off.
+// at rec_off; subst != 0: the level pairs by substitution (cr_subst), else
+// by Householder QR (cr_pair).
 struct CRSeg {
-  int in_off, out_off, rec_off, nrows;
+  int in_off, out_off, rec_off, nrows, subst;
 };
 
 // -------------------------------------------------------------------- fill
@@ -112,16 +121,18 @@ __device__ __forceinline__ void residual(int nint, Batch bt, const double* __res
                          double* __restrict__ ytil, double* __restrict__ fa, double* __restrict__ fb,
                          double* __restrict__ rbc, unsigned long long* __restrict__ norm) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nst = nint * M;
+  const int nst = launch_intervals(bt, nint) * M;
   double amax = 0.0;
   int b = -1;
+  int i = 0, j = 0, st = 0;   // interval, stage, stage row
   if (gid < nst) {
-    b = bt.iinst[gid / M];
+    i = listed_interval(bt, gid / M);
+    j = gid - (gid / M) * M;
+    st = i * M + j;
+    b = bt.iinst[i];
     if (skipped(bt, b)) b = -1;
   }
   if (b >= 0) {
-    const int i = gid / M;
-    const int j = gid - i * M;
     const int nd = i + b;
     const double h = t[nd + 1] - t[nd];
     const double* yi = y + (size_t)nd * NY;
@@ -137,10 +148,10 @@ __device__ __forceinline__ void residual(int nint, Batch bt, const double* __res
       yt[l] = yi[l] + h * s;
     }
 #pragma unroll
-    for (int l = 0; l < NZ; ++l) zt[l] = ztil[(size_t)gid * NZ + l];
+    for (int l = 0; l < NZ; ++l) zt[l] = ztil[(size_t)st * NZ + l];
     prob::f(yt, zt, pb, cb, fo);
     prob::g(yt, zt, pb, cb, alpha, go);
-    double* fai = fa + (size_t)gid * NYZ;
+    double* fai = fa + (size_t)st * NYZ;
 #pragma unroll
     for (int l = 0; l < NY; ++l) {
       const double v = fo[l] - yd[j * NY + l];
@@ -154,7 +165,7 @@ __device__ __forceinline__ void residual(int nint, Batch bt, const double* __res
     }
     if (FULL) {
 #pragma unroll
-      for (int l = 0; l < NY; ++l) ytil[(size_t)gid * NY + l] = yt[l];
+      for (int l = 0; l < NY; ++l) ytil[(size_t)st * NY + l] = yt[l];
     }
     if (j == M - 1) {
 #pragma unroll
@@ -190,16 +201,18 @@ __device__ __forceinline__ void jacobian(int nint, Batch bt, const double* __res
                          const double* __restrict__ p, const double* __restrict__ c, double alpha,
                          double* __restrict__ jac, double* __restrict__ Dr) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nst = nint * M;
+  const int nst = launch_intervals(bt, nint) * M;
   if (gid < nst) {
-    const int b = bt.iinst[gid / M];
+    const int i = listed_interval(bt, gid / M);
+    const int st = i * M + (gid - (gid / M) * M);
+    const int b = bt.iinst[i];
     if (skipped(bt, b)) return;
     double yt[NY], zt[NZ1];
 #pragma unroll
-    for (int l = 0; l < NY; ++l) yt[l] = ytil[(size_t)gid * NY + l];
+    for (int l = 0; l < NY; ++l) yt[l] = ytil[(size_t)st * NY + l];
 #pragma unroll
-    for (int l = 0; l < NZ; ++l) zt[l] = ztil[(size_t)gid * NZ + l];
-    double* out = jac + (size_t)gid * JSTAGE;
+    for (int l = 0; l < NZ; ++l) zt[l] = ztil[(size_t)st * NZ + l];
+    double* out = jac + (size_t)st * JSTAGE;
     const double* pb = p + (size_t)b * NP;
     const double* cb = c + (size_t)b * NC;
     prob::Df(yt, zt, pb, cb, out);
@@ -346,13 +359,15 @@ __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__
 //   parameter columns), b = -f_b + U_{M-1}[:, NVA-1].  Only the NPA parameters
 //   the stage functions depend on get columns (g_p = f_p = 0 for the rest).
 template <int CW>
-__device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst, const double* __restrict__ t,
-                         const double* __restrict__ jac,
+__device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst, const int* __restrict__ ilist,
+                         const double* __restrict__ t, const double* __restrict__ jac,
                          const double* __restrict__ fa, const double* __restrict__ fb,
                          double* __restrict__ rows, double* __restrict__ rec, int* __restrict__ status,
                          const int* __restrict__ only) {
-  // only != nullptr: redo just the intervals the series kernel left (only[i] != 0)
-  if (only && !only[blockIdx.x]) return;
+  // workgroup k condenses interval ilist[k] (k without a list); only !=
+  // nullptr: just the intervals the series kernel left (only[i] != 0)
+  const int i = ilist ? ilist[blockIdx.x] : blockIdx.x;
+  if (only && !only[i]) return;
   using Solver = GJB<NS, NS + NVA, CW, 16>;
   constexpr int NT = CW * WAVE;
   // sT (phases 1-2) and the elimination's panel / pivot-row buffers (phase 3)
@@ -366,7 +381,6 @@ __device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst
   __shared__ int invperm[Solver::RQ * WAVE];
   __shared__ int sbad;
 
-  const int i = blockIdx.x;
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = wave_id();
   if (tid == 0) sbad = 0;
@@ -499,7 +513,7 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
   double* sT = smem_ns + NSK * NY * NVAP;
   __shared__ double sred[2 * NWV];
   __shared__ int sbad;
-  const int i = blockIdx.x;
+  const int i = listed_interval(bt, blockIdx.x);
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = wave_id();
   const int ib = bt.iinst[i];
@@ -672,6 +686,7 @@ __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRS
   const int ib = oinst[blockIdx.x];
   if (act && !act[ib]) return;
   const CRSeg sg = seg[ib];
+  if (sg.subst) return;
   const int k = blockIdx.x - sg.out_off;
   const int nrows = sg.nrows;
   rows_in += (size_t)sg.in_off * NY * CRW;
@@ -779,6 +794,112 @@ __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRS
       if (r < NY) rec[r * RECW + cidx] = a[i];
       else if (r < R2 && cidx >= NY) out[(r - NY) * CRW + cidx - NY] = a[i];
     }
+  }
+}
+
+// Pairing by substitution, for levels whose rows all have C = I (the
+// condensation gives C_i = I, and this step keeps it): row a
+// G_a x_L + x_M + H_a p = b_a gives x_M = b_a - G_a x_L - H_a p, and row b
+// G_b x_M + C_b x_R + H_b p = b_b becomes
+//   (-G_b G_a) x_L + C_b x_R + (H_b - G_b H_a) p = b_b - G_b b_a.
+// The record is [R = I | Gp = G_a | Ep = 0 | Jp = H_a | d = b_a], which
+// cr_back solves like a Householder record.  One product G_b [G_a | H_a | b_a]
+// (NY x NY times NY x (NY+NP+1)) per pair instead of NY dependent reflector
+// steps.  Used only on the finest levels (a pair spans a small fraction of
+// the horizon, so the transition products stay near the identity); the
+// coarse levels keep the Householder step.  Thread (rg, cg) of the
+// workgroup computes a 4 x 4 tile of the product from LDS.
+constexpr int CS_XC = NY + NP + 1;                    // columns of [G_a | H_a | b_a]
+constexpr int CS_CG = (CS_XC + 3) / 4;                // column groups of 4
+constexpr int CS_RG = (NY + 3) / 4;                   // row groups of 4
+constexpr int CS_THREADS = 256;
+static_assert(CS_RG * CS_CG <= 4 * CS_THREADS, "cr_subst tiles");
+__device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
+                                         const int* __restrict__ act, const double* __restrict__ rows_in,
+                                         double* __restrict__ rows_out, double* __restrict__ recs) {
+  constexpr int XP = CS_CG * 4;                         // padded columns
+  __shared__ double sG[CS_RG * 4][NY];                  // G_b (rows padded to a multiple of 4)
+  __shared__ double sX[NY][XP];                         // [G_a | H_a | b_a]
+  const int ib = oinst[blockIdx.x];
+  if (act && !act[ib]) return;
+  const CRSeg sg = seg[ib];
+  if (!sg.subst) return;
+  const int k = blockIdx.x - sg.out_off;
+  const int nrows = sg.nrows;
+  rows_in += (size_t)sg.in_off * NY * CRW;
+  rows_out += (size_t)sg.out_off * NY * CRW;
+  recs += (size_t)sg.rec_off * NY * RECW;
+  const int npairs = nrows >> 1;
+  const int tid = threadIdx.x;
+  if (k == npairs) {  // carry the odd last row
+    const double* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
+    double* dst = rows_out + (size_t)npairs * NY * CRW;
+    for (int e = tid; e < NY * CRW; e += blockDim.x) dst[e] = src[e];
+    return;
+  }
+  const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
+  const double* rb = ra + NY * CRW;
+  double* out = rows_out + (size_t)k * NY * CRW;
+  double* rec = recs + (size_t)k * NY * RECW;
+  for (int e = tid; e < CS_RG * 4 * NY; e += blockDim.x) {
+    const int r = e / NY, c = e - (e / NY) * NY;
+    (&sG[0][0])[e] = r < NY ? rb[r * CRW + c] : 0.0;
+  }
+  for (int e = tid; e < NY * XP; e += blockDim.x) {
+    const int r = e / XP, c = e - (e / XP) * XP;
+    double v = 0.0;
+    if (c < NY) v = ra[r * CRW + c];                       // G_a
+    else if (c < NY + NP) v = ra[r * CRW + NY + c];        // H_a (after C_a)
+    else if (c == NY + NP) v = ra[r * CRW + CRW - 1];      // b_a
+    sX[r][c] = v;
+  }
+  // the record: [I | G_a | 0 | H_a | b_a]
+  for (int e = tid; e < NY * RECW; e += blockDim.x) {
+    const int r = e / RECW, c = e - (e / RECW) * RECW;
+    double v;
+    if (c < NY) v = (c == r) ? 1.0 : 0.0;
+    else if (c < 2 * NY) v = ra[r * CRW + c - NY];
+    else if (c < 3 * NY) v = 0.0;
+    else if (c < 3 * NY + NP) v = ra[r * CRW + c - NY];    // H_a: column 2NY + j of the row
+    else v = ra[r * CRW + CRW - 1];
+    rec[e] = v;
+  }
+  __syncthreads();
+  for (int t = tid; t < CS_RG * CS_CG; t += blockDim.x) {
+    const int rg = t / CS_CG, cg = t - (t / CS_CG) * CS_CG;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+#pragma unroll 4
+    for (int q = 0; q < NY; ++q) {
+      double g[4], x[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) g[a] = sG[rg * 4 + a][q];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) x[c] = sX[q][cg * 4 + c];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = fma(g[a], x[c], acc[a][c]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int r = rg * 4 + a;
+      if (r >= NY) continue;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int col = cg * 4 + c;
+        if (col < NY) out[r * CRW + col] = -acc[a][c];                                   // G'
+        else if (col < NY + NP) out[r * CRW + NY + col] = rb[r * CRW + NY + col] - acc[a][c];   // H'
+        else if (col == NY + NP) out[r * CRW + CRW - 1] = rb[r * CRW + CRW - 1] - acc[a][c];  // b'
+      }
+    }
+  }
+  for (int e = tid; e < NY * NY; e += blockDim.x) {        // C' = C_b
+    const int r = e / NY, c = e - (e / NY) * NY;
+    out[r * CRW + NY + c] = rb[r * CRW + NY + c];
   }
 }
 
@@ -901,7 +1022,7 @@ __device__ __forceinline__ void recover(int nint, Batch bt, const double* __rest
                         const double* __restrict__ dy, const double* __restrict__ dp,
                         double* __restrict__ dydot, double* __restrict__ dz) {
   __shared__ double sw[M][NVA];
-  const int i = blockIdx.x;
+  const int i = listed_interval(bt, blockIdx.x);
   const int ib = bt.iinst[i];
   if (skipped(bt, ib)) return;
   dy += (size_t)ib * NY;    // left node of interval i is i + ib
@@ -1166,9 +1287,10 @@ __global__ __launch_bounds__(256) void colloc_jacobian(int nint, colloc::Batch b
   colloc::jacobian(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
 }
 
-__global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const int* iinst, const double* t,
-    const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status, const int* only) {
-  colloc::condense<Cfg::CW>(nint, iinst, t, jac, fa, fb, rows, rec, status, only);
+__global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const int* iinst, const int* ilist,
+    const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
+    const int* only) {
+  colloc::condense<Cfg::CW>(nint, iinst, ilist, t, jac, fa, fb, rows, rec, status, only);
 }
 __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nint, colloc::Batch bt,
     const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
@@ -1176,8 +1298,9 @@ __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nin
   if constexpr (colloc::NS_OK) {
     colloc::condense_ns(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
   } else if (threadIdx.x == 0) {
-    const bool skip = bt.act && !bt.act[bt.iinst[blockIdx.x]];
-    fallback[blockIdx.x] = skip ? 0 : 1;
+    const int i = colloc::listed_interval(bt, blockIdx.x);
+    const bool skip = bt.act && !bt.act[bt.iinst[i]];
+    fallback[i] = skip ? 0 : 1;
     if (!skip) atomicAdd(counters, 1ull);
   }
 }
@@ -1185,6 +1308,11 @@ __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nin
 __global__ __launch_bounds__(colloc::CR_THREADS) void colloc_cr_pair(const int* oinst, const colloc::CRSeg* seg,
     const int* act, const double* rows_in, double* rows_out, double* recs) {
   colloc::cr_pair(oinst, seg, act, rows_in, rows_out, recs);
+}
+
+__global__ __launch_bounds__(colloc::CS_THREADS) void colloc_cr_subst(const int* oinst, const colloc::CRSeg* seg,
+    const int* act, const double* rows_in, double* rows_out, double* recs) {
+  colloc::cr_subst(oinst, seg, act, rows_in, rows_out, recs);
 }
 
 __global__ __launch_bounds__(Cfg::FW * 64) void colloc_cr_final(colloc::Batch bt, const double* rows,

@@ -254,6 +254,12 @@ class CollocationSolver:
         check(lib().cs_solve_babd(self.h, ptr(dy), ptr(dp)))
         return dy, dp
 
+    def set_babd_substitution(self, min_spans):
+        """Cyclic-reduction pairing rule: substitution while a pair spans at
+        most 1/min_spans of an instance's intervals, Householder QR above
+        (0: Householder everywhere, as the reference)."""
+        check(lib().cs_set_babd_substitution(self.h, int(min_spans)))
+
     def recover_delta_k(self):
         """recover_delta_k_parallel (solver:1580): delta_y_dot, delta_z_tilde."""
         nst = (self.N - self.B) * self.m

@@ -153,6 +153,16 @@ int cs_reduce_jacobian(cs_handle* h, double* A, double* H, double* b);
  * delta_p[B][np]. */
 int cs_solve_babd(cs_handle* h, double* delta_y, double* delta_p);
 
+/* Pairing rule of the BABD solve's cyclic reduction.  Every block row that
+ * the condensation produces has C_i = I; while a pair of rows spans at most
+ * 1/min_spans of its instance's intervals, the middle node is eliminated by
+ * substitution (x_M = b_a - G_a x_L - H_a p, one small matrix product per
+ * pair, and C stays I), and on the coarser levels by the Householder step of
+ * partition_factorization_kernel (solve_babd_system.py:184-246).
+ * min_spans = 0: Householder on every level.  Default 64.  The reference
+ * always uses Householder QR. */
+int cs_set_babd_substitution(cs_handle* h, int min_spans);
+
 /* recover_delta_k_parallel (solver:1580-1650).  Outputs (may be NULL):
  * delta_y_dot[(N-1)*m][ny], delta_z_tilde[(N-1)*m][nz]. */
 int cs_recover_delta_k(cs_handle* h, double* delta_y_dot, double* delta_z_tilde);

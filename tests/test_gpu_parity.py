@@ -322,3 +322,39 @@ def test_condensation_paths(nint, series):
     assert rel_err(A, A_o) < 1e-11
     assert rel_err(b, b_o) < 1e-11
     assert np.all(H == 0.0)
+
+
+@pytest.mark.parametrize("spans", [64, 0, 8])
+def test_babd_substitution_levels(spans):
+    """Cyclic reduction with the finest levels paired by substitution (C = I
+    rows): at Nint=512 the default (a pair spans <= 1/64 of the intervals)
+    gives 3 substitution levels, 8 gives 6, 0 none (Householder only, as the
+    reference); the Newton direction agrees with the oracle's partitioned
+    QR solve of the same BABD system to 1e-10 in every case."""
+    inst, m = problems.config(1, nint=512)
+    bvp = solver.BvpDae(inst.ocp_text())
+    prob = family_of(inst)
+    N = bvp.N
+    S = solver.CollocationSolver(bvp, m, n_max=N)
+    t = np.linspace(0.0, inst.T, N)
+    y, yd, zt, p = seeded_state(prob, m, N, 11, scale=0.3)
+    S.set_mesh(t)
+    S.set_babd_substitution(spans)
+    S.set_state(y, yd, zt, p)
+    say("babd substitution spans=%d" % spans)
+    S.compute_f_q(0.1, outputs=False)
+    S.construct_jacobian(0.1, outputs=False)
+    S.reduce_jacobian(outputs=False)
+    dy, dp = S.solve_babd()
+    dyd, dz = S.recover_delta_k()
+    assert S.status() == 0
+    S.close()
+    rk = otables.lobatto(m)
+    f_q, yt_o, fa_o, fb_o, rbc_o = osolver.compute_f_q(prob, m, N, rk, t, y, yd, zt, p, 0.1)
+    J, V, D, W, B0, Bn, Vn = osolver.construct_jacobian(prob, m, N, rk, t, y, yt_o, zt, p, 0.1)
+    A_o, C_o, H_o, b_o = osolver.reduce_jacobian(prob, m, N, W, D, J, V, fa_o, fb_o)
+    dy_o, dp_o = obabd.solve_babd(prob.size_y, prob.size_p, N, A_o, C_o, H_o, b_o, B0, Bn, Vn, rbc_o, M=8)
+    assert rel_err(dy, dy_o) < 1e-10
+    assert rel_err(dp, dp_o) < 1e-10
+    _, dyd_o, dz_o = osolver.recover_delta_k(prob, m, N, dy_o, dp_o, fa_o, J, V, W)
+    assert rel_err(dyd, dyd_o) < 1e-10 and rel_err(dz, dz_o) < 1e-10
