@@ -30,6 +30,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "collocation-solver-cuda_amd")
 FP64_PEAK_TFLOPS = 78.6     # MI355X, FP64 vector = FP64 matrix peak
+FP32_PEAK_TFLOPS = 157.3    # MI355X, FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 
 
@@ -116,8 +117,10 @@ def cpu_baseline(inst, m, nint_sample, nint_target):
             "seconds": dt, "converged": ok}
 
 
-def condense_roofline(S, before, after, cond_ms, cond_n, ny, nz, np_, npa, m, traffic):
-    """Roofline entry of the condensation from the executed-flop counters."""
+def condense_roofline(S, before, after, cond_ms, cond_n, ny, nz, np_, npa, m, traffic, peak=None):
+    """Roofline entry of the condensation from the executed-flop counters
+    (peak: the arithmetic's peak, FP64 78.6 or FP32 157.3 TFLOP/s)."""
+    peak = peak or FP64_PEAK_TFLOPS
     fb = after[0] - before[0]
     terms = after[1] - before[1]
     ncond = after[2] - before[2]
@@ -127,8 +130,8 @@ def condense_roofline(S, before, after, cond_ms, cond_n, ny, nz, np_, npa, m, tr
     avg_s = cond_ms / max(cond_n, 1) / 1e3
     achieved = fl / avg_s / 1e12 if avg_s > 0 else 0.0
     dense = dense_condense_flops(ny, nz, np_, m) * ncond / (cond_ms / 1e3) / 1e12 if cond_ms > 0 else 0.0
-    return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
+    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak,
             "traffic": traffic * ncond / max(cond_n, 1) if traffic else None,
             "traffic_bytes_per_interval": traffic,
             "kernel": "colloc_condense_ns (+ colloc_condense for intervals the series leaves)",
@@ -160,6 +163,10 @@ def main():
                     help="batch: network warm start (default) or the plain guess with continuation")
     ap.add_argument("--model", default=None, help="batch, learned: trained model (default: the family's)")
     ap.add_argument("--plain-pass", type=int, default=1, help="batch, learned: also time one plain batched pass")
+    ap.add_argument("--precision", type=int, choices=(32, 64), default=32,
+                    help="batch: arithmetic of the Newton direction's condensation (32: the FP32 direction)")
+    ap.add_argument("--fp64-pass", type=int, default=1,
+                    help="batch, learned, precision 32: also time one learned pass with the float64 direction")
     ap.add_argument("--warm-cap", type=int, default=12,
                     help="batch, learned: Newton iterations from the network's guess before the fallback")
     ap.add_argument("--cpu-sample", type=int, default=128, help="Nint of the CPU baseline sample (0: skip)")
@@ -191,6 +198,7 @@ def batch_workload(args):
     chunks = solver.plan_chunks(nodes, args.chunk_nodes, len(insts))
     S = solver.CollocationSolver(bvp, m, n_max=max(sum(nodes[i] for i in ch) for ch in chunks),
                                  batch=max(len(ch) for ch in chunks))
+    S.set_direction_precision(args.precision)
     ts_all = [np.linspace(0.0, inst.T, inst.nodes) for inst in insts]
     cons_all = np.array([inst.constant_values() for inst in insts])
     hom_all = [np.array([inst.constant_values(s=0.0) for inst in insts])]
@@ -259,15 +267,26 @@ def batch_workload(args):
     conv, its, n_back = results[-1]
     npa = len(jit.unit_info(bvp.ocp_text, m)["active_params"])
     roof = condense_roofline(S, st0, st1, prof["condense"][0], prof["condense"][1], ny, nz, np_, npa, m,
-                             measured_traffic("batch_%s" % args.form))
+                             measured_traffic("batch_%s" % args.form),
+                             peak=FP32_PEAK_TFLOPS if args.precision == 32 else FP64_PEAK_TFLOPS)
+    roof["arithmetic"] = "fp%d" % args.precision
     extra = {}
+    if args.form == "learned" and args.precision == 32 and args.fp64_pass:
+        # the same learned + batched solve with the float64 direction
+        S.set_direction_precision(64)
+        t0 = time.perf_counter()
+        c64, it64, b64 = learned_pass()
+        lt = time.perf_counter() - t0
+        extra.update({"learned_fp64_solves_per_s": c64.sum() / lt, "learned_fp64_converged": int(c64.sum()),
+                      "learned_fp64_fallback": b64})
     if args.form == "learned" and args.plain_pass:
-        # the plain batched form on the same instances, for the split of the gain
+        # the plain batched form (float64 direction) on the same instances, for the split of the gain
+        S.set_direction_precision(64)
         t0 = time.perf_counter()
         pconv, pits, _ = plain_pass()
         pt = time.perf_counter() - t0
-        extra = {"plain_batched_solves_per_s": pconv.sum() / pt, "plain_batched_converged": int(pconv.sum()),
-                 "plain_newton_iterations_mean": pits.sum(axis=0).mean()}
+        extra.update({"plain_batched_solves_per_s": pconv.sum() / pt, "plain_batched_converged": int(pconv.sum()),
+                      "plain_newton_iterations_mean": pits.sum(axis=0).mean()})
     S.close()
 
     # the plain single-instance solve looped over a sample of the same instances
@@ -293,9 +312,12 @@ def batch_workload(args):
                 "solver solutions) + at most %d Newton iterations at the final constants, plain guess + homotopy as "
                 "the fallback" % args.warm_cap if args.form == "learned"
                 else "plain all-ones guess + homotopy S: 0->0.5")
+    form_txt += ("; Newton direction condensed in float32 (residuals, BABD, convergence test in float64)"
+                 if args.precision == 32 else "; float64 throughout")
     line = {
         "metric": "converged BVP solves/s @ residual <=1e-8, %s vs plain FP64, 1 GPU"
-                  % ("learned+batched" if args.form == "learned" else "batched"),
+                  % (("learned+batched" if args.form == "learned" else "batched")
+                     + ("+FP32-direction" if args.precision == 32 else "")),
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -305,12 +327,12 @@ def batch_workload(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == 64 else "f64 (residuals, BABD, convergence) + f32 (direction condensation)",
         "data": "synthetic (seeded OCP family, generated in-process)",
         "config": {"workload": "batched family: %d instances of the tanh OCP n_x=20 n_u=10 (ny=40 nz=10 np=40), "
                                "shared A,B,C (seed 1), x0~U[-1,1]^20, T~U[0.5,2], Nint~U{256..2048}, m=%d; %s; "
                                "%d chunk(s) of <= %d nodes" % (len(insts), m, form_txt, len(chunks), args.chunk_nodes),
-                   "form": args.form, "instances": len(insts), "mean_nint": mean_nint, "m": m, "tolerance": tol,
+                   "form": args.form, "direction_precision": args.precision, "instances": len(insts), "mean_nint": mean_nint, "m": m, "tolerance": tol,
                    "converged_per_pass": int(conv.sum()), "fallback_per_pass": n_back, "chunks": len(chunks),
                    "newton_iterations_mean": float(it_mean),
                    "looped_plain_solves_per_s": loop_rate, "looped_sample": len(sample),

@@ -57,13 +57,14 @@ struct CRSeg {
 };
 
 enum Fn {
-  F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_COND, F_COND_NS, F_CR_PAIR, F_CR_SUBST, F_CR_FINAL, F_CR_BACK,
-  F_RECOVER, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
+  F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_COND, F_COND_NS, F_COND_NS32, F_CR_PAIR, F_CR_SUBST, F_CR_FINAL,
+  F_CR_BACK, F_RECOVER, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
   "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian",
-  "colloc_condense", "colloc_condense_ns", "colloc_cr_pair", "colloc_cr_subst", "colloc_cr_final",
-  "colloc_cr_back", "colloc_recover", "colloc_axpy4", "colloc_init_input", "colloc_segres", "colloc_recover_z"};
+  "colloc_condense", "colloc_condense_ns", "colloc_condense_ns32", "colloc_cr_pair", "colloc_cr_subst",
+  "colloc_cr_final", "colloc_cr_back", "colloc_recover", "colloc_axpy4", "colloc_init_input", "colloc_segres",
+  "colloc_recover_z"};
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
@@ -123,6 +124,7 @@ struct cs_handle {
   std::vector<int> lev_nout;          // output rows per level
   std::vector<char> lev_qr, lev_subst;  // level has instances paired by QR / by substitution
   int subst_spans = kDefaultSubstSpans; // see cs_set_babd_substitution (0: Householder everywhere)
+  int dir_bits = 64;                    // condensation arithmetic of the Newton direction (64 or 32)
   std::vector<size_t> lev_obase;      // offset of the level in cr_oinst
   std::vector<size_t> lev_sbase;      // offset of the level in cr_seg
   unsigned long long cond_fallback = 0;   // intervals condensed by Gauss-Jordan since creation
@@ -297,7 +299,7 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
   TRY(fill_u64(h, counters, 2));
   Batch bt = h->batch_listed(mask);
   void* args_ns[] = {&nint, &bt, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &h->fbk, &counters};
-  TRY(launch(h, F_COND_NS, (unsigned)h->launch_intervals(), kNsThreads, args_ns));
+  TRY(launch(h, h->dir_bits == 32 ? F_COND_NS32 : F_COND_NS, (unsigned)h->launch_intervals(), kNsThreads, args_ns));
   unsigned long long cnt[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -1019,6 +1021,13 @@ int cs_node_solution_device(cs_handle* h, const double** y, const double** z, co
   if (y) *y = h->y();
   if (z) *z = h->zbuf;
   if (p) *p = h->p();
+  return CS_OK;
+}
+
+int cs_set_direction_precision(cs_handle* h, int bits) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (bits != 32 && bits != 64) return fail(CS_E_ARG, "precision %d bits (32 or 64)", bits);
+  h->dir_bits = bits;
   return CS_OK;
 }
 

@@ -117,11 +117,23 @@ __device__ __forceinline__ double fast_rcp(double x) {
 
 // NaN-propagating max (fmax would drop a NaN and hide a diverged residual).
 __device__ __forceinline__ double nmax(double a, double b) { return (a != a || a > b) ? a : b; }
+__device__ __forceinline__ float nmax(float a, float b) { return (a != a || a > b) ? a : b; }
 
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off));
   return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off));
+  return v;
+}
+
+// Broadcast a value held by lane `src` (uniform), either precision.
+__device__ __forceinline__ double readlane_r(double v, int src) { return readlane_d(v, src); }
+__device__ __forceinline__ float readlane_r(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

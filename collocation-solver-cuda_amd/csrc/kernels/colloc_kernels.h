@@ -226,12 +226,13 @@ __device__ __forceinline__ void jacobian(int nint, Batch bt, const double* __res
 }
 
 // Stage eliminations of one interval (the first phase of both condensation
-// kernels): wave j eliminates the algebraic unknowns of stage j.  Writes T_j
-// to sT (LDS, rows of NVA) and recT, G_j to recG; returns nonzero on a zero
-// pivot in some g_z.
-template <int NW>
+// kernels): wave j eliminates the algebraic unknowns of stage j, computing
+// in R (double, or float for the FP32 direction).  Writes T_j to sT (LDS,
+// rows of NVA) and recT, G_j to recG (float64 records); returns nonzero on a
+// zero pivot in some g_z.
+template <class R, int NW>
 __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__ jac, const double* __restrict__ fa,
-                                               double* sT, double* __restrict__ recT, double* __restrict__ recG) {
+                                               R* sT, double* __restrict__ recT, double* __restrict__ recG) {
   const int lane = lane_id(), wave = wave_id();
   int bad = 0;
   // Lane l holds columns l + 64 s of [g_z | g_y g_p f_a,g] (NZ rows each);
@@ -244,28 +245,28 @@ __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__
     const double* Dfj = jac + ((size_t)i * M + j) * JSTAGE;
     const double* Dgj = Dfj + NY * NJC;
     const double* faj = fa + (size_t)i * M * NYZ + j * NYZ;
-    double gc[SL][NZ1];
+    R gc[SL][NZ1];
 #pragma unroll
     for (int sl = 0; sl < SL; ++sl) {
       const int cc = lane + WAVE * sl;
 #pragma unroll
       for (int r = 0; r < NZ; ++r) {
-        double v = 0.0;
+        double v = 0.0;   // loaded in float64, kept in R
         if (cc < NZ) v = Dgj[r * NJC + NY + cc];
         else if (cc < NCZ) {
           const int q = cc - NZ;
           v = q < NY ? Dgj[r * NJC + q] : (q < NY + NPA ? Dgj[r * NJC + NY + NZ + pact(q - NY)] : faj[NY + r]);
         }
-        gc[sl][r] = v;
+        gc[sl][r] = (R)v;
       }
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
-      double ck[NZ1];
+      R ck[NZ1];
 #pragma unroll
-      for (int r = 0; r < NZ; ++r) ck[r] = readlane_d(gc[0][r], k);
+      for (int r = 0; r < NZ; ++r) ck[r] = readlane_r(gc[0][r], k);
       int pr = k;
-      double best = fabs(ck[k]);
+      R best = fabs(ck[k]);
 #pragma unroll
       for (int r = k + 1; r < NZ; ++r)
         if (fabs(ck[r]) > best) { best = fabs(ck[r]); pr = r; }
@@ -273,22 +274,22 @@ __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__
 #pragma unroll
       for (int r = k + 1; r < NZ; ++r) {
         const bool sw = r == pr;
-        const double t0 = ck[r];
+        const R t0 = ck[r];
         ck[r] = sw ? ck[k] : ck[r];
         ck[k] = sw ? t0 : ck[k];
 #pragma unroll
         for (int sl = 0; sl < SL; ++sl) {
-          const double t1 = gc[sl][r];
+          const R t1 = gc[sl][r];
           gc[sl][r] = sw ? gc[sl][k] : gc[sl][r];
           gc[sl][k] = sw ? t1 : gc[sl][k];
         }
       }
-      const double cpr = ck[k];
-      if (!(fabs(cpr) > 0.0)) bad |= 1;
-      const double inv = 1.0 / cpr;
+      const R cpr = ck[k];
+      if (!(fabs(cpr) > (R)0)) bad |= 1;
+      const R inv = (R)1 / cpr;
 #pragma unroll
       for (int sl = 0; sl < SL; ++sl) {
-        const double pk = gc[sl][k] * inv;
+        const R pk = gc[sl][k] * inv;
 #pragma unroll
         for (int r = 0; r < NZ; ++r) gc[sl][r] = (r == k) ? pk : fma(-ck[r], pk, gc[sl][r]);
       }
@@ -299,25 +300,25 @@ __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__
       const int q = lane + WAVE * sl - NZ;
       if (q < 0 || q >= NVA) continue;
 #pragma unroll
-      for (int r = 0; r < NZ; ++r) recG[(j * NZ + r) * NVA + q] = gc[sl][r];
+      for (int r = 0; r < NZ; ++r) recG[(j * NZ + r) * NVA + q] = (double)gc[sl][r];
     }
     // T_j = [h_y | h_p | f_a,h] - h_z G_j, rows in batches of TB whose
     // Jacobian loads are all issued before any of them is used
     constexpr int TB = 5;
     for (int r0 = 0; r0 < NY; r0 += TB) {
-      double hz[TB][NZ1], hb[TB][SL];
+      R hz[TB][NZ1], hb[TB][SL];
 #pragma unroll
       for (int rr = 0; rr < TB; ++rr) {
         const int r = r0 + rr < NY ? r0 + rr : NY - 1;
         const double* hrow = Dfj + r * NJC;
 #pragma unroll
-        for (int z = 0; z < NZ; ++z) hz[rr][z] = hrow[NY + z];
+        for (int z = 0; z < NZ; ++z) hz[rr][z] = (R)hrow[NY + z];
 #pragma unroll
         for (int sl = 0; sl < SL; ++sl) {
           const int q = lane + WAVE * sl - NZ;
           // h_y, then the active h_p columns after the NZ z-columns
           const int src = q < NY ? q : NY + NZ + pact(q - NY < NPA ? q - NY : 0);
-          hb[rr][sl] = (q >= 0 && q < NY + NPA) ? hrow[src] : (q == NY + NPA ? faj[r] : 0.0);
+          hb[rr][sl] = (R)((q >= 0 && q < NY + NPA) ? hrow[src] : (q == NY + NPA ? faj[r] : 0.0));
         }
       }
 #pragma unroll
@@ -328,11 +329,11 @@ __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__
         for (int sl = 0; sl < SL; ++sl) {
           const int q = lane + WAVE * sl - NZ;
           if (q < 0 || q >= NVA) continue;
-          double v = hb[rr][sl];
+          R v = hb[rr][sl];
 #pragma unroll
           for (int z = 0; z < NZ; ++z) v = fma(-hz[rr][z], gc[sl][z], v);
           sT[(j * NY + r) * NVA + q] = v;
-          recT[(j * NY + r) * NVA + q] = v;
+          recT[(j * NY + r) * NVA + q] = (double)v;
         }
       }
     }
@@ -392,7 +393,7 @@ __device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst
   int bad = 0;
 
   // ---- phase 1: wave j eliminates the algebraic unknowns of stage j.
-  bad |= stage_eliminate<CW>(i, jac, fa, sT, recT, recG);
+  bad |= stage_eliminate<double, CW>(i, jac, fa, sT, recT, recG);
   __syncthreads();
 
   // ---- phase 2: assemble [S | RHS] into registers
@@ -496,9 +497,22 @@ constexpr int NS_CPT = 24;                                 // columns per work i
 constexpr int NS_NH = (NVA + NS_CPT - 1) / NS_CPT;
 constexpr int NVAP = NS_NH * NS_CPT;                       // padded row of the U buffer
 constexpr int NS_ITEMS = NSK * NY * NS_NH;
-constexpr bool NS_OK = NY <= 48 && NS_ITEMS <= NS_THREADS &&
-                       (M * NY * NVA + NSK * NY * NVAP) * 8 <= 150 * 1024;
+template <class R>
+constexpr bool ns_ok() {
+  return NY <= 48 && NS_ITEMS <= NS_THREADS && (M * NY * NVA + NSK * NY * NVAP) * (int)sizeof(R) <= 150 * 1024;
+}
+constexpr bool NS_OK = ns_ok<double>();
+// Series convergence: a term changes U by at most this fraction of its
+// largest entry (a few units of the arithmetic's rounding).
+template <class R> __device__ __forceinline__ R ns_tol();
+template <> __device__ __forceinline__ double ns_tol<double>() { return 0x1p-50; }
+template <> __device__ __forceinline__ float ns_tol<float>() { return 0x1p-21f; }
 
+// R = double: the float64 condensation.  R = float: the FP32 direction (the
+// stage eliminations and the series in float, the records and the BABD row
+// stored in float64); an interval whose float series does not converge is
+// left to the float64 Gauss-Jordan kernel, like a float64 one.
+template <class R>
 __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __restrict__ t,
                                             const double* __restrict__ jac,
                                             const double* __restrict__ fa, const double* __restrict__ fb,
@@ -508,10 +522,10 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
   constexpr int NWV = NS_THREADS / WAVE;
   // one LDS array with buf at its start, so that the U-buffer addresses fit
   // the 16-bit offsets of the LDS instructions
-  __shared__ double smem_ns[NSK * NY * NVAP + M * NY * NVA];
-  double* buf = smem_ns;
-  double* sT = smem_ns + NSK * NY * NVAP;
-  __shared__ double sred[2 * NWV];
+  __shared__ R smem_ns[NSK * NY * NVAP + M * NY * NVA];
+  R* buf = smem_ns;
+  R* sT = smem_ns + NSK * NY * NVAP;
+  __shared__ R sred[2 * NWV];
   __shared__ int sbad;
   const int i = listed_interval(bt, blockIdx.x);
   const int tid = threadIdx.x;
@@ -522,11 +536,11 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
     return;
   }
   if (tid == 0) sbad = 0;
-  const double h = t[i + ib + 1] - t[i + ib];
+  const R h = (R)(t[i + ib + 1] - t[i + ib]);
   double* recU = rec + (size_t)i * REC;
   double* recT = recU + NS * NVA;
   double* recG = recT + M * NY * NVA;
-  const int bad = stage_eliminate<NWV>(i, jac, fa, sT, recT, recG);
+  const int bad = stage_eliminate<R, NWV>(i, jac, fa, sT, recT, recG);
   __syncthreads();
 
   const bool act = tid < NS_ITEMS;
@@ -536,24 +550,24 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
   const int r = rem / NS_NH;
   const int c0 = (rem - r * NS_NH) * NS_CPT;
   const int js = k + 1;
-  double wk[NSK];                      // h A[k+1][k'+1]
+  R wk[NSK];                           // h A[k+1][k'+1]
 #pragma unroll
-  for (int q = 0; q < NSK; ++q) wk[q] = h * c_rkA[js * M + q + 1];
+  for (int q = 0; q < NSK; ++q) wk[q] = h * (R)c_rkA[js * M + q + 1];
   // R = h A[k+1][0] T_0 + (c < NY ? I : h sum_j A[k+1][j] T_j), kept in
   // registers; it is also the first iterate
-  double rv[NS_CPT], xo[NS_CPT];
-  double* mine = buf + (k * NY + r) * NVAP;
+  R rv[NS_CPT], xo[NS_CPT];
+  R* mine = buf + (k * NY + r) * NVAP;
 #pragma unroll
   for (int cc = 0; cc < NS_CPT; ++cc) {
     const int c = c0 + cc < NVA ? c0 + cc : NVA - 1;
-    double v = h * c_rkA[js * M] * sT[r * NVA + c];
+    R v = h * (R)c_rkA[js * M] * sT[r * NVA + c];
     if (c < NY) {
-      if (c == r) v += 1.0;
+      if (c == r) v += (R)1;
     } else {
 #pragma unroll
-      for (int j = 1; j < M; ++j) v += h * c_rkA[js * M + j] * sT[(j * NY + r) * NVA + c];
+      for (int j = 1; j < M; ++j) v += h * (R)c_rkA[js * M + j] * sT[(j * NY + r) * NVA + c];
     }
-    if (c0 + cc >= NVA) v = 0.0;       // padding columns stay zero
+    if (c0 + cc >= NVA) v = (R)0;      // padding columns stay zero
     rv[cc] = v;
     xo[cc] = v;
     if (act) mine[c0 + cc] = v;
@@ -567,17 +581,17 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
     // LDS every term through an opaque offset (not hoisted into registers).
     int moff = (js * NY + r) * NVA;
     asm volatile("" : "+v"(moff));
-    double acc[NS_CPT];
+    R acc[NS_CPT];
 #pragma unroll
-    for (int cc = 0; cc < NS_CPT; ++cc) acc[cc] = 0.0;
-    const double* Uk = buf + k * NY * NVAP + c0;
+    for (int cc = 0; cc < NS_CPT; ++cc) acc[cc] = (R)0;
+    const R* Uk = buf + k * NY * NVAP + c0;
     // a runtime loop over the rows of U_k, two rows per step: each step's
     // loads and FMAs are scheduled together (a fully unrolled product keeps
     // every load live)
 #pragma nounroll
     for (int q = 0; q < NY; ++q) {
-      const double mq = sT[moff + q];
-      double u[NS_CPT];
+      const R mq = sT[moff + q];
+      R u[NS_CPT];
 #pragma unroll
       for (int cc = 0; cc < NS_CPT; ++cc) u[cc] = Uk[q * NVAP + cc];
 #pragma unroll
@@ -589,10 +603,10 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
       if (act) mine[c0 + cc] = acc[cc];
     __syncthreads();
     // U_new = R + h (A^ x I) Y
-    double dmax = 0.0, xmax = 0.0;
+    R dmax = (R)0, xmax = (R)0;
 #pragma unroll
     for (int cc = 0; cc < NS_CPT; ++cc) {
-      double v = rv[cc];
+      R v = rv[cc];
 #pragma unroll
       for (int q = 0; q < NSK; ++q) v = fma(wk[q], buf[(q * NY + r) * NVAP + c0 + cc], v);
       acc[cc] = v;
@@ -613,13 +627,13 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
       sred[NWV + wave] = xmax;
     }
     __syncthreads();
-    double dm = sred[0], xm = sred[NWV];
+    R dm = sred[0], xm = sred[NWV];
 #pragma unroll
     for (int w = 1; w < NWV; ++w) {
       dm = nmax(sred[w], dm);
       xm = nmax(sred[NWV + w], xm);
     }
-    if (dm <= 0x1p-50 * xm) {
+    if (dm <= ns_tol<R>() * xm) {
       converged = true;
       break;
     }
@@ -641,7 +655,7 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
     for (int cc = 0; cc < NS_CPT; ++cc) {
       const int c = c0 + cc;
       if (c >= NVA) continue;
-      const double v = xo[cc];
+      const double v = (double)xo[cc];
       recU[(k * NY + r) * NVA + c] = v;
       if (k == NSK - 1) {
         if (c < NY) rowi[r * CRW + c] = -v;
@@ -1296,7 +1310,22 @@ __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nin
     const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
     int* fallback, unsigned long long* counters) {
   if constexpr (colloc::NS_OK) {
-    colloc::condense_ns(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
+    colloc::condense_ns<double>(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
+  } else if (threadIdx.x == 0) {
+    const int i = colloc::listed_interval(bt, blockIdx.x);
+    const bool skip = bt.act && !bt.act[bt.iinst[i]];
+    fallback[i] = skip ? 0 : 1;
+    if (!skip) atomicAdd(counters, 1ull);
+  }
+}
+
+// The FP32-direction condensation: two 512-thread workgroups per CU (half the
+// LDS and registers of the float64 kernel).
+__global__ __launch_bounds__(colloc::NS_THREADS, 4) void colloc_condense_ns32(int nint, colloc::Batch bt,
+    const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
+    int* fallback, unsigned long long* counters) {
+  if constexpr (colloc::ns_ok<float>()) {
+    colloc::condense_ns<float>(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
   } else if (threadIdx.x == 0) {
     const int i = colloc::listed_interval(bt, blockIdx.x);
     const bool skip = bt.act && !bt.act[bt.iinst[i]];

@@ -260,6 +260,12 @@ class CollocationSolver:
         (0: Householder everywhere, as the reference)."""
         check(lib().cs_set_babd_substitution(self.h, int(min_spans)))
 
+    def set_direction_precision(self, bits):
+        """Condensation arithmetic of the Newton direction: 64 (default) or
+        32 (an inexact-Newton direction; residuals, BABD solve and the
+        convergence test stay float64)."""
+        check(lib().cs_set_direction_precision(self.h, int(bits)))
+
     def recover_delta_k(self):
         """recover_delta_k_parallel (solver:1580): delta_y_dot, delta_z_tilde."""
         nst = (self.N - self.B) * self.m
