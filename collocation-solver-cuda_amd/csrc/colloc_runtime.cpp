@@ -57,12 +57,13 @@ struct CRSeg {
 };
 
 enum Fn {
-  F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_COND, F_COND_NS, F_COND_NS32, F_CR_PAIR, F_CR_SUBST, F_CR_FINAL,
-  F_CR_BACK, F_RECOVER, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
+  F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_JAC32, F_COND, F_COND32, F_COND_NS, F_COND_NS32, F_CR_PAIR, F_CR_SUBST,
+  F_CR_FINAL, F_CR_BACK, F_RECOVER, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
-  "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian",
-  "colloc_condense", "colloc_condense_ns", "colloc_condense_ns32", "colloc_cr_pair", "colloc_cr_subst",
+  "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian", "colloc_jacobian32",
+  "colloc_condense", "colloc_condense32", "colloc_condense_ns", "colloc_condense_ns32", "colloc_cr_pair",
+  "colloc_cr_subst",
   "colloc_cr_final", "colloc_cr_back", "colloc_recover", "colloc_axpy4", "colloc_init_input", "colloc_segres",
   "colloc_recover_z"};
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
@@ -125,6 +126,7 @@ struct cs_handle {
   std::vector<char> lev_qr, lev_subst;  // level has instances paired by QR / by substitution
   int subst_spans = kDefaultSubstSpans; // see cs_set_babd_substitution (0: Householder everywhere)
   int dir_bits = 64;                    // condensation arithmetic of the Newton direction (64 or 32)
+  int jac_bits = 64;                    // stored type of the last stage Jacobians (64 or 32)
   std::vector<size_t> lev_obase;      // offset of the level in cr_oinst
   std::vector<size_t> lev_sbase;      // offset of the level in cr_seg
   unsigned long long cond_fallback = 0;   // intervals condensed by Gauss-Jordan since creation
@@ -284,7 +286,9 @@ int jacobian(cs_handle* h, double alpha, const int* mask) {
   Batch bt = h->batch_listed(mask);
   void* args[] = {&nint, &bt, &y, &h->ytil, &zt, &p, &h->c, &alpha, &h->jac, &h->Dr};
   unsigned nthreads = (unsigned)(h->launch_intervals() * h->m + h->B);
-  return launch(h, F_JAC, (nthreads + 255) / 256, 256, args);
+  // the FP32 direction stores the stage Jacobians in float32 (in the same buffer)
+  h->jac_bits = h->dir_bits;
+  return launch(h, h->jac_bits == 32 ? F_JAC32 : F_JAC, (nthreads + 255) / 256, 256, args);
 }
 
 // Condensation: the series kernel handles every interval whose stage system
@@ -299,7 +303,9 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
   TRY(fill_u64(h, counters, 2));
   Batch bt = h->batch_listed(mask);
   void* args_ns[] = {&nint, &bt, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &h->fbk, &counters};
-  TRY(launch(h, h->dir_bits == 32 ? F_COND_NS32 : F_COND_NS, (unsigned)h->launch_intervals(), kNsThreads, args_ns));
+  // the arithmetic follows the stored type of the stage Jacobians
+  const bool f32 = h->jac_bits == 32;
+  TRY(launch(h, f32 ? F_COND_NS32 : F_COND_NS, (unsigned)h->launch_intervals(), kNsThreads, args_ns));
   unsigned long long cnt[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -310,7 +316,7 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
   const int* only = h->fbk;
   const int* ilist = bt.ilist;
   void* args[] = {&nint, &h->iinst, &ilist, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &only};
-  return launch(h, F_COND, (unsigned)h->launch_intervals(), (unsigned)(h->cw * 64), args);
+  return launch(h, f32 ? F_COND32 : F_COND, (unsigned)h->launch_intervals(), (unsigned)(h->cw * 64), args);
 }
 
 // Cyclic-reduction schedule of the current mesh (host side, once per mesh):
@@ -910,7 +916,14 @@ int cs_construct_jacobian(cs_handle* h, double alpha, double* Df, double* Dg, do
   const size_t nf = (size_t)h->ny * h->nj, ng = (size_t)h->nz * h->nj;
   if (Df || Dg) {
     std::vector<double> buf(nst * h->jstage);
-    TRY(copy_out(h, buf.data(), h->jac, buf.size()));
+    if (h->jac_bits == 32) {   // stored in float32: widen
+      std::vector<float> fbuf(buf.size());
+      HIPCHK(hipMemcpyAsync(fbuf.data(), h->jac, fbuf.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+      TRY(sync(h));
+      for (size_t e = 0; e < buf.size(); ++e) buf[e] = fbuf[e];
+    } else {
+      TRY(copy_out(h, buf.data(), h->jac, buf.size()));
+    }
     TRY(sync(h));
     for (size_t s = 0; s < nst; ++s) {
       if (Df) std::memcpy(Df + s * nf, buf.data() + s * h->jstage, nf * sizeof(double));

@@ -10,7 +10,8 @@
 // end to end (N nodes and Nint = N - B intervals in total, see Batch):
 //   t[N]; y[N][NY]; ydot[Nint*M][NY]; ztil[Nint*M][NZ]; p[B][NP]
 //   ytil[Nint*M][NY]; fa[Nint][M*(NY+NZ)]; fb[Nint][NY]; rbc[B][NY+NP]
-//   jac[Nint*M][JSTAGE]      stage block = Df (NY x NJC) then Dg (NZ x NJC)
+//   jac[Nint*M][JSTAGE]      stage block = Df (NY x NJC) then Dg (NZ x NJC), float64
+//                            (float32 for the FP32 direction)
 //   rows[Nint][NY][CRW]      BABD row  [G | C | H | b]  (G x_left + C x_right + H p = b)
 //   rec[Nint][REC]           condensation record [U (NS x NVA) | T (M NY x NVA) | G (M NZ x NVA)]
 //   crrec[Nint-1][NY][RECW]  cyclic-reduction record [R | Gp | Ep | Jp | d]
@@ -196,10 +197,13 @@ __device__ __forceinline__ void residual(int nint, Batch bt, const double* __res
 // One thread per stage writes Df, Dg densely; thread Nint*M + b writes Dr of
 // instance b.  Restates construct_jacobian_kernel (solver:680-797) without
 // materialising J, V, D, W (the condensation kernel assembles what it needs).
+// TJ: the stored type of the stage Jacobians (double, or float for the FP32
+// direction; evaluated in float64 either way).
+template <class TJ>
 __device__ __forceinline__ void jacobian(int nint, Batch bt, const double* __restrict__ y,
                          const double* __restrict__ ytil, const double* __restrict__ ztil,
                          const double* __restrict__ p, const double* __restrict__ c, double alpha,
-                         double* __restrict__ jac, double* __restrict__ Dr) {
+                         TJ* __restrict__ jac, double* __restrict__ Dr) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nst = launch_intervals(bt, nint) * M;
   if (gid < nst) {
@@ -212,7 +216,7 @@ __device__ __forceinline__ void jacobian(int nint, Batch bt, const double* __res
     for (int l = 0; l < NY; ++l) yt[l] = ytil[(size_t)st * NY + l];
 #pragma unroll
     for (int l = 0; l < NZ; ++l) zt[l] = ztil[(size_t)st * NZ + l];
-    double* out = jac + (size_t)st * JSTAGE;
+    TJ* out = jac + (size_t)st * JSTAGE;
     const double* pb = p + (size_t)b * NP;
     const double* cb = c + (size_t)b * NC;
     prob::Df(yt, zt, pb, cb, out);
@@ -230,8 +234,8 @@ __device__ __forceinline__ void jacobian(int nint, Batch bt, const double* __res
 // in R (double, or float for the FP32 direction).  Writes T_j to sT (LDS,
 // rows of NVA) and recT, G_j to recG (float64 records); returns nonzero on a
 // zero pivot in some g_z.
-template <class R, int NW>
-__device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__ jac, const double* __restrict__ fa,
+template <class R, int NW, class TJ>
+__device__ __forceinline__ int stage_eliminate(int i, const TJ* __restrict__ jac, const double* __restrict__ fa,
                                                R* sT, double* __restrict__ recT, double* __restrict__ recG) {
   const int lane = lane_id(), wave = wave_id();
   int bad = 0;
@@ -242,8 +246,8 @@ __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__
   constexpr int NCZ = NZ + NVA;
   constexpr int SL = (NCZ + WAVE - 1) / WAVE;
   for (int j = wave; j < M; j += NW) {
-    const double* Dfj = jac + ((size_t)i * M + j) * JSTAGE;
-    const double* Dgj = Dfj + NY * NJC;
+    const TJ* Dfj = jac + ((size_t)i * M + j) * JSTAGE;
+    const TJ* Dgj = Dfj + NY * NJC;
     const double* faj = fa + (size_t)i * M * NYZ + j * NYZ;
     R gc[SL][NZ1];
 #pragma unroll
@@ -251,11 +255,12 @@ __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__
       const int cc = lane + WAVE * sl;
 #pragma unroll
       for (int r = 0; r < NZ; ++r) {
-        double v = 0.0;   // loaded in float64, kept in R
-        if (cc < NZ) v = Dgj[r * NJC + NY + cc];
+        double v = 0.0;   // loaded as stored, kept in R
+        if (cc < NZ) v = (double)Dgj[r * NJC + NY + cc];
         else if (cc < NCZ) {
           const int q = cc - NZ;
-          v = q < NY ? Dgj[r * NJC + q] : (q < NY + NPA ? Dgj[r * NJC + NY + NZ + pact(q - NY)] : faj[NY + r]);
+          v = q < NY ? (double)Dgj[r * NJC + q]
+                     : (q < NY + NPA ? (double)Dgj[r * NJC + NY + NZ + pact(q - NY)] : faj[NY + r]);
         }
         gc[sl][r] = (R)v;
       }
@@ -310,7 +315,7 @@ __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__
 #pragma unroll
       for (int rr = 0; rr < TB; ++rr) {
         const int r = r0 + rr < NY ? r0 + rr : NY - 1;
-        const double* hrow = Dfj + r * NJC;
+        const TJ* hrow = Dfj + r * NJC;
 #pragma unroll
         for (int z = 0; z < NZ; ++z) hz[rr][z] = (R)hrow[NY + z];
 #pragma unroll
@@ -318,7 +323,7 @@ __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__
           const int q = lane + WAVE * sl - NZ;
           // h_y, then the active h_p columns after the NZ z-columns
           const int src = q < NY ? q : NY + NZ + pact(q - NY < NPA ? q - NY : 0);
-          hb[rr][sl] = (R)((q >= 0 && q < NY + NPA) ? hrow[src] : (q == NY + NPA ? faj[r] : 0.0));
+          hb[rr][sl] = (R)((q >= 0 && q < NY + NPA) ? (double)hrow[src] : (q == NY + NPA ? faj[r] : 0.0));
         }
       }
 #pragma unroll
@@ -359,9 +364,9 @@ __device__ __forceinline__ int stage_eliminate(int i, const double* __restrict__
 //   G = -U_{M-1}[:, :NY], C = I, H = -U_{M-1}[:, NY:NY+NPA] (zero in the other
 //   parameter columns), b = -f_b + U_{M-1}[:, NVA-1].  Only the NPA parameters
 //   the stage functions depend on get columns (g_p = f_p = 0 for the rest).
-template <int CW>
+template <int CW, class TJ>
 __device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst, const int* __restrict__ ilist,
-                         const double* __restrict__ t, const double* __restrict__ jac,
+                         const double* __restrict__ t, const TJ* __restrict__ jac,
                          const double* __restrict__ fa, const double* __restrict__ fb,
                          double* __restrict__ rows, double* __restrict__ rec, int* __restrict__ status,
                          const int* __restrict__ only) {
@@ -393,7 +398,7 @@ __device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst
   int bad = 0;
 
   // ---- phase 1: wave j eliminates the algebraic unknowns of stage j.
-  bad |= stage_eliminate<double, CW>(i, jac, fa, sT, recT, recG);
+  bad |= stage_eliminate<double, CW, TJ>(i, jac, fa, sT, recT, recG);
   __syncthreads();
 
   // ---- phase 2: assemble [S | RHS] into registers
@@ -514,7 +519,7 @@ template <> __device__ __forceinline__ float ns_tol<float>() { return 0x1p-21f; 
 // left to the float64 Gauss-Jordan kernel, like a float64 one.
 template <class R>
 __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __restrict__ t,
-                                            const double* __restrict__ jac,
+                                            const R* __restrict__ jac,
                                             const double* __restrict__ fa, const double* __restrict__ fb,
                                             double* __restrict__ rows, double* __restrict__ rec,
                                             int* __restrict__ status, int* __restrict__ fallback,
@@ -540,7 +545,7 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
   double* recU = rec + (size_t)i * REC;
   double* recT = recU + NS * NVA;
   double* recG = recT + M * NY * NVA;
-  const int bad = stage_eliminate<R, NWV>(i, jac, fa, sT, recT, recG);
+  const int bad = stage_eliminate<R, NWV, R>(i, jac, fa, sT, recT, recG);
   __syncthreads();
 
   const bool act = tid < NS_ITEMS;
@@ -1298,13 +1303,24 @@ __global__ __launch_bounds__(256) void colloc_residual_norm(int nint, colloc::Ba
 
 __global__ __launch_bounds__(256) void colloc_jacobian(int nint, colloc::Batch bt, const double* y,
     const double* ytil, const double* ztil, const double* p, const double* c, double alpha, double* jac, double* Dr) {
-  colloc::jacobian(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
+  colloc::jacobian<double>(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
+}
+// stage Jacobians stored in float32 (the FP32 direction)
+__global__ __launch_bounds__(256) void colloc_jacobian32(int nint, colloc::Batch bt, const double* y,
+    const double* ytil, const double* ztil, const double* p, const double* c, double alpha, float* jac, double* Dr) {
+  colloc::jacobian<float>(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
 }
 
 __global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const int* iinst, const int* ilist,
     const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
     const int* only) {
-  colloc::condense<Cfg::CW>(nint, iinst, ilist, t, jac, fa, fb, rows, rec, status, only);
+  colloc::condense<Cfg::CW, double>(nint, iinst, ilist, t, jac, fa, fb, rows, rec, status, only);
+}
+// the float64 Gauss-Jordan fallback of the FP32 direction (float32 stage Jacobians)
+__global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense32(int nint, const int* iinst, const int* ilist,
+    const double* t, const float* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
+    const int* only) {
+  colloc::condense<Cfg::CW, float>(nint, iinst, ilist, t, jac, fa, fb, rows, rec, status, only);
 }
 __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nint, colloc::Batch bt,
     const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
@@ -1322,7 +1338,7 @@ __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nin
 // The FP32-direction condensation: two 512-thread workgroups per CU (half the
 // LDS and registers of the float64 kernel).
 __global__ __launch_bounds__(colloc::NS_THREADS, 4) void colloc_condense_ns32(int nint, colloc::Batch bt,
-    const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
+    const double* t, const float* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
     int* fallback, unsigned long long* counters) {
   if constexpr (colloc::ns_ok<float>()) {
     colloc::condense_ns<float>(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);

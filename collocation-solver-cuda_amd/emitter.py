@@ -351,8 +351,15 @@ def emit_problem(sym):
              "const double* __restrict__ _c"
     H = sym.hoist_fg
 
-    def fn(name, extra, target, rows, binds, hoister):
-        lines = ["__device__ __forceinline__ void %s(%s%s, double* __restrict__ %s) {" % (name, sig_fg, extra, target)]
+    def fn(name, extra, target, rows, binds, hoister, out_type=False):
+        # out_type: a template on the output type (the Jacobians are stored in
+        # float64, or float32 for the FP32 direction; evaluated in float64)
+        if out_type:
+            lines = ["template <class TO>",
+                     "__device__ __forceinline__ void %s(%s%s, TO* __restrict__ %s) {" % (name, sig_fg, extra, target)]
+        else:
+            lines = ["__device__ __forceinline__ void %s(%s%s, double* __restrict__ %s) {" % (name, sig_fg, extra,
+                                                                                            target)]
         lines.append("  (void)_y; (void)_z; (void)_p; (void)_c;")
         if "_alpha" in extra:
             lines.append("  (void)_alpha;")
@@ -363,9 +370,9 @@ def emit_problem(sym):
     fn("f", "", "_f", [("_f[%d]" % i, e) for i, e in enumerate(sym.f_exprs)], yb + zb + pb, H)
     fn("g", ", double _alpha", "_g", [("_g[%d]" % i, e) for i, e in enumerate(sym.g_exprs)], yb + zb + pb + ab, H)
     fn("Df", "", "_Df", [("_Df[%d]" % (i * nj + j), e) for i, row in enumerate(sym.Df_exprs)
-                         for j, e in enumerate(row)], yb + zb + pb, H)
+                         for j, e in enumerate(row)], yb + zb + pb, H, out_type=True)
     fn("Dg", ", double _alpha", "_Dg", [("_Dg[%d]" % (i * nj + j), e) for i, row in enumerate(sym.Dg_exprs)
-                                        for j, e in enumerate(row)], yb + zb + pb + ab, H)
+                                        for j, e in enumerate(row)], yb + zb + pb + ab, H, out_type=True)
     # r and Dr: initial rows see y(t_i) = _y0, final rows y(t_f) = _y1
     sig_r = "const double* __restrict__ _y0, const double* __restrict__ _y1, const double* __restrict__ _p, " \
             "const double* __restrict__ _c"

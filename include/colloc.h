@@ -155,11 +155,13 @@ int cs_solve_babd(cs_handle* h, double* delta_y, double* delta_p);
 
 /* Arithmetic of the Newton direction's condensation (reduce_jacobian,
  * solver:840-1001): 64 (default; float64 throughout, as the reference) or 32
- * (the stage eliminations and the series solve of the stage systems in
- * float32, an inexact-Newton direction).  Residuals, the BABD solve, the
- * recovery, the line-search merit and the convergence test stay float64;
- * intervals whose float32 series does not converge are condensed in float64.
- * No counterpart in the reference. */
+ * (the stage Jacobians stored in float32 -- evaluated in float64 -- and the
+ * stage eliminations and the series solve of the stage systems in float32:
+ * an inexact-Newton direction).  Residuals, the BABD solve, the recovery,
+ * the line-search merit and the convergence test stay float64; intervals
+ * whose float32 series does not converge are condensed in float64.  Takes
+ * effect at the next Jacobian; cs_construct_jacobian returns the stored
+ * Jacobians widened to float64.  No counterpart in the reference. */
 int cs_set_direction_precision(cs_handle* h, int bits);
 
 /* Pairing rule of the BABD solve's cyclic reduction.  Every block row that
