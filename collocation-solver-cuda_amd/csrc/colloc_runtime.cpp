@@ -58,14 +58,14 @@ struct CRSeg {
 
 enum Fn {
   F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_JAC32, F_COND, F_COND32, F_COND_NS, F_COND_NS32, F_CR_PAIR, F_CR_SUBST,
-  F_CR_FINAL, F_CR_BACK, F_RECOVER, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
+  F_CR_FINAL, F_CR_BACK, F_RECOVER, F_RECOVER32, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
   "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian", "colloc_jacobian32",
   "colloc_condense", "colloc_condense32", "colloc_condense_ns", "colloc_condense_ns32", "colloc_cr_pair",
   "colloc_cr_subst",
-  "colloc_cr_final", "colloc_cr_back", "colloc_recover", "colloc_axpy4", "colloc_init_input", "colloc_segres",
-  "colloc_recover_z"};
+  "colloc_cr_final", "colloc_cr_back", "colloc_recover", "colloc_recover32", "colloc_axpy4", "colloc_init_input",
+  "colloc_segres", "colloc_recover_z"};
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
@@ -127,6 +127,7 @@ struct cs_handle {
   int subst_spans = kDefaultSubstSpans; // see cs_set_babd_substitution (0: Householder everywhere)
   int dir_bits = 64;                    // condensation arithmetic of the Newton direction (64 or 32)
   int jac_bits = 64;                    // stored type of the last stage Jacobians (64 or 32)
+  int rec_bits = 64;                    // stored type of the last condensation records (64 or 32)
   std::vector<size_t> lev_obase;      // offset of the level in cr_oinst
   std::vector<size_t> lev_sbase;      // offset of the level in cr_seg
   unsigned long long cond_fallback = 0;   // intervals condensed by Gauss-Jordan since creation
@@ -303,8 +304,10 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
   TRY(fill_u64(h, counters, 2));
   Batch bt = h->batch_listed(mask);
   void* args_ns[] = {&nint, &bt, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &h->fbk, &counters};
-  // the arithmetic follows the stored type of the stage Jacobians
+  // the arithmetic follows the stored type of the stage Jacobians, and so
+  // does the records' type
   const bool f32 = h->jac_bits == 32;
+  h->rec_bits = h->jac_bits;
   TRY(launch(h, f32 ? F_COND_NS32 : F_COND_NS, (unsigned)h->launch_intervals(), kNsThreads, args_ns));
   unsigned long long cnt[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
@@ -429,7 +432,7 @@ int recover(cs_handle* h, const int* mask) {
   double* dz = h->dir + h->st_z;
   Batch bt = h->batch_listed(mask);
   void* args[] = {&nint, &bt, &h->rec, &dy, &dp, &dyd, &dz};
-  return launch(h, F_RECOVER, (unsigned)h->launch_intervals(), 256, args);
+  return launch(h, h->rec_bits == 32 ? F_RECOVER32 : F_RECOVER, (unsigned)h->launch_intervals(), 256, args);
 }
 
 Seg4 state_segs(cs_handle* h) {

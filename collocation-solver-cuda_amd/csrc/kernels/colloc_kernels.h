@@ -13,7 +13,8 @@
 //   jac[Nint*M][JSTAGE]      stage block = Df (NY x NJC) then Dg (NZ x NJC), float64
 //                            (float32 for the FP32 direction)
 //   rows[Nint][NY][CRW]      BABD row  [G | C | H | b]  (G x_left + C x_right + H p = b)
-//   rec[Nint][REC]           condensation record [U (NS x NVA) | T (M NY x NVA) | G (M NZ x NVA)]
+//   rec[Nint][REC]           condensation record [U (NS x NVA) | T (M NY x NVA) | G (M NZ x NVA)],
+//                            float64 (float32 for the FP32 direction)
 //   crrec[Nint-1][NY][RECW]  cyclic-reduction record [R | Gp | Ep | Jp | d]
 #pragma once
 #include "colloc_common.h"
@@ -232,11 +233,11 @@ __device__ __forceinline__ void jacobian(int nint, Batch bt, const double* __res
 // Stage eliminations of one interval (the first phase of both condensation
 // kernels): wave j eliminates the algebraic unknowns of stage j, computing
 // in R (double, or float for the FP32 direction).  Writes T_j to sT (LDS,
-// rows of NVA) and recT, G_j to recG (float64 records); returns nonzero on a
-// zero pivot in some g_z.
-template <class R, int NW, class TJ>
+// rows of NVA) and recT, G_j to recG (records of type TR); returns nonzero on
+// a zero pivot in some g_z.
+template <class R, int NW, class TJ, class TR>
 __device__ __forceinline__ int stage_eliminate(int i, const TJ* __restrict__ jac, const double* __restrict__ fa,
-                                               R* sT, double* __restrict__ recT, double* __restrict__ recG) {
+                                               R* sT, TR* __restrict__ recT, TR* __restrict__ recG) {
   const int lane = lane_id(), wave = wave_id();
   int bad = 0;
   // Lane l holds columns l + 64 s of [g_z | g_y g_p f_a,g] (NZ rows each);
@@ -305,7 +306,7 @@ __device__ __forceinline__ int stage_eliminate(int i, const TJ* __restrict__ jac
       const int q = lane + WAVE * sl - NZ;
       if (q < 0 || q >= NVA) continue;
 #pragma unroll
-      for (int r = 0; r < NZ; ++r) recG[(j * NZ + r) * NVA + q] = (double)gc[sl][r];
+      for (int r = 0; r < NZ; ++r) recG[(j * NZ + r) * NVA + q] = (TR)gc[sl][r];
     }
     // T_j = [h_y | h_p | f_a,h] - h_z G_j, rows in batches of TB whose
     // Jacobian loads are all issued before any of them is used
@@ -338,7 +339,7 @@ __device__ __forceinline__ int stage_eliminate(int i, const TJ* __restrict__ jac
 #pragma unroll
           for (int z = 0; z < NZ; ++z) v = fma(-hz[rr][z], gc[sl][z], v);
           sT[(j * NY + r) * NVA + q] = v;
-          recT[(j * NY + r) * NVA + q] = (double)v;
+          recT[(j * NY + r) * NVA + q] = (TR)v;
         }
       }
     }
@@ -368,7 +369,7 @@ template <int CW, class TJ>
 __device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst, const int* __restrict__ ilist,
                          const double* __restrict__ t, const TJ* __restrict__ jac,
                          const double* __restrict__ fa, const double* __restrict__ fb,
-                         double* __restrict__ rows, double* __restrict__ rec, int* __restrict__ status,
+                         double* __restrict__ rows, TJ* __restrict__ rec, int* __restrict__ status,
                          const int* __restrict__ only) {
   // workgroup k condenses interval ilist[k] (k without a list); only !=
   // nullptr: just the intervals the series kernel left (only[i] != 0)
@@ -392,13 +393,13 @@ __device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst
   if (tid == 0) sbad = 0;
   const int nd = i + iinst[i];
   const double h = t[nd + 1] - t[nd];
-  double* recU = rec + (size_t)i * REC;
-  double* recT = recU + NS * NVA;
-  double* recG = recT + M * NY * NVA;
+  TJ* recU = rec + (size_t)i * REC;   // records stored like the Jacobians
+  TJ* recT = recU + NS * NVA;
+  TJ* recG = recT + M * NY * NVA;
   int bad = 0;
 
   // ---- phase 1: wave j eliminates the algebraic unknowns of stage j.
-  bad |= stage_eliminate<double, CW, TJ>(i, jac, fa, sT, recT, recG);
+  bad |= stage_eliminate<double, CW, TJ, TJ>(i, jac, fa, sT, recT, recG);
   __syncthreads();
 
   // ---- phase 2: assemble [S | RHS] into registers
@@ -460,7 +461,7 @@ __device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst
       if (cc < NS || cc >= NS + NVA) continue;
       const int qc = cc - NS;
       const double v = gj.a[q][cq] / pivval[kk];
-      recU[kk * NVA + qc] = v;
+      recU[kk * NVA + qc] = (TJ)v;
       if (last) {
         if (qc < NY) rowi[a * CRW + qc] = -v;
         else if (qc < NY + NPA) rowi[a * CRW + 2 * NY + pact(qc - NY)] = -v;
@@ -521,7 +522,7 @@ template <class R>
 __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __restrict__ t,
                                             const R* __restrict__ jac,
                                             const double* __restrict__ fa, const double* __restrict__ fb,
-                                            double* __restrict__ rows, double* __restrict__ rec,
+                                            double* __restrict__ rows, R* __restrict__ rec,
                                             int* __restrict__ status, int* __restrict__ fallback,
                                             unsigned long long* __restrict__ counters) {
   constexpr int NWV = NS_THREADS / WAVE;
@@ -542,10 +543,10 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
   }
   if (tid == 0) sbad = 0;
   const R h = (R)(t[i + ib + 1] - t[i + ib]);
-  double* recU = rec + (size_t)i * REC;
-  double* recT = recU + NS * NVA;
-  double* recG = recT + M * NY * NVA;
-  const int bad = stage_eliminate<R, NWV, R>(i, jac, fa, sT, recT, recG);
+  R* recU = rec + (size_t)i * REC;
+  R* recT = recU + NS * NVA;
+  R* recG = recT + M * NY * NVA;
+  const int bad = stage_eliminate<R, NWV, R, R>(i, jac, fa, sT, recT, recG);
   __syncthreads();
 
   const bool act = tid < NS_ITEMS;
@@ -661,7 +662,7 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
       const int c = c0 + cc;
       if (c >= NVA) continue;
       const double v = (double)xo[cc];
-      recU[(k * NY + r) * NVA + c] = v;
+      recU[(k * NY + r) * NVA + c] = xo[cc];
       if (k == NSK - 1) {
         if (c < NY) rowi[r * CRW + c] = -v;
         else if (c < NY + NPA) rowi[r * CRW + 2 * NY + pact(c - NY)] = -v;
@@ -1037,7 +1038,8 @@ __device__ __forceinline__ void cr_back(int level, Batch bt, const int* __restri
 // ---------------------------------------------------------------- recover
 // One workgroup per interval (recover_delta_k_kernel, solver:1619-1650):
 //   u_0 = dy_i, u_j = U_j v,  dydot_j = T_j [u_j; dp; 1],  dz_j = -G_j [u_j; dp; 1].
-__device__ __forceinline__ void recover(int nint, Batch bt, const double* __restrict__ rec,
+template <class TR>
+__device__ __forceinline__ void recover(int nint, Batch bt, const TR* __restrict__ rec,
                         const double* __restrict__ dy, const double* __restrict__ dp,
                         double* __restrict__ dydot, double* __restrict__ dz) {
   __shared__ double sw[M][NVA];
@@ -1049,9 +1051,9 @@ __device__ __forceinline__ void recover(int nint, Batch bt, const double* __rest
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = wave_id();
   const int nw = blockDim.x / WAVE;
-  const double* recU = rec + (size_t)i * REC;
-  const double* recT = recU + NS * NVA;
-  const double* recG = recT + M * NY * NVA;
+  const TR* recU = rec + (size_t)i * REC;
+  const TR* recT = recU + NS * NVA;
+  const TR* recG = recT + M * NY * NVA;
   for (int e = tid; e < M * NVA; e += blockDim.x) {
     const int j = e / NVA, q = e - (e / NVA) * NVA;
     double v = 0.0;
@@ -1064,7 +1066,7 @@ __device__ __forceinline__ void recover(int nint, Batch bt, const double* __rest
   // u_j rows: one wave per row, lanes across the NVA columns
   for (int rr = wave; rr < NS; rr += nw) {
     double s = 0.0;
-    for (int q = lane; q < NVA; q += WAVE) s = fma(recU[rr * NVA + q], sw[0][q], s);
+    for (int q = lane; q < NVA; q += WAVE) s = fma((double)recU[rr * NVA + q], sw[0][q], s);
     s = wave_sum(s);
     if (lane == 0) sw[rr / NY + 1][rr % NY] = s;
   }
@@ -1072,9 +1074,9 @@ __device__ __forceinline__ void recover(int nint, Batch bt, const double* __rest
   for (int rr = wave; rr < M * (NY + NZ); rr += nw) {
     const bool isz = rr >= M * NY;
     const int j = isz ? (rr - M * NY) / NZ1 : rr / NY;
-    const double* rowp = isz ? recG + (size_t)(rr - M * NY) * NVA : recT + (size_t)rr * NVA;
+    const TR* rowp = isz ? recG + (size_t)(rr - M * NY) * NVA : recT + (size_t)rr * NVA;
     double s = 0.0;
-    for (int q = lane; q < NVA; q += WAVE) s = fma(rowp[q], sw[j][q], s);
+    for (int q = lane; q < NVA; q += WAVE) s = fma((double)rowp[q], sw[j][q], s);
     s = wave_sum(s);
     if (lane == 0) {
       if (isz) {
@@ -1318,7 +1320,7 @@ __global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const 
 }
 // the float64 Gauss-Jordan fallback of the FP32 direction (float32 stage Jacobians)
 __global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense32(int nint, const int* iinst, const int* ilist,
-    const double* t, const float* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
+    const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
     const int* only) {
   colloc::condense<Cfg::CW, float>(nint, iinst, ilist, t, jac, fa, fb, rows, rec, status, only);
 }
@@ -1338,7 +1340,7 @@ __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nin
 // The FP32-direction condensation: two 512-thread workgroups per CU (half the
 // LDS and registers of the float64 kernel).
 __global__ __launch_bounds__(colloc::NS_THREADS, 4) void colloc_condense_ns32(int nint, colloc::Batch bt,
-    const double* t, const float* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
+    const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
     int* fallback, unsigned long long* counters) {
   if constexpr (colloc::ns_ok<float>()) {
     colloc::condense_ns<float>(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
@@ -1372,7 +1374,12 @@ __global__ __launch_bounds__(64) void colloc_cr_back(int level, colloc::Batch bt
 
 __global__ __launch_bounds__(256) void colloc_recover(int nint, colloc::Batch bt, const double* rec,
     const double* dy, const double* dp, double* dydot, double* dz) {
-  colloc::recover(nint, bt, rec, dy, dp, dydot, dz);
+  colloc::recover<double>(nint, bt, rec, dy, dp, dydot, dz);
+}
+// condensation records stored in float32 (the FP32 direction)
+__global__ __launch_bounds__(256) void colloc_recover32(int nint, colloc::Batch bt, const float* rec,
+    const double* dy, const double* dp, double* dydot, double* dz) {
+  colloc::recover<float>(nint, bt, rec, dy, dp, dydot, dz);
 }
 
 __global__ __launch_bounds__(256) void colloc_axpy4(colloc::Seg4 s, colloc::Batch bt, const double* x,
