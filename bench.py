@@ -167,6 +167,8 @@ def main():
                     help="batch: arithmetic of the Newton direction's condensation (32: the FP32 direction)")
     ap.add_argument("--fp64-pass", type=int, default=1,
                     help="batch, learned, precision 32: also time one learned pass with the float64 direction")
+    ap.add_argument("--babd-spans", type=int, default=64,
+                    help="batch: substitution pairing while a pair spans <= 1/N of an instance (0: Householder only)")
     ap.add_argument("--warm-cap", type=int, default=12,
                     help="batch, learned: Newton iterations from the network's guess before the fallback")
     ap.add_argument("--cpu-sample", type=int, default=128, help="Nint of the CPU baseline sample (0: skip)")
@@ -199,6 +201,8 @@ def batch_workload(args):
     S = solver.CollocationSolver(bvp, m, n_max=max(sum(nodes[i] for i in ch) for ch in chunks),
                                  batch=max(len(ch) for ch in chunks))
     S.set_direction_precision(args.precision)
+    if args.babd_spans != 64:
+        S.set_babd_substitution(args.babd_spans)
     ts_all = [np.linspace(0.0, inst.T, inst.nodes) for inst in insts]
     cons_all = np.array([inst.constant_values() for inst in insts])
     hom_all = [np.array([inst.constant_values(s=0.0) for inst in insts])]

@@ -1049,8 +1049,6 @@ __device__ __forceinline__ void recover(int nint, Batch bt, const TR* __restrict
   dy += (size_t)ib * NY;    // left node of interval i is i + ib
   dp += (size_t)ib * NP;
   const int tid = threadIdx.x;
-  const int lane = lane_id(), wave = wave_id();
-  const int nw = blockDim.x / WAVE;
   const TR* recU = rec + (size_t)i * REC;
   const TR* recT = recU + NS * NVA;
   const TR* recG = recT + M * NY * NVA;
@@ -1063,29 +1061,39 @@ __device__ __forceinline__ void recover(int nint, Batch bt, const TR* __restrict
     sw[j][q] = v;
   }
   __syncthreads();
-  // u_j rows: one wave per row, lanes across the NVA columns
-  for (int rr = wave; rr < NS; rr += nw) {
-    double s = 0.0;
-    for (int q = lane; q < NVA; q += WAVE) s = fma((double)recU[rr * NVA + q], sw[0][q], s);
-    s = wave_sum(s);
-    if (lane == 0) sw[rr / NY + 1][rr % NY] = s;
+  // u_j rows: one thread per record row (a row's NVA entries are read in
+  // sequence; the wave's rows share cache lines across the iterations), the
+  // vector [dy_i; dp; 1] is an LDS broadcast
+  for (int rr = tid; rr < NS; rr += blockDim.x) {
+    const TR* rowp = recU + (size_t)rr * NVA;
+    double s0 = 0.0, s1 = 0.0;
+    int q = 0;
+    for (; q + 1 < NVA; q += 2) {
+      s0 = fma((double)rowp[q], sw[0][q], s0);
+      s1 = fma((double)rowp[q + 1], sw[0][q + 1], s1);
+    }
+    if (q < NVA) s0 = fma((double)rowp[q], sw[0][q], s0);
+    sw[rr / NY + 1][rr % NY] = s0 + s1;
   }
   __syncthreads();
-  for (int rr = wave; rr < M * (NY + NZ); rr += nw) {
+  for (int rr = tid; rr < M * (NY + NZ); rr += blockDim.x) {
     const bool isz = rr >= M * NY;
     const int j = isz ? (rr - M * NY) / NZ1 : rr / NY;
     const TR* rowp = isz ? recG + (size_t)(rr - M * NY) * NVA : recT + (size_t)rr * NVA;
-    double s = 0.0;
-    for (int q = lane; q < NVA; q += WAVE) s = fma((double)rowp[q], sw[j][q], s);
-    s = wave_sum(s);
-    if (lane == 0) {
-      if (isz) {
-        const int a = (rr - M * NY) - j * NZ;
-        dz[((size_t)i * M + j) * NZ + a] = -s;
-      } else {
-        const int a = rr - j * NY;
-        dydot[((size_t)i * M + j) * NY + a] = s;
-      }
+    double s0 = 0.0, s1 = 0.0;
+    int q = 0;
+    for (; q + 1 < NVA; q += 2) {
+      s0 = fma((double)rowp[q], sw[j][q], s0);
+      s1 = fma((double)rowp[q + 1], sw[j][q + 1], s1);
+    }
+    if (q < NVA) s0 = fma((double)rowp[q], sw[j][q], s0);
+    const double s = s0 + s1;
+    if (isz) {
+      const int a = (rr - M * NY) - j * NZ;
+      dz[((size_t)i * M + j) * NZ + a] = -s;
+    } else {
+      const int a = rr - j * NY;
+      dydot[((size_t)i * M + j) * NY + a] = s;
     }
   }
 }
