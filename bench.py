@@ -169,6 +169,10 @@ def main():
                     help="batch, learned, precision 32: also time one learned pass with the float64 direction")
     ap.add_argument("--babd-spans", type=int, default=64,
                     help="batch: substitution pairing while a pair spans <= 1/N of an instance (0: Householder only)")
+    ap.add_argument("--warm-patience", type=int, default=8,
+                    help="batch, learned: from this warm iteration on, stop instances whose norm falls by less than "
+                         "--warm-ratio per iteration (0: off)")
+    ap.add_argument("--warm-ratio", type=float, default=2.0)
     ap.add_argument("--warm-cap", type=int, default=12,
                     help="batch, learned: Newton iterations from the network's guess before the fallback")
     ap.add_argument("--cpu-sample", type=int, default=128, help="Nint of the CPU baseline sample (0: skip)")
@@ -239,16 +243,22 @@ def batch_workload(args):
         conv = np.zeros(len(insts), dtype=bool)
         its = np.zeros(len(insts), dtype=int)
         newton_ok = np.zeros(len(insts), dtype=bool)
+        t_warm = time.perf_counter()
         for ch in chunks:
             r = learned.warm_solve(ws, S, [ts_all[i] for i in ch], feats_all[ch], cons_all[ch], hom_all[0][ch],
-                                   tol=tol, max_iter=max_iter, max_iter_warm=args.warm_cap)
+                                   tol=tol, max_iter=max_iter, max_iter_warm=args.warm_cap,
+                                   patience=args.warm_patience, min_ratio=args.warm_ratio)
             conv[ch] = r["converged"]
             its[ch] = r["iterations"][0] + r["lq_iterations"]
             newton_ok[ch] = r["norm"] < tol
         back = [i for i in range(len(insts)) if not newton_ok[i]]
+        t_back = time.perf_counter()
+        phase_s["warm"] += t_back - t_warm
         if back:
             c2, _ = plain_batch(back)
             conv[back] = c2
+            phase_s["fallback_converged"] += int(c2.sum())
+        phase_s["fallback"] += time.perf_counter() - t_back
         return conv, its, len(back)
 
     def plain_pass():
@@ -256,8 +266,10 @@ def batch_workload(args):
         return conv, its, 0
 
     one_pass = learned_pass if args.form == "learned" else plain_pass
+    phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0}
     for _ in range(args.warmup):
         one_pass()
+    phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0}
     S.profile(True)
     st0 = S.condense_stats()
     t0 = time.perf_counter()
@@ -266,6 +278,7 @@ def batch_workload(args):
     st1 = S.condense_stats()
     prof = {k: S.profile_get(k) for k in ("residual", "jacobian", "condense", "babd", "recover")}
     S.profile(False)
+    phases = {k: v / steps for k, v in phase_s.items()}
     converged = int(sum(r[0].sum() for r in results))
     value = converged / elapsed
     conv, its, n_back = results[-1]
@@ -310,11 +323,15 @@ def batch_workload(args):
     print("batch (%s): %d chunks, converged %d/%d per pass, Newton iterations mean %.2f, fallback to the plain "
           "path %d" % (args.form, len(chunks), conv.sum(), len(insts), it_mean, n_back), file=sys.stderr)
     print("kernel totals over %d passes (ms, launches): %s" % (steps, prof), file=sys.stderr)
+    if args.form == "learned":
+        print("per pass: warm-started solves %.2f s, fallback %.2f s (%.1f converged)"
+              % (phases["warm"], phases["fallback"], phases["fallback_converged"]), file=sys.stderr)
     print("looped plain solve: %d instances in %.2f s (%d converged)" % (len(sample), loop_s, loop_ok),
           file=sys.stderr)
     form_txt = ("LQ solve (1 Newton iteration) + network correction (MLP 21->4x1024->3250, bf16, trained on "
-                "solver solutions) + at most %d Newton iterations at the final constants, plain guess + homotopy as "
-                "the fallback" % args.warm_cap if args.form == "learned"
+                "solver solutions) + at most %d Newton iterations at the final constants (stopped from iteration "
+                "%d on when the norm falls by less than %gx per iteration), plain guess + homotopy as the fallback"
+                % (args.warm_cap, args.warm_patience, args.warm_ratio) if args.form == "learned"
                 else "plain all-ones guess + homotopy S: 0->0.5")
     form_txt += ("; Newton direction condensed in float32 (residuals, BABD, convergence test in float64)"
                  if args.precision == 32 else "; float64 throughout")
@@ -339,6 +356,9 @@ def batch_workload(args):
                    "form": args.form, "direction_precision": args.precision, "instances": len(insts), "mean_nint": mean_nint, "m": m, "tolerance": tol,
                    "converged_per_pass": int(conv.sum()), "fallback_per_pass": n_back, "chunks": len(chunks),
                    "newton_iterations_mean": float(it_mean),
+                   "seconds_per_pass": {"warm_start_solves": phases["warm"], "fallback": phases["fallback"]}
+                   if args.form == "learned" else None,
+                   "fallback_converged_per_pass": phases["fallback_converged"] if args.form == "learned" else None,
                    "looped_plain_solves_per_s": loop_rate, "looped_sample": len(sample),
                    "looped_sample_converged": loop_ok,
                    "vs_looped_plain": value / loop_rate if loop_rate > 0 else None,

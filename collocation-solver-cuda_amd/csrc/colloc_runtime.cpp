@@ -815,10 +815,15 @@ struct LaunchList {
   }
 };
 
+// patience > 0: an instance whose norm, from iteration `patience` on, falls
+// by less than a factor min_ratio in one iteration stops there (reported
+// with its loop counter at that point and not converged); off for the
+// reference's loop (patience = 0).
 int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time, double* norm, double* hist,
-           int hist_len) {
+           int hist_len, int patience = 0, double min_ratio = 0.0) {
   const int B = h->B;
-  std::vector<double> nrm(B, 0.0), nn(B, 0.0), step(B, 0.0), taken(B, 0.0);
+  std::vector<double> nrm(B, 0.0), nn(B, 0.0), step(B, 0.0), taken(B, 0.0), prev(B, 0.0);
+  std::vector<int> gave_up(B, -1);
   std::vector<int> active(B, 1), search(B, 0), done(B, -1);
   LaunchList lst(h);
   for (int it = 0; it < max_iter; ++it) {
@@ -834,9 +839,13 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
       if (cur[b] < tol) {
         active[b] = 0;
         done[b] = it;
+      } else if (patience > 0 && it >= patience && !(cur[b] * min_ratio <= prev[b])) {
+        active[b] = 0;      // too slow: not converging the way a good start does
+        gave_up[b] = it;
       } else {
         ++nact;
       }
+      prev[b] = cur[b];
     }
     if (nact == 0) break;
     TRY(upload_ints(h, h->act, active));
@@ -887,7 +896,7 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
   }
   for (int b = 0; b < B; ++b) {
     // the reference's loop variable stops at max_iter - 1
-    if (iter_time) iter_time[b] = done[b] >= 0 ? done[b] : max_iter - 1;
+    if (iter_time) iter_time[b] = done[b] >= 0 ? done[b] : (gave_up[b] >= 0 ? gave_up[b] : max_iter - 1);
     if (norm) norm[b] = nrm[b];
   }
   return sync(h);
@@ -983,6 +992,17 @@ int cs_newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_ti
   if (max_iter < 1) return fail(CS_E_ARG, "max_iter must be >= 1");
   if (hist_len < 0) return fail(CS_E_ARG, "hist_len must be >= 0");
   return newton(h, alpha, tol, max_iter, iter_time, norm, norms, hist_len);
+}
+
+int cs_newton_progress(cs_handle* h, double alpha, double tol, int max_iter, int patience, double min_ratio,
+                       int* iter_time, double* norm, double* norms, int hist_len) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  if (max_iter < 1) return fail(CS_E_ARG, "max_iter must be >= 1");
+  if (hist_len < 0) return fail(CS_E_ARG, "hist_len must be >= 0");
+  if (patience < 1 || !(min_ratio >= 1.0)) return fail(CS_E_ARG, "patience %d (>= 1), min_ratio %g (>= 1)", patience,
+                                                       min_ratio);
+  return newton(h, alpha, tol, max_iter, iter_time, norm, norms, hist_len, patience, min_ratio);
 }
 
 int cs_segment_residual_ex(cs_handle* h, double alpha, double tol, int residual_type, int scale_by_time,

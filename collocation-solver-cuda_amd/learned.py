@@ -365,13 +365,16 @@ class WarmStart:
 
 
 def warm_solve(ws, solv, t_spans, feats, constants, lq_constants, tol=1e-8, max_iter=200, max_iter_warm=12,
-               alpha=0.1):
+               alpha=0.1, patience=8, min_ratio=2.0):
     """Warm-started batched solve of the instances (meshes t_spans): for an
     "lq" network, one Newton solve of the LQ problems from the all-ones
     guess (converged after one iteration), then the network's guess and at
-    most max_iter_warm Newton iterations at the instances' constants.
-    Returns the solve_batch-style result dict; "lq_iterations" holds the
-    iterations of the LQ stage."""
+    most max_iter_warm Newton iterations at the instances' constants; from
+    iteration `patience` on, an instance whose norm falls by less than a
+    factor min_ratio in an iteration stops (not converged: its fallback
+    takes over; patience=0 disables the test).  Returns the
+    solve_batch-style result dict; "lq_iterations" holds the iterations of
+    the LQ stage."""
     torch = _torch()
     B = len(t_spans)
     solv.set_batch_mesh(t_spans)
@@ -387,7 +390,7 @@ def warm_solve(ws, solv, t_spans, feats, constants, lq_constants, tol=1e-8, max_
         it, _, _ = solv.newton(alpha, tol, max_iter)
         lq_its = np.atleast_1d(it)
     ws.guess_on_meshes(solv, feats)
-    r = solver.newton_stages(solv, constants, [], alpha, tol, max_iter_warm)
+    r = solver.newton_stages(solv, constants, [], alpha, tol, max_iter_warm, patience=patience, min_ratio=min_ratio)
     r["lq_iterations"] = lq_its
     return r
 

@@ -273,14 +273,19 @@ class CollocationSolver:
         check(lib().cs_recover_delta_k(self.h, ptr(dyd), ptr(dz)))
         return dyd, dz
 
-    def newton(self, alpha, tol, max_iter):
+    def newton(self, alpha, tol, max_iter, patience=0, min_ratio=0.0):
         """Damped Newton loop (solver:109-174): (iter_time, norm, norm history).
         For a batch: arrays of B iteration counts and norms, and a list of B
-        histories."""
+        histories.  patience > 0 adds the progress test of cs_newton_progress
+        (for warm-started solves; the reference has none)."""
         it = np.zeros(self.B, dtype=np.int32)
         nrm = np.zeros(self.B)
         hist = np.zeros((self.B, max_iter))
-        check(lib().cs_newton(self.h, alpha, tol, max_iter, native.iptr(it), ptr(nrm), ptr(hist), max_iter))
+        if patience > 0:
+            check(lib().cs_newton_progress(self.h, alpha, tol, max_iter, int(patience), float(min_ratio),
+                                           native.iptr(it), ptr(nrm), ptr(hist), max_iter))
+        else:
+            check(lib().cs_newton(self.h, alpha, tol, max_iter, native.iptr(it), ptr(nrm), ptr(hist), max_iter))
         hists = [hist[b, :min(int(it[b]) + 1, max_iter)].copy() for b in range(self.B)]
         if self.B == 1:
             return int(it[0]), float(nrm[0]), hists[0]
@@ -441,12 +446,18 @@ def solve_batch(solver, t_spans, Y0s, Z0s, P0s, constants=None, homotopy=(), alp
 
 
 def newton_stages(solver, constants, homotopy, alpha, tol, max_iter, residual_type=1, scale_by_time=True,
-                   residual2=False):
+                   residual2=False, patience=0, min_ratio=0.0):
+    """Newton at each homotopy constant vector in turn, then at `constants`,
+    then the acceptance residual (see solve_batch).  patience / min_ratio:
+    the progress test of cs_newton_progress for the final solve."""
     B = solver.B
     its = []
     nrm = None
 
-    def run():
+    def run(final=False):
+        if final and patience > 0:
+            it, n, _ = solver.newton(alpha, tol, max_iter, patience, min_ratio)
+            return np.atleast_1d(it), np.atleast_1d(n)
         it, n, _ = solver.newton(alpha, tol, max_iter)
         return np.atleast_1d(it), np.atleast_1d(n)
     for cst in homotopy:
@@ -455,7 +466,7 @@ def newton_stages(solver, constants, homotopy, alpha, tol, max_iter, residual_ty
         its.append(it)
     if constants is not None:
         solver.set_batch_constants(constants)
-    it, nrm = run()
+    it, nrm = run(final=True)
     its.append(it)
     _, mx = solver.segment_residual(alpha, tol, residual_type, scale_by_time, residual2)
     mx = np.atleast_1d(mx)

@@ -191,6 +191,16 @@ int cs_recover_delta_k(cs_handle* h, double* delta_y_dot, double* delta_z_tilde)
 int cs_newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
               double* norm, double* norms, int hist_len);
 
+/* cs_newton with a progress test, for Newton solves started from a guess
+ * that should already be in the quadratic basin (a learned warm start): an
+ * instance whose ||f_q||_inf, from iteration `patience` on, falls by less
+ * than a factor min_ratio (>= 1) in one iteration stops there, not
+ * converged, and iter_time[b] is that iteration.  The caller falls back to
+ * its plain solve for it.  No counterpart in the reference, which iterates to
+ * max_iter. */
+int cs_newton_progress(cs_handle* h, double alpha, double tol, int max_iter, int patience, double min_ratio,
+                       int* iter_time, double* norm, double* norms, int hist_len);
+
 /* compute_segment_residual_parallel (solver:1770-1892) with residual_type 1
  * and scale_by_time.  residual[N] (may be NULL; the last node of each
  * instance holds its ||r||_inf / tol), max_residual[B]. */
