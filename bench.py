@@ -175,7 +175,7 @@ def main():
     ap.add_argument("--warm-ratio", type=float, default=2.0)
     ap.add_argument("--warm-cap", type=int, default=12,
                     help="batch, learned: Newton iterations from the network's guess before the fallback")
-    ap.add_argument("--cpu-sample", type=int, default=128, help="Nint of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=640, help="Nint of the CPU baseline sample (0: skip)")
     args = ap.parse_args()
     check_gpus(args)
     load_package()
