@@ -175,6 +175,19 @@ def main():
     ap.add_argument("--warm-ratio", type=float, default=2.0)
     ap.add_argument("--warm-cap", type=int, default=12,
                     help="batch, learned: Newton iterations from the network's guess before the fallback")
+    ap.add_argument("--warm-cont", default="10,5,2",
+                    help="batch, learned: the instances the capped warm stage leaves continue Newton from there "
+                         "for the rest of the iteration budget, with the stall test start,window,ratio (an "
+                         "instance stops when its norm fell by less than ratio over the last window "
+                         "iterations); 0: no continuation")
+    ap.add_argument("--fallback", choices=("none", "plain"), default="none",
+                    help="batch, learned: what the instances the warm-started Newton leaves get: nothing, or the "
+                         "plain path again (all-ones guess + continuation)")
+    ap.add_argument("--fallback-scales", default="0",
+                    help="batch, learned: tanh scales of the plain fallback's continuation stages before the "
+                         "final one")
+    ap.add_argument("--fallback-stall", default="10,5,2",
+                    help="batch, learned: the plain fallback's stall test start,window,ratio; 0: off")
     ap.add_argument("--cpu-sample", type=int, default=640, help="Nint of the CPU baseline sample (0: skip)")
     args = ap.parse_args()
     check_gpus(args)
@@ -211,16 +224,28 @@ def batch_workload(args):
     cons_all = np.array([inst.constant_values() for inst in insts])
     hom_all = [np.array([inst.constant_values(s=0.0) for inst in insts])]
 
-    def plain_batch(idx):
-        """Plain all-ones guess + continuation for the instances idx, chunked."""
+    fb_scales = [float(v) for v in args.fallback_scales.split(",") if v.strip()]
+    fb_hom = [np.array([inst.constant_values(s=v) for inst in insts]) for v in fb_scales]
+    def stall_rule(txt):
+        if txt == "0":
+            return None
+        v = [float(x) for x in txt.split(",")]
+        return int(v[0]), int(v[1]), v[2]
+    fb_stall = stall_rule(args.fallback_stall)
+    warm_cont = stall_rule(args.warm_cont)
+
+    def plain_batch(idx, hom=None, stall=None):
+        """All-ones guess + continuation for the instances idx, chunked: by
+        default the plain path (homotopy S: 0 -> final, no stall test)."""
+        hom = hom_all if hom is None else hom
         conv = np.zeros(len(idx), dtype=bool)
-        its = np.zeros((2, len(idx)), dtype=int)
+        its = np.zeros((len(hom) + 1, len(idx)), dtype=int)
         for ch in solver.plan_chunks([nodes[i] for i in idx], S.n_max, S.batch):
             sel = [idx[k] for k in ch]
             Nt = sum(nodes[i] for i in sel)
             S.set_batch_mesh([ts_all[i] for i in sel])
             S.form_initial_input(np.ones((Nt, ny)), np.ones((Nt, nz)), np.ones((len(sel), np_)))
-            r = solver.newton_stages(S, cons_all[sel], [h[sel] for h in hom_all], 0.1, tol, max_iter)
+            r = solver.newton_stages(S, cons_all[sel], [h[sel] for h in hom], 0.1, tol, max_iter, stall=stall)
             conv[ch] = r["converged"]
             its[:, ch] = r["iterations"]
         return conv, its
@@ -247,15 +272,16 @@ def batch_workload(args):
         for ch in chunks:
             r = learned.warm_solve(ws, S, [ts_all[i] for i in ch], feats_all[ch], cons_all[ch], hom_all[0][ch],
                                    tol=tol, max_iter=max_iter, max_iter_warm=args.warm_cap,
-                                   patience=args.warm_patience, min_ratio=args.warm_ratio)
+                                   patience=args.warm_patience, min_ratio=args.warm_ratio, cont=warm_cont)
             conv[ch] = r["converged"]
-            its[ch] = r["iterations"][0] + r["lq_iterations"]
+            its[ch] = r["iterations"].sum(axis=0) + r["lq_iterations"]
             newton_ok[ch] = r["norm"] < tol
+            phase_s["warm_converged"] += int(r["warm_converged"].sum())
         back = [i for i in range(len(insts)) if not newton_ok[i]]
         t_back = time.perf_counter()
         phase_s["warm"] += t_back - t_warm
-        if back:
-            c2, _ = plain_batch(back)
+        if back and args.fallback == "plain":
+            c2, _ = plain_batch(back, fb_hom, fb_stall)
             conv[back] = c2
             phase_s["fallback_converged"] += int(c2.sum())
         phase_s["fallback"] += time.perf_counter() - t_back
@@ -266,10 +292,10 @@ def batch_workload(args):
         return conv, its, 0
 
     one_pass = learned_pass if args.form == "learned" else plain_pass
-    phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0}
+    phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0, "warm_converged": 0}
     for _ in range(args.warmup):
         one_pass()
-    phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0}
+    phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0, "warm_converged": 0}
     S.profile(True)
     st0 = S.condense_stats()
     t0 = time.perf_counter()
@@ -295,7 +321,7 @@ def batch_workload(args):
         c64, it64, b64 = learned_pass()
         lt = time.perf_counter() - t0
         extra.update({"learned_fp64_solves_per_s": c64.sum() / lt, "learned_fp64_converged": int(c64.sum()),
-                      "learned_fp64_fallback": b64})
+                      "learned_fp64_unconverged_after_newton": b64})
     if args.form == "learned" and args.plain_pass:
         # the plain batched form (float64 direction) on the same instances, for the split of the gain
         S.set_direction_precision(64)
@@ -320,18 +346,31 @@ def batch_workload(args):
     loop_rate = loop_ok / loop_s
     mean_nint = float(np.mean([n - 1 for n in nodes]))
     it_mean = np.asarray(its).reshape(-1, len(insts)).sum(axis=0).mean()
-    print("batch (%s): %d chunks, converged %d/%d per pass, Newton iterations mean %.2f, fallback to the plain "
-          "path %d" % (args.form, len(chunks), conv.sum(), len(insts), it_mean, n_back), file=sys.stderr)
+    print("batch (%s): %d chunks, converged %d/%d per pass, Newton iterations mean %.2f, unconverged after the "
+          "warm-started Newton %d" % (args.form, len(chunks), conv.sum(), len(insts), it_mean, n_back),
+          file=sys.stderr)
     print("kernel totals over %d passes (ms, launches): %s" % (steps, prof), file=sys.stderr)
     if args.form == "learned":
-        print("per pass: warm-started solves %.2f s, fallback %.2f s (%.1f converged)"
-              % (phases["warm"], phases["fallback"], phases["fallback_converged"]), file=sys.stderr)
+        print("per pass: warm-started solves %.2f s (%.1f converged in the capped stage), %s fallback %.2f s "
+              "(%.1f converged)" % (phases["warm"], phases["warm_converged"], args.fallback, phases["fallback"],
+                                    phases["fallback_converged"]), file=sys.stderr)
     print("looped plain solve: %d instances in %.2f s (%d converged)" % (len(sample), loop_s, loop_ok),
           file=sys.stderr)
     form_txt = ("LQ solve (1 Newton iteration) + network correction (MLP 21->4x1024->3250, bf16, trained on "
                 "solver solutions) + at most %d Newton iterations at the final constants (stopped from iteration "
-                "%d on when the norm falls by less than %gx per iteration), plain guess + homotopy as the fallback"
-                % (args.warm_cap, args.warm_patience, args.warm_ratio) if args.form == "learned"
+                "%d on when the norm falls by less than %gx per iteration)%s; %s"
+                % (args.warm_cap, args.warm_patience, args.warm_ratio,
+                   "" if warm_cont is None else
+                   (", the instances it leaves continuing damped Newton for the rest of the %d-iteration budget "
+                    "(stopped from iteration %d on when the norm fell by less than %gx over the last %d "
+                    "iterations)" % (max_iter, warm_cont[0], warm_cont[2], warm_cont[1])),
+                   "no fallback" if args.fallback == "none" else
+                   ("fallback: plain guess + continuation S = %s -> final%s"
+                    % (args.fallback_scales,
+                       "" if fb_stall is None else (", each stage stopping an instance from iteration %d on when "
+                                                     "its norm fell by less than %gx over the last %d iterations"
+                                                     % (fb_stall[0], fb_stall[2], fb_stall[1])))))
+                if args.form == "learned"
                 else "plain all-ones guess + homotopy S: 0->0.5")
     form_txt += ("; Newton direction condensed in float32 (residuals, BABD, convergence test in float64)"
                  if args.precision == 32 else "; float64 throughout")
@@ -354,8 +393,11 @@ def batch_workload(args):
                                "shared A,B,C (seed 1), x0~U[-1,1]^20, T~U[0.5,2], Nint~U{256..2048}, m=%d; %s; "
                                "%d chunk(s) of <= %d nodes" % (len(insts), m, form_txt, len(chunks), args.chunk_nodes),
                    "form": args.form, "direction_precision": args.precision, "instances": len(insts), "mean_nint": mean_nint, "m": m, "tolerance": tol,
-                   "converged_per_pass": int(conv.sum()), "fallback_per_pass": n_back, "chunks": len(chunks),
+                   "converged_per_pass": int(conv.sum()), "chunks": len(chunks),
                    "newton_iterations_mean": float(it_mean),
+                   "warm_stage_converged_per_pass": phases["warm_converged"] if args.form == "learned" else None,
+                   "unconverged_after_newton_per_pass": n_back if args.form == "learned" else None,
+                   "fallback": args.fallback if args.form == "learned" else None,
                    "seconds_per_pass": {"warm_start_solves": phases["warm"], "fallback": phases["fallback"]}
                    if args.form == "learned" else None,
                    "fallback_converged_per_pass": phases["fallback_converged"] if args.form == "learned" else None,

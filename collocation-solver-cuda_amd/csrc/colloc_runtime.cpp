@@ -816,13 +816,16 @@ struct LaunchList {
 };
 
 // patience > 0: an instance whose norm, from iteration `patience` on, falls
-// by less than a factor min_ratio in one iteration stops there (reported
-// with its loop counter at that point and not converged); off for the
-// reference's loop (patience = 0).
+// by less than a factor min_ratio over the last `window` iterations stops
+// there (reported with its loop counter at that point and not converged);
+// off for the reference's loop (patience = 0).
 int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time, double* norm, double* hist,
-           int hist_len, int patience = 0, double min_ratio = 0.0) {
+           int hist_len, int patience = 0, double min_ratio = 0.0, int window = 1) {
   const int B = h->B;
-  std::vector<double> nrm(B, 0.0), nn(B, 0.0), step(B, 0.0), taken(B, 0.0), prev(B, 0.0);
+  std::vector<double> nrm(B, 0.0), nn(B, 0.0), step(B, 0.0), taken(B, 0.0);
+  // the last window + 1 norms of each instance (ring), for the progress test
+  const int ring = patience > 0 ? window + 1 : 1;
+  std::vector<double> past((size_t)B * ring, 0.0);
   std::vector<int> gave_up(B, -1);
   std::vector<int> active(B, 1), search(B, 0), done(B, -1);
   LaunchList lst(h);
@@ -839,13 +842,14 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
       if (cur[b] < tol) {
         active[b] = 0;
         done[b] = it;
-      } else if (patience > 0 && it >= patience && !(cur[b] * min_ratio <= prev[b])) {
+      } else if (patience > 0 && it >= patience && it >= window &&
+                 !(cur[b] * min_ratio <= past[(size_t)b * ring + (it - window) % ring])) {
         active[b] = 0;      // too slow: not converging the way a good start does
         gave_up[b] = it;
       } else {
         ++nact;
       }
-      prev[b] = cur[b];
+      past[(size_t)b * ring + it % ring] = cur[b];
     }
     if (nact == 0) break;
     TRY(upload_ints(h, h->act, active));
@@ -1003,6 +1007,17 @@ int cs_newton_progress(cs_handle* h, double alpha, double tol, int max_iter, int
   if (patience < 1 || !(min_ratio >= 1.0)) return fail(CS_E_ARG, "patience %d (>= 1), min_ratio %g (>= 1)", patience,
                                                        min_ratio);
   return newton(h, alpha, tol, max_iter, iter_time, norm, norms, hist_len, patience, min_ratio);
+}
+
+int cs_newton_stall(cs_handle* h, double alpha, double tol, int max_iter, int start, int window, double min_ratio,
+                    int* iter_time, double* norm, double* norms, int hist_len) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  if (max_iter < 1) return fail(CS_E_ARG, "max_iter must be >= 1");
+  if (hist_len < 0) return fail(CS_E_ARG, "hist_len must be >= 0");
+  if (start < 1 || window < 1 || window > max_iter || !(min_ratio >= 1.0))
+    return fail(CS_E_ARG, "start %d (>= 1), window %d (1..max_iter), min_ratio %g (>= 1)", start, window, min_ratio);
+  return newton(h, alpha, tol, max_iter, iter_time, norm, norms, hist_len, start, min_ratio, window);
 }
 
 int cs_segment_residual_ex(cs_handle* h, double alpha, double tol, int residual_type, int scale_by_time,

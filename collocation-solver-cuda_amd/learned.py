@@ -365,16 +365,21 @@ class WarmStart:
 
 
 def warm_solve(ws, solv, t_spans, feats, constants, lq_constants, tol=1e-8, max_iter=200, max_iter_warm=12,
-               alpha=0.1, patience=8, min_ratio=2.0):
+               alpha=0.1, patience=8, min_ratio=2.0, cont=(10, 5, 2.0)):
     """Warm-started batched solve of the instances (meshes t_spans): for an
     "lq" network, one Newton solve of the LQ problems from the all-ones
     guess (converged after one iteration), then the network's guess and at
     most max_iter_warm Newton iterations at the instances' constants; from
     iteration `patience` on, an instance whose norm falls by less than a
-    factor min_ratio in an iteration stops (not converged: its fallback
-    takes over; patience=0 disables the test).  Returns the
-    solve_batch-style result dict; "lq_iterations" holds the iterations of
-    the LQ stage."""
+    factor min_ratio in an iteration stops (patience=0 disables the test).
+    cont = (start, window, ratio): the instances that stage leaves
+    unconverged continue damped Newton from where they stopped, for the
+    rest of the max_iter budget, each stopping when its norm fell by less
+    than `ratio` over the last `window` iterations from iteration `start` on
+    (cs_newton_stall); None: no continuation (a caller's fallback takes
+    over).  Returns the solve_batch-style result dict ("iterations" has one
+    row per stage run; "warm_converged" is the acceptance after the capped
+    stage); "lq_iterations" holds the iterations of the LQ stage."""
     torch = _torch()
     B = len(t_spans)
     solv.set_batch_mesh(t_spans)
@@ -392,6 +397,17 @@ def warm_solve(ws, solv, t_spans, feats, constants, lq_constants, tol=1e-8, max_
     ws.guess_on_meshes(solv, feats)
     r = solver.newton_stages(solv, constants, [], alpha, tol, max_iter_warm, patience=patience, min_ratio=min_ratio)
     r["lq_iterations"] = lq_its
+    r["warm_converged"] = r["converged"].copy()
+    left = max_iter - max_iter_warm
+    if cont is not None and left >= 2 and not r["converged"].all():
+        it, nrm, _ = solv.newton(alpha, tol, left, int(cont[0]), float(cont[2]), window=int(cont[1]))
+        it, nrm = np.atleast_1d(it), np.atleast_1d(nrm)
+        _, mx = solv.segment_residual(alpha, tol, 1, True, False)
+        mx = np.atleast_1d(mx)
+        r["iterations"] = np.vstack([r["iterations"], it[None, :]])
+        r["norm"] = nrm
+        r["max_residual"] = mx
+        r["converged"] = (nrm < tol) & (mx <= 1.0) & (it < left - 1)
     return r
 
 

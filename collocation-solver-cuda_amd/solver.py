@@ -273,15 +273,20 @@ class CollocationSolver:
         check(lib().cs_recover_delta_k(self.h, ptr(dyd), ptr(dz)))
         return dyd, dz
 
-    def newton(self, alpha, tol, max_iter, patience=0, min_ratio=0.0):
+    def newton(self, alpha, tol, max_iter, patience=0, min_ratio=0.0, window=1):
         """Damped Newton loop (solver:109-174): (iter_time, norm, norm history).
         For a batch: arrays of B iteration counts and norms, and a list of B
         histories.  patience > 0 adds the progress test of cs_newton_progress
-        (for warm-started solves; the reference has none)."""
+        (window 1, for warm-started solves) or, for window > 1, the stall test
+        of cs_newton_stall from iteration `patience` on (for fallback
+        continuation stages); the reference has neither."""
         it = np.zeros(self.B, dtype=np.int32)
         nrm = np.zeros(self.B)
         hist = np.zeros((self.B, max_iter))
-        if patience > 0:
+        if patience > 0 and window > 1:
+            check(lib().cs_newton_stall(self.h, alpha, tol, max_iter, int(patience), int(window), float(min_ratio),
+                                        native.iptr(it), ptr(nrm), ptr(hist), max_iter))
+        elif patience > 0:
             check(lib().cs_newton_progress(self.h, alpha, tol, max_iter, int(patience), float(min_ratio),
                                            native.iptr(it), ptr(nrm), ptr(hist), max_iter))
         else:
@@ -446,10 +451,13 @@ def solve_batch(solver, t_spans, Y0s, Z0s, P0s, constants=None, homotopy=(), alp
 
 
 def newton_stages(solver, constants, homotopy, alpha, tol, max_iter, residual_type=1, scale_by_time=True,
-                   residual2=False, patience=0, min_ratio=0.0):
+                   residual2=False, patience=0, min_ratio=0.0, stall=None):
     """Newton at each homotopy constant vector in turn, then at `constants`,
     then the acceptance residual (see solve_batch).  patience / min_ratio:
-    the progress test of cs_newton_progress for the final solve."""
+    the progress test of cs_newton_progress for the final solve.  stall =
+    (start, window, min_ratio): the stall test of cs_newton_stall in every
+    stage (an instance stopped by it goes on to the next stage from where it
+    stopped)."""
     B = solver.B
     its = []
     nrm = None
@@ -457,6 +465,9 @@ def newton_stages(solver, constants, homotopy, alpha, tol, max_iter, residual_ty
     def run(final=False):
         if final and patience > 0:
             it, n, _ = solver.newton(alpha, tol, max_iter, patience, min_ratio)
+            return np.atleast_1d(it), np.atleast_1d(n)
+        if stall is not None:
+            it, n, _ = solver.newton(alpha, tol, max_iter, stall[0], stall[2], window=stall[1])
             return np.atleast_1d(it), np.atleast_1d(n)
         it, n, _ = solver.newton(alpha, tol, max_iter)
         return np.atleast_1d(it), np.atleast_1d(n)

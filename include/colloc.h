@@ -201,6 +201,16 @@ int cs_newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_ti
 int cs_newton_progress(cs_handle* h, double alpha, double tol, int max_iter, int patience, double min_ratio,
                        int* iter_time, double* norm, double* norms, int hist_len);
 
+/* cs_newton with a stall test over a window, for the continuation stages
+ * of a fallback solve: from iteration `start` on, an instance whose
+ * ||f_q||_inf at iteration k is not below its norm at iteration k - window
+ * divided by min_ratio (>= 1) stops there, not converged, and iter_time[b]
+ * is k.  cs_newton_progress is (start = patience, window = 1).  Damped
+ * iterations that are still descending pass; instances on a plateau stop
+ * instead of running to max_iter.  No counterpart in the reference. */
+int cs_newton_stall(cs_handle* h, double alpha, double tol, int max_iter, int start, int window, double min_ratio,
+                    int* iter_time, double* norm, double* norms, int hist_len);
+
 /* compute_segment_residual_parallel (solver:1770-1892) with residual_type 1
  * and scale_by_time.  residual[N] (may be NULL; the last node of each
  * instance holds its ||r||_inf / tol), max_residual[B]. */

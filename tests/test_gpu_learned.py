@@ -231,3 +231,23 @@ def test_warm_start_gains_over_plain_guess():
             for k in np.nonzero(both)[0]]
     say("same solution from both starts: %d of %d" % (sum(same), len(same)))
     assert sum(same) >= 0.9 * len(same)
+    # a tightly capped warm stage, alone and with the continuation of the
+    # instances it leaves (cs_newton_stall from where they stopped): the
+    # continuation leaves the converged instances untouched and adds others
+    feats = learned.features([i.x0 for i in held], [i.T for i in held])
+    lq = np.array([i.constant_values(s=0.0) for i in held])
+    capped = learned.warm_solve(ws, S, ts, feats, cons, lq, tol=bvp.tolerance, max_iter=bvp.maximum_newton_iterations,
+                                max_iter_warm=3, patience=2, min_ratio=2.0, cont=None)
+    sol_capped = solver.batch_solutions(S)
+    cont = learned.warm_solve(ws, S, ts, feats, cons, lq, tol=bvp.tolerance, max_iter=bvp.maximum_newton_iterations,
+                              max_iter_warm=3, patience=2, min_ratio=2.0, cont=(10, 5, 2.0))
+    sol_cont = solver.batch_solutions(S)
+    say("capped warm stage: %d converged; with the continuation: %d"
+        % (capped["converged"].sum(), cont["converged"].sum()))
+    np.testing.assert_array_equal(cont["warm_converged"], capped["converged"])
+    assert np.all(cont["converged"][capped["converged"]])
+    for k in np.nonzero(capped["converged"])[0]:
+        for key in ("y", "z", "p"):
+            np.testing.assert_array_equal(sol_cont[k][key], sol_capped[k][key])
+    assert cont["converged"].sum() > capped["converged"].sum()
+    assert cont["iterations"].shape[0] == 2
