@@ -175,7 +175,7 @@ def main():
     ap.add_argument("--warm-ratio", type=float, default=2.0)
     ap.add_argument("--warm-cap", type=int, default=12,
                     help="batch, learned: Newton iterations from the network's guess before the fallback")
-    ap.add_argument("--warm-cont", default="10,5,2",
+    ap.add_argument("--warm-cont", default="5,5,2",
                     help="batch, learned: the instances the capped warm stage leaves continue Newton from there "
                          "for the rest of the iteration budget, with the stall test start,window,ratio (an "
                          "instance stops when its norm fell by less than ratio over the last window "

@@ -58,14 +58,15 @@ struct CRSeg {
 
 enum Fn {
   F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_JAC32, F_COND, F_COND32, F_COND_NS, F_COND_NS32, F_CR_PAIR, F_CR_SUBST,
-  F_CR_FINAL, F_CR_BACK, F_RECOVER, F_RECOVER32, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_COUNT
+  F_CR_FINAL, F_CR_BACK, F_RECOVER, F_RECOVER32, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_JAC_STATIC, F_JAC_STATIC32,
+  F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
   "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian", "colloc_jacobian32",
   "colloc_condense", "colloc_condense32", "colloc_condense_ns", "colloc_condense_ns32", "colloc_cr_pair",
   "colloc_cr_subst",
   "colloc_cr_final", "colloc_cr_back", "colloc_recover", "colloc_recover32", "colloc_axpy4", "colloc_init_input",
-  "colloc_segres", "colloc_recover_z"};
+  "colloc_segres", "colloc_recover_z", "colloc_jacobian_static", "colloc_jacobian_static32"};
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
@@ -127,6 +128,7 @@ struct cs_handle {
   int subst_spans = kDefaultSubstSpans; // see cs_set_babd_substitution (0: Householder everywhere)
   int dir_bits = 64;                    // condensation arithmetic of the Newton direction (64 or 32)
   int jac_bits = 64;                    // stored type of the last stage Jacobians (64 or 32)
+  int jac_static_bits = 0;              // stored type whose static Jacobian entries fill the buffer (0: none)
   int rec_bits = 64;                    // stored type of the last condensation records (64 or 32)
   std::vector<size_t> lev_obase;      // offset of the level in cr_oinst
   std::vector<size_t> lev_sbase;      // offset of the level in cr_seg
@@ -289,6 +291,15 @@ int jacobian(cs_handle* h, double alpha, const int* mask) {
   unsigned nthreads = (unsigned)(h->launch_intervals() * h->m + h->B);
   // the FP32 direction stores the stage Jacobians in float32 (in the same buffer)
   h->jac_bits = h->dir_bits;
+  if (h->jac_static_bits != h->jac_bits) {
+    // the static entries, once for the whole buffer in this stored type (the
+    // other type's layout overwrote them)
+    long long nst = (long long)(h->n_max - 1) * h->m;
+    void* sargs[] = {&nst, &h->jac};
+    const unsigned blocks = (unsigned)std::min<long long>(nst, 16384);
+    TRY(launch(h, h->jac_bits == 32 ? F_JAC_STATIC32 : F_JAC_STATIC, blocks, 256, sargs));
+    h->jac_static_bits = h->jac_bits;
+  }
   return launch(h, h->jac_bits == 32 ? F_JAC32 : F_JAC, (nthreads + 255) / 256, 256, args);
 }
 

@@ -220,13 +220,29 @@ __device__ __forceinline__ void jacobian(int nint, Batch bt, const double* __res
     TJ* out = jac + (size_t)st * JSTAGE;
     const double* pb = p + (size_t)b * NP;
     const double* cb = c + (size_t)b * NC;
-    prob::Df(yt, zt, pb, cb, out);
-    prob::Dg(yt, zt, pb, cb, alpha, out + NY * NJC);
+    // the dynamic entries only: the static ones (zeros, compile-time
+    // constants) are in place since jacobian_static filled the buffer
+    prob::Df_dyn(yt, zt, pb, cb, out);
+    prob::Dg_dyn(yt, zt, pb, cb, alpha, out + NY * NJC);
   } else if (gid < nst + bt.B) {
     const int b = gid - nst;
     if (skipped(bt, b)) return;
     prob::Dr(y + (size_t)bt.noff[b] * NY, y + (size_t)(bt.noff[b + 1] - 1) * NY, p + (size_t)b * NP,
              c + (size_t)b * NC, Dr + (size_t)b * DRSZ);
+  }
+}
+
+// The static entries of every stage Jacobian block (prob::jac_static_mask /
+// jac_static_val: entries that depend on nothing but the problem's
+// compile-time constants) for stages [0, nst) of the buffer, written once per
+// buffer and stored type; workgroup g fills stages g, g + G, ... with
+// consecutive lanes on consecutive entries.
+template <class TJ>
+__device__ __forceinline__ void jacobian_static(long long nst, TJ* __restrict__ jac) {
+  for (long long st = blockIdx.x; st < nst; st += gridDim.x) {
+    TJ* out = jac + (size_t)st * JSTAGE;
+    for (int k = threadIdx.x; k < JSTAGE; k += blockDim.x)
+      if (prob::jac_static_mask[k]) out[k] = (TJ)prob::jac_static_val[k];
   }
 }
 
@@ -1319,6 +1335,12 @@ __global__ __launch_bounds__(256) void colloc_jacobian(int nint, colloc::Batch b
 __global__ __launch_bounds__(256) void colloc_jacobian32(int nint, colloc::Batch bt, const double* y,
     const double* ytil, const double* ztil, const double* p, const double* c, double alpha, float* jac, double* Dr) {
   colloc::jacobian<float>(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
+}
+__global__ __launch_bounds__(256) void colloc_jacobian_static(long long nst, double* jac) {
+  colloc::jacobian_static<double>(nst, jac);
+}
+__global__ __launch_bounds__(256) void colloc_jacobian_static32(long long nst, float* jac) {
+  colloc::jacobian_static<float>(nst, jac);
 }
 
 __global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const int* iinst, const int* ilist,

@@ -369,10 +369,26 @@ def emit_problem(sym):
 
     fn("f", "", "_f", [("_f[%d]" % i, e) for i, e in enumerate(sym.f_exprs)], yb + zb + pb, H)
     fn("g", ", double _alpha", "_g", [("_g[%d]" % i, e) for i, e in enumerate(sym.g_exprs)], yb + zb + pb + ab, H)
-    fn("Df", "", "_Df", [("_Df[%d]" % (i * nj + j), e) for i, row in enumerate(sym.Df_exprs)
-                         for j, e in enumerate(row)], yb + zb + pb, H, out_type=True)
-    fn("Dg", ", double _alpha", "_Dg", [("_Dg[%d]" % (i * nj + j), e) for i, row in enumerate(sym.Dg_exprs)
-                                        for j, e in enumerate(row)], yb + zb + pb + ab, H, out_type=True)
+    # The stage Jacobian block [Df; Dg] splits into static entries (zeros and
+    # expressions of the problem's compile-time constants only) and dynamic
+    # ones (depending on the state, the parameters, the instance constants or
+    # alpha).  Df_dyn / Dg_dyn write the dynamic entries at every Newton
+    # iteration; the static ones are tabulated (jac_static_val where
+    # jac_static_mask is set) and written once per Jacobian buffer.
+    cvals = {s: sym.constants[s] for s in consts}
+    cset = set(consts)
+    block = [e for row in sym.Df_exprs for e in row] + [e for row in sym.Dg_exprs for e in row]
+    static = [not (e.free_symbols - cset) for e in block]
+    fn("Df_dyn", "", "_Df", [("_Df[%d]" % (i * nj + j), e) for i, row in enumerate(sym.Df_exprs)
+                             for j, e in enumerate(row) if not static[i * nj + j]], yb + zb + pb, H, out_type=True)
+    nf = ny * nj
+    fn("Dg_dyn", ", double _alpha", "_Dg", [("_Dg[%d]" % (i * nj + j), e) for i, row in enumerate(sym.Dg_exprs)
+                                            for j, e in enumerate(row) if not static[nf + i * nj + j]],
+       yb + zb + pb + ab, H, out_type=True)
+    out.append(_table("__device__ const unsigned char jac_static_mask", ["%d" % (1 if st else 0) for st in static]))
+    out.append(_table("__device__ const double jac_static_val",
+                      [repr(float(e.subs(cvals))) if st else "0.0" for e, st in zip(block, static)]))
+    out.append("constexpr int JAC_DYNAMIC = %d;   // dynamic entries per stage block" % static.count(False))
     # r and Dr: initial rows see y(t_i) = _y0, final rows y(t_f) = _y1
     sig_r = "const double* __restrict__ _y0, const double* __restrict__ _y1, const double* __restrict__ _p, " \
             "const double* __restrict__ _c"
@@ -411,6 +427,11 @@ def emit_problem(sym):
     out.append("}")
     out.append("}  // namespace prob")
     return "\n".join(out) + "\n"
+
+
+def _table(decl, items, per_line=16):
+    rows = [", ".join(items[k:k + per_line]) for k in range(0, len(items), per_line)]
+    return "%s[%d] = {\n  %s};" % (decl, len(items), ",\n  ".join(rows))
 
 
 def _carray(name, vals):

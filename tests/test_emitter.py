@@ -148,15 +148,45 @@ def test_general_ocp_jacobians_match_finite_differences():
         np.testing.assert_allclose(Dn, central_jacobian(fun, w0), rtol=1e-6, atol=1e-7)
 
 
+def _table(text, name):
+    """Values of an emitted table `name[N] = {...};`."""
+    i = text.index(name + "[")
+    body = text[text.index("{", i) + 1:text.index("};", i)]
+    return [float(v) for v in body.replace("\n", " ").split(",")]
+
+
+def _body(text, fn):
+    i = text.index("void %s(" % fn)
+    return text[i:text.index("\n}\n", i)]
+
+
 def test_emitted_code_is_well_formed():
     sym = emitter.BvpDaeSymbolic(ocpmod.parse_ocp(GENERAL))
     text = emitter.emit_problem(sym)
     assert text.count("{") == text.count("}")
     ny, nz, np_ = sym.size_y, sym.size_z, sym.size_p
     nj = ny + nz + np_
-    for name, n in (("_f", ny), ("_g", nz), ("_Df", ny * nj), ("_Dg", nz * nj)):
+    for name, n in (("_f", ny), ("_g", nz)):
         for i in range(n):
             assert ("%s[%d] =" % (name, i)) in text, (name, i)
+    # every entry of the stage Jacobian block [Df; Dg] is either written by
+    # Df_dyn / Dg_dyn or static: tabulated with its value, and then a
+    # function of the compile-time constants only
+    mask = _table(text, "jac_static_mask")
+    vals = _table(text, "jac_static_val")
+    assert len(mask) == len(vals) == (ny + nz) * nj
+    block = [e for row in sym.Df_exprs for e in row] + [e for row in sym.Dg_exprs for e in row]
+    dyn_f = _body(text, "Df_dyn")
+    dyn_g = _body(text, "Dg_dyn")
+    consts = {s: v for s, v in sym.constants.items() if s not in sym.instance_constants}
+    for k, e in enumerate(block):
+        name, i, body = ("_Df", k, dyn_f) if k < ny * nj else ("_Dg", k - ny * nj, dyn_g)
+        written = ("%s[%d] =" % (name, i)) in body
+        assert written != bool(mask[k]), (name, i)
+        if mask[k]:
+            assert not (e.free_symbols - set(consts)), (name, i)
+            assert vals[k] == float(e.subs(consts)), (name, i)
+    assert any(mask) and not all(mask)
     assert "K_c1" in text and "K_c2" in text
     # deterministic: the code text (and so the code-object cache key) is stable
     assert text == emitter.emit_problem(emitter.BvpDaeSymbolic(ocpmod.parse_ocp(GENERAL)))
