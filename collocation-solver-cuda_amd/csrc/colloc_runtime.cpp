@@ -59,14 +59,15 @@ struct CRSeg {
 enum Fn {
   F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_JAC32, F_COND, F_COND32, F_COND_NS, F_COND_NS32, F_CR_PAIR, F_CR_SUBST,
   F_CR_FINAL, F_CR_BACK, F_RECOVER, F_RECOVER32, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_JAC_STATIC, F_JAC_STATIC32,
-  F_COUNT
+  F_AXPY_LISTED, F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
   "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian", "colloc_jacobian32",
   "colloc_condense", "colloc_condense32", "colloc_condense_ns", "colloc_condense_ns32", "colloc_cr_pair",
   "colloc_cr_subst",
   "colloc_cr_final", "colloc_cr_back", "colloc_recover", "colloc_recover32", "colloc_axpy4", "colloc_init_input",
-  "colloc_segres", "colloc_recover_z", "colloc_jacobian_static", "colloc_jacobian_static32"};
+  "colloc_segres", "colloc_recover_z", "colloc_jacobian_static", "colloc_jacobian_static32",
+  "colloc_axpy_listed"};
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
@@ -458,6 +459,11 @@ Seg4 state_segs(cs_handle* h) {
 // out = x + steps[b] d per instance b (steps on the device).
 int axpy_state(cs_handle* h, const double* x, const double* d, double* out) {
   Seg4 s = state_segs(h);
+  if (h->nalist > 0) {   // a launch list is set: just the listed intervals' entries
+    Batch bt = h->batch_listed(nullptr);
+    void* args[] = {&s, &bt, &x, &d, &h->steps, &out};
+    return launch(h, F_AXPY_LISTED, (unsigned)h->nalist, 64, args);
+  }
   long long total = s.len[0] + s.len[1] + s.len[2] + s.len[3];
   Batch bt = h->batch(nullptr);
   void* args[] = {&s, &bt, &x, &d, &h->steps, &out};

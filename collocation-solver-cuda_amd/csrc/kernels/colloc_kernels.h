@@ -1142,6 +1142,29 @@ __device__ __forceinline__ void axpy4(Seg4 s, Batch bt, const double* x, const d
   }
 }
 
+// The same update over the intervals of a launch list (bt.ilist set): one
+// workgroup per listed interval i of instance b updates its stage blocks
+// (y-dot, z-tilde) and its left node; the instance's last interval also its
+// right node and p.  Instances with steps[b] == 0 are left unchanged.
+__device__ __forceinline__ void axpy_listed(Seg4 s, Batch bt, const double* x, const double* __restrict__ d,
+                                            const double* __restrict__ steps, double* out) {
+  const int i = bt.ilist[blockIdx.x];
+  const int b = bt.iinst[i];
+  const double a = steps[b];
+  if (a == 0.0) return;
+  const bool last = i + b + 2 == bt.noff[b + 1];
+  constexpr int NYD = M * NY, NZD = M * NZ;
+  const int n = NYD + NZD + NY + (last ? NY + NP : 0);
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    long long k;
+    if (e < NYD) k = s.off[1] + (long long)i * NYD + e;
+    else if (e < NYD + NZD) k = s.off[2] + (long long)i * NZD + (e - NYD);
+    else if (e < NYD + NZD + 2 * NY) k = s.off[0] + (long long)(i + b) * NY + (e - NYD - NZD);   // left, right node
+    else k = s.off[3] + (long long)b * NP + (e - NYD - NZD - 2 * NY);
+    out[k] = fma(a, d[k], x[k]);
+  }
+}
+
 // -------------------------------------------------------- initial input
 // form_initial_input_kernel (solver:316-356).
 __device__ __forceinline__ void init_input(int nint, Batch bt, const double* __restrict__ y0,
@@ -1415,6 +1438,11 @@ __global__ __launch_bounds__(256) void colloc_recover32(int nint, colloc::Batch 
 __global__ __launch_bounds__(256) void colloc_axpy4(colloc::Seg4 s, colloc::Batch bt, const double* x,
     const double* d, const double* steps, double* out) {
   colloc::axpy4(s, bt, x, d, steps, out);
+}
+
+__global__ __launch_bounds__(64) void colloc_axpy_listed(colloc::Seg4 s, colloc::Batch bt, const double* x,
+    const double* d, const double* steps, double* out) {
+  colloc::axpy_listed(s, bt, x, d, steps, out);
 }
 
 __global__ __launch_bounds__(256) void colloc_init_input(int nint, colloc::Batch bt, const double* y0,
