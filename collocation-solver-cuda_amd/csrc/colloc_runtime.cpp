@@ -116,8 +116,10 @@ struct cs_handle {
   int* act = nullptr;        // instances still iterating
   int* srch = nullptr;       // instances still line-searching
   int* fbk = nullptr;        // per interval: 1 if the series condensation left it to Gauss-Jordan
-  int* alist = nullptr;      // intervals of the instances still iterating (launch list)
+  int* alist = nullptr;      // the current launch list of intervals (one of the two buffers below)
   int nalist = 0;            // its length; 0: launch over all intervals
+  int* alist_buf = nullptr;  // intervals of the instances still iterating
+  int* slist_buf = nullptr;  // intervals of the instances still line-searching
   // cyclic-reduction schedule of the current mesh: per level, the instance
   // of each output row and each instance's CRSeg
   int* cr_oinst = nullptr;
@@ -271,11 +273,11 @@ int residual(cs_handle* h, const double* st, double alpha, bool full, const int*
   const double* zt = st + h->st_z;
   const double* p = st + h->st_p;
   unsigned long long* slot = h->scal;
-  // the full residual (start of an iteration) runs over the listed intervals;
-  // the line-search residual over all, masked by the searching instances
-  Batch bt = full ? h->batch_listed(mask) : h->batch(mask);
+  // over the listed intervals when a launch list is set (the instances still
+  // iterating, a superset of the line search's), masked by `mask`
+  Batch bt = h->batch_listed(mask);
   void* args[] = {&nint, &bt, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &h->ytil, &h->fa, &h->fb, &h->rbc, &slot};
-  const int nlaunch = full ? h->launch_intervals() : nint;
+  const int nlaunch = h->launch_intervals();
   unsigned nthreads = (unsigned)(nlaunch * h->m + h->B);
   TRY(launch(h, full ? F_RES_FULL : F_RES_NORM, (nthreads + 255) / 256, 256, args));
   return read_slots(h, h->scal, norms);
@@ -586,7 +588,8 @@ int cs_create(const cs_dims* d, cs_handle** out) {
   I(&h->act, bm);
   I(&h->srch, bm);
   I(&h->fbk, nint);
-  I(&h->alist, nint);
+  I(&h->alist_buf, nint);
+  I(&h->slist_buf, nint);
   if (r == CS_OK) {
     hipError_t e = hipMalloc((void**)&h->scal, (2 * bm + 2) * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&h->status, sizeof(int));
@@ -612,7 +615,7 @@ void cs_destroy(cs_handle* h) {
                     h->rows0, h->rows1, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps};
   for (double* b : bufs)
     if (b) hipFree(b);
-  int* ibufs[] = {h->iinst, h->ninst, h->noff, h->act, h->srch, h->fbk, h->alist, h->cr_oinst};
+  int* ibufs[] = {h->iinst, h->ninst, h->noff, h->act, h->srch, h->fbk, h->alist_buf, h->slist_buf, h->cr_oinst};
   for (int* b : ibufs)
     if (b) hipFree(b);
   if (h->cr_seg) hipFree(h->cr_seg);
@@ -806,29 +809,46 @@ int init_from_saved(cs_handle* h) {
 // every later iteration of that instance would repeat this one exactly: the
 // instance stops here with the outcome those iterations would give (loop
 // counter max_iter - 1, the same norm, the norm history continued).
-// Launch list of the intervals of the active instances (none, i.e. every
-// interval, when all are active); re-uploaded only when the set changed.
+// Launch lists of the intervals of the active instances (set) and of the
+// line-searching ones (search), each in its own buffer and re-uploaded only
+// when its set changed; none (every interval) when all instances are in the
+// set.  The handle's current list is the one selected last.
 struct LaunchList {
   cs_handle* h;
-  std::vector<int> last;
-  explicit LaunchList(cs_handle* h_) : h(h_) {}
-  ~LaunchList() { h->nalist = 0; }   // other entry points launch over every interval
-  int set(const std::vector<int>& active) {
-    if (active == last) return CS_OK;
-    last = active;
-    int nact = 0;
-    for (int a : active) nact += a;
-    if (nact == h->B || nact == 0) {
-      h->nalist = 0;
-      return CS_OK;
+  struct Slot {
+    int* buf;
+    std::vector<int> last;
+    int n = 0;   // list length; 0: no list
+  };
+  Slot act_, srch_;
+  explicit LaunchList(cs_handle* h_) : h(h_), act_{h_->alist_buf, {}, 0}, srch_{h_->slist_buf, {}, 0} {}
+  ~LaunchList() {   // other entry points launch over every interval
+    h->alist = nullptr;
+    h->nalist = 0;
+  }
+  int fill(Slot& sl, const std::vector<int>& in) {
+    if (in != sl.last) {
+      sl.last = in;
+      int nin = 0;
+      for (int a : in) nin += a;
+      sl.n = 0;
+      if (nin != h->B && nin != 0) {
+        std::vector<int> list;
+        for (int b = 0; b < h->B; ++b)
+          if (in[b])
+            for (int i = h->hnoff[b] - b; i < h->hnoff[b + 1] - b - 1; ++i) list.push_back(i);
+        TRY(upload_ints(h, sl.buf, list));
+        sl.n = (int)list.size();
+      }
     }
-    std::vector<int> list;
-    for (int b = 0; b < h->B; ++b)
-      if (active[b])
-        for (int i = h->hnoff[b] - b; i < h->hnoff[b + 1] - b - 1; ++i) list.push_back(i);
-    TRY(upload_ints(h, h->alist, list));
-    h->nalist = (int)list.size();
+    h->alist = sl.n ? sl.buf : nullptr;
+    h->nalist = sl.n;
     return CS_OK;
+  }
+  int set(const std::vector<int>& active) { return fill(act_, active); }
+  int search(const std::vector<int>& searching) {
+    if (searching == act_.last) return fill(act_, searching);   // the same set: its list
+    return fill(srch_, searching);
   }
 };
 
@@ -886,6 +906,7 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
     bool any_taken = false;
     for (int ls = 0; ls < 20; ++ls) {
       TRY(upload_ints(h, h->srch, search));
+      TRY(lst.search(search));
       TRY(upload_doubles(h, h->steps, step));
       TRY(axpy_state(h, h->state, h->dir, h->trial));
       TRY(residual(h, h->trial, alpha, false, h->srch, nn.data()));
@@ -904,6 +925,7 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
       }
       if (left == 0) break;
     }
+    TRY(lst.set(active));
     if (any_taken) {
       TRY(upload_doubles(h, h->steps, taken));
       TRY(axpy_state(h, h->state, h->dir, h->state));
