@@ -102,8 +102,8 @@ struct cs_handle {
   double* rbc = nullptr;
   double* jac = nullptr;
   double* Dr = nullptr;
-  double* rows0 = nullptr;
-  double* rows1 = nullptr;
+  double* rows0 = nullptr;   // BABD rows of every cyclic-reduction level, level after level
+  size_t rows_cap = 0;       // rows0's capacity in rows
   double* rec = nullptr;
   double* crrec = nullptr;
   double* segres = nullptr;
@@ -135,6 +135,7 @@ struct cs_handle {
   int rec_bits = 64;                    // stored type of the last condensation records (64 or 32)
   std::vector<size_t> lev_obase;      // offset of the level in cr_oinst
   std::vector<size_t> lev_sbase;      // offset of the level in cr_seg
+  std::vector<size_t> lev_rbase;      // row offset of the level's input rows in rows0 (one more: the last output)
   unsigned long long cond_fallback = 0;   // intervals condensed by Gauss-Jordan since creation
   unsigned long long cond_terms = 0;      // series terms summed over intervals since creation
   unsigned long long cond_intervals = 0;  // intervals condensed since creation
@@ -358,6 +359,9 @@ int build_cr_schedule(cs_handle* h) {
   h->lev_nout.clear();
   h->lev_obase.clear();
   h->lev_sbase.clear();
+  h->lev_rbase.assign(1, 0);   // level 0 reads the condensation's rows, one per interval
+  size_t nin = 0;
+  for (int b = 0; b < B; ++b) nin += (size_t)n[b];
   h->lev_qr.clear();
   h->lev_subst.clear();
   size_t rec_base = 0;
@@ -384,7 +388,14 @@ int build_cr_schedule(cs_handle* h) {
     h->lev_nout.push_back(out);
     h->lev_qr.push_back(any_qr);
     h->lev_subst.push_back(any_sub);
+    // every level's rows stay in place: the substitution levels' back
+    // substitution reads its input rows instead of a record
+    h->lev_rbase.push_back(h->lev_rbase.back() + nin);
+    nin = (size_t)out;
   }
+  if (h->lev_rbase.back() + nin > h->rows_cap)
+    return fail(CS_E_SHAPE, "cyclic reduction needs %zu BABD rows, the handle holds %zu",
+                h->lev_rbase.back() + nin, h->rows_cap);
   if (oinst.size() > h->cr_oinst_cap) {
     if (h->cr_oinst) hipFree(h->cr_oinst);
     h->cr_oinst = nullptr;
@@ -411,27 +422,29 @@ int babd(cs_handle* h, const int* mask) {
   double* dy = h->dir + h->st_y;
   double* dp = h->dir + h->st_p;
   Batch bt = h->batch(mask);
-  double* rin = h->rows0;
-  double* rout = h->rows1;
+  const size_t rowsz = (size_t)h->ny * h->crw;
   unsigned pair_block = (unsigned)((h->recw + 15) / 16 * 64);   // 4 lanes per record column
   const int nlev = (int)h->lev_nout.size();
   for (int l = 0; l < nlev; ++l) {
     const int* oinst = h->cr_oinst + h->lev_obase[l];
     const CRSeg* seg = h->cr_seg + h->lev_sbase[l];
     const int* act = mask;
+    double* rin = h->rows0 + h->lev_rbase[l] * rowsz;
+    double* rout = h->rows0 + h->lev_rbase[l + 1] * rowsz;
     void* args[] = {&oinst, &seg, &act, &rin, &rout, &h->crrec};
     if (h->lev_subst[l]) TRY(launch(h, F_CR_SUBST, (unsigned)h->lev_nout[l], kSubstThreads, args));
     if (h->lev_qr[l]) TRY(launch(h, F_CR_PAIR, (unsigned)h->lev_nout[l], pair_block, args));
-    std::swap(rin, rout);
   }
   {
+    double* rin = h->rows0 + h->lev_rbase[nlev] * rowsz;
     void* args[] = {&bt, &rin, &h->Dr, &h->rbc, &dy, &dp, &h->status};
     TRY(launch(h, F_CR_FINAL, (unsigned)h->B, (unsigned)(h->fw * 64), args));
   }
   for (int l = nlev - 1; l >= 0; --l) {
     const int* oinst = h->cr_oinst + h->lev_obase[l];
     const CRSeg* seg = h->cr_seg + h->lev_sbase[l];
-    void* args[] = {&l, &bt, &oinst, &seg, &h->crrec, &dy, &dp};
+    double* rin = h->rows0 + h->lev_rbase[l] * rowsz;
+    void* args[] = {&l, &bt, &oinst, &seg, &h->crrec, &rin, &dy, &dp};
     TRY(launch(h, F_CR_BACK, (unsigned)h->lev_nout[l], 64, args));
   }
   return CS_OK;
@@ -575,8 +588,10 @@ int cs_create(const cs_dims* d, cs_handle** out) {
   A(&h->rbc, bm * h->nrb);
   A(&h->jac, nint * m * (size_t)h->jstage);
   A(&h->Dr, bm * h->drsz);
-  A(&h->rows0, nint * ny * (size_t)h->crw);
-  A(&h->rows1, (nint / 2 + bm) * ny * (size_t)h->crw);   // a level's output: <= nint/2 + B rows
+  // the rows of all levels: an instance of n rows has at most n + 32 output
+  // rows over its levels (ceil(n / 2^k) <= n / 2^k + 1, at most 32 levels)
+  h->rows_cap = 2 * nint + 32 * bm;
+  A(&h->rows0, h->rows_cap * ny * (size_t)h->crw);
   A(&h->crrec, nint * ny * (size_t)h->recw);
   A(&h->segres, nmax);
   A(&h->zbuf, nmax * std::max(nz, 1));
@@ -612,7 +627,7 @@ void cs_destroy(cs_handle* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
   double* bufs[] = {h->t, h->state, h->trial, h->dir, h->c, h->ytil, h->fa, h->fb, h->rbc, h->jac, h->Dr,
-                    h->rows0, h->rows1, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps};
+                    h->rows0, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps};
   for (double* b : bufs)
     if (b) hipFree(b);
   int* ibufs[] = {h->iinst, h->ninst, h->noff, h->act, h->srch, h->fbk, h->alist_buf, h->slist_buf, h->cr_oinst};

@@ -864,7 +864,7 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
   const int nrows = sg.nrows;
   rows_in += (size_t)sg.in_off * NY * CRW;
   rows_out += (size_t)sg.out_off * NY * CRW;
-  recs += (size_t)sg.rec_off * NY * RECW;
+  (void)recs;   // the substitution leaves no record
   const int npairs = nrows >> 1;
   const int tid = threadIdx.x;
   if (k == npairs) {  // carry the odd last row
@@ -876,7 +876,6 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
   const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
   const double* rb = ra + NY * CRW;
   double* out = rows_out + (size_t)k * NY * CRW;
-  double* rec = recs + (size_t)k * NY * RECW;
   for (int e = tid; e < CS_RG * 4 * NY; e += blockDim.x) {
     const int r = e / NY, c = e - (e / NY) * NY;
     (&sG[0][0])[e] = r < NY ? rb[r * CRW + c] : 0.0;
@@ -889,17 +888,8 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
     else if (c == NY + NP) v = ra[r * CRW + CRW - 1];      // b_a
     sX[r][c] = v;
   }
-  // the record: [I | G_a | 0 | H_a | b_a]
-  for (int e = tid; e < NY * RECW; e += blockDim.x) {
-    const int r = e / RECW, c = e - (e / RECW) * RECW;
-    double v;
-    if (c < NY) v = (c == r) ? 1.0 : 0.0;
-    else if (c < 2 * NY) v = ra[r * CRW + c - NY];
-    else if (c < 3 * NY) v = 0.0;
-    else if (c < 3 * NY + NP) v = ra[r * CRW + c - NY];    // H_a: column 2NY + j of the row
-    else v = ra[r * CRW + CRW - 1];
-    rec[e] = v;
-  }
+  // no record: back substitution reads row a itself (x_M = b_a - G_a x_L - H_a p),
+  // which stays in place in this level's input rows
   __syncthreads();
   for (int t = tid; t < CS_RG * CS_CG; t += blockDim.x) {
     const int rg = t / CS_CG, cg = t - (t / CS_CG) * CS_CG;
@@ -1013,7 +1003,8 @@ __device__ __forceinline__ void cr_final(Batch bt, const double* __restrict__ ro
 // no record and return.
 __device__ __forceinline__ void cr_back(int level, Batch bt, const int* __restrict__ oinst,
                         const CRSeg* __restrict__ seg, const double* __restrict__ recs,
-                        double* __restrict__ dy, const double* __restrict__ dp) {
+                        const double* __restrict__ rows_in, double* __restrict__ dy,
+                        const double* __restrict__ dp) {
   const int ib = oinst[blockIdx.x];
   if (skipped(bt, ib)) return;
   const CRSeg sg = seg[ib];
@@ -1025,10 +1016,22 @@ __device__ __forceinline__ void cr_back(int level, Batch bt, const int* __restri
   const int mid = base + ((2 * k + 1) << level);
   const int left = base + ((2 * k) << level);
   const int right = base + min((2 * k + 2) << level, nint);
-  const double* rec = recs + (size_t)(sg.rec_off + k) * NY * RECW;
   const double* xl = dy + (size_t)left * NY;
   const double* xr = dy + (size_t)right * NY;
   dp += (size_t)ib * NP;
+  if (sg.subst) {
+    // substitution level: row a of the pair, [G_a | I | H_a | b_a] (this
+    // level's input, kept in place), gives x_M = b_a - G_a x_L - H_a p
+    const double* ra = rows_in + (size_t)(sg.in_off + 2 * k) * NY * CRW + (lane < NY ? lane : 0) * CRW;
+    double rhs = ra[CRW - 1];
+#pragma unroll 8
+    for (int c = 0; c < NY; ++c) rhs = fma(-ra[c], xl[c], rhs);
+#pragma unroll 8
+    for (int c = 0; c < NP; ++c) rhs = fma(-ra[2 * NY + c], dp[c], rhs);
+    if (lane < NY) dy[(size_t)mid * NY + lane] = rhs;
+    return;
+  }
+  const double* rec = recs + (size_t)(sg.rec_off + k) * NY * RECW;
   // lane r holds row r of R and the right-hand side d - Gp x_L - Ep x_R - Jp p
   const double* rw = rec + (lane < NY ? lane : 0) * RECW;
   double Rr[NY];
@@ -1421,8 +1424,8 @@ __global__ __launch_bounds__(Cfg::FW * 64) void colloc_cr_final(colloc::Batch bt
 }
 
 __global__ __launch_bounds__(64) void colloc_cr_back(int level, colloc::Batch bt, const int* oinst,
-    const colloc::CRSeg* seg, const double* recs, double* dy, const double* dp) {
-  colloc::cr_back(level, bt, oinst, seg, recs, dy, dp);
+    const colloc::CRSeg* seg, const double* recs, const double* rows_in, double* dy, const double* dp) {
+  colloc::cr_back(level, bt, oinst, seg, recs, rows_in, dy, dp);
 }
 
 __global__ __launch_bounds__(256) void colloc_recover(int nint, colloc::Batch bt, const double* rec,
