@@ -375,7 +375,8 @@ int build_cr_schedule(cs_handle* h) {
     char any_qr = 0, any_sub = 0;
     for (int b = 0; b < B; ++b) {
       const int nout = n[b] / 2 + (n[b] & 1);
-      const int sub = lev < nsub[b] ? 1 : 0;
+      // 2: the next level substitutes too (it does not read C, which stays I)
+      const int sub = lev < nsub[b] ? (lev + 1 < nsub[b] ? 2 : 1) : 0;
       segs.push_back(CRSeg{in_off[b], out, (int)rec_base, n[b], sub});
       if (sub) any_sub = 1;
       else any_qr = 1;

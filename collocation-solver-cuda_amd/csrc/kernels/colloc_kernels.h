@@ -83,7 +83,8 @@ __device__ __forceinline__ int listed_interval(const Batch& bt, int k) { return 
 // One CR level of one instance: its rows start at in_off of the input
 // buffer, its ceil(nrows / 2) output rows at out_off, its nrows / 2 records
 // at rec_off; subst != 0: the level pairs by substitution (cr_subst), else
-// by Householder QR (cr_pair).
+// by Householder QR (cr_pair); subst == 2: the instance's next level pairs
+// by substitution too, so the output rows' C block (I) is left unwritten.
 struct CRSeg {
   int in_off, out_off, rec_off, nrows, subst;
 };
@@ -867,10 +868,17 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
   (void)recs;   // the substitution leaves no record
   const int npairs = nrows >> 1;
   const int tid = threadIdx.x;
+  // C = I on every row of a substitution level, so it is not read here;
+  // it is written (as I) only where the next level is not one (subst == 1)
+  const bool wc = sg.subst == 1;
   if (k == npairs) {  // carry the odd last row
     const double* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
     double* dst = rows_out + (size_t)npairs * NY * CRW;
-    for (int e = tid; e < NY * CRW; e += blockDim.x) dst[e] = src[e];
+    for (int e = tid; e < NY * CRW; e += blockDim.x) {
+      const int r = e / CRW, c = e - (e / CRW) * CRW;
+      if (c < NY || c >= 2 * NY) dst[e] = src[e];
+      else if (wc) dst[e] = (c - NY == r) ? 1.0 : 0.0;
+    }
     return;
   }
   const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
@@ -923,9 +931,11 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
       }
     }
   }
-  for (int e = tid; e < NY * NY; e += blockDim.x) {        // C' = C_b
-    const int r = e / NY, c = e - (e / NY) * NY;
-    out[r * CRW + NY + c] = rb[r * CRW + NY + c];
+  if (wc) {
+    for (int e = tid; e < NY * NY; e += blockDim.x) {      // C' = C_b = I
+      const int r = e / NY, c = e - (e / NY) * NY;
+      out[r * CRW + NY + c] = (r == c) ? 1.0 : 0.0;
+    }
   }
 }
 
