@@ -107,3 +107,38 @@ def test_fp32_direction_solve(nint):
     f_q = osolver.compute_f_q(prob, m, r32["N"], otables.lobatto(m), r32["t"], r32["y"], r32["y_dot"],
                               r32["z_tilde"], r32["p"], 0.1)[0]
     assert np.max(np.abs(f_q)) < bvp.tolerance
+
+
+def test_precision_switches_refill_static_jacobian_entries():
+    """One handle switching 32 -> 64 -> 32: the stage Jacobian buffer holds
+    float32 or float64 entries, and its static entries (zeros, compile-time
+    constants) are written once per stored type.  After every switch the
+    whole Jacobian must match the oracle's (float32 storage: to float32
+    rounding; float64: to 1e-13)."""
+    inst, m = problems.config(1, nint=64)
+    bvp = solver.BvpDae(inst.ocp_text())
+    prob = family_of(inst)
+    N = bvp.N
+    S = solver.CollocationSolver(bvp, m, n_max=N)
+    t = np.linspace(0.0, inst.T, N)
+    rng = np.random.default_rng(5)
+    y = rng.normal(scale=0.3, size=(N, prob.size_y))
+    yd = rng.normal(scale=0.3, size=((N - 1) * m, prob.size_y))
+    zt = rng.normal(scale=0.3, size=((N - 1) * m, prob.size_z))
+    p = rng.normal(scale=0.3, size=prob.size_p)
+    S.set_mesh(t)
+    S.set_state(y, yd, zt, p)
+    _, yt, _, _, _ = S.compute_f_q(0.1)
+    Df_o, Dg_o = prob.Df(yt, zt, p), prob.Dg(yt, zt, p)
+    for bits in (32, 64, 32, 64):
+        S.set_direction_precision(bits)
+        Df, Dg, _ = S.construct_jacobian(0.1)
+        errs = (rel_err(Df, Df_o), rel_err(Dg, Dg_o))
+        say("precision %d: Jacobian errors %.1e %.1e" % (bits, errs[0], errs[1]))
+        bound = 1e-7 if bits == 32 else 1e-13
+        assert errs[0] < bound and errs[1] < bound, (bits, errs)
+        # entries zero in every stage (structural zeros) are exactly zero
+        zero = np.all(Df_o == 0, axis=0)
+        assert zero.any() and np.all(Df[:, zero] == 0)
+    assert S.status() == 0
+    S.close()
