@@ -167,8 +167,9 @@ def main():
                     help="batch: arithmetic of the Newton direction's condensation (32: the FP32 direction)")
     ap.add_argument("--fp64-pass", type=int, default=1,
                     help="batch, learned, precision 32: also time one learned pass with the float64 direction")
-    ap.add_argument("--babd-spans", type=int, default=64,
-                    help="batch: substitution pairing while a pair spans <= 1/N of an instance (0: Householder only)")
+    ap.add_argument("--babd-spans", type=int, default=16,
+                    help="batch: substitution pairing while a pair spans <= 1/N of an instance (0: Householder only; "
+                         "the library's default is 64)")
     ap.add_argument("--warm-patience", type=int, default=8,
                     help="batch, learned: from this warm iteration on, stop instances whose norm falls by less than "
                          "--warm-ratio per iteration (0: off)")
@@ -218,8 +219,7 @@ def batch_workload(args):
     S = solver.CollocationSolver(bvp, m, n_max=max(sum(nodes[i] for i in ch) for ch in chunks),
                                  batch=max(len(ch) for ch in chunks))
     S.set_direction_precision(args.precision)
-    if args.babd_spans != 64:
-        S.set_babd_substitution(args.babd_spans)
+    S.set_babd_substitution(args.babd_spans)
     ts_all = [np.linspace(0.0, inst.T, inst.nodes) for inst in insts]
     cons_all = np.array([inst.constant_values() for inst in insts])
     hom_all = [np.array([inst.constant_values(s=0.0) for inst in insts])]
@@ -374,6 +374,8 @@ def batch_workload(args):
                 else "plain all-ones guess + homotopy S: 0->0.5")
     form_txt += ("; Newton direction condensed in float32 (residuals, BABD, convergence test in float64)"
                  if args.precision == 32 else "; float64 throughout")
+    form_txt += ("; BABD pairs by substitution while a pair spans <= 1/%d of its instance, by Householder QR above"
+                 % args.babd_spans if args.babd_spans > 0 else "; BABD pairs by Householder QR on every level")
     line = {
         "metric": "converged BVP solves/s @ residual <=1e-8, %s vs plain FP64, 1 GPU"
                   % (("learned+batched" if args.form == "learned" else "batched")
