@@ -143,8 +143,8 @@ def condense_roofline(S, before, after, cond_ms, cond_n, ny, nz, np_, npa, m, tr
 
 def check_gpus(args):
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (the solve shards by "
-              "interval ranges or by instance; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (a batch would shard by instance, "
+              "one chunk per GPU, with no communication inside a solve). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
 
