@@ -200,7 +200,8 @@ __device__ __forceinline__ void residual(int nint, Batch bt, const double* __res
 // instance b.  Restates construct_jacobian_kernel (solver:680-797) without
 // materialising J, V, D, W (the condensation kernel assembles what it needs).
 // TJ: the stored type of the stage Jacobians (double, or float for the FP32
-// direction; evaluated in float64 either way).
+// direction), which is also the type their state-dependent entries are
+// evaluated in.
 template <class TJ>
 __device__ __forceinline__ void jacobian(int nint, Batch bt, const double* __restrict__ y,
                          const double* __restrict__ ytil, const double* __restrict__ ztil,

@@ -287,6 +287,50 @@ class _Printer(C99CodePrinter):
         return "((%s) > 0.0 ? 1.0 : ((%s) < 0.0 ? 0.0 : 0.5))" % (a, a)
 
 
+class _TypedPrinter(_Printer):
+    """Prints expressions evaluated in the C type `tc` (a template parameter
+    such as TO): literals and compile-time constants are cast to it, so a
+    float instantiation computes in float throughout."""
+
+    def __init__(self, names, tc):
+        super().__init__(names)
+        self._tc = tc
+
+    def _c(self, lit):
+        return "((%s)%s)" % (self._tc, lit)
+
+    def _print_Pow(self, e):
+        b, x = e.base, e.exp
+        if x.is_Integer and -4 <= int(x) <= -1:
+            s = "(" + self._print(b) + ")"
+            return "(%s/(%s))" % (self._c("1"), "*".join([s] * (-int(x))))
+        if x == sp.Rational(-1, 2):
+            return "(%s/sqrt(%s))" % (self._c("1"), self._print(b))
+        return super()._print_Pow(e)
+
+    def _print_Float(self, e):
+        return self._c(repr(float(e)))
+
+    def _print_Rational(self, e):
+        return self._c(repr(float(e.p) / float(e.q)))
+
+    def _print_Integer(self, e):
+        return self._c("%d" % int(e))
+
+    def _print_sign(self, e):
+        a = self._print(e.args[0])
+        return "((%s) > %s ? %s : ((%s) < %s ? %s : %s))" % (a, self._c(0), self._c(1), a, self._c(0), self._c(-1),
+                                                             self._c(0))
+
+    def _print_DiracDelta(self, e):
+        return self._c(0)
+
+    def _print_Heaviside(self, e):
+        a = self._print(e.args[0])
+        return "((%s) > %s ? %s : ((%s) < %s ? %s : %s))" % (a, self._c(0), self._c(1), a, self._c(0), self._c(0),
+                                                             self._c("0.5"))
+
+
 def _cname(sym):
     """C name of a symbol: generated symbols (leading '_') get prefix 'w', user
     symbols 'u_', constants are printed as 'K_<name>' (distinct namespaces)."""
@@ -294,8 +338,11 @@ def _cname(sym):
     return "w" + n if n.startswith("_") else "u_" + n
 
 
-def _emit_block(lines, exprs_out, hoister, bindings, constants, inst_consts, indent="  "):
-    """Append C statements computing exprs_out = [(target, expr), ...]."""
+def _emit_block(lines, exprs_out, hoister, bindings, constants, inst_consts, indent="  ", ctype=None):
+    """Append C statements computing exprs_out = [(target, expr), ...]; with
+    ctype (a C type name) every intermediate is computed in that type,
+    else in double."""
+    td = ctype or "double"
     exprs = [e for _, e in exprs_out]
     hdefs = hoister.needed(exprs) if hoister is not None else []
     used = set()
@@ -304,33 +351,34 @@ def _emit_block(lines, exprs_out, hoister, bindings, constants, inst_consts, ind
     for _, d in hdefs:
         used |= d.free_symbols
     names = {}
+    cast = (lambda src: "(%s)(%s)" % (ctype, src)) if ctype else (lambda src: src)
     for s, src in bindings:
         names[s] = _cname(s)
         if s in used:
-            lines.append("%sconst double %s = %s;" % (indent, names[s], src))
+            lines.append("%sconst %s %s = %s;" % (indent, td, names[s], cast(src)))
     for k, s in enumerate(inst_consts):
         names[s] = _cname(s)
         if s in used:
-            lines.append("%sconst double %s = _c[%d];" % (indent, names[s], k))
+            lines.append("%sconst %s %s = %s;" % (indent, td, names[s], cast("_c[%d]" % k)))
     for s in constants:
         if s not in names:
-            names[s] = "K_" + s.name
-    pr = _Printer(names)
+            names[s] = ("((%s)K_%s)" % (ctype, s.name)) if ctype else "K_" + s.name
+    pr = _TypedPrinter(names, ctype) if ctype else _Printer(names)
     for s, d in hdefs:
         names[s] = _cname(s)
-        lines.append("%sconst double %s = %s;" % (indent, names[s], pr.doprint(d)))
+        lines.append("%sconst %s %s = %s;" % (indent, td, names[s], pr.doprint(d)))
     nonzero = [(tgt, e) for tgt, e in exprs_out if e != 0]
     if nonzero:
         reps, red = sp.cse([e for _, e in nonzero], symbols=sp.numbered_symbols("_t", real=True),
                            optimizations=None, order="none")
         for s, d in reps:
             names[s] = _cname(s)
-            lines.append("%sconst double %s = %s;" % (indent, names[s], pr.doprint(d)))
+            lines.append("%sconst %s %s = %s;" % (indent, td, names[s], pr.doprint(d)))
         for (tgt, _), e in zip(nonzero, red):
             lines.append("%s%s = %s;" % (indent, tgt, pr.doprint(e)))
     for tgt, e in exprs_out:
         if e == 0:
-            lines.append("%s%s = 0.0;" % (indent, tgt))
+            lines.append("%s%s = %s;" % (indent, tgt, "(%s)0" % ctype if ctype else "0.0"))
 
 
 def emit_problem(sym):
@@ -352,8 +400,8 @@ def emit_problem(sym):
     H = sym.hoist_fg
 
     def fn(name, extra, target, rows, binds, hoister, out_type=False):
-        # out_type: a template on the output type (the Jacobians are stored in
-        # float64, or float32 for the FP32 direction; evaluated in float64)
+        # out_type: a template on the output type (the Jacobians are stored,
+        # and evaluated, in float64, or float32 for the FP32 direction)
         if out_type:
             lines = ["template <class TO>",
                      "__device__ __forceinline__ void %s(%s%s, TO* __restrict__ %s) {" % (name, sig_fg, extra, target)]
@@ -363,7 +411,9 @@ def emit_problem(sym):
         lines.append("  (void)_y; (void)_z; (void)_p; (void)_c;")
         if "_alpha" in extra:
             lines.append("  (void)_alpha;")
-        _emit_block(lines, rows, hoister, binds, consts, ic)
+        # the Jacobians compute in their stored type (float for the FP32
+        # direction: an inexact-Newton direction), the residuals in double
+        _emit_block(lines, rows, hoister, binds, consts, ic, ctype="TO" if out_type else None)
         lines.append("}")
         out.extend(lines)
 

@@ -155,7 +155,7 @@ int cs_solve_babd(cs_handle* h, double* delta_y, double* delta_p);
 
 /* Arithmetic of the Newton direction's condensation (reduce_jacobian,
  * solver:840-1001): 64 (default; float64 throughout, as the reference) or 32
- * (the stage Jacobians stored in float32 -- evaluated in float64 -- and the
+ * (the stage Jacobians evaluated and stored in float32, and the
  * stage eliminations and the series solve of the stage systems in float32:
  * an inexact-Newton direction).  Residuals, the BABD solve, the recovery,
  * the line-search merit and the convergence test stay float64; intervals

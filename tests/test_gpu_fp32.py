@@ -113,8 +113,9 @@ def test_precision_switches_refill_static_jacobian_entries():
     """One handle switching 32 -> 64 -> 32: the stage Jacobian buffer holds
     float32 or float64 entries, and its static entries (zeros, compile-time
     constants) are written once per stored type.  After every switch the
-    whole Jacobian must match the oracle's (float32 storage: to float32
-    rounding; float64: to 1e-13)."""
+    whole Jacobian must match the oracle's (float32 evaluation and storage:
+    2e-6 relative to the largest entry, a few float32 roundings of O(1)
+    terms; float64: 1e-13)."""
     inst, m = problems.config(1, nint=64)
     bvp = solver.BvpDae(inst.ocp_text())
     prob = family_of(inst)
@@ -135,7 +136,7 @@ def test_precision_switches_refill_static_jacobian_entries():
         Df, Dg, _ = S.construct_jacobian(0.1)
         errs = (rel_err(Df, Df_o), rel_err(Dg, Dg_o))
         say("precision %d: Jacobian errors %.1e %.1e" % (bits, errs[0], errs[1]))
-        bound = 1e-7 if bits == 32 else 1e-13
+        bound = 2e-6 if bits == 32 else 1e-13
         assert errs[0] < bound and errs[1] < bound, (bits, errs)
         # entries zero in every stage (structural zeros) are exactly zero
         zero = np.all(Df_o == 0, axis=0)
