@@ -3,13 +3,17 @@
 Default workload (--workload batch): the batched family, 1024 instances of
 the n_x=20, n_u=10 tanh OCP sharing A, B, C (seed 1), each with its own
 x0 ~ U[-1,1]^20, T ~ U[0.5, 2] and uniform mesh of Nint ~ U{256..2048}
-intervals, m=5 Lobatto IIIA stages, FP64.  One step = every instance solved
-once from the plain all-ones guess with the two-step homotopy in the tanh
-scale (0 -> 0.5): the instances are split into chunks that fit one handle,
-and each chunk runs as one batch (every launch covers all its instances).
-A solve counts only if it converged (residual < 1e-8, max segment
-residual <= 1).  Beside it, the plain single-instance solve looped over a
-sample of the same instances.
+intervals, m=5 Lobatto IIIA stages.  One step = every instance solved once,
+in chunks that fit one handle, each chunk one batch (every launch covers all
+its instances).  The default form (--form learned --precision 32): the LQ
+solve, the trained network's correction, Newton capped at 12 iterations with
+a progress test, then the continuation of the instances it leaves, with the
+Newton direction condensed in float32 (residuals, BABD solve and
+convergence test in float64).  --form plain: the all-ones guess with the
+two-step homotopy in the tanh scale (0 -> 0.5), float64.  A solve counts
+only if it converged (residual < 1e-8, max segment residual <= 1).  Beside
+it: the same learned pass in float64, one plain batched pass, and the plain
+single-instance solve looped over a sample of the same instances.
 
 --workload single: one instance, Nint=4096 (the single-instance
 configuration), one complete solve per step.
@@ -176,7 +180,7 @@ def main():
     ap.add_argument("--warm-ratio", type=float, default=2.0)
     ap.add_argument("--warm-cap", type=int, default=12,
                     help="batch, learned: Newton iterations from the network's guess before the fallback")
-    ap.add_argument("--warm-cont", default="5,5,2",
+    ap.add_argument("--warm-cont", default="8,5,2",
                     help="batch, learned: the instances the capped warm stage leaves continue Newton from there "
                          "for the rest of the iteration budget, with the stall test start,window,ratio (an "
                          "instance stops when its norm fell by less than ratio over the last window "

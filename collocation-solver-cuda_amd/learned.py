@@ -365,7 +365,7 @@ class WarmStart:
 
 
 def warm_solve(ws, solv, t_spans, feats, constants, lq_constants, tol=1e-8, max_iter=200, max_iter_warm=12,
-               alpha=0.1, patience=8, min_ratio=2.0, cont=(5, 5, 2.0)):
+               alpha=0.1, patience=8, min_ratio=2.0, cont=(8, 5, 2.0)):
     """Warm-started batched solve of the instances (meshes t_spans): for an
     "lq" network, one Newton solve of the LQ problems from the all-ones
     guess (converged after one iteration), then the network's guess and at
