@@ -667,6 +667,9 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
                 meta[2], meta[3], meta[4], h->ny, h->nz, h->np, h->m, h->nc);
   h->cw = meta[5];
   h->fw = meta[6];
+  // the static stage-Jacobian entries are the new module's constants: the
+  // buffer holds the previous module's until they are written again
+  h->jac_static_bits = 0;
   // the condensation record keeps the columns [dy_i | active dp | 1]
   const int npa = meta[7];
   if (npa < 0 || npa > h->np) return fail(CS_E_SHAPE, "module has %d active parameters of %d", npa, h->np);

@@ -530,9 +530,13 @@ def fit_family(n_train=50000, nint=256, epochs=200, batch=512, lr=1e-3, width=10
     path = path or FAMILY_MODEL
     os.makedirs(os.path.dirname(path), exist_ok=True)
     ws.save(path)
+    import hashlib
+    with open(path, "rb") as f:
+        digest = hashlib.sha256(f.read()).hexdigest()
     info = {"n_train": int(n_train), "converged": int(ok.sum()), "nint": nint, "epochs": epochs, "batch": batch,
             "width": width, "depth": depth, "base": base, "seconds_generate": t_gen, "seconds_train": t_train,
-            "final_mse": loss}
+            "final_mse": loss, "family_seed": 1, "members": [TRAIN_OFFSET, TRAIN_OFFSET + int(n_train)],
+            "training_seed": 0, "lr": lr, "sha256": digest}
     with open(path + ".json", "w") as f:
         import json
         json.dump(info, f, indent=1)

@@ -148,6 +148,14 @@ def config(index, nint=None, nx=None, nu=None):
     return inst, m
 
 
+def reseeded(inst, seed):
+    """The same family and sizes as inst with A, B, C, x0 drawn from another
+    seed: a different generated program of identical sizes."""
+    A, B, C, x0 = draw_family(inst.nx, inst.nu, seed, inst.C is not None)
+    return FamilyInstance(nx=inst.nx, nu=inst.nu, A=A, B=B, C=C, x0=x0, s=inst.s, q=inst.q, c_u=inst.c_u, T=inst.T,
+                          nodes=inst.nodes, tolerance=inst.tolerance, homotopy=inst.homotopy)
+
+
 def batch_family(count, seed=1, nx=20, nu=10, nint_range=(256, 2048)):
     """Batched family: shared A, B, C from `seed`; per instance i,
     default_rng([seed, i]) draws x0 ~ U[-1,1]^nx, T ~ U[0.5, 2] and
@@ -200,6 +208,7 @@ def standard_units():
     yield "lq4_m4", config(0)[0], 4
     small, _ = config(1, nx=4, nu=2, nint=32)
     yield "tanh4_m5", small, 5
+    yield "tanh4_seed2_m5", reseeded(small, 2), 5
     yield "tanh20_m5", config(1)[0], 5
     yield "param_osc_m4", TextProblem(PARAM_OSC_OCP), 4
     # batched families: x0 (and the tanh scale) are instance constants, so

@@ -110,19 +110,27 @@ class CollocationSolver:
         self.ny, self.nz, self.np = bvp.size_y, bvp.size_z, bvp.size_p
         self.n_max = int(n_max or max(bvp.maximum_nodes, bvp.N))
         self.batch = int(batch)
-        code = jit.code_object(bvp.ocp_text, m)
         L = lib()
         dims = CSDims(self.ny, self.nz, self.np, m, self.n_max, bvp.size_c, self.batch)
         h = native.c_void_p()
         check(L.cs_create(native.ctypes.byref(dims), native.ctypes.byref(h)))
         self.h = h
-        buf = native.ctypes.create_string_buffer(code, len(code))
-        check(L.cs_load_module(self.h, buf, len(code)))
-        if bvp.size_c:
-            self.set_constants(bvp.constants)
+        self.load_problem(bvp)
         self.N = 0
         self.B = 1
         self.nodes = np.zeros(1, dtype=np.int32)
+
+    def load_problem(self, bvp):
+        """Load the device code of a problem of this handle's sizes (the
+        generated callbacks and the kernels specialised to them); the mesh and
+        state are kept.  The role of importing a generated bvp_problem module
+        (solver:10-17)."""
+        code = jit.code_object(bvp.ocp_text, self.m)
+        buf = native.ctypes.create_string_buffer(code, len(code))
+        check(lib().cs_load_module(self.h, buf, len(code)))
+        self.bvp = bvp
+        if bvp.size_c:
+            self.set_constants(bvp.constants)
 
     def set_constants(self, values):
         """Per-instance constants of the problem (InstanceConstants order),

@@ -18,14 +18,21 @@ Pinning (what checks each part independently of the formulas restated here):
   ``tests/golden/reference_tables.json``, extracted from the reference's text).
 * ``families`` -- hand-derived closed forms of f, g, r and their Jacobians for
   the linear-quadratic and tanh OCP families, checked by central finite
-  differences in the tests.
-* ``abvp``    -- the generic OCP -> BVP-DAE restatement is checked against the
-  closed-form families.
+  differences and the Hamiltonian identities in the tests.
 * ``solver``/``babd`` -- the condensed BABD direction equals the solution of the
   fully assembled Newton matrix (np.linalg.solve) for small meshes; the Newton
   solve of the LQ problem equals the matrix-exponential closed form; the
   partition QR satisfies Q^T Q = I and the BABD residual invariant.
+* ``solver.line_search`` / ``newton_solve`` / ``collocation_solver`` (the
+  driver) -- hand-made merit sequences (strict decrease, 20 trials, first
+  decrease taken); a scalar BVP whose Newton iterates and step halvings follow
+  a scalar closed form; an ascent direction that fails all 20 trials and
+  leaves the state unchanged; the alpha continuation 0.1 * 0.9^k down to
+  1e-6 on a one-inequality OCP whose constrained optimum is known in closed
+  form; remeshing of a stiff LQ problem until the segment residual is <= 1.
 * ``solver.compute_segment_residual`` -- hand-computed tiny cases, including
   the reference's in-loop time-scaling quirk.
 * ``solver.remesh`` -- hand-built residual patterns.
+* ``mlp`` (the warm-start network, no reference counterpart) -- central finite
+  differences of the loss, GELU's closed forms, torch's bf16 rounding.
 """

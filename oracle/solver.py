@@ -151,18 +151,28 @@ def newton_solve(prob, m, rk, t_span, y, y_dot, z_tilde, p, alpha, tol, max_iter
             break
         dy, dyd, dz, dp = newton_direction(prob, m, N, rk, t_span, y, y_tilde, z_tilde, p, alpha,
                                            f_a, f_b, r_bc, M=M)
-        alpha0 = 1.0
-        for _ in range(MAX_LINESEARCH):
-            y_n = y + alpha0 * dy
-            yd_n = y_dot + alpha0 * dyd
-            z_n = z_tilde + alpha0 * dz
-            p_n = p + alpha0 * dp
-            f_q_new = compute_f_q(prob, m, N, rk, t_span, y_n, yd_n, z_n, p_n, alpha)[0]
-            if np.max(np.abs(f_q_new)) < norm_f_q:
-                y, y_dot, z_tilde, p = y_n, yd_n, z_n, p_n
-                break
-            alpha0 /= 2.0
+
+        def trial_norm(a0):
+            f_q_new = compute_f_q(prob, m, N, rk, t_span, y + a0 * dy, y_dot + a0 * dyd, z_tilde + a0 * dz,
+                                  p + a0 * dp, alpha)[0]
+            return np.max(np.abs(f_q_new))
+        a0 = line_search(trial_norm, norm_f_q)
+        if a0 is not None:
+            y, y_dot, z_tilde, p = y + a0 * dy, y_dot + a0 * dyd, z_tilde + a0 * dz, p + a0 * dp
     return y, y_dot, z_tilde, p, iter_time, norms
+
+
+def line_search(trial_norm, norm0, max_trials=MAX_LINESEARCH):
+    """The reference's step rule (solver:155-174): alpha0 = 1, 1/2, ...,
+    2^-(max_trials-1); the first step whose trial norm is strictly below
+    norm0 is taken.  Returns that step, or None when no trial decreased the
+    norm (the reference then leaves the iterate unchanged)."""
+    a0 = 1.0
+    for _ in range(max_trials):
+        if trial_norm(a0) < norm0:
+            return a0
+        a0 /= 2.0
+    return None
 
 
 def _gauss_point_values(prob, m, N, t_span, y, y_dot, z_tilde, i_gp, gc):

@@ -100,6 +100,31 @@ def test_stage_parity(small, seed):
     assert S.status() == 0
 
 
+def test_module_reload_refreshes_static_jacobian_entries(small):
+    """A second problem of the same sizes loaded into the same handle: its
+    compile-time constants (A, B, C of another seed) are static stage-Jacobian
+    entries, which must replace the first problem's."""
+    inst, m, bvp, S = small
+    other = problems.reseeded(inst, 2)
+    N = bvp.N
+    t = np.linspace(0.0, inst.T, N)
+    y, yd, zt, p = seeded_state(family_of(inst), m, N, 7)
+    S.set_mesh(t)
+    S.set_state(y, yd, zt, p)
+    try:
+        for cur in (inst, other, inst):
+            say("reload: seed-%s problem" % ("1" if cur is inst else "2"))
+            S.load_problem(solver.BvpDae(cur.ocp_text()))
+            S.set_state(y, yd, zt, p)
+            prob = family_of(cur)
+            _, yt, _, _, _ = S.compute_f_q(0.1)
+            Df, Dg, Dr = S.construct_jacobian(0.1)
+            assert rel_err(Df, prob.Df(yt, zt, p)) < 1e-13
+            assert rel_err(Dg, prob.Dg(yt, zt, p)) < 1e-13
+    finally:
+        S.load_problem(bvp)
+
+
 def test_initial_input_and_segment_residual(small):
     inst, m, bvp, S = small
     prob = family_of(inst)
