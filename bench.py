@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--plain-pass", type=int, default=1, help="batch, learned: also time one plain batched pass")
     ap.add_argument("--precision", type=int, choices=(32, 64), default=32,
                     help="batch: arithmetic of the Newton direction's condensation (32: the FP32 direction)")
+    ap.add_argument("--condense-kernel", type=int, choices=(0, 1), default=0,
+                    help="batch, precision 32: series condensation on matrix cores (0) or the vector ALU (1)")
     ap.add_argument("--fp64-pass", type=int, default=1,
                     help="batch, learned, precision 32: also time one learned pass with the float64 direction")
     ap.add_argument("--babd-spans", type=int, default=16,
@@ -223,6 +225,7 @@ def batch_workload(args):
     S = solver.CollocationSolver(bvp, m, n_max=max(sum(nodes[i] for i in ch) for ch in chunks),
                                  batch=max(len(ch) for ch in chunks))
     S.set_direction_precision(args.precision)
+    S.set_condense_kernel(args.condense_kernel)
     S.set_babd_substitution(args.babd_spans)
     ts_all = [np.linspace(0.0, inst.T, inst.nodes) for inst in insts]
     cons_all = np.array([inst.constant_values() for inst in insts])

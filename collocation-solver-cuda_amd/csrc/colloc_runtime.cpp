@@ -59,7 +59,7 @@ struct CRSeg {
 enum Fn {
   F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_JAC32, F_COND, F_COND32, F_COND_NS, F_COND_NS32, F_CR_PAIR, F_CR_SUBST,
   F_CR_FINAL, F_CR_BACK, F_RECOVER, F_RECOVER32, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_JAC_STATIC, F_JAC_STATIC32,
-  F_AXPY_LISTED, F_COUNT
+  F_AXPY_LISTED, F_COND_MF32, F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
   "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian", "colloc_jacobian32",
@@ -67,7 +67,7 @@ const char* kFnNames[F_COUNT] = {
   "colloc_cr_subst",
   "colloc_cr_final", "colloc_cr_back", "colloc_recover", "colloc_recover32", "colloc_axpy4", "colloc_init_input",
   "colloc_segres", "colloc_recover_z", "colloc_jacobian_static", "colloc_jacobian_static32",
-  "colloc_axpy_listed"};
+  "colloc_axpy_listed", "colloc_condense_mf32"};
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
@@ -133,6 +133,8 @@ struct cs_handle {
   int jac_bits = 64;                    // stored type of the last stage Jacobians (64 or 32)
   int jac_static_bits = 0;              // stored type whose static Jacobian entries fill the buffer (0: none)
   int rec_bits = 64;                    // stored type of the last condensation records (64 or 32)
+  int mf_ok = 0;                        // the module's series condensation on matrix cores fits its sizes
+  int cond_kind = 0;                    // see cs_set_condense_kernel
   std::vector<size_t> lev_obase;      // offset of the level in cr_oinst
   std::vector<size_t> lev_sbase;      // offset of the level in cr_seg
   std::vector<size_t> lev_rbase;      // row offset of the level's input rows in rows0 (one more: the last output)
@@ -323,7 +325,10 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
   // does the records' type
   const bool f32 = h->jac_bits == 32;
   h->rec_bits = h->jac_bits;
-  TRY(launch(h, f32 ? F_COND_NS32 : F_COND_NS, (unsigned)h->launch_intervals(), kNsThreads, args_ns));
+  if (f32 && h->mf_ok && h->cond_kind == 0)   // the series products on matrix cores, one wave per stage block
+    TRY(launch(h, F_COND_MF32, (unsigned)h->launch_intervals(), (unsigned)(h->m - 1) * 64, args_ns));
+  else
+    TRY(launch(h, f32 ? F_COND_NS32 : F_COND_NS, (unsigned)h->launch_intervals(), kNsThreads, args_ns));
   unsigned long long cnt[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -667,6 +672,10 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
                 meta[2], meta[3], meta[4], h->ny, h->nz, h->np, h->m, h->nc);
   h->cw = meta[5];
   h->fw = meta[6];
+  HIPCHK(hipModuleGetGlobal(&gp, &gsz, h->module, "colloc_caps"));
+  int caps[4] = {};
+  HIPCHK(hipMemcpy(caps, (void*)gp, sizeof caps, hipMemcpyDeviceToHost));
+  h->mf_ok = caps[0];
   // the static stage-Jacobian entries are the new module's constants: the
   // buffer holds the previous module's until they are written again
   h->jac_static_bits = 0;
@@ -1137,6 +1146,13 @@ int cs_set_direction_precision(cs_handle* h, int bits) {
   if (!h) return fail(CS_E_ARG, "null handle");
   if (bits != 32 && bits != 64) return fail(CS_E_ARG, "precision %d bits (32 or 64)", bits);
   h->dir_bits = bits;
+  return CS_OK;
+}
+
+int cs_set_condense_kernel(cs_handle* h, int kind) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (kind != 0 && kind != 1) return fail(CS_E_ARG, "condensation kernel %d (0 or 1)", kind);
+  h->cond_kind = kind;
   return CS_OK;
 }
 

@@ -700,6 +700,198 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
   }
 }
 
+// ------------------------------------- series condensation on matrix cores
+// The series of condense_ns with its products Y_k = M_{k+1} U_k on
+// v_mfma_f32_16x16x4f32 (float32, the FP32 direction).  One wave per stage
+// block k (NSK waves per interval); wave k holds, in registers,
+//   M_{k+1} as MFMA A operands  (MF_RT row tiles x MF_KS k-steps, 1 value each)
+//   R_k, U_k and Y_k in the accumulator layout of MF_RT x MF_CT 16x16 tiles
+//   (4 values per lane and tile).
+// Accumulator layout: lane l, value v of tile (ri, ci) is row
+// 16 ri + 4 v + l / 16, column 16 ci + l % 16 of the block.  With this row
+// numbering, value v of row tile ri is exactly the B operand of k-step
+// 4 ri + v (lane l: row 4 ks + l / 16, column l % 16), so U_k feeds the
+// next product from the registers it is formed in.  The A operand of row
+// tile ri holds, in lane l, row 16 ri + 4 (l % 4) + (l % 16) / 4 of M (the
+// output row the MFMA writes to value (l % 16) % 4 of lane group
+// (l % 16) / 4) and column 4 ks + l / 16.  Rows and columns past NY / NVA
+// are zero, so the padded entries of every tile stay zero.
+// A term: the products, Y_k through LDS to every wave, then
+//   U_k <- R_k + sum_q h A^[k][q] Y_q
+// and the workgroup max of the change (two barriers per term).
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+constexpr int MF_RT = (NY + 15) / 16;           // row tiles of a stage block
+constexpr int MF_CT = (NVA + 15) / 16;          // column tiles
+constexpr int MF_KS = (NY + 3) / 4;             // k-steps of a product
+constexpr int MF_T = MF_RT * MF_CT;
+constexpr int MF_THREADS = NSK * WAVE;
+constexpr int MF_LDS = (M * NY * NVA > NSK * MF_T * WAVE * 4) ? M * NY * NVA : NSK * MF_T * WAVE * 4;
+constexpr bool MF_OK = NSK >= 1 && NSK <= 16 && MF_T * 12 + MF_RT * MF_KS <= 176 && MF_LDS * 4 <= 64 * 1024;
+
+__device__ __forceinline__ void condense_mf32(int nint, Batch bt, const double* __restrict__ t,
+                                              const float* __restrict__ jac, const double* __restrict__ fa,
+                                              const double* __restrict__ fb, double* __restrict__ rows,
+                                              float* __restrict__ rec, int* __restrict__ status,
+                                              int* __restrict__ fallback, unsigned long long* __restrict__ counters) {
+  // sT (stage eliminations) and, once M and R are in registers, the
+  // exchange buffer of the products
+  __shared__ __attribute__((aligned(16))) float smem_mf[MF_LDS];
+  __shared__ float sred[2 * NSK];
+  __shared__ int sbad;
+  float* sT = smem_mf;
+  mf_f4* ybuf = reinterpret_cast<mf_f4*>(smem_mf);
+  const int i = listed_interval(bt, blockIdx.x);
+  const int tid = threadIdx.x;
+  const int lane = lane_id(), k = wave_id();
+  const int ib = bt.iinst[i];
+  if (skipped(bt, ib)) {
+    if (tid == 0) fallback[i] = 0;
+    return;
+  }
+  if (tid == 0) sbad = 0;
+  const float h = (float)(t[i + ib + 1] - t[i + ib]);
+  float* recU = rec + (size_t)i * REC;
+  float* recT = recU + NS * NVA;
+  float* recG = recT + M * NY * NVA;
+  const int bad = stage_eliminate<float, NSK, float, float>(i, jac, fa, sT, recT, recG);
+  __syncthreads();
+
+  const int js = k + 1;
+  const int g = lane >> 4, c16 = lane & 15;
+  // M_{k+1} as A operands
+  float am[MF_RT][MF_KS];
+  {
+    const int rl = 4 * (c16 & 3) + (c16 >> 2);
+#pragma unroll
+    for (int ri = 0; ri < MF_RT; ++ri)
+#pragma unroll
+      for (int ks = 0; ks < MF_KS; ++ks) {
+        const int row = 16 * ri + rl, col = 4 * ks + g;
+        am[ri][ks] = (row < NY && col < NY) ? sT[(js * NY + row) * NVA + col] : 0.0f;
+      }
+  }
+  float wk[NSK];   // h A^[k][q]
+#pragma unroll
+  for (int q = 0; q < NSK; ++q) wk[q] = h * (float)c_rkA[js * M + q + 1];
+  // R_k = h A[k+1][0] T_0 + (c < NY ? I : h sum_{j>=1} A[k+1][j] T_j); the first iterate
+  mf_f4 rv[MF_T], u[MF_T];
+#pragma unroll
+  for (int ri = 0; ri < MF_RT; ++ri)
+#pragma unroll
+    for (int ci = 0; ci < MF_CT; ++ci)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * ri + 4 * v + g, col = 16 * ci + c16;
+        float val = 0.0f;
+        if (row < NY && col < NVA) {
+          val = h * (float)c_rkA[js * M] * sT[row * NVA + col];
+          if (col < NY) {
+            if (col == row) val += 1.0f;
+          } else {
+#pragma unroll
+            for (int j = 1; j < M; ++j) val += h * (float)c_rkA[js * M + j] * sT[(j * NY + row) * NVA + col];
+          }
+        }
+        rv[ri * MF_CT + ci][v] = val;
+        u[ri * MF_CT + ci][v] = val;
+      }
+  __syncthreads();   // sT is overwritten by the exchange buffer from here on
+
+  bool converged = false;
+  int terms = 0;
+  for (int term = 0; term < NS_MAX_TERMS; ++term) {
+    ++terms;
+    mf_f4 y[MF_T];
+#pragma unroll
+    for (int tt = 0; tt < MF_T; ++tt) y[tt] = mf_f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < MF_KS; ++ks)
+#pragma unroll
+      for (int ri = 0; ri < MF_RT; ++ri)
+#pragma unroll
+        for (int ci = 0; ci < MF_CT; ++ci)
+          y[ri * MF_CT + ci] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[ri][ks], u[(ks >> 2) * MF_CT + ci][ks & 3],
+                                                                     y[ri * MF_CT + ci], 0, 0, 0);
+#pragma unroll
+    for (int tt = 0; tt < MF_T; ++tt) ybuf[(k * MF_T + tt) * WAVE + lane] = y[tt];
+    __syncthreads();
+    // U_k <- R_k + sum_q h A^[k][q] Y_q (padded entries stay zero)
+    float dmax = 0.0f, xmax = 0.0f;
+#pragma unroll
+    for (int tt = 0; tt < MF_T; ++tt) {
+      mf_f4 acc = rv[tt];
+#pragma unroll
+      for (int q = 0; q < NSK; ++q) {
+        const mf_f4 yq = ybuf[(q * MF_T + tt) * WAVE + lane];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] = fmaf(wk[q], yq[v], acc[v]);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        dmax = nmax(fabsf(acc[v] - u[tt][v]), dmax);
+        xmax = nmax(fabsf(acc[v]), xmax);
+      }
+      u[tt] = acc;
+    }
+    dmax = wave_max(dmax);
+    xmax = wave_max(xmax);
+    if (lane == 0) {
+      sred[k] = dmax;
+      sred[NSK + k] = xmax;
+    }
+    __syncthreads();   // also: every wave has read ybuf before the next term writes it
+    float dm = sred[0], xm = sred[NSK];
+#pragma unroll
+    for (int w = 1; w < NSK; ++w) {
+      dm = nmax(sred[w], dm);
+      xm = nmax(sred[NSK + w], xm);
+    }
+    if (dm <= ns_tol<float>() * xm) {
+      converged = true;
+      break;
+    }
+  }
+  if (tid == 0) {
+    fallback[i] = converged ? 0 : 1;
+    if (!converged) atomicAdd(counters, 1ull);
+    atomicAdd(counters + 1, (unsigned long long)terms);
+  }
+  if (bad) atomicOr(&sbad, bad);
+  __syncthreads();
+  if (tid == 0 && sbad) atomicOr(status, sbad);
+  if (!converged) return;
+  // outputs: U_k, the BABD row from the last stage block, C = I, zero H columns
+  double* rowi = rows + (size_t)i * NY * CRW;
+  const bool last = k == NSK - 1;
+#pragma unroll
+  for (int ri = 0; ri < MF_RT; ++ri)
+#pragma unroll
+    for (int ci = 0; ci < MF_CT; ++ci)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * ri + 4 * v + g, col = 16 * ci + c16;
+        if (row >= NY || col >= NVA) continue;
+        const float x = u[ri * MF_CT + ci][v];
+        recU[(k * NY + row) * NVA + col] = x;
+        if (last) {
+          const double dv = (double)x;
+          if (col < NY) rowi[row * CRW + col] = -dv;
+          else if (col < NY + NPA) rowi[row * CRW + 2 * NY + pact(col - NY)] = -dv;
+          else rowi[row * CRW + CRW - 1] = dv - fb[(size_t)i * NY + row];
+        }
+      }
+  for (int e = tid; e < NY * NY; e += MF_THREADS) {
+    const int a = e / NY, q = e - (e / NY) * NY;
+    rowi[a * CRW + NY + q] = (a == q) ? 1.0 : 0.0;
+  }
+  if (NPA < NP) {
+    for (int e = tid; e < NY * NP; e += MF_THREADS) {
+      const int a = e / NP, j = e - (e / NP) * NP;
+      if (!p_active(j)) rowi[a * CRW + 2 * NY + j] = 0.0;
+    }
+  }
+}
+
 // ------------------------------------------------------ cyclic reduction
 // One workgroup per pair of adjacent BABD rows (level l):
 //   row a: G_a x_L + C_a x_M + H_a p = b_a,   row b: G_b x_M + C_b x_R + H_b p = b_b.
@@ -1403,6 +1595,25 @@ __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nin
     if (!skip) atomicAdd(counters, 1ull);
   }
 }
+
+// The FP32-direction condensation with the series products on matrix cores:
+// one wave per stage block (the runtime launches it when colloc_caps[0] says
+// the problem's blocks fit, else colloc_condense_ns32).
+__global__ __launch_bounds__(colloc::MF_THREADS) void colloc_condense_mf32(int nint, colloc::Batch bt,
+    const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
+    int* fallback, unsigned long long* counters) {
+  if constexpr (colloc::MF_OK) {
+    colloc::condense_mf32(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
+  } else if (threadIdx.x == 0) {
+    const int i = colloc::listed_interval(bt, blockIdx.x);
+    const bool skip = bt.act && !bt.act[bt.iinst[i]];
+    fallback[i] = skip ? 0 : 1;
+    if (!skip) atomicAdd(counters, 1ull);
+  }
+}
+// capabilities of this problem's kernels: [0] colloc_condense_mf32 runs the
+// matrix-core series (else it leaves every interval to Gauss-Jordan)
+__device__ int colloc_caps[4] = {colloc::MF_OK ? 1 : 0, 0, 0, 0};
 
 // The FP32-direction condensation: two 512-thread workgroups per CU (half the
 // LDS and registers of the float64 kernel).

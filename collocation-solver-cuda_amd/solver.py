@@ -274,6 +274,11 @@ class CollocationSolver:
         convergence test stay float64)."""
         check(lib().cs_set_direction_precision(self.h, int(bits)))
 
+    def set_condense_kernel(self, kind):
+        """Series condensation kernel of the FP32 direction: 0 matrix cores
+        (default, where the stage blocks fit), 1 vector ALU."""
+        check(lib().cs_set_condense_kernel(self.h, int(kind)))
+
     def recover_delta_k(self):
         """recover_delta_k_parallel (solver:1580): delta_y_dot, delta_z_tilde."""
         nst = (self.N - self.B) * self.m

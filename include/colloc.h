@@ -164,6 +164,14 @@ int cs_solve_babd(cs_handle* h, double* delta_y, double* delta_p);
  * Jacobians widened to float64.  No counterpart in the reference. */
 int cs_set_direction_precision(cs_handle* h, int bits);
 
+/* Kernel of the series condensation in the FP32 direction: 0 (default) the
+ * products of the series on matrix cores (v_mfma_f32_16x16x4f32, one wave
+ * per stage block) where the problem's stage blocks fit (ny <= 48 and
+ * ny + active parameters + 1 <= 48, up to 16 stage blocks), 1 the vector-ALU
+ * series kernel.  Both solve the same stage systems of reduce_jacobian
+ * (solver:887-1001) to the same tolerance.  No counterpart in the reference. */
+int cs_set_condense_kernel(cs_handle* h, int kind);
+
 /* Pairing rule of the BABD solve's cyclic reduction.  Every block row that
  * the condensation produces has C_i = I; while a pair of rows spans at most
  * 1/min_spans of its instance's intervals, the middle node is eliminated by

@@ -40,13 +40,16 @@ def family_of(inst):
     return families.TanhFamily(inst.A, inst.B, inst.x0, C=inst.C, s=inst.s, q=inst.q, c_u=inst.c_u)
 
 
-def test_fp32_direction_stages_against_oracle():
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_fp32_direction_stages_against_oracle(kernel):
+    """kernel 0: the series products on matrix cores; 1: the vector-ALU series."""
     inst, m = problems.config(1, nint=512)
     bvp = solver.BvpDae(inst.ocp_text())
     prob = family_of(inst)
     N = bvp.N
     S = solver.CollocationSolver(bvp, m, n_max=N)
     S.set_direction_precision(32)
+    S.set_condense_kernel(kernel)
     t = np.linspace(0.0, inst.T, N)
     rng = np.random.default_rng(21)
     y = rng.normal(scale=0.3, size=(N, prob.size_y))
@@ -55,7 +58,7 @@ def test_fp32_direction_stages_against_oracle():
     p = rng.normal(scale=0.3, size=prob.size_p)
     S.set_mesh(t)
     S.set_state(y, yd, zt, p)
-    say("fp32 direction stages N=%d" % N)
+    say("fp32 direction stages N=%d condensation kernel %d" % (N, kernel))
     S.compute_f_q(0.1, outputs=False)
     S.construct_jacobian(0.1, outputs=False)
     A, C, H, b = S.reduce_jacobian()
@@ -80,6 +83,52 @@ def test_fp32_direction_stages_against_oracle():
     assert errs["A"] < 1e-5 and errs["b"] < 1e-5
     for k in ("dy", "dp", "dyd", "dz"):
         assert errs[k] < 1e-4, k
+
+
+@pytest.mark.parametrize("label", ["tanh4_m5", "param_osc_m4", "tanh4_batch_m5"])
+def test_matrix_core_condensation_small_blocks(label):
+    """The matrix-core series on stage blocks smaller than one 16 x 16 tile
+    (ny = 8, 4; an active parameter column; three stage blocks with m = 4),
+    against the float64 oracle's condensed rows (float32 arithmetic: 1e-5
+    relative to the largest entry) and against the vector-ALU series kernel
+    (the same float32 system, solved to the same series tolerance: 1e-5)."""
+    from symbolic_problem import SymbolicProblem
+    inst, m = [(i, mm) for lab, i, mm in problems.standard_units() if lab == label][0]
+    text = inst.ocp_text() if hasattr(inst, "ocp_text") else inst.text
+    bvp = solver.BvpDae(text)
+    prob = SymbolicProblem(text)
+    N = 65
+    S = solver.CollocationSolver(bvp, m, n_max=N)
+    S.set_direction_precision(32)
+    rng = np.random.default_rng(3)
+    t = np.sort(np.concatenate([[0.0, 1.0], rng.uniform(0.0, 1.0, N - 2)]))
+    y = rng.normal(scale=0.3, size=(N, prob.size_y))
+    yd = rng.normal(scale=0.3, size=((N - 1) * m, prob.size_y))
+    zt = rng.normal(scale=0.3, size=((N - 1) * m, prob.size_z))
+    p = rng.normal(scale=0.3, size=prob.size_p)
+    S.set_mesh(t)
+    S.set_state(y, yd, zt, p)
+    out = {}
+    for kernel in (0, 1):
+        say("%s: condensation kernel %d" % (label, kernel))
+        S.set_condense_kernel(kernel)
+        S.compute_f_q(0.1, outputs=False)
+        S.construct_jacobian(0.1, outputs=False)
+        out[kernel] = S.reduce_jacobian()
+        assert S.condense_fallbacks() == 0 and S.status() == 0
+    S.close()
+    # the handle's instance constants are the text's values, which the
+    # oracle's problem substitutes as well
+    rk = otables.lobatto(m)
+    f_q, yt_o, fa_o, fb_o, rbc_o = osolver.compute_f_q(prob, m, N, rk, t, y, yd, zt, p, 0.1)
+    J, V, D, W, B0, Bn, Vn = osolver.construct_jacobian(prob, m, N, rk, t, y, yt_o, zt, p, 0.1)
+    A_o, C_o, H_o, b_o = osolver.reduce_jacobian(prob, m, N, W, D, J, V, fa_o, fb_o)
+    for kernel, (A, C, H, b) in out.items():
+        errs = (rel_err(A, A_o), rel_err(H, H_o) if np.any(H_o) else np.max(np.abs(H)), rel_err(b, b_o))
+        say("kernel %d: A, H, b errors vs the float64 oracle %.1e %.1e %.1e" % ((kernel,) + errs))
+        assert max(errs) < 1e-5, (kernel, errs)
+    for k in (0, 2, 3):
+        assert rel_err(out[0][k], out[1][k]) < 1e-5
 
 
 @pytest.mark.parametrize("nint", [200, 1024])
