@@ -34,7 +34,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "collocation-solver-cuda_amd")
 FP64_PEAK_TFLOPS = 78.6     # MI355X, FP64 vector = FP64 matrix peak
-FP32_PEAK_TFLOPS = 157.3    # MI355X, FP32 vector peak
+FP32_PEAK_TFLOPS = 157.3    # MI355X, FP32 peak (matrix cores; vector with packed FP32)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -81,17 +81,32 @@ def dense_condense_flops(ny, nz, np_, m):
     return (2.0 / 3.0) * K ** 3 + 2 * K * K * (ny + np_ + 1) + 2 * m * ny * (ny + np_ + 1)
 
 
-def measured_traffic(workload):
-    """HBM bytes per interval condensed by colloc_condense_ns at this workload,
-    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated
-    against a copy kernel); None when no measurement exists for it."""
+def measured_traffic(workload, kernel):
+    """HBM bytes per interval condensed by `kernel` at this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a
+    copy kernel); None when no measurement exists for it."""
     path = os.path.join(ROOT, "profiles", "traffic_%s_m5.md" % workload)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_interval colloc_condense_ns:"):
+        if line.startswith("traffic_bytes_per_interval %s:" % kernel):
             return float(line.split(":")[1])
     return None
+
+
+# condensation kernel -> (bound, peak TFLOP/s).  The FP32 vector peak without
+# packed FP32 math (disabled in this build) is half the published 157.3.
+CONDENSE_KERNELS = {
+    "colloc_condense_mf32": ("mfma", FP32_PEAK_TFLOPS),
+    "colloc_condense_ns32": ("valu", FP32_PEAK_TFLOPS / 2),
+    "colloc_condense_ns": ("valu", FP64_PEAK_TFLOPS),
+}
+
+
+def condense_kernel_name(precision, kind):
+    if precision == 64:
+        return "colloc_condense_ns"
+    return "colloc_condense_mf32" if kind == 0 else "colloc_condense_ns32"
 
 
 def cpu_baseline(inst, m, nint_sample, nint_target):
@@ -121,10 +136,61 @@ def cpu_baseline(inst, m, nint_sample, nint_target):
             "seconds": dt, "converged": ok}
 
 
-def condense_roofline(S, before, after, cond_ms, cond_n, ny, nz, np_, npa, m, traffic, peak=None):
-    """Roofline entry of the condensation from the executed-flop counters
-    (peak: the arithmetic's peak, FP64 78.6 or FP32 157.3 TFLOP/s)."""
-    peak = peak or FP64_PEAK_TFLOPS
+def cpu_protocol(runs=5):
+    """The oracle timed as the measurement plan asks: the configuration-0
+    solve (LQ, n_x=4, Nint=128, m=4) as the median of `runs` wall-clock runs
+    with BLAS on all the process's cores and on one thread, and one Newton
+    iteration (residual, direction, full-step trial) of the single-instance
+    configuration (n_x=20, m=5) on all cores, timed on a 256-interval mesh
+    and scaled to its 4096 intervals (every step of an iteration is linear
+    in the interval count)."""
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    from threadpoolctl import threadpool_limits
+    from colloc_amd import problems
+    from oracle import families
+    from oracle import solver as osolver
+    from oracle import tables as otables
+    cores = len(os.sched_getaffinity(0))
+    inst, m = problems.config(0)
+    prob = families.TanhFamily(inst.A, inst.B, inst.x0, C=None, q=inst.q, c_u=inst.c_u)
+    N = inst.nodes
+    t = np.linspace(0.0, inst.T, N)
+
+    def cfg0():
+        t0 = time.perf_counter()
+        r = osolver.collocation_solver(prob, m, t, np.ones((N, prob.size_y)), np.ones((N, prob.size_z)),
+                                       np.ones(prob.size_p), tol=inst.tolerance)
+        assert r["success"] == 1
+        return time.perf_counter() - t0
+    out = {}
+    for label, nthreads in (("all_cores", cores), ("1_thread", 1)):
+        with threadpool_limits(nthreads):
+            out["cfg0_solve_s_median_%s" % label] = float(np.median([cfg0() for _ in range(runs)]))
+    sample_nint = 256
+    inst1, m1 = problems.config(1, nint=sample_nint)
+    prob1 = families.TanhFamily(inst1.A, inst1.B, inst1.x0, C=inst1.C, s=inst1.s, q=inst1.q, c_u=inst1.c_u)
+    N1 = inst1.nodes
+    t1 = np.linspace(0.0, inst1.T, N1)
+    rk = otables.lobatto(m1)
+    y, yd, zt = osolver.form_initial_input(prob1, m1, N1, np.ones((N1, prob1.size_y)), np.ones((N1, prob1.size_z)),
+                                           np.ones(prob1.size_p), rk)
+    p = np.ones(prob1.size_p)
+    t0 = time.perf_counter()
+    f_q, yt, fa, fb, rbc = osolver.compute_f_q(prob1, m1, N1, rk, t1, y, yd, zt, p, 0.1)
+    dy, dyd, dz, dp = osolver.newton_direction(prob1, m1, N1, rk, t1, y, yt, zt, p, 0.1, fa, fb, rbc)
+    osolver.compute_f_q(prob1, m1, N1, rk, t1, y + dy, yd + dyd, zt + dz, p + dp, 0.1)
+    out["cfg1_newton_iteration_s_all_cores"] = (time.perf_counter() - t0) * 4096 / sample_nint
+    out["cfg1_sample_nint"] = sample_nint
+    out["cores"] = cores
+    return out
+
+
+def condense_roofline(S, before, after, cond_ms, cond_n, ny, nz, np_, npa, m, traffic, kernel):
+    """Roofline entry of the condensation from the algorithmic-flop counters
+    (stage eliminations, series terms, Gauss-Jordan fallbacks) against the
+    peak of the unit the kernel's products run on."""
+    bound, peak = CONDENSE_KERNELS[kernel]
     fb = after[0] - before[0]
     terms = after[1] - before[1]
     ncond = after[2] - before[2]
@@ -134,11 +200,11 @@ def condense_roofline(S, before, after, cond_ms, cond_n, ny, nz, np_, npa, m, tr
     avg_s = cond_ms / max(cond_n, 1) / 1e3
     achieved = fl / avg_s / 1e12 if avg_s > 0 else 0.0
     dense = dense_condense_flops(ny, nz, np_, m) * ncond / (cond_ms / 1e3) / 1e12 if cond_ms > 0 else 0.0
-    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+    return {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak,
             "traffic": traffic * ncond / max(cond_n, 1) if traffic else None,
             "traffic_bytes_per_interval": traffic,
-            "kernel": "colloc_condense_ns (+ colloc_condense for intervals the series leaves)",
+            "kernel": "%s (+ colloc_condense for intervals the series leaves)" % kernel,
             "flops_per_launch": fl, "avg_launch_ms": avg_s * 1e3, "launches": cond_n,
             "intervals_per_launch": ncond / max(cond_n, 1),
             "series_terms_per_interval": terms / max(ncond, 1), "gauss_jordan_intervals": fb,
@@ -187,7 +253,7 @@ def main():
                          "for the rest of the iteration budget, with the stall test start,window,ratio (an "
                          "instance stops when its norm fell by less than ratio over the last window "
                          "iterations); 0: no continuation")
-    ap.add_argument("--fallback", choices=("none", "plain"), default="none",
+    ap.add_argument("--fallback", choices=("none", "plain"), default="plain",
                     help="batch, learned: what the instances the warm-started Newton leaves get: nothing, or the "
                          "plain path again (all-ones guess + continuation)")
     ap.add_argument("--fallback-scales", default="0",
@@ -284,9 +350,11 @@ def batch_workload(args):
             its[ch] = r["iterations"].sum(axis=0) + r["lq_iterations"]
             newton_ok[ch] = r["norm"] < tol
             phase_s["warm_converged"] += int(r["warm_converged"].sum())
+            phase_s["learned_converged"] += int(r["converged"].sum())
         back = [i for i in range(len(insts)) if not newton_ok[i]]
         t_back = time.perf_counter()
         phase_s["warm"] += t_back - t_warm
+        phase_s["fallback_instances"] += len(back)
         if back and args.fallback == "plain":
             c2, _ = plain_batch(back, fb_hom, fb_stall)
             conv[back] = c2
@@ -299,10 +367,12 @@ def batch_workload(args):
         return conv, its, 0
 
     one_pass = learned_pass if args.form == "learned" else plain_pass
-    phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0, "warm_converged": 0}
+    phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0, "warm_converged": 0, "learned_converged": 0,
+               "fallback_instances": 0}
     for _ in range(args.warmup):
         one_pass()
-    phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0, "warm_converged": 0}
+    phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0, "warm_converged": 0, "learned_converged": 0,
+               "fallback_instances": 0}
     S.profile(True)
     st0 = S.condense_stats()
     t0 = time.perf_counter()
@@ -316,9 +386,9 @@ def batch_workload(args):
     value = converged / elapsed
     conv, its, n_back = results[-1]
     npa = len(jit.unit_info(bvp.ocp_text, m)["active_params"])
+    kname = condense_kernel_name(args.precision, args.condense_kernel)
     roof = condense_roofline(S, st0, st1, prof["condense"][0], prof["condense"][1], ny, nz, np_, npa, m,
-                             measured_traffic("batch_%s" % args.form),
-                             peak=FP32_PEAK_TFLOPS if args.precision == 32 else FP64_PEAK_TFLOPS)
+                             measured_traffic("batch_%s" % args.form, kname), kname)
     roof["arithmetic"] = "fp%d" % args.precision
     extra = {}
     if args.form == "learned" and args.precision == 32 and args.fp64_pass:
@@ -410,6 +480,10 @@ def batch_workload(args):
                    "seconds_per_pass": {"warm_start_solves": phases["warm"], "fallback": phases["fallback"]}
                    if args.form == "learned" else None,
                    "fallback_converged_per_pass": phases["fallback_converged"] if args.form == "learned" else None,
+                   "fallback_rate": phases["fallback_instances"] / len(insts) if args.form == "learned" else None,
+                   "no_fallback_solves_per_s": (phases["learned_converged"] / phases["warm"])
+                   if args.form == "learned" and phases["warm"] > 0 else None,
+                   "no_fallback_converged_per_pass": phases["learned_converged"] if args.form == "learned" else None,
                    "looped_plain_solves_per_s": loop_rate, "looped_sample": len(sample),
                    "looped_sample_converged": loop_ok,
                    "vs_looped_plain": value / loop_rate if loop_rate > 0 else None,
@@ -420,6 +494,7 @@ def batch_workload(args):
     }
     if args.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(insts[0], m, args.cpu_sample, mean_nint)
+        line["cpu_baseline"]["protocol"] = cpu_protocol()
     return line
 
 
@@ -465,7 +540,7 @@ def single_workload(args):
     ny, nz, np_ = bvp.size_y, bvp.size_z, bvp.size_p
     npa = len(jit.unit_info(bvp.ocp_text, m)["active_params"])
     roof = condense_roofline(S, st0, st1, prof["condense"][0], prof["condense"][1], ny, nz, np_, npa, m,
-                             measured_traffic("tanh20_nint%d" % nint))
+                             measured_traffic("tanh20_nint%d" % nint, "colloc_condense_ns"), "colloc_condense_ns")
     print("per-solve Newton iterations %s, final residual %.2e, max segment residual %.3g"
           % (results[-1][1], results[-1][2], results[-1][3]), file=sys.stderr)
     print("kernel totals over %d solves (ms, launches): %s" % (steps, prof), file=sys.stderr)

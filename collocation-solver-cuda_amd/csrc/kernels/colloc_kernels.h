@@ -365,6 +365,345 @@ __device__ __forceinline__ int stage_eliminate(int i, const TJ* __restrict__ jac
   return bad;
 }
 
+// Stage eliminations with every global load issued up front: rows [r0, r1)
+// of stage j of interval i (the whole stage when r0 = 0, r1 = NY), computing
+// in R.  Each lane slice holds one column of [g_z | g_y g_p f_a,g] / of
+// [h_z | h_y h_p f_a,h] (same column map for the g and h rows), loaded from
+// a clamped, always valid address so that no load sits behind a branch;
+// h_z of the rows and f_a of the stage go to the wave's LDS scratch `scr`
+// (SE_SCR values) and are read back as broadcasts.  RL: a compile-time bound
+// on r1 - r0.  G_j is written when write_g.  Returns nonzero on a zero pivot.
+constexpr int SE_SCR = NY * NZ1 + NYZ;
+template <class R, int RL, class TJ, class TR>
+__device__ __forceinline__ int stage_rows(int i, int j, int r0, int r1, bool write_g, const TJ* __restrict__ jac,
+                                          const double* __restrict__ fa, R* sT, TR* __restrict__ recT,
+                                          TR* __restrict__ recG, R* scr) {
+  constexpr int NCZ = NZ + NVA;
+  constexpr int SL = (NCZ + WAVE - 1) / WAVE;
+  const int lane = lane_id();
+  const TJ* Dfj = jac + ((size_t)i * M + j) * JSTAGE;
+  const TJ* Dgj = Dfj + NY * NJC;
+  const double* faj = fa + (size_t)i * M * NYZ + j * NYZ;
+  int src[SL];
+  bool rhs[SL];
+#pragma unroll
+  for (int sl = 0; sl < SL; ++sl) {
+    const int cc = lane + WAVE * sl, q = cc - NZ;
+    src[sl] = cc < NZ ? NY + cc : (q < NY ? q : (q < NY + NPA ? NY + NZ + pact(q - NY < NPA ? q - NY : 0) : 0));
+    rhs[sl] = q == NY + NPA;
+  }
+  // every load first: g rows, this range's h rows, h_z, f_a
+  R gc[SL][NZ1];
+  R hc[RL][SL];
+#pragma unroll
+  for (int r = 0; r < NZ; ++r)
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) gc[sl][r] = (R)Dgj[r * NJC + src[sl]];
+#pragma unroll
+  for (int rr = 0; rr < RL; ++rr) {
+    const int r = r0 + rr < NY ? r0 + rr : NY - 1;
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) hc[rr][sl] = (R)Dfj[r * NJC + src[sl]];
+  }
+  constexpr int NHZ = RL * NZ;
+  R hz[(NHZ + WAVE - 1) / WAVE];
+  R fv[(NYZ + WAVE - 1) / WAVE];
+#pragma unroll
+  for (int k = 0; k < (NHZ + WAVE - 1) / WAVE; ++k) {
+    const int e = lane + WAVE * k;
+    const int r = r0 + e / NZ1 < NY ? r0 + e / NZ1 : NY - 1;
+    hz[k] = (R)Dfj[r * NJC + NY + (e % NZ1 < NZ ? e % NZ1 : 0)];
+  }
+#pragma unroll
+  for (int k = 0; k < (NYZ + WAVE - 1) / WAVE; ++k) {
+    const int e = lane + WAVE * k;
+    fv[k] = (R)faj[e < NYZ ? e : 0];
+  }
+#pragma unroll
+  for (int k = 0; k < (NHZ + WAVE - 1) / WAVE; ++k)
+    if (lane + WAVE * k < NHZ) scr[lane + WAVE * k] = hz[k];
+#pragma unroll
+  for (int k = 0; k < (NYZ + WAVE - 1) / WAVE; ++k)
+    if (lane + WAVE * k < NYZ) scr[NY * NZ1 + lane + WAVE * k] = fv[k];
+  // the rhs column of the g rows from f_a,g
+#pragma unroll
+  for (int sl = 0; sl < SL; ++sl)
+#pragma unroll
+    for (int r = 0; r < NZ; ++r)
+      if (rhs[sl]) gc[sl][r] = readlane_r(fv[(NY + r) / WAVE], (NY + r) % WAVE);
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");   // the scratch writes above happen before the reads below (one wave: LDS in order)
+  // Gauss-Jordan on [g_z | ...] with partial pivoting (lanes = columns)
+  int bad = 0;
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    R ck[NZ1];
+#pragma unroll
+    for (int r = 0; r < NZ; ++r) ck[r] = readlane_r(gc[0][r], k);
+    int pr = k;
+    R best = fabs(ck[k]);
+#pragma unroll
+    for (int r = k + 1; r < NZ; ++r)
+      if (fabs(ck[r]) > best) { best = fabs(ck[r]); pr = r; }
+#pragma unroll
+    for (int r = k + 1; r < NZ; ++r) {
+      const bool sw = r == pr;
+      const R t0 = ck[r];
+      ck[r] = sw ? ck[k] : ck[r];
+      ck[k] = sw ? t0 : ck[k];
+#pragma unroll
+      for (int sl = 0; sl < SL; ++sl) {
+        const R t1 = gc[sl][r];
+        gc[sl][r] = sw ? gc[sl][k] : gc[sl][r];
+        gc[sl][k] = sw ? t1 : gc[sl][k];
+      }
+    }
+    const R cpr = ck[k];
+    if (!(fabs(cpr) > (R)0)) bad |= 1;
+    const R inv = (R)1 / cpr;
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) {
+      const R pk = gc[sl][k] * inv;
+#pragma unroll
+      for (int r = 0; r < NZ; ++r) gc[sl][r] = (r == k) ? pk : fma(-ck[r], pk, gc[sl][r]);
+    }
+  }
+  if (write_g) {
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) {
+      const int q = lane + WAVE * sl - NZ;
+      if (q < 0 || q >= NVA) continue;
+#pragma unroll
+      for (int r = 0; r < NZ; ++r) recG[(j * NZ + r) * NVA + q] = (TR)gc[sl][r];
+    }
+  }
+  // T_j rows: [h_y | h_p | f_a,h] - h_z G_j
+#pragma unroll
+  for (int rr = 0; rr < RL; ++rr) {
+    const int r = r0 + rr;
+    if (r >= r1) break;
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) {
+      const int q = lane + WAVE * sl - NZ;
+      R v = rhs[sl] ? scr[NY * NZ1 + r] : hc[rr][sl];
+#pragma unroll
+      for (int z = 0; z < NZ; ++z) v = fma(-scr[rr * NZ + z], gc[sl][z], v);
+      if (q >= 0 && q < NVA) {
+        sT[(j * NY + r) * NVA + q] = v;
+        recT[(j * NY + r) * NVA + q] = (TR)v;
+      }
+    }
+  }
+  return bad;
+}
+
+// The stage eliminations of one interval over NW waves: with NW >= M, wave j
+// does stage j; otherwise every wave does whole stages w, w + NW, ... below
+// (M / NW) NW, and the remaining stages are split by rows over all waves
+// (each wave repeats that stage's small g_z elimination).  scr: NW * SE_SCR
+// values of LDS.
+template <class R, int NW, class TJ, class TR>
+__device__ __forceinline__ int stage_eliminate_all(int i, const TJ* __restrict__ jac, const double* __restrict__ fa,
+                                                   R* sT, TR* __restrict__ recT, TR* __restrict__ recG, R* scr) {
+  const int wave = wave_id();
+  R* my = scr + wave * SE_SCR;
+  int bad = 0;
+  if constexpr (NW >= M) {
+    if (wave < M) bad |= stage_rows<R, NY, TJ, TR>(i, wave, 0, NY, true, jac, fa, sT, recT, recG, my);
+  } else {
+    constexpr int FULL = (M / NW) * NW;
+    constexpr int RS = (NY + NW - 1) / NW;
+    for (int j = wave; j < FULL; j += NW)
+      bad |= stage_rows<R, NY, TJ, TR>(i, j, 0, NY, true, jac, fa, sT, recT, recG, my);
+    for (int j = FULL; j < M; ++j) {
+      const int r0 = wave * RS < NY ? wave * RS : NY;
+      const int r1 = r0 + RS < NY ? r0 + RS : NY;
+      __builtin_amdgcn_wave_barrier();
+      asm volatile("" ::: "memory");   // the previous stage's scratch reads are done
+      bad |= stage_rows<R, RS, TJ, TR>(i, j, r0, r1, wave == 0, jac, fa, sT, recT, recG, my);
+    }
+  }
+  return bad;
+}
+
+// stage_rows for float32 with T_j = [h_y | h_p | f_a,h] - h_z G_j on matrix
+// cores: h_z rows as A operands (row 16 ri + l % 16, column 4 ks + l / 16),
+// G_j as B operands from the wave's LDS copy, and [h_y | h_p | f_a,h] loaded
+// straight into the accumulator layout (lane l, value v of tile (ri, ci):
+// row r0 + 16 ri + 4 (l / 16) + v, column 16 ci + l % 16).  The g_z
+// elimination is stage_rows'.  scr: SE_SCR_MF floats of LDS.
+constexpr int SE_KZ = (NZ + 3) / 4;                 // k-steps over the algebraic unknowns
+constexpr int SE_GW = ((NVA + 15) / 16) * 16;       // padded columns of the LDS copy of G_j
+constexpr int SE_SCR_MF = NYZ + SE_KZ * 4 * SE_GW;
+template <int RL, class TR>
+__device__ __forceinline__ int stage_rows_mf(int i, int j, int r0, int r1, bool write_g, const float* __restrict__ jac,
+                                             const double* __restrict__ fa, float* sT, TR* __restrict__ recT,
+                                             TR* __restrict__ recG, float* scr) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  constexpr int NCZ = NZ + NVA;
+  constexpr int SL = (NCZ + WAVE - 1) / WAVE;
+  constexpr int RTL = (RL + 15) / 16;
+  constexpr int CT = SE_GW / 16;
+  const int lane = lane_id(), g = lane >> 4, c16 = lane & 15;
+  const float* Dfj = jac + ((size_t)i * M + j) * JSTAGE;
+  const float* Dgj = Dfj + NY * NJC;
+  const double* faj = fa + (size_t)i * M * NYZ + j * NYZ;
+  float* sfa = scr;             // f_a of the stage
+  float* sG = scr + NYZ;        // G_j, SE_KZ * 4 rows of SE_GW
+  int src[SL];
+  bool rhs[SL];
+#pragma unroll
+  for (int sl = 0; sl < SL; ++sl) {
+    const int cc = lane + WAVE * sl, q = cc - NZ;
+    src[sl] = cc < NZ ? NY + cc : (q < NY ? q : (q < NY + NPA ? NY + NZ + pact(q - NY < NPA ? q - NY : 0) : 0));
+    rhs[sl] = q == NY + NPA;
+  }
+  // every load first: g rows, h_z as A operands, [h_y | h_p] in the accumulator layout, f_a
+  float gc[SL][NZ1];
+#pragma unroll
+  for (int r = 0; r < NZ; ++r)
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) gc[sl][r] = Dgj[r * NJC + src[sl]];
+  float az[RTL][SE_KZ];
+#pragma unroll
+  for (int ri = 0; ri < RTL; ++ri)
+#pragma unroll
+    for (int ks = 0; ks < SE_KZ; ++ks) {
+      const int row = r0 + 16 * ri + c16, z = 4 * ks + g;
+      const float v = Dfj[(row < NY ? row : NY - 1) * NJC + NY + (z < NZ ? z : 0)];
+      az[ri][ks] = (row < r1 && z < NZ) ? v : 0.0f;
+    }
+  f4 hv[RTL][CT];
+#pragma unroll
+  for (int ri = 0; ri < RTL; ++ri)
+#pragma unroll
+    for (int ci = 0; ci < CT; ++ci) {
+      const int q = 16 * ci + c16;
+      const int sq = q < NY ? q : (q < NY + NPA ? NY + NZ + pact(q - NY < NPA ? q - NY : 0) : 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + 16 * ri + 4 * g + v;
+        hv[ri][ci][v] = Dfj[(row < NY ? row : NY - 1) * NJC + sq];
+      }
+    }
+  float fv[(NYZ + WAVE - 1) / WAVE];
+#pragma unroll
+  for (int k = 0; k < (NYZ + WAVE - 1) / WAVE; ++k) {
+    const int e = lane + WAVE * k;
+    fv[k] = (float)faj[e < NYZ ? e : 0];
+  }
+#pragma unroll
+  for (int k = 0; k < (NYZ + WAVE - 1) / WAVE; ++k)
+    if (lane + WAVE * k < NYZ) sfa[lane + WAVE * k] = fv[k];
+#pragma unroll
+  for (int sl = 0; sl < SL; ++sl)
+#pragma unroll
+    for (int r = 0; r < NZ; ++r)
+      if (rhs[sl]) gc[sl][r] = readlane_r(fv[(NY + r) / WAVE], (NY + r) % WAVE);
+  // Gauss-Jordan on [g_z | ...] with partial pivoting (lanes = columns)
+  int bad = 0;
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    float ck[NZ1];
+#pragma unroll
+    for (int r = 0; r < NZ; ++r) ck[r] = readlane_r(gc[0][r], k);
+    int pr = k;
+    float best = fabsf(ck[k]);
+#pragma unroll
+    for (int r = k + 1; r < NZ; ++r)
+      if (fabsf(ck[r]) > best) { best = fabsf(ck[r]); pr = r; }
+#pragma unroll
+    for (int r = k + 1; r < NZ; ++r) {
+      const bool sw = r == pr;
+      const float t0 = ck[r];
+      ck[r] = sw ? ck[k] : ck[r];
+      ck[k] = sw ? t0 : ck[k];
+#pragma unroll
+      for (int sl = 0; sl < SL; ++sl) {
+        const float t1 = gc[sl][r];
+        gc[sl][r] = sw ? gc[sl][k] : gc[sl][r];
+        gc[sl][k] = sw ? t1 : gc[sl][k];
+      }
+    }
+    const float cpr = ck[k];
+    if (!(fabsf(cpr) > 0.0f)) bad |= 1;
+    const float inv = 1.0f / cpr;
+#pragma unroll
+    for (int sl = 0; sl < SL; ++sl) {
+      const float pk = gc[sl][k] * inv;
+#pragma unroll
+      for (int r = 0; r < NZ; ++r) gc[sl][r] = (r == k) ? pk : fmaf(-ck[r], pk, gc[sl][r]);
+    }
+  }
+  // G_j to the records and to the LDS copy (zero padding rows and columns)
+  for (int e = lane; e < SE_KZ * 4 * SE_GW; e += WAVE) sG[e] = 0.0f;
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+#pragma unroll
+  for (int sl = 0; sl < SL; ++sl) {
+    const int q = lane + WAVE * sl - NZ;
+    if (q < 0 || q >= NVA) continue;
+#pragma unroll
+    for (int r = 0; r < NZ; ++r) {
+      sG[r * SE_GW + q] = gc[sl][r];
+      if (write_g) recG[(j * NZ + r) * NVA + q] = (TR)gc[sl][r];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");   // one wave: the LDS writes above precede the reads below
+  float bg[SE_KZ][CT];
+#pragma unroll
+  for (int ks = 0; ks < SE_KZ; ++ks)
+#pragma unroll
+    for (int ci = 0; ci < CT; ++ci) bg[ks][ci] = sG[(4 * ks + g) * SE_GW + 16 * ci + c16];
+  // T = H - h_z G, tile by tile
+#pragma unroll
+  for (int ri = 0; ri < RTL; ++ri)
+#pragma unroll
+    for (int ci = 0; ci < CT; ++ci) {
+      f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < SE_KZ; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(az[ri][ks], bg[ks][ci], acc, 0, 0, 0);
+      const int q = 16 * ci + c16;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + 16 * ri + 4 * g + v;
+        if (row >= r1 || q >= NVA) continue;
+        const float h = (q == NY + NPA) ? sfa[row] : hv[ri][ci][v];
+        const float tv = h - acc[v];
+        sT[(j * NY + row) * NVA + q] = tv;
+        recT[(j * NY + row) * NVA + q] = (TR)tv;
+      }
+    }
+  return bad;
+}
+
+// stage_eliminate_all for the matrix-core kernel (float32): the same
+// assignment of stages and row ranges to waves.  scr: NW * SE_SCR_MF floats.
+template <int NW, class TR>
+__device__ __forceinline__ int stage_eliminate_mf(int i, const float* __restrict__ jac, const double* __restrict__ fa,
+                                                  float* sT, TR* __restrict__ recT, TR* __restrict__ recG, float* scr) {
+  const int wave = wave_id();
+  float* my = scr + wave * SE_SCR_MF;
+  int bad = 0;
+  if constexpr (NW >= M) {
+    if (wave < M) bad |= stage_rows_mf<NY, TR>(i, wave, 0, NY, true, jac, fa, sT, recT, recG, my);
+  } else {
+    constexpr int FULL = (M / NW) * NW;
+    constexpr int RS = (NY + NW - 1) / NW;
+    for (int j = wave; j < FULL; j += NW)
+      bad |= stage_rows_mf<NY, TR>(i, j, 0, NY, true, jac, fa, sT, recT, recG, my);
+    for (int j = FULL; j < M; ++j) {
+      const int r0 = wave * RS < NY ? wave * RS : NY;
+      const int r1 = r0 + RS < NY ? r0 + RS : NY;
+      __builtin_amdgcn_wave_barrier();
+      asm volatile("" ::: "memory");   // the previous stage's scratch reads are done
+      bad |= stage_rows_mf<RS, TR>(i, j, r0, r1, wave == 0, jac, fa, sT, recT, recG, my);
+    }
+  }
+  return bad;
+}
+
 // --------------------------------------------------------------- condense
 // One workgroup per interval.  Condenses the stage unknowns out of the
 // interval's linearised collocation equations (the role of
@@ -564,7 +903,8 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
   R* recU = rec + (size_t)i * REC;
   R* recT = recU + NS * NVA;
   R* recG = recT + M * NY * NVA;
-  const int bad = stage_eliminate<R, NWV, R, R>(i, jac, fa, sT, recT, recG);
+  static_assert(NWV * SE_SCR <= NSK * NY * NVAP || !NS_OK, "stage scratch inside the U buffer");
+  const int bad = stage_eliminate_all<R, NWV, R, R>(i, jac, fa, sT, recT, recG, buf);
   __syncthreads();
 
   const bool act = tid < NS_ITEMS;
@@ -725,7 +1065,8 @@ constexpr int MF_CT = (NVA + 15) / 16;          // column tiles
 constexpr int MF_KS = (NY + 3) / 4;             // k-steps of a product
 constexpr int MF_T = MF_RT * MF_CT;
 constexpr int MF_THREADS = NSK * WAVE;
-constexpr int MF_LDS = (M * NY * NVA > NSK * MF_T * WAVE * 4) ? M * NY * NVA : NSK * MF_T * WAVE * 4;
+constexpr int MF_LDS0 = M * NY * NVA + NSK * SE_SCR_MF;   // sT and the stage eliminations' scratch
+constexpr int MF_LDS = (MF_LDS0 > NSK * MF_T * WAVE * 4) ? MF_LDS0 : NSK * MF_T * WAVE * 4;
 constexpr bool MF_OK = NSK >= 1 && NSK <= 16 && MF_T * 12 + MF_RT * MF_KS <= 176 && MF_LDS * 4 <= 64 * 1024;
 
 __device__ __forceinline__ void condense_mf32(int nint, Batch bt, const double* __restrict__ t,
@@ -753,7 +1094,7 @@ __device__ __forceinline__ void condense_mf32(int nint, Batch bt, const double* 
   float* recU = rec + (size_t)i * REC;
   float* recT = recU + NS * NVA;
   float* recG = recT + M * NY * NVA;
-  const int bad = stage_eliminate<float, NSK, float, float>(i, jac, fa, sT, recT, recG);
+  const int bad = stage_eliminate_mf<NSK, float>(i, jac, fa, sT, recT, recG, sT + M * NY * NVA);
   __syncthreads();
 
   const int js = k + 1;
