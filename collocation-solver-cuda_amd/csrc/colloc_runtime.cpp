@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -71,6 +72,34 @@ const char* kFnNames[F_COUNT] = {
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
+
+// Per-instance residual-norm slots of cs_compute_f_q_noalloc: a fixed device
+// array of this library (64 KiB on every GPU it runs on, held until the
+// process ends), so that entry point allocates nothing.  Host code reaches
+// it only through the address hipGetSymbolAddress returns for the current
+// device (g_scratch_dev), never through the array's name.
+constexpr int kScratchSlots = 8192;
+static __device__ __align__(256) unsigned long long scratch[kScratchSlots];
+constexpr int kMaxDevices = 64;
+void* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// The device address of `scratch` on the current device, looked up once per
+// device (cs_create calls this with the handle's device current).
+int scratch_address(void** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(CS_E_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(CS_E_HIP, "hipGetSymbolAddress returned no address for the scratch slots");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return CS_OK;
+}
 
 }  // namespace
 
@@ -145,6 +174,7 @@ struct cs_handle {
   // [2 bmax, 2 bmax + 2): condensation counters
   unsigned long long* scal = nullptr;
   int* status = nullptr;
+  unsigned long long* scratch_slots = nullptr;   // this device's `scratch` (cs_compute_f_q_noalloc)
   // profiling
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[CS_KERNEL_COUNT];
@@ -266,23 +296,27 @@ int upload_doubles(cs_handle* h, double* dev, const std::vector<double>& v) {
 
 // Residual of a packed state for the instances in mask (null: all); writes
 // f_a/f_b/y_tilde/r_bc when full; per-instance norms to norms[B].
-int residual(cs_handle* h, const double* st, double alpha, bool full, const int* mask, double* norms) {
+int residual_to(cs_handle* h, const double* st, double alpha, bool full, const int* mask,
+                unsigned long long* slot) {
   TRY(need_mesh(h));
   ProfScope ps(h, CS_KERNEL_RESIDUAL);
-  TRY(fill_u64(h, h->scal, h->B));
+  TRY(fill_u64(h, slot, h->B));
   int nint = h->nint;
   const double* y = st + h->st_y;
   const double* yd = st + h->st_yd;
   const double* zt = st + h->st_z;
   const double* p = st + h->st_p;
-  unsigned long long* slot = h->scal;
   // over the listed intervals when a launch list is set (the instances still
   // iterating, a superset of the line search's), masked by `mask`
   Batch bt = h->batch_listed(mask);
   void* args[] = {&nint, &bt, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &h->ytil, &h->fa, &h->fb, &h->rbc, &slot};
   const int nlaunch = h->launch_intervals();
   unsigned nthreads = (unsigned)(nlaunch * h->m + h->B);
-  TRY(launch(h, full ? F_RES_FULL : F_RES_NORM, (nthreads + 255) / 256, 256, args));
+  return launch(h, full ? F_RES_FULL : F_RES_NORM, (nthreads + 255) / 256, 256, args);
+}
+
+int residual(cs_handle* h, const double* st, double alpha, bool full, const int* mask, double* norms) {
+  TRY(residual_to(h, st, alpha, full, mask, h->scal));
   return read_slots(h, h->scal, norms);
 }
 
@@ -618,6 +652,11 @@ int cs_create(const cs_dims* d, cs_handle** out) {
     if (e == hipSuccess) e = hipMemset(h->status, 0, sizeof(int));
     if (e == hipSuccess) e = hipMemset(h->c, 0, sizeof(double) * bm * std::max(h->nc, 1));
     if (e != hipSuccess) r = fail(CS_E_HIP, "allocation failed: %s", hipGetErrorString(e));
+  }
+  if (r == CS_OK) {
+    void* sp = nullptr;
+    r = scratch_address(&sp);
+    h->scratch_slots = (unsigned long long*)sp;
   }
   if (r != CS_OK) {
     std::string msg = g_err;
@@ -988,6 +1027,19 @@ int cs_compute_f_q(cs_handle* h, double alpha, double* norm_inf, double* y_tilde
   TRY(copy_out(h, f_a, h->fa, nint * h->m * (h->ny + h->nz)));
   TRY(copy_out(h, f_b, h->fb, nint * h->ny));
   TRY(copy_out(h, r_bc, h->rbc, (size_t)h->B * h->nrb));
+  return sync(h);
+}
+
+int cs_compute_f_q_noalloc(cs_handle* h, double alpha, double* norm_inf) {
+  if (!h || !norm_inf) return fail(CS_E_ARG, "null argument");
+  TRY(need_mesh(h));
+  if (h->B > kScratchSlots)
+    return fail(CS_E_SHAPE, "B=%d instances: cs_compute_f_q_noalloc holds at most %d", h->B, kScratchSlots);
+  // slots cleared by a fill kernel, the norms copied straight into the
+  // caller's array: no allocation on the device or the host
+  TRY(residual_to(h, h->state, alpha, true, nullptr, h->scratch_slots));
+  HIPCHK(hipMemcpyAsync(norm_inf, h->scratch_slots, (size_t)h->B * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
   return sync(h);
 }
 

@@ -38,6 +38,7 @@ SIGNATURES = {
     "cs_form_initial_input_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "cs_restart": (c_int, [c_void_p]),
     "cs_compute_f_q": (c_int, [c_void_p, c_double, P_D, P_D, P_D, P_D, P_D]),
+    "cs_compute_f_q_noalloc": (c_int, [c_void_p, c_double, P_D]),
     "cs_construct_jacobian": (c_int, [c_void_p, c_double, P_D, P_D, P_D]),
     "cs_reduce_jacobian": (c_int, [c_void_p, P_D, P_D, P_D]),
     "cs_solve_babd": (c_int, [c_void_p, P_D, P_D]),

@@ -135,6 +135,18 @@ int cs_restart(cs_handle* h);
 int cs_compute_f_q(cs_handle* h, double alpha, double* norm_inf, double* y_tilde,
                    double* f_a, double* f_b, double* r_bc);
 
+/* cs_compute_f_q that allocates nothing, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid it).  No counterpart in the reference.  It evaluates the same
+ * residual into the handle's f_a / f_b / y_tilde / r_bc buffers and writes
+ * norm_inf[b] (host array of B values), with the per-instance max-norm slots
+ * in a fixed 64 KiB device array of the library instead of the handle's
+ * workspace: at most 8192 instances (a larger batch is rejected with
+ * CS_E_SHAPE).  All handles on one GPU share that array, which keeps the last
+ * call's contents (each call clears the slots it uses first): calls that
+ * overlap on one GPU, from several threads, race. */
+int cs_compute_f_q_noalloc(cs_handle* h, double alpha, double* norm_inf);
+
 /* construct_jacobian_parallel (solver:611-797): stage Jacobians at the
  * stage values of the last cs_compute_f_q call.  Kept on the device; when
  * non-NULL, Df[(N-B)*m][ny][ny+nz+np], Dg[(N-B)*m][nz][ny+nz+np] and
