@@ -164,6 +164,9 @@ struct cs_handle {
   int rec_bits = 64;                    // stored type of the last condensation records (64 or 32)
   int mf_ok = 0;                        // the module's series condensation on matrix cores fits its sizes
   int cond_kind = 0;                    // see cs_set_condense_kernel
+  int pair_threads = 0;                 // block size of colloc_cr_pair (from the module)
+  int h_zero = 0;                       // the module's BABD rows carry no H (identically zero)
+  int jac_parts = 1;                    // threads per stage of the Jacobian kernel (from the module)
   std::vector<size_t> lev_obase;      // offset of the level in cr_oinst
   std::vector<size_t> lev_sbase;      // offset of the level in cr_seg
   std::vector<size_t> lev_rbase;      // row offset of the level's input rows in rows0 (one more: the last output)
@@ -340,7 +343,7 @@ int jacobian(cs_handle* h, double alpha, const int* mask) {
     TRY(launch(h, h->jac_bits == 32 ? F_JAC_STATIC32 : F_JAC_STATIC, blocks, 256, sargs));
     h->jac_static_bits = h->jac_bits;
   }
-  return launch(h, h->jac_bits == 32 ? F_JAC32 : F_JAC, (nthreads + 255) / 256, 256, args);
+  return launch(h, h->jac_bits == 32 ? F_JAC32 : F_JAC, (nthreads + 255) / 256 * (unsigned)h->jac_parts, 256, args);
 }
 
 // Condensation: the series kernel handles every interval whose stage system
@@ -463,7 +466,7 @@ int babd(cs_handle* h, const int* mask) {
   double* dp = h->dir + h->st_p;
   Batch bt = h->batch(mask);
   const size_t rowsz = (size_t)h->ny * h->crw;
-  unsigned pair_block = (unsigned)((h->recw + 15) / 16 * 64);   // 4 lanes per record column
+  const unsigned pair_block = (unsigned)h->pair_threads;   // 4 lanes per record column
   const int nlev = (int)h->lev_nout.size();
   for (int l = 0; l < nlev; ++l) {
     const int* oinst = h->cr_oinst + h->lev_obase[l];
@@ -715,6 +718,9 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
   int caps[4] = {};
   HIPCHK(hipMemcpy(caps, (void*)gp, sizeof caps, hipMemcpyDeviceToHost));
   h->mf_ok = caps[0];
+  h->pair_threads = caps[1];
+  h->h_zero = caps[2];
+  h->jac_parts = caps[3] > 0 ? caps[3] : 1;
   // the static stage-Jacobian entries are the new module's constants: the
   // buffer holds the previous module's until they are written again
   h->jac_static_bits = 0;
@@ -1082,7 +1088,10 @@ int cs_reduce_jacobian(cs_handle* h, double* A, double* H, double* b) {
       for (int r = 0; r < h->ny; ++r) {
         const double* row = buf.data() + (i * h->ny + r) * h->crw;
         if (A) std::memcpy(A + (i * h->ny + r) * h->ny, row, h->ny * sizeof(double));
-        if (H && h->np) std::memcpy(H + (i * h->ny + r) * h->np, row + 2 * h->ny, h->np * sizeof(double));
+        if (H && h->np) {
+          if (h->h_zero) std::memset(H + (i * h->ny + r) * h->np, 0, h->np * sizeof(double));
+          else std::memcpy(H + (i * h->ny + r) * h->np, row + 2 * h->ny, h->np * sizeof(double));
+        }
         if (b) b[i * h->ny + r] = row[h->crw - 1];
       }
   }

@@ -39,7 +39,14 @@ constexpr int NFIN = 2 * NY + NP;          // boundary (final) system size
 constexpr int CR_PARTS = 4;                          // lanes per record column in cr_pair (one per 16-lane row)
 constexpr int CR_COLS = WAVE / CR_PARTS;              // columns per wave
 constexpr int CR_ROWS = (2 * NY + CR_PARTS - 1) / CR_PARTS;   // rows per lane
-constexpr int CR_THREADS = (RECW + CR_COLS - 1) / CR_COLS * WAVE;
+// No stage function depends on a parameter (NPA = 0): every H block of the
+// BABD rows is identically zero at every level (the condensation gives
+// H = 0 and both pairings keep it), so the kernels neither form, read nor
+// write it; NPH is the count of H columns they carry.
+constexpr bool HZ = NPA == 0;
+constexpr int NPH = HZ ? 0 : NP;
+constexpr int RECW_L = 3 * NY + NPH + 1;   // record columns cr_pair carries
+constexpr int CR_THREADS = (RECW_L + CR_COLS - 1) / CR_COLS * WAVE;
 
 static_assert(M >= 2, "need at least two collocation stages");
 
@@ -196,10 +203,45 @@ __device__ __forceinline__ void residual(int nint, Batch bt, const double* __res
 }
 
 // ---------------------------------------------------------------- jacobian
-// One thread per stage writes Df, Dg densely; thread Nint*M + b writes Dr of
-// instance b.  Restates construct_jacobian_kernel (solver:680-797) without
-// materialising J, V, D, W (the condensation kernel assembles what it needs).
-// TJ: the stored type of the stage Jacobians (double, or float for the FP32
+// The values of a wave's 64 stages go to the stage blocks in groups of
+// JAC_CHUNK = 16 (prob::jac_part<k> hands them on in that order): each lane
+// puts its 16 values in LDS, then lane l stores value l % 16 of the stages
+// (l / 16) + 4 t, t < 16, so 16 consecutive lanes write 16 entries of one
+// stage (consecutive addresses where the entries are) instead of 64 lanes
+// writing 64 stage blocks apart.
+template <class TJ>
+struct JacSink {
+  TJ* jac;
+  TJ* buf;                       // this wave's LDS: [WAVE][JAC_CHUNK + 1]
+  size_t st_t[16];               // stage of the lanes (l / 16) + 4 t of this wave
+  unsigned valid;                // bit t: that stage is evaluated
+  template <int N>
+  __device__ __forceinline__ void put(int first, const TJ* v) {
+    static_assert(prob::JAC_CHUNK == 16, "four 16-lane rows per store");
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < N; ++j) buf[lane * 17 + j] = v[j];
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");   // one wave: the LDS writes above precede the reads below
+    const int j = lane & 15;
+    if (j < N) {
+      const size_t k = (size_t)prob::jac_dyn_idx[first + j];
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        if (valid & (1u << t)) jac[st_t[t] * JSTAGE + k] = buf[((lane >> 4) + 4 * t) * 17 + j];
+    }
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");   // and the reads precede the next put's writes
+  }
+};
+
+// prob::JAC_PARTS threads share a stage: workgroup g evaluates part
+// g % JAC_PARTS of the dynamic entries (the same part in the whole
+// workgroup, so no divergence) for the stages of workgroup g / JAC_PARTS.
+// One thread per stage; thread Nint*M + b of part 0 writes Dr of instance b.
+// Restates construct_jacobian_kernel (solver:680-797) without materialising
+// J, V, D, W (the condensation kernel assembles what it needs).  TJ: the
+// stored type of the stage Jacobians (double, or float for the FP32
 // direction), which is also the type their state-dependent entries are
 // evaluated in.
 template <class TJ>
@@ -207,30 +249,45 @@ __device__ __forceinline__ void jacobian(int nint, Batch bt, const double* __res
                          const double* __restrict__ ytil, const double* __restrict__ ztil,
                          const double* __restrict__ p, const double* __restrict__ c, double alpha,
                          TJ* __restrict__ jac, double* __restrict__ Dr) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ TJ jbuf[256 / WAVE][WAVE * 17];
+  const int part = blockIdx.x % prob::JAC_PARTS;
+  const int gid = (blockIdx.x / prob::JAC_PARTS) * blockDim.x + threadIdx.x;
   const int nst = launch_intervals(bt, nint) * M;
-  if (gid < nst) {
+  bool valid = gid < nst;
+  int st = 0, b = 0;
+  if (valid) {
     const int i = listed_interval(bt, gid / M);
-    const int st = i * M + (gid - (gid / M) * M);
-    const int b = bt.iinst[i];
-    if (skipped(bt, b)) return;
+    st = i * M + (gid - (gid / M) * M);
+    b = bt.iinst[i];
+    valid = !skipped(bt, b);
+    if (!valid) { st = 0; b = 0; }
+  }
+  if (__ballot(valid) != 0) {   // the whole wave runs the part (its lanes exchange values)
+    JacSink<TJ> sink;
+    sink.jac = jac;
+    sink.buf = jbuf[wave_id()];
+    sink.valid = 0;
+    const int lane = lane_id();
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int src = (lane >> 4) + 4 * t;
+      sink.st_t[t] = (size_t)__shfl(st, src);
+      if (__shfl((int)valid, src)) sink.valid |= 1u << t;
+    }
     double yt[NY], zt[NZ1];
 #pragma unroll
     for (int l = 0; l < NY; ++l) yt[l] = ytil[(size_t)st * NY + l];
 #pragma unroll
     for (int l = 0; l < NZ; ++l) zt[l] = ztil[(size_t)st * NZ + l];
-    TJ* out = jac + (size_t)st * JSTAGE;
-    const double* pb = p + (size_t)b * NP;
-    const double* cb = c + (size_t)b * NC;
     // the dynamic entries only: the static ones (zeros, compile-time
     // constants) are in place since jacobian_static filled the buffer
-    prob::Df_dyn(yt, zt, pb, cb, out);
-    prob::Dg_dyn(yt, zt, pb, cb, alpha, out + NY * NJC);
-  } else if (gid < nst + bt.B) {
-    const int b = gid - nst;
-    if (skipped(bt, b)) return;
-    prob::Dr(y + (size_t)bt.noff[b] * NY, y + (size_t)(bt.noff[b + 1] - 1) * NY, p + (size_t)b * NP,
-             c + (size_t)b * NC, Dr + (size_t)b * DRSZ);
+    prob::jac_dyn_part<TJ>(part, yt, zt, p + (size_t)b * NP, c + (size_t)b * NC, alpha, sink);
+  }
+  if (gid >= nst && gid < nst + bt.B && part == 0) {
+    const int ib = gid - nst;
+    if (skipped(bt, ib)) return;
+    prob::Dr(y + (size_t)bt.noff[ib] * NY, y + (size_t)(bt.noff[ib + 1] - 1) * NY, p + (size_t)ib * NP,
+             c + (size_t)ib * NC, Dr + (size_t)ib * DRSZ);
   }
 }
 
@@ -830,7 +887,7 @@ __device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst
     const int r = e / NY, q = e - (e / NY) * NY;
     rowi[r * CRW + NY + q] = (r == q) ? 1.0 : 0.0;
   }
-  if (NPA < NP) {   // H columns of parameters the stage functions do not depend on
+  if (!HZ && NPA < NP) {   // H columns of parameters the stage functions do not depend on
     for (int e = tid; e < NY * NP; e += NT) {
       const int r = e / NP, j = e - (e / NP) * NP;
       if (!p_active(j)) rowi[r * CRW + 2 * NY + j] = 0.0;
@@ -1032,7 +1089,7 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
     const int a = e / NY, q = e - (e / NY) * NY;
     rowi[a * CRW + NY + q] = (a == q) ? 1.0 : 0.0;
   }
-  if (NPA < NP) {
+  if (!HZ && NPA < NP) {
     for (int e = tid; e < NY * NP; e += NS_THREADS) {
       const int a = e / NP, j = e - (e / NP) * NP;
       if (!p_active(j)) rowi[a * CRW + 2 * NY + j] = 0.0;
@@ -1225,7 +1282,7 @@ __device__ __forceinline__ void condense_mf32(int nint, Batch bt, const double* 
     const int a = e / NY, q = e - (e / NY) * NY;
     rowi[a * CRW + NY + q] = (a == q) ? 1.0 : 0.0;
   }
-  if (NPA < NP) {
+  if (!HZ && NPA < NP) {
     for (int e = tid; e < NY * NP; e += MF_THREADS) {
       const int a = e / NP, j = e - (e / NP) * NP;
       if (!p_active(j)) rowi[a * CRW + 2 * NY + j] = 0.0;
@@ -1275,7 +1332,10 @@ __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRS
   const int lane = lane_id();
   const int part = lane / CR_COLS;                 // which CR_ROWS-row slice
   const int cidx = wave_id() * CR_COLS + (lane % CR_COLS);
-  const bool live = cidx < RECW;
+  const bool live = cidx < RECW_L;
+  // the carried column's place in the record ([R | Gp | Ep | Jp | d]; Jp
+  // absent when HZ) and in the output row (column - NY)
+  const int pcol = cidx < 3 * NY + NPH ? cidx : RECW - 1;
   const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
   const double* rb = ra + NY * CRW;
   // column layout: [x_M (NY) | x_L (NY) | x_R (NY) | p (NP) | rhs]
@@ -1290,7 +1350,7 @@ __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRS
       if (cidx < NY) v = top ? src[NY + cidx] : src[cidx];
       else if (cidx < 2 * NY) v = top ? src[cidx - NY] : 0.0;
       else if (cidx < 3 * NY) v = top ? 0.0 : src[cidx - NY];
-      else v = src[cidx - NY];
+      else v = src[pcol - NY];
     }
     a[i] = v;
   }
@@ -1362,8 +1422,8 @@ __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRS
 #pragma unroll
     for (int i = 0; i < CR_ROWS; ++i) {
       const int r = part * CR_ROWS + i;
-      if (r < NY) rec[r * RECW + cidx] = a[i];
-      else if (r < R2 && cidx >= NY) out[(r - NY) * CRW + cidx - NY] = a[i];
+      if (r < NY) rec[r * RECW + pcol] = a[i];
+      else if (r < R2 && cidx >= NY) out[(r - NY) * CRW + pcol - NY] = a[i];
     }
   }
 }
@@ -1380,7 +1440,7 @@ __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRS
 // the horizon, so the transition products stay near the identity); the
 // coarse levels keep the Householder step.  Thread (rg, cg) of the
 // workgroup computes a 4 x 4 tile of the product from LDS.
-constexpr int CS_XC = NY + NP + 1;                    // columns of [G_a | H_a | b_a]
+constexpr int CS_XC = NY + NPH + 1;                   // columns of [G_a | H_a | b_a] (no H_a when HZ)
 constexpr int CS_CG = (CS_XC + 3) / 4;                // column groups of 4
 constexpr int CS_RG = (NY + 3) / 4;                   // row groups of 4
 constexpr int CS_THREADS = 256;
@@ -1410,8 +1470,8 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
     double* dst = rows_out + (size_t)npairs * NY * CRW;
     for (int e = tid; e < NY * CRW; e += blockDim.x) {
       const int r = e / CRW, c = e - (e / CRW) * CRW;
-      if (c < NY || c >= 2 * NY) dst[e] = src[e];
-      else if (wc) dst[e] = (c - NY == r) ? 1.0 : 0.0;
+      if (c < NY || c == CRW - 1 || (!HZ && c >= 2 * NY)) dst[e] = src[e];
+      else if (wc && c < 2 * NY) dst[e] = (c - NY == r) ? 1.0 : 0.0;
     }
     return;
   }
@@ -1426,8 +1486,8 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
     const int r = e / XP, c = e - (e / XP) * XP;
     double v = 0.0;
     if (c < NY) v = ra[r * CRW + c];                       // G_a
-    else if (c < NY + NP) v = ra[r * CRW + NY + c];        // H_a (after C_a)
-    else if (c == NY + NP) v = ra[r * CRW + CRW - 1];      // b_a
+    else if (c < NY + NPH) v = ra[r * CRW + NY + c];       // H_a (after C_a)
+    else if (c == NY + NPH) v = ra[r * CRW + CRW - 1];     // b_a
     sX[r][c] = v;
   }
   // no record: back substitution reads row a itself (x_M = b_a - G_a x_L - H_a p),
@@ -1460,8 +1520,8 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
       for (int c = 0; c < 4; ++c) {
         const int col = cg * 4 + c;
         if (col < NY) out[r * CRW + col] = -acc[a][c];                                   // G'
-        else if (col < NY + NP) out[r * CRW + NY + col] = rb[r * CRW + NY + col] - acc[a][c];   // H'
-        else if (col == NY + NP) out[r * CRW + CRW - 1] = rb[r * CRW + CRW - 1] - acc[a][c];  // b'
+        else if (col < NY + NPH) out[r * CRW + NY + col] = rb[r * CRW + NY + col] - acc[a][c];  // H'
+        else if (col == NY + NPH) out[r * CRW + CRW - 1] = rb[r * CRW + CRW - 1] - acc[a][c];  // b'
       }
     }
   }
@@ -1507,7 +1567,7 @@ __device__ __forceinline__ void cr_final(Batch bt, const double* __restrict__ ro
       if (r < NY) {
         if (cc < NY) v = row[r * CRW + NY + cc];                 // C
         else if (cc < 2 * NY) v = row[r * CRW + cc - NY];        // G
-        else if (cc < NFIN) v = row[r * CRW + cc];               // H
+        else if (cc < NFIN) v = HZ ? 0.0 : row[r * CRW + cc];    // H
         else if (cc == NFIN) v = row[r * CRW + CRW - 1];         // b
       } else if (r < NFIN) {
         const int rr = r - NY;
@@ -1571,7 +1631,7 @@ __device__ __forceinline__ void cr_back(int level, Batch bt, const int* __restri
 #pragma unroll 8
     for (int c = 0; c < NY; ++c) rhs = fma(-ra[c], xl[c], rhs);
 #pragma unroll 8
-    for (int c = 0; c < NP; ++c) rhs = fma(-ra[2 * NY + c], dp[c], rhs);
+    for (int c = 0; c < NPH; ++c) rhs = fma(-ra[2 * NY + c], dp[c], rhs);
     if (lane < NY) dy[(size_t)mid * NY + lane] = rhs;
     return;
   }
@@ -1587,7 +1647,7 @@ __device__ __forceinline__ void cr_back(int level, Batch bt, const int* __restri
 #pragma unroll 8
   for (int c = 0; c < NY; ++c) rhs = fma(-rw[2 * NY + c], xr[c], rhs);
 #pragma unroll 8
-  for (int c = 0; c < NP; ++c) rhs = fma(-rw[3 * NY + c], dp[c], rhs);
+  for (int c = 0; c < NPH; ++c) rhs = fma(-rw[3 * NY + c], dp[c], rhs);
   double x = 0.0;
 #pragma unroll
   for (int r = NY - 1; r >= 0; --r) {
@@ -1953,8 +2013,11 @@ __global__ __launch_bounds__(colloc::MF_THREADS) void colloc_condense_mf32(int n
   }
 }
 // capabilities of this problem's kernels: [0] colloc_condense_mf32 runs the
-// matrix-core series (else it leaves every interval to Gauss-Jordan)
-__device__ int colloc_caps[4] = {colloc::MF_OK ? 1 : 0, 0, 0, 0};
+// matrix-core series (else it leaves every interval to Gauss-Jordan);
+// [1] the block size of colloc_cr_pair; [2] the BABD rows carry no H block
+// (no stage function depends on a parameter: H is identically zero);
+// [3] workgroups per 256 stages of colloc_jacobian (threads per stage)
+__device__ int colloc_caps[4] = {colloc::MF_OK ? 1 : 0, colloc::CR_THREADS, colloc::HZ ? 1 : 0, prob::JAC_PARTS};
 
 // The FP32-direction condensation: two 512-thread workgroups per CU (half the
 // LDS and registers of the float64 kernel).

@@ -338,10 +338,12 @@ def _cname(sym):
     return "w" + n if n.startswith("_") else "u_" + n
 
 
-def _emit_block(lines, exprs_out, hoister, bindings, constants, inst_consts, indent="  ", ctype=None):
+def _emit_block(lines, exprs_out, hoister, bindings, constants, inst_consts, indent="  ", ctype=None, chunk=None):
     """Append C statements computing exprs_out = [(target, expr), ...]; with
     ctype (a C type name) every intermediate is computed in that type,
-    else in double."""
+    else in double.  chunk = (C, flush): the targets are ignored, the values
+    go to _v[0..C) in groups of C and each group is handed on by the
+    statement flush(index of its first value, group size)."""
     td = ctype or "double"
     exprs = [e for _, e in exprs_out]
     hdefs = hoister.needed(exprs) if hoister is not None else []
@@ -374,9 +376,21 @@ def _emit_block(lines, exprs_out, hoister, bindings, constants, inst_consts, ind
         for s, d in reps:
             names[s] = _cname(s)
             lines.append("%sconst %s %s = %s;" % (indent, td, names[s], pr.doprint(d)))
-        for (tgt, _), e in zip(nonzero, red):
-            lines.append("%s%s = %s;" % (indent, tgt, pr.doprint(e)))
+        if chunk is None:
+            for (tgt, _), e in zip(nonzero, red):
+                lines.append("%s%s = %s;" % (indent, tgt, pr.doprint(e)))
+        else:
+            C, flush = chunk
+            if len(nonzero) != len(exprs_out):
+                raise ValueError("chunked output needs non-zero expressions")
+            lines.append("%s%s _v[%d];" % (indent, td, C))
+            for q, e in enumerate(red):
+                lines.append("%s_v[%d] = %s;" % (indent, q % C, pr.doprint(e)))
+                if q % C == C - 1 or q == len(red) - 1:
+                    lines.append(indent + flush(q - q % C, q % C + 1))
     for tgt, e in exprs_out:
+        if chunk is not None:
+            break
         if e == 0:
             lines.append("%s%s = %s;" % (indent, tgt, "(%s)0" % ctype if ctype else "0.0"))
 
@@ -399,10 +413,14 @@ def emit_problem(sym):
              "const double* __restrict__ _c"
     H = sym.hoist_fg
 
-    def fn(name, extra, target, rows, binds, hoister, out_type=False):
+    def fn(name, extra, target, rows, binds, hoister, out_type=False, chunk=None):
         # out_type: a template on the output type (the Jacobians are stored,
-        # and evaluated, in float64, or float32 for the FP32 direction)
-        if out_type:
+        # and evaluated, in float64, or float32 for the FP32 direction);
+        # chunk: the values go to a sink object (template SINK) in groups
+        if chunk is not None:
+            lines = ["template <class TO, class SINK>",
+                     "__device__ __forceinline__ void %s(%s%s, SINK& _sink) {" % (name, sig_fg, extra)]
+        elif out_type:
             lines = ["template <class TO>",
                      "__device__ __forceinline__ void %s(%s%s, TO* __restrict__ %s) {" % (name, sig_fg, extra, target)]
         else:
@@ -413,7 +431,7 @@ def emit_problem(sym):
             lines.append("  (void)_alpha;")
         # the Jacobians compute in their stored type (float for the FP32
         # direction: an inexact-Newton direction), the residuals in double
-        _emit_block(lines, rows, hoister, binds, consts, ic, ctype="TO" if out_type else None)
+        _emit_block(lines, rows, hoister, binds, consts, ic, ctype="TO" if out_type else None, chunk=chunk)
         lines.append("}")
         out.extend(lines)
 
@@ -422,19 +440,38 @@ def emit_problem(sym):
     # The stage Jacobian block [Df; Dg] splits into static entries (zeros and
     # expressions of the problem's compile-time constants only) and dynamic
     # ones (depending on the state, the parameters, the instance constants or
-    # alpha).  Df_dyn / Dg_dyn write the dynamic entries at every Newton
-    # iteration; the static ones are tabulated (jac_static_val where
+    # alpha).  jac_part<k> (JAC_PARTS groups) write the dynamic entries at
+    # every Newton iteration; the static ones are tabulated (jac_static_val where
     # jac_static_mask is set) and written once per Jacobian buffer.
     cvals = {s: sym.constants[s] for s in consts}
     cset = set(consts)
     block = [e for row in sym.Df_exprs for e in row] + [e for row in sym.Dg_exprs for e in row]
     static = [not (e.free_symbols - cset) for e in block]
-    fn("Df_dyn", "", "_Df", [("_Df[%d]" % (i * nj + j), e) for i, row in enumerate(sym.Df_exprs)
-                             for j, e in enumerate(row) if not static[i * nj + j]], yb + zb + pb, H, out_type=True)
-    nf = ny * nj
-    fn("Dg_dyn", ", double _alpha", "_Dg", [("_Dg[%d]" % (i * nj + j), e) for i, row in enumerate(sym.Dg_exprs)
-                                            for j, e in enumerate(row) if not static[nf + i * nj + j]],
-       yb + zb + pb + ab, H, out_type=True)
+    # The dynamic entries are split into JAC_PARTS groups of about equal
+    # operation count, each its own function (jac_part<k>), so that several
+    # threads share a stage and none holds the whole block's intermediates.
+    # Each part hands its values on in groups of JAC_CHUNK to a sink of the
+    # kernel's (SINK::put<n>(first, values)), value q of the part being entry
+    # jac_dyn_idx[offset of the part + q] of the stage block, so the kernel
+    # can store a group for many stages at once.
+    dyn = [(k, e) for k, (e, st) in enumerate(zip(block, static)) if not st]
+    parts = partition_entries(dyn)
+    offs, idx = [], []
+    for q, group in enumerate(parts):
+        offs.append(len(idx))
+        idx += [k for k, _ in group]
+        base = offs[-1]
+        fn("jac_part%d" % q, ", double _alpha", "_J", [("_J[%d]" % k, e) for k, e in group], yb + zb + pb + ab, H,
+           out_type=True, chunk=(JAC_CHUNK, lambda first, n, base=base:
+                                 "_sink.template put<%d>(%d, _v);" % (n, base + first)))
+    out.append("constexpr int JAC_PARTS = %d;   // threads per stage of the Jacobian kernel" % len(parts))
+    out.append("constexpr int JAC_CHUNK = %d;   // values per sink call" % JAC_CHUNK)
+    out.append(_table("__device__ const int jac_dyn_idx", ["%d" % k for k in idx] or ["0"]))
+    out.append("template <class TO, class SINK>\n__device__ __forceinline__ void jac_dyn_part(int part, %s, "
+               "double _alpha, SINK& _sink) {\n  switch (part) {" % sig_fg)
+    for q in range(len(parts)):
+        out.append("    case %d: jac_part%d<TO>(_y, _z, _p, _c, _alpha, _sink); break;" % (q, q))
+    out.append("    default: break;\n  }\n}")
     out.append(_table("__device__ const unsigned char jac_static_mask", ["%d" % (1 if st else 0) for st in static]))
     out.append(_table("__device__ const double jac_static_val",
                       [repr(float(e.subs(cvals))) if st else "0.0" for e, st in zip(block, static)]))
@@ -477,6 +514,31 @@ def emit_problem(sym):
     out.append("}")
     out.append("}  // namespace prob")
     return "\n".join(out) + "\n"
+
+
+JAC_OPS_PER_PART = 20000
+JAC_MAX_PARTS = 8
+JAC_CHUNK = 16
+
+
+def partition_entries(dyn):
+    """Split the dynamic Jacobian entries [(index, expr)] into contiguous
+    groups of about equal operation count (sympy's count_ops), one group per
+    JAC_OPS_PER_PART operations, at most JAC_MAX_PARTS groups."""
+    if not dyn:
+        return [[]]
+    cost = [sp.count_ops(e) + 1 for _, e in dyn]
+    total = sum(cost)
+    n = max(1, min(JAC_MAX_PARTS, -(-total // JAC_OPS_PER_PART), len(dyn)))
+    groups, cur, acc = [], [], 0.0
+    for (k, e), c in zip(dyn, cost):
+        cur.append((k, e))
+        acc += c
+        if acc >= total * (len(groups) + 1) / n and len(groups) < n - 1:
+            groups.append(cur)
+            cur = []
+    groups.append(cur)
+    return groups
 
 
 def _table(decl, items, per_line=16):

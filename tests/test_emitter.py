@@ -170,19 +170,22 @@ def test_emitted_code_is_well_formed():
         for i in range(n):
             assert ("%s[%d] =" % (name, i)) in text, (name, i)
     # every entry of the stage Jacobian block [Df; Dg] is either written by
-    # Df_dyn / Dg_dyn or static: tabulated with its value, and then a
-    # function of the compile-time constants only
+    # exactly one of the jac_part<k> functions or static: tabulated with its
+    # value, and then a function of the compile-time constants only
     mask = _table(text, "jac_static_mask")
     vals = _table(text, "jac_static_val")
     assert len(mask) == len(vals) == (ny + nz) * nj
     block = [e for row in sym.Df_exprs for e in row] + [e for row in sym.Dg_exprs for e in row]
-    dyn_f = _body(text, "Df_dyn")
-    dyn_g = _body(text, "Dg_dyn")
+    # the parts hand their values on in the order of jac_dyn_idx
+    idx = [int(v) for v in _table(text, "jac_dyn_idx")]
+    assert sorted(idx) == idx and len(set(idx)) == len(idx)
+    nparts = int(text.split("constexpr int JAC_PARTS = ")[1].split(";")[0])
+    puts = sum(_body(text, "jac_part%d" % q).count("_sink.template put<") for q in range(nparts))
+    assert puts >= -(-len(idx) // emitter.JAC_CHUNK)
     consts = {s: v for s, v in sym.constants.items() if s not in sym.instance_constants}
     for k, e in enumerate(block):
-        name, i, body = ("_Df", k, dyn_f) if k < ny * nj else ("_Dg", k - ny * nj, dyn_g)
-        written = ("%s[%d] =" % (name, i)) in body
-        assert written != bool(mask[k]), (name, i)
+        assert (k in idx) != bool(mask[k]), k
+        name, i = ("_Df", k) if k < ny * nj else ("_Dg", k - ny * nj)
         if mask[k]:
             assert not (e.free_symbols - set(consts)), (name, i)
             assert vals[k] == float(e.subs(consts)), (name, i)
@@ -190,3 +193,20 @@ def test_emitted_code_is_well_formed():
     assert "K_c1" in text and "K_c2" in text
     # deterministic: the code text (and so the code-object cache key) is stable
     assert text == emitter.emit_problem(emitter.BvpDaeSymbolic(ocpmod.parse_ocp(GENERAL)))
+
+
+def test_jacobian_entries_partition():
+    """Dynamic entries are split into contiguous groups of about equal
+    operation count: every entry in exactly one group, in order, one group
+    per JAC_OPS_PER_PART operations up to JAC_MAX_PARTS."""
+    x = sp.symbols("x0:40", real=True)
+    heavy = [(k, sum(sp.tanh(x[j]) * x[(j + k) % 40] for j in range(40))) for k in range(400)]
+    light = [(400 + k, x[k % 40] * 2) for k in range(100)]
+    groups = emitter.partition_entries(light + heavy)
+    flat = [k for g in groups for k, _ in g]
+    assert flat == [k for k, _ in light + heavy]
+    total = sum(sp.count_ops(e) + 1 for _, e in light + heavy)
+    assert len(groups) == min(emitter.JAC_MAX_PARTS, -(-total // emitter.JAC_OPS_PER_PART))
+    cost = [sum(sp.count_ops(e) + 1 for _, e in g) for g in groups]
+    assert max(cost) < 1.3 * total / len(groups)
+    assert emitter.partition_entries(light[:3]) == [light[:3]]   # a small block stays whole
