@@ -917,9 +917,11 @@ constexpr int NS_CPT = 24;                                 // columns per work i
 constexpr int NS_NH = (NVA + NS_CPT - 1) / NS_CPT;
 constexpr int NVAP = NS_NH * NS_CPT;                       // padded row of the U buffer
 constexpr int NS_ITEMS = NSK * NY * NS_NH;
+// the U buffer, which also holds the stage eliminations' scratch before the series
+constexpr int NS_BUF = NSK * NY * NVAP > (NS_THREADS / WAVE) * SE_SCR ? NSK * NY * NVAP : (NS_THREADS / WAVE) * SE_SCR;
 template <class R>
 constexpr bool ns_ok() {
-  return NY <= 48 && NS_ITEMS <= NS_THREADS && (M * NY * NVA + NSK * NY * NVAP) * (int)sizeof(R) <= 150 * 1024;
+  return NY <= 48 && NS_ITEMS <= NS_THREADS && (M * NY * NVA + NS_BUF) * (int)sizeof(R) <= 150 * 1024;
 }
 constexpr bool NS_OK = ns_ok<double>();
 // Series convergence: a term changes U by at most this fraction of its
@@ -942,9 +944,9 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
   constexpr int NWV = NS_THREADS / WAVE;
   // one LDS array with buf at its start, so that the U-buffer addresses fit
   // the 16-bit offsets of the LDS instructions
-  __shared__ R smem_ns[NSK * NY * NVAP + M * NY * NVA];
+  __shared__ R smem_ns[NS_BUF + M * NY * NVA];
   R* buf = smem_ns;
-  R* sT = smem_ns + NSK * NY * NVAP;
+  R* sT = smem_ns + NS_BUF;
   __shared__ R sred[2 * NWV];
   __shared__ int sbad;
   const int i = listed_interval(bt, blockIdx.x);
@@ -960,7 +962,6 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
   R* recU = rec + (size_t)i * REC;
   R* recT = recU + NS * NVA;
   R* recG = recT + M * NY * NVA;
-  static_assert(NWV * SE_SCR <= NSK * NY * NVAP || !NS_OK, "stage scratch inside the U buffer");
   const int bad = stage_eliminate_all<R, NWV, R, R>(i, jac, fa, sT, recT, recG, buf);
   __syncthreads();
 

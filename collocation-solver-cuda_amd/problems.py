@@ -191,6 +191,94 @@ Tolerance = 1e-8;
 """
 
 
+# Constrained problems of the reference's example set (Notebook/source), the
+# same mathematics written here as OCP text: a trolley carrying a pendulum
+# load with a bounded control (exTrolley), a rocket in free space with control
+# and state-variable bounds and a free final time (exFreespaceRocket), a
+# container crane with a free final time (exContainerTransfer) and an
+# underwater vehicle with bounded controls (exUnderwaterVehicle).  Their
+# InputFiles are not shipped with the reference, so they start from the
+# all-ones guess.
+TROLLEY_OCP = """
+Constants = [pi = 3.14159265358979, m1 = 0.3, m2 = 0.5, L = 0.75, Iy = 0.002, g = 9.81];
+Variables = [den, num3, num4];
+StateVariables = [x1, x2, x3, x4];
+ControlVariables = [u];
+den = -m1*m2*L*L - m1*Iy - m2*m2*L*L - m2*Iy + m2*m2*L*L*cos(x2)*cos(x2);
+num3 = m2*m2*L*L*L*sin(x2)*x4*x4 - m2*L*L*u + m2*Iy*L*x4*x4*sin(x2) - Iy*u + m2*m2*L*L*g*cos(x2)*sin(x2);
+num4 = m2*L*(m2*L*cos(x2)*x4*x4*sin(x2) - cos(x2)*u + g*sin(x2)*(m1 + m2));
+DifferentialEquations = [x3, x4, num3/den, num4/den];
+CostFunctional = 0.5*(u*u + 5*x4*x4);
+InitialConstraints = [x1, x2, x3, x4];
+TerminalConstraints = [x1 - 1, x2, x3, x4];
+InequalityConstraints = [u - 0.5, -0.5 - u];
+InitialTime = 0.0;
+FinalTime = 2.7;
+Nodes = 271;
+Tolerance = 1.0e-6;
+MaximumNodes = 1000;
+MaximumMeshRefinements = 20;
+"""
+
+ROCKET_OCP = """
+StateVariables = [y1, y2, y3];
+ControlVariables = [u];
+ParameterVariables = [p];
+DifferentialEquations = [p*y2, p*(u - 0.2*y2*y2)/y3, -0.01*p*u*u];
+CostFunctional = p*(1 + 0.1*u*u);
+InitialConstraints = [y1, y2, y3 - 1.0];
+TerminalConstraints = [y1 - 10, y2];
+InequalityConstraints = [u - 1.1, -1.1 - u];
+StateVariableInequalityConstraints = [y2 - 1.7, -0.1 - y2];
+InitialTime = 0.0;
+FinalTime = 1.0;
+Nodes = 101;
+Tolerance = 1.0e-6;
+"""
+
+CONTAINER_OCP = """
+Constants = [rho = 1.0];
+StateVariables = [x1, x2, x3, x4, x5, x6];
+ControlVariables = [u1, u2];
+ParameterVariables = [p];
+DifferentialEquations = [p*x4, p*x5, p*x6, p*(u1 + 17.2656*x3), p*u2, -p*(u1 + 27.0756*x3 + 2.0*x5*x6)/x2];
+CostFunctional = p*(rho + 0.01*(u1*u1 + u2*u2));
+InitialConstraints = [x1, x2 - 22.0, x3, x4, x5 + 1.0, x6];
+TerminalConstraints = [x1 - 10.0, x2 - 14.0, x3, x4 - 2.5, x5, x6];
+InequalityConstraints = [u1 - 2.83374, -2.83374 - u1, u2 - 0.71265, -0.80865 - u2];
+ParameterEstimate = [9.0];
+Tolerance = 1.0e-6;
+MaximumNodes = 2000;
+MaximumNewtonIterations = 500;
+MaximumMeshRefinements = 25;
+"""
+
+UNDERWATER_OCP = """
+Constants = [pi = 3.14159265358979, uxmax = 2.0, uzmax = 1.0, cx = 0.5, cz = 0.1, rx0 = 0.1];
+Variables = [d1, d2, rx, rz];
+StateVariables = [x1, x2, x3, x4, x5, x6, x7, x8, x9, x10];
+ControlVariables = [u1, u2, u3, u4];
+d1 = (x1 - cx)/rx0;
+d2 = (x3 - cz)/cz;
+rx = -uxmax*exp(-d1*d1)*(x1 - cx)*d2*d2;
+rz = -uzmax*exp(-d1*d1)*d2*d2;
+DifferentialEquations = [cos(x6)*cos(x5)*x7 + rx, sin(x6)*cos(x5)*x7, -sin(x5)*x7 + rz,
+                         x8 + sin(x4)*tan(x5)*x9 + cos(x4)*tan(x5)*x10, cos(x4)*x9 - sin(x4)*x10,
+                         (x9*sin(x4) + x10*cos(x4))/cos(x5), u1, u2, u3, u4];
+CostFunctional = u1*u1 + u2*u2 + u3*u3 + u4*u4;
+InitialConstraints = [x1, x2, x3 - 0.02, x4 - pi/2.0, x5 - 0.1, x6 + pi/4.0, x7 - 1.0, x8, x9 - 0.5, x10 - 0.1];
+TerminalConstraints = [x1 - 1.0, x2 - 0.5, x3, x4 - pi/2.0, x5, x6, x7, x8, x9, x10];
+InequalityConstraints = [u1 - 15.0, -15.0 - u1, u2 - 15.0, -15.0 - u2, u3 - 15.0, -15.0 - u3, u4 - 15.0, -15.0 - u4];
+Tolerance = 1.0e-6;
+MaximumNodes = 2000;
+MaximumNewtonIterations = 500;
+MaximumMeshRefinements = 20;
+"""
+
+CONSTRAINED_EXAMPLES = {"trolley": TROLLEY_OCP, "rocket": ROCKET_OCP, "container": CONTAINER_OCP,
+                        "underwater": UNDERWATER_OCP}
+
+
 class TextProblem:
     """An OCP given as text (no generated family data)."""
 
@@ -211,6 +299,8 @@ def standard_units():
     yield "tanh4_seed2_m5", reseeded(small, 2), 5
     yield "tanh20_m5", config(1)[0], 5
     yield "param_osc_m4", TextProblem(PARAM_OSC_OCP), 4
+    for name, text in CONSTRAINED_EXAMPLES.items():
+        yield "%s_m4" % name, TextProblem(text), 4
     # batched families: x0 (and the tanh scale) are instance constants, so
     # instance 0's text stands for every member
     yield "tanh4_batch_m5", batch_family(1, nx=4, nu=2, nint_range=(16, 64))[0], 5
