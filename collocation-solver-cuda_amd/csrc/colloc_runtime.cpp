@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -52,6 +53,36 @@ struct Batch {
   int B;
   const int* ilist;
   int nlist;
+  const int* nlistp;
+};
+// Per-instance state of the device-resident Newton loop (struct NtState of
+// the kernels).
+struct NtState {
+  double* nrm;
+  double* taken;
+  int* trials;
+  int* done;
+  int* gave;
+  double* hist;
+  double* ring;
+  int* it;
+  unsigned long long* cnt;
+  int* nlist;
+  int* lofs;
+  int hl, window, patience, max_iter, max_trials;
+  double tol, min_ratio;
+};
+// A captured Newton phase and the timed kernel groups inside it: group
+// (kernel id, first stamp slot, last stamp slot) of the graph's slots.
+constexpr int kStampSlots = 64;   // timestamp slots per graph
+struct StampPair {
+  int k, s0, s1;
+};
+struct Graph {
+  hipGraphExec_t exec = nullptr;
+  int base = 0;                    // the graph's first slot in the handle's stamp buffer
+  int used = 0;                    // slots it writes
+  std::vector<StampPair> prof;
 };
 struct CRSeg {
   int in_off, out_off, rec_off, nrows, subst;
@@ -60,7 +91,7 @@ struct CRSeg {
 enum Fn {
   F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_JAC32, F_COND, F_COND32, F_COND_NS, F_COND_NS32, F_CR_PAIR, F_CR_SUBST,
   F_CR_FINAL, F_CR_BACK, F_RECOVER, F_RECOVER32, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_JAC_STATIC, F_JAC_STATIC32,
-  F_AXPY_LISTED, F_COND_MF32, F_COUNT
+  F_AXPY_LISTED, F_COND_MF32, F_NT_CHECK, F_NT_SCAN, F_NT_FILL, F_NT_LS, F_NT_ACCEPT, F_NT_TICK, F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
   "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian", "colloc_jacobian32",
@@ -68,7 +99,8 @@ const char* kFnNames[F_COUNT] = {
   "colloc_cr_subst",
   "colloc_cr_final", "colloc_cr_back", "colloc_recover", "colloc_recover32", "colloc_axpy4", "colloc_init_input",
   "colloc_segres", "colloc_recover_z", "colloc_jacobian_static", "colloc_jacobian_static32",
-  "colloc_axpy_listed", "colloc_condense_mf32"};
+  "colloc_axpy_listed", "colloc_condense_mf32", "colloc_nt_check", "colloc_nt_list_scan", "colloc_nt_list_fill",
+  "colloc_nt_line_search", "colloc_nt_accept", "colloc_nt_tick"};
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
@@ -167,6 +199,25 @@ struct cs_handle {
   int pair_threads = 0;                 // block size of colloc_cr_pair (from the module)
   int h_zero = 0;                       // the module's BABD rows carry no H (identically zero)
   int jac_parts = 1;                    // threads per stage of the Jacobian kernel (from the module)
+  // device-resident Newton loop (cs_set_newton_graph)
+  bool use_graph = false;
+  const int* nalistp = nullptr;         // set: the launch list alist is built on the device, its length here
+  int* dlist = nullptr;                 // the device-built launch list (intervals of the iterating instances)
+  double* nt_nrm = nullptr;
+  double* nt_taken = nullptr;
+  int* nt_ints = nullptr;               // trials | done | gave | lofs (bmax each) | it | nlist
+  unsigned long long* nt_cnt = nullptr; // [0] iterating, [1] searching, [2] listed intervals, [3] spare
+  double* nt_hist = nullptr;
+  size_t nt_hist_cap = 0;
+  double* nt_ring = nullptr;
+  size_t nt_ring_cap = 0;
+  unsigned mesh_version = 0;            // bumped whenever the mesh or the BABD schedule changes
+  std::vector<double> graph_key;        // what the captured graphs were made for
+  Graph g_dir, g_trial, g_acc;
+  bool capturing = false;               // a Newton graph is being captured
+  Graph* cap_prof = nullptr;            // the graph being captured with profiling on
+  unsigned long long* stamps = nullptr; // kStampSlots per graph, written by colloc_stamp inside the graphs
+  double stamp_khz = 0.0;               // the device's wall clock
   std::vector<size_t> lev_obase;      // offset of the level in cr_oinst
   std::vector<size_t> lev_sbase;      // offset of the level in cr_seg
   std::vector<size_t> lev_rbase;      // row offset of the level's input rows in rows0 (one more: the last output)
@@ -189,12 +240,16 @@ struct cs_handle {
   double* yd() const { return state + st_yd; }
   double* zt() const { return state + st_z; }
   double* p() const { return state + st_p; }
-  Batch batch(const int* mask) const { return Batch{iinst, ninst, noff, mask, B, nullptr, 0}; }
+  Batch batch(const int* mask) const { return Batch{iinst, ninst, noff, mask, B, nullptr, 0, nullptr}; }
   // with the launch list of the instances still iterating, when one is set
+  // (a device-built list: every interval is launched, the work units past
+  // the list's length return at once)
   Batch batch_listed(const int* mask) const {
-    return Batch{iinst, ninst, noff, mask, B, nalist ? alist : nullptr, nalist};
+    if (nalistp) return Batch{iinst, ninst, noff, mask, B, alist, 0, nalistp};
+    return Batch{iinst, ninst, noff, mask, B, nalist ? alist : nullptr, nalist, nullptr};
   }
-  int launch_intervals() const { return nalist ? nalist : nint; }
+  int launch_intervals() const { return nalistp ? nint : (nalist ? nalist : nint); }
+  bool listed() const { return nalistp || nalist > 0; }
 };
 
 namespace {
@@ -210,19 +265,44 @@ hipEvent_t take_event(cs_handle* h) {
   return e;
 }
 
+// The device's wall clock (constant rate, stamp_khz) into slot p[0]: a
+// captured graph's kernel groups are timed between two of these (an
+// event record inside a graph does not give elapsed times reliably).
+__global__ void colloc_stamp(unsigned long long* p) {
+  if (threadIdx.x == 0) p[0] = wall_clock64();
+}
+
+int stamp(cs_handle* h, int slot) {
+  unsigned long long* p = h->stamps + slot;
+  hipLaunchKernelGGL(colloc_stamp, dim3(1), dim3(64), 0, h->stream, p);
+  HIPCHK(hipGetLastError());
+  return CS_OK;
+}
+
+// Times the launches made in its scope (kernel group k): HIP events around
+// them, or inside a graph capture two wall-clock stamps read after each
+// replay.
 struct ProfScope {
   cs_handle* h;
   int k;
   hipEvent_t a = nullptr, b = nullptr;
+  int s0 = -1;
   ProfScope(cs_handle* h_, int k_) : h(h_), k(k_) {
-    if (h->prof) {
-      a = take_event(h);
-      b = take_event(h);
-      hipEventRecord(a, h->stream);
+    if (!h->prof) return;
+    if (h->cap_prof) {
+      Graph* g = h->cap_prof;
+      if (g->used + 2 <= kStampSlots && stamp(h, g->base + g->used) == CS_OK) s0 = g->used++;
+      return;
     }
+    a = take_event(h);
+    b = take_event(h);
+    hipEventRecord(a, h->stream);
   }
   ~ProfScope() {
-    if (a) {
+    if (s0 >= 0) {
+      Graph* g = h->cap_prof;
+      if (stamp(h, g->base + g->used) == CS_OK) g->prof.push_back({k, s0, g->used++});
+    } else if (a) {
       hipEventRecord(b, h->stream);
       h->pending[k].push_back({a, b});
     }
@@ -323,6 +403,8 @@ int residual(cs_handle* h, const double* st, double alpha, bool full, const int*
   return read_slots(h, h->scal, norms);
 }
 
+int jacobian_static(cs_handle* h);
+
 int jacobian(cs_handle* h, double alpha, const int* mask) {
   ProfScope ps(h, CS_KERNEL_JACOBIAN);
   int nint = h->nint;
@@ -334,6 +416,11 @@ int jacobian(cs_handle* h, double alpha, const int* mask) {
   unsigned nthreads = (unsigned)(h->launch_intervals() * h->m + h->B);
   // the FP32 direction stores the stage Jacobians in float32 (in the same buffer)
   h->jac_bits = h->dir_bits;
+  TRY(jacobian_static(h));
+  return launch(h, h->jac_bits == 32 ? F_JAC32 : F_JAC, (nthreads + 255) / 256 * (unsigned)h->jac_parts, 256, args);
+}
+
+int jacobian_static(cs_handle* h) {
   if (h->jac_static_bits != h->jac_bits) {
     // the static entries, once for the whole buffer in this stored type (the
     // other type's layout overwrote them)
@@ -343,7 +430,7 @@ int jacobian(cs_handle* h, double alpha, const int* mask) {
     TRY(launch(h, h->jac_bits == 32 ? F_JAC_STATIC32 : F_JAC_STATIC, blocks, 256, sargs));
     h->jac_static_bits = h->jac_bits;
   }
-  return launch(h, h->jac_bits == 32 ? F_JAC32 : F_JAC, (nthreads + 255) / 256 * (unsigned)h->jac_parts, 256, args);
+  return CS_OK;
 }
 
 // Condensation: the series kernel handles every interval whose stage system
@@ -355,7 +442,11 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
   h->cond_intervals += (unsigned long long)ncond;
   int nint = h->nint;
   unsigned long long* counters = h->scal + 2 * h->bmax;   // [intervals left to Gauss-Jordan, series terms]
-  TRY(fill_u64(h, counters, 2));
+  // captured in a graph: the counters add up over the whole Newton solve
+  // and the Gauss-Jordan kernel is always launched (its workgroups return
+  // for the intervals the series condensed)
+  const bool in_graph = h->capturing || h->nalistp != nullptr;
+  if (!in_graph) TRY(fill_u64(h, counters, 2));
   Batch bt = h->batch_listed(mask);
   void* args_ns[] = {&nint, &bt, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &h->fbk, &counters};
   // the arithmetic follows the stored type of the stage Jacobians, and so
@@ -366,16 +457,19 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
     TRY(launch(h, F_COND_MF32, (unsigned)h->launch_intervals(), (unsigned)(h->m - 1) * 64, args_ns));
   else
     TRY(launch(h, f32 ? F_COND_NS32 : F_COND_NS, (unsigned)h->launch_intervals(), kNsThreads, args_ns));
-  unsigned long long cnt[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  h->cond_fallback += cnt[0];
-  h->cond_terms += cnt[1];
-  const unsigned long long nleft = cnt[0];
-  if (nleft == 0) return CS_OK;
+  if (!in_graph) {
+    unsigned long long cnt[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->cond_fallback += cnt[0];
+    h->cond_terms += cnt[1];
+    if (cnt[0] == 0) return CS_OK;
+  }
   const int* only = h->fbk;
   const int* ilist = bt.ilist;
-  void* args[] = {&nint, &h->iinst, &ilist, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &only};
+  const int* nlistp = bt.nlistp;
+  void* args[] = {&nint, &h->iinst, &ilist, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &only,
+                  &nlistp};
   return launch(h, f32 ? F_COND32 : F_COND, (unsigned)h->launch_intervals(), (unsigned)(h->cw * 64), args);
 }
 
@@ -385,6 +479,7 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
 // instance by instance, so after the last level instance b's row is row b.
 int build_cr_schedule(cs_handle* h) {
   const int B = h->B;
+  ++h->mesh_version;
   std::vector<int> n(B), in_off(B);
   for (int b = 0; b < B; ++b) {
     n[b] = h->hnoff[b + 1] - h->hnoff[b] - 1;
@@ -517,10 +612,10 @@ Seg4 state_segs(cs_handle* h) {
 // out = x + steps[b] d per instance b (steps on the device).
 int axpy_state(cs_handle* h, const double* x, const double* d, double* out) {
   Seg4 s = state_segs(h);
-  if (h->nalist > 0) {   // a launch list is set: just the listed intervals' entries
+  if (h->listed()) {   // a launch list is set: just the listed intervals' entries
     Batch bt = h->batch_listed(nullptr);
     void* args[] = {&s, &bt, &x, &d, &h->steps, &out};
-    return launch(h, F_AXPY_LISTED, (unsigned)h->nalist, 64, args);
+    return launch(h, F_AXPY_LISTED, (unsigned)h->launch_intervals(), 64, args);
   }
   long long total = s.len[0] + s.len[1] + s.len[2] + s.len[3];
   Batch bt = h->batch(nullptr);
@@ -648,6 +743,23 @@ int cs_create(const cs_dims* d, cs_handle** out) {
   I(&h->fbk, nint);
   I(&h->alist_buf, nint);
   I(&h->slist_buf, nint);
+  I(&h->dlist, nint);
+  A(&h->nt_nrm, bm);
+  A(&h->nt_taken, bm);
+  I(&h->nt_ints, 4 * bm + 2);
+  if (r == CS_OK) {
+    hipError_t e = hipMalloc((void**)&h->nt_cnt, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&h->stamps, 3 * kStampSlots * sizeof(unsigned long long));
+    int dev = 0, khz = 0;
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+    if (e == hipSuccess && khz <= 0) e = hipErrorInvalidValue;
+    if (e != hipSuccess) r = fail(CS_E_HIP, "allocation failed: %s", hipGetErrorString(e));
+    h->stamp_khz = (double)khz;
+    h->g_dir.base = 0;
+    h->g_trial.base = kStampSlots;
+    h->g_acc.base = 2 * kStampSlots;
+  }
   if (r == CS_OK) {
     hipError_t e = hipMalloc((void**)&h->scal, (2 * bm + 2) * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&h->status, sizeof(int));
@@ -674,11 +786,17 @@ int cs_create(const cs_dims* d, cs_handle** out) {
 void cs_destroy(cs_handle* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
+  for (Graph* g : {&h->g_dir, &h->g_trial, &h->g_acc})
+    if (g->exec) hipGraphExecDestroy(g->exec);
+  if (h->stamps) hipFree(h->stamps);
   double* bufs[] = {h->t, h->state, h->trial, h->dir, h->c, h->ytil, h->fa, h->fb, h->rbc, h->jac, h->Dr,
-                    h->rows0, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps};
+                    h->rows0, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps, h->nt_nrm, h->nt_taken,
+                    h->nt_hist, h->nt_ring};
   for (double* b : bufs)
     if (b) hipFree(b);
-  int* ibufs[] = {h->iinst, h->ninst, h->noff, h->act, h->srch, h->fbk, h->alist_buf, h->slist_buf, h->cr_oinst};
+  int* ibufs[] = {h->iinst, h->ninst, h->noff, h->act, h->srch, h->fbk, h->alist_buf, h->slist_buf, h->cr_oinst,
+                  h->dlist, h->nt_ints};
+  if (h->nt_cnt) hipFree(h->nt_cnt);
   for (int* b : ibufs)
     if (b) hipFree(b);
   if (h->cr_seg) hipFree(h->cr_seg);
@@ -692,8 +810,22 @@ void cs_destroy(cs_handle* h) {
   delete h;
 }
 
+// The captured Newton graphs name the module's kernels and the buffers of
+// the moment: they go with the module.
+static void drop_graphs(cs_handle* h) {
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (Graph* g : {&h->g_dir, &h->g_trial, &h->g_acc}) {
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    g->exec = nullptr;
+    g->prof.clear();
+    g->used = 0;
+  }
+  h->graph_key.clear();
+}
+
 int cs_load_module(cs_handle* h, const void* code, size_t size) {
   if (!h || !code || size == 0) return fail(CS_E_ARG, "null argument");
+  drop_graphs(h);
   if (h->module) {
     hipModuleUnload(h->module);
     h->module = nullptr;
@@ -845,6 +977,7 @@ int set_batch_mesh(cs_handle* h, int B, const int* nodes, const double* t) {
   h->nint = nint;
   h->hnoff = noff;
   h->init_N = 0;
+  ++h->mesh_version;
   TRY(copy_in(h, h->t, t, (size_t)N));
   TRY(upload_ints(h, h->noff, noff));
   TRY(upload_ints(h, h->iinst, iinst));
@@ -929,8 +1062,13 @@ struct LaunchList {
 // by less than a factor min_ratio over the last `window` iterations stops
 // there (reported with its loop counter at that point and not converged);
 // off for the reference's loop (patience = 0).
+int newton_graph(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time, double* norm, double* hist,
+                 int hist_len, int patience, double min_ratio, int window);
+
 int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time, double* norm, double* hist,
            int hist_len, int patience = 0, double min_ratio = 0.0, int window = 1) {
+  if (h->use_graph) return newton_graph(h, alpha, tol, max_iter, iter_time, norm, hist, hist_len, patience, min_ratio,
+                                        window);
   const int B = h->B;
   std::vector<double> nrm(B, 0.0), nn(B, 0.0), step(B, 0.0), taken(B, 0.0);
   // the last window + 1 norms of each instance (ring), for the progress test
@@ -1016,6 +1154,225 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
     if (norm) norm[b] = nrm[b];
   }
   return sync(h);
+}
+// ------------------------------------------- device-resident Newton loop
+// The same loop as newton() (identical arithmetic and decisions) with every
+// per-instance decision made by kernels (colloc_nt_*) and the launch list
+// of the iterating instances' intervals built on the device; the work of
+// an iteration is captured once per mesh and settings as three HIP graphs:
+//   direction: residual of the iterate, the convergence / progress check,
+//     the launch list, Jacobian, condensation, BABD, recovery, and the
+//     first trial step of the line search;
+//   trial: one more trial step (replayed while instances are searching);
+//   accept: the accepted steps added to the state, the iteration counter.
+// The host reads three counters per replay of the direction graph (and per
+// extra trial) instead of synchronising for every residual and the
+// condensation's counters.
+NtState nt_state(cs_handle* h, double tol, int max_iter, int patience, double min_ratio, int window, int hl) {
+  const size_t bm = (size_t)h->bmax;
+  NtState st;
+  st.nrm = h->nt_nrm;
+  st.taken = h->nt_taken;
+  st.trials = h->nt_ints;
+  st.done = h->nt_ints + bm;
+  st.gave = h->nt_ints + 2 * bm;
+  st.lofs = h->nt_ints + 3 * bm;
+  st.it = h->nt_ints + 4 * bm;
+  st.nlist = h->nt_ints + 4 * bm + 1;
+  st.hist = h->nt_hist;
+  st.ring = h->nt_ring;
+  st.cnt = h->nt_cnt;
+  st.hl = hl;
+  st.window = window;
+  st.patience = patience;
+  st.max_iter = max_iter;
+  st.max_trials = 20;   // solver:64
+  st.tol = tol;
+  st.min_ratio = min_ratio;
+  return st;
+}
+
+int nt_list(cs_handle* h, const NtState& st) {
+  Batch bt = h->batch(h->act);
+  NtState s2 = st;
+  void* a1[] = {&bt, &s2};
+  TRY(launch(h, F_NT_SCAN, 1, 1024, a1));
+  int nint = h->nint;
+  int* list = h->dlist;
+  void* a2[] = {&nint, &bt, &s2, &list};
+  return launch(h, F_NT_FILL, (unsigned)((nint + 255) / 256), 256, a2);
+}
+
+// One trial step of the line search: trial state, its residual norms, the
+// decision (the still-searching count in cnt[1]).
+int nt_trial(cs_handle* h, const NtState& st, double alpha) {
+  TRY(axpy_state(h, h->state, h->dir, h->trial));
+  TRY(residual_to(h, h->trial, alpha, false, h->srch, h->scal));
+  TRY(fill_u64(h, h->nt_cnt + 1, 1));
+  NtState s2 = st;
+  int B = h->B;
+  unsigned long long* slot = h->scal;
+  void* a[] = {&B, &s2, &slot, &h->srch, &h->steps};
+  return launch(h, F_NT_LS, (unsigned)((B + 255) / 256), 256, a);
+}
+
+int nt_capture(cs_handle* h, Graph& g, const std::function<int()>& body) {
+  if (g.exec) {
+    HIPCHK(hipGraphExecDestroy(g.exec));
+    g.exec = nullptr;
+  }
+  g.prof.clear();
+  g.used = 0;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+  h->capturing = true;
+  h->cap_prof = h->prof ? &g : nullptr;
+  const int r = body();
+  h->cap_prof = nullptr;
+  h->capturing = false;
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  if (r != CS_OK) {
+    if (e == hipSuccess && graph) hipGraphDestroy(graph);
+    return r;
+  }
+  if (e != hipSuccess) return fail(CS_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(CS_E_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+  return CS_OK;
+}
+
+int nt_replay(cs_handle* h, Graph& g, unsigned long long* cnt) {
+  HIPCHK(hipGraphLaunch(g.exec, h->stream));
+  HIPCHK(hipMemcpyAsync(cnt, h->nt_cnt, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
+  unsigned long long ts[kStampSlots];
+  if (!g.prof.empty())
+    HIPCHK(hipMemcpyAsync(ts, h->stamps + g.base, g.used * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (const StampPair& pr : g.prof) {
+    h->prof_ms[pr.k] += (double)(ts[pr.s1] - ts[pr.s0]) / h->stamp_khz;
+    h->prof_n[pr.k] += 1;
+  }
+  return CS_OK;
+}
+
+int newton_graph(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time, double* norm, double* hist,
+                 int hist_len, int patience, double min_ratio, int window) {
+  const int B = h->B;
+  const int hl = std::max(hist_len, 1);
+  const int rs = patience > 0 ? window + 1 : 1;
+  // history and progress ring (grown as needed; outside any graph)
+  if ((size_t)B * hl > h->nt_hist_cap) {
+    if (h->nt_hist) hipFree(h->nt_hist);
+    h->nt_hist = nullptr;
+    h->nt_hist_cap = 0;
+    TRY(dmalloc(&h->nt_hist, (size_t)h->bmax * hl));
+    h->nt_hist_cap = (size_t)h->bmax * hl;
+    h->graph_key.clear();
+  }
+  if ((size_t)B * rs > h->nt_ring_cap) {
+    if (h->nt_ring) hipFree(h->nt_ring);
+    h->nt_ring = nullptr;
+    h->nt_ring_cap = 0;
+    TRY(dmalloc(&h->nt_ring, (size_t)h->bmax * rs));
+    h->nt_ring_cap = (size_t)h->bmax * rs;
+    h->graph_key.clear();
+  }
+  const NtState st = nt_state(h, tol, max_iter, patience, min_ratio, window, hl);
+  // start: every instance iterating, nothing converged, iteration 0, the
+  // list of all intervals; static Jacobian entries in place
+  {
+    std::vector<int> ints(4 * (size_t)h->bmax + 2, 0);
+    for (int b = 0; b < B; ++b) {
+      ints[h->bmax + b] = -1;
+      ints[2 * h->bmax + b] = -1;
+    }
+    HIPCHK(hipMemcpyAsync(h->nt_ints, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    TRY(upload_ints(h, h->act, std::vector<int>(B, 1)));
+  }
+  h->jac_bits = h->dir_bits;
+  TRY(jacobian_static(h));
+  TRY(fill_u64(h, h->scal + 2 * h->bmax, 2));   // condensation counters, summed over the solve
+  TRY(nt_list(h, st));
+  // (re)capture when the mesh or any setting baked into the graphs changed
+  const std::vector<double> key = {(double)h->mesh_version, (double)B, (double)h->N, alpha, tol, (double)max_iter,
+                                   (double)patience, min_ratio, (double)window, (double)hl, (double)h->dir_bits,
+                                   (double)h->cond_kind, (double)h->prof, (double)h->subst_spans,
+                                   (double)(size_t)h->nt_hist, (double)(size_t)h->nt_ring};
+  h->alist = h->dlist;
+  h->nalist = 0;
+  h->nalistp = st.nlist;
+  struct Unlist {
+    cs_handle* h;
+    ~Unlist() { h->alist = nullptr; h->nalist = 0; h->nalistp = nullptr; }
+  } unlist{h};
+  if (key != h->graph_key) {
+    h->graph_key.clear();
+    TRY(nt_capture(h, h->g_dir, [&]() -> int {
+      TRY(fill_u64(h, h->nt_cnt, 4));
+      TRY(residual_to(h, h->state, alpha, true, h->act, h->scal));
+      int Bv = B;
+      NtState s2 = st;
+      unsigned long long* slot = h->scal;
+      void* a[] = {&Bv, &s2, &slot, &h->act, &h->srch, &h->steps};
+      TRY(launch(h, F_NT_CHECK, (unsigned)((B + 255) / 256), 256, a));
+      TRY(nt_list(h, st));
+      TRY(jacobian(h, alpha, h->act));
+      TRY(condense(h, h->act, 0));
+      TRY(babd(h, h->act));
+      TRY(recover(h, h->act));
+      return nt_trial(h, st, alpha);
+    }));
+    TRY(nt_capture(h, h->g_trial, [&]() -> int { return nt_trial(h, st, alpha); }));
+    TRY(nt_capture(h, h->g_acc, [&]() -> int {
+      int Bv = B;
+      NtState s2 = st;
+      void* a[] = {&Bv, &s2, &h->act, &h->steps};
+      TRY(launch(h, F_NT_ACCEPT, (unsigned)((B + 255) / 256), 256, a));
+      TRY(axpy_state(h, h->state, h->dir, h->state));
+      void* t[] = {&s2};
+      return launch(h, F_NT_TICK, 1, 64, t);
+    }));
+    h->graph_key = key;
+  }
+  unsigned long long cnt[3] = {0, 0, 0};
+  for (int it = 0; it < max_iter; ++it) {
+    TRY(nt_replay(h, h->g_dir, cnt));
+    if (cnt[0] == 0) break;
+    h->cond_intervals += cnt[2];
+    while (cnt[1] > 0) TRY(nt_replay(h, h->g_trial, cnt));
+    HIPCHK(hipGraphLaunch(h->g_acc.exec, h->stream));
+  }
+  // results, as newton() reports them
+  std::vector<int> ints(4 * (size_t)h->bmax + 2);
+  std::vector<double> nrm(B);
+  std::vector<unsigned long long> ctr(2);
+  HIPCHK(hipMemcpyAsync(ints.data(), h->nt_ints, ints.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(nrm.data(), h->nt_nrm, B * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(ctr.data(), h->scal + 2 * h->bmax, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        h->stream));
+  std::vector<double> dh;
+  if (hist && hist_len > 0) {
+    dh.resize((size_t)B * hl);
+    HIPCHK(hipMemcpyAsync(dh.data(), h->nt_hist, dh.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->cond_fallback += ctr[0];
+  h->cond_terms += ctr[1];
+  const int last_all = std::min(max_iter, hist_len) - 1;
+  for (int b = 0; b < B; ++b) {
+    const int done = ints[h->bmax + b], gave = ints[2 * h->bmax + b];
+    if (iter_time) iter_time[b] = done >= 0 ? done : (gave >= 0 ? gave : max_iter - 1);
+    if (norm) norm[b] = nrm[b];
+    if (!dh.empty()) {
+      // the entries newton() writes: up to the stopping iteration, or the
+      // whole budget (run out, or a stall continued at the same norm)
+      const int last = done >= 0 ? std::min(done, hist_len - 1) : (gave >= 0 ? std::min(gave, hist_len - 1) : last_all);
+      for (int k = 0; k <= last; ++k) hist[(size_t)b * hist_len + k] = dh[(size_t)b * hl + k];
+    }
+  }
+  return CS_OK;
 }
 }  // namespace
 
@@ -1207,6 +1564,12 @@ int cs_set_direction_precision(cs_handle* h, int bits) {
   if (!h) return fail(CS_E_ARG, "null handle");
   if (bits != 32 && bits != 64) return fail(CS_E_ARG, "precision %d bits (32 or 64)", bits);
   h->dir_bits = bits;
+  return CS_OK;
+}
+
+int cs_set_newton_graph(cs_handle* h, int on) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  h->use_graph = on != 0;
   return CS_OK;
 }
 

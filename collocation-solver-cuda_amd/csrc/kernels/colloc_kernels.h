@@ -72,7 +72,10 @@ constexpr int NRB = NY + NP;                       // one instance's r_bc
 // kernel works on; the others are skipped and their outputs left as they are.
 // ilist (may be null) lists the intervals a per-interval or per-stage kernel
 // is launched over (nlist of them, those of the instances still iterating),
-// so that converged instances cost no workgroups; null: all intervals.
+// so that converged instances cost no workgroups; null: all intervals.  When
+// the list is built on the device (the graph-replayed Newton iteration), its
+// length is read from *nlistp and the launch covers every interval, the
+// work units past the list returning at once.
 struct Batch {
   const int* iinst;
   const int* ninst;
@@ -81,11 +84,18 @@ struct Batch {
   int B;
   const int* ilist;
   int nlist;
+  const int* nlistp;
 };
 __device__ __forceinline__ bool skipped(const Batch& bt, int b) { return bt.act && !bt.act[b]; }
 // Intervals a launch covers and the interval of its k-th work unit.
-__device__ __forceinline__ int launch_intervals(const Batch& bt, int nint) { return bt.ilist ? bt.nlist : nint; }
+__device__ __forceinline__ int launch_intervals(const Batch& bt, int nint) {
+  return bt.ilist ? (bt.nlistp ? *bt.nlistp : bt.nlist) : nint;
+}
 __device__ __forceinline__ int listed_interval(const Batch& bt, int k) { return bt.ilist ? bt.ilist[k] : k; }
+// A workgroup-per-interval kernel's block past the launch list.
+__device__ __forceinline__ bool beyond_list(const Batch& bt, int nint) {
+  return (int)blockIdx.x >= launch_intervals(bt, nint);
+}
 
 // One CR level of one instance: its rows start at in_off of the input
 // buffer, its ceil(nrows / 2) output rows at out_off, its nrows / 2 records
@@ -784,9 +794,11 @@ __device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst
                          const double* __restrict__ t, const TJ* __restrict__ jac,
                          const double* __restrict__ fa, const double* __restrict__ fb,
                          double* __restrict__ rows, TJ* __restrict__ rec, int* __restrict__ status,
-                         const int* __restrict__ only) {
-  // workgroup k condenses interval ilist[k] (k without a list); only !=
-  // nullptr: just the intervals the series kernel left (only[i] != 0)
+                         const int* __restrict__ only, const int* __restrict__ nlistp) {
+  // workgroup k condenses interval ilist[k] (k without a list; *nlistp, when
+  // given, is the list's length); only != nullptr: just the intervals the
+  // series kernel left (only[i] != 0)
+  if (ilist && nlistp && (int)blockIdx.x >= *nlistp) return;
   const int i = ilist ? ilist[blockIdx.x] : blockIdx.x;
   if (only && !only[i]) return;
   using Solver = GJB<NS, NS + NVA, CW, 16>;
@@ -1756,6 +1768,7 @@ __device__ __forceinline__ void axpy4(Seg4 s, Batch bt, const double* x, const d
 // right node and p.  Instances with steps[b] == 0 are left unchanged.
 __device__ __forceinline__ void axpy_listed(Seg4 s, Batch bt, const double* x, const double* __restrict__ d,
                                             const double* __restrict__ steps, double* out) {
+  if ((int)blockIdx.x >= (bt.nlistp ? *bt.nlistp : bt.nlist)) return;
   const int i = bt.ilist[blockIdx.x];
   const int b = bt.iinst[i];
   const double a = steps[b];
@@ -1940,11 +1953,182 @@ __device__ __forceinline__ void recover_z(int nnodes, Batch bt, const double* __
   }
 }
 
+// ------------------------------------------- device-resident Newton loop
+// The per-instance decisions of the damped Newton loop (solver:109-174) as
+// kernels, so that one iteration of every instance of a batch runs from
+// device memory without a host round trip (replayed as a HIP graph).
+// nrm[b]: ||f_q||_inf of instance b's iterate; taken[b]: its accepted step
+// of this iteration (0: none yet); trials[b]: failed trial steps;
+// done / gave: the iteration at which it converged / the progress test
+// stopped it (-1: not); hist[b][hl]: norm history; ring[b][window + 1]: the
+// last norms for the progress test; *it: the iteration; cnt: [0] instances
+// still iterating after the check, [1] still searching after a trial, [2]
+// intervals in the launch list.
+struct NtState {
+  double* nrm;
+  double* taken;
+  int* trials;
+  int* done;
+  int* gave;
+  double* hist;
+  double* ring;
+  int* it;
+  unsigned long long* cnt;
+  int* nlist;
+  int* lofs;
+  int hl, window, patience, max_iter, max_trials;
+  double tol, min_ratio;
+};
+
+// After the residual of the iterate (norms in slot[b]): record it, stop the
+// instances that converged (||f_q|| < tol, before the step, solver:116-120)
+// or that the progress test stops, and start the line search of the others
+// at step 1.
+__device__ __forceinline__ void nt_check(int B, NtState st, const unsigned long long* __restrict__ slot, int* act,
+                                         int* srch, double* steps) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  if (!act[b]) {
+    srch[b] = 0;
+    steps[b] = 0.0;
+    return;
+  }
+  const int it = *st.it;
+  const double cur = __longlong_as_double((long long)slot[b]);
+  st.nrm[b] = cur;
+  if (it < st.hl) st.hist[(size_t)b * st.hl + it] = cur;
+  const int rs = st.patience > 0 ? st.window + 1 : 1;
+  bool on = true;
+  if (cur < st.tol) {
+    on = false;
+    st.done[b] = it;
+  } else if (st.patience > 0 && it >= st.patience && it >= st.window &&
+             !(cur * st.min_ratio <= st.ring[(size_t)b * rs + (it - st.window) % rs])) {
+    on = false;
+    st.gave[b] = it;
+  }
+  st.ring[(size_t)b * rs + it % rs] = cur;
+  act[b] = on ? 1 : 0;
+  srch[b] = on ? 1 : 0;
+  steps[b] = on ? 1.0 : 0.0;
+  st.taken[b] = 0.0;
+  st.trials[b] = 0;
+  if (on) {
+    atomicAdd(st.cnt, 1ull);
+    atomicAdd(st.cnt + 1, 1ull);
+  }
+}
+
+// Launch list of the intervals of the instances still iterating: one
+// workgroup scans the instances' interval counts (lofs[b]: the first slot
+// of instance b), then every interval of an active instance takes its slot.
+constexpr int NT_SCAN = 1024;
+__device__ __forceinline__ void nt_list_scan(Batch bt, NtState st) {
+  __shared__ int part[NT_SCAN];
+  const int tid = threadIdx.x;
+  const int per = (bt.B + NT_SCAN - 1) / NT_SCAN;
+  const int b0 = tid * per;
+  int sum = 0;
+  for (int b = b0; b < b0 + per && b < bt.B; ++b)
+    if (bt.act[b]) sum += bt.noff[b + 1] - bt.noff[b] - 1;
+  part[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < NT_SCAN; off <<= 1) {   // inclusive scan (Hillis-Steele)
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - sum;
+  for (int b = b0; b < b0 + per && b < bt.B; ++b) {
+    st.lofs[b] = run;
+    if (bt.act[b]) run += bt.noff[b + 1] - bt.noff[b] - 1;
+  }
+  if (tid == NT_SCAN - 1) {
+    *st.nlist = part[tid];
+    st.cnt[2] = (unsigned long long)part[tid];
+  }
+}
+
+__device__ __forceinline__ void nt_list_fill(int nint, Batch bt, NtState st, int* list) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nint) return;
+  const int b = bt.iinst[i];
+  if (bt.act[b]) list[st.lofs[b] + (i - (bt.noff[b] - b))] = i;
+}
+
+// After a trial step's residual (norms in slot[b]): the first step that
+// strictly decreases the norm is taken (solver:155-174); otherwise the step
+// is halved, and after max_trials failures the search ends without a step.
+__device__ __forceinline__ void nt_line_search(int B, NtState st, const unsigned long long* __restrict__ slot,
+                                               int* srch, double* steps) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B || !srch[b]) return;
+  const double cur = __longlong_as_double((long long)slot[b]);
+  if (cur < st.nrm[b]) {
+    st.taken[b] = steps[b];
+    srch[b] = 0;
+    steps[b] = 0.0;
+    return;
+  }
+  const int tr = st.trials[b] + 1;
+  st.trials[b] = tr;
+  if (tr >= st.max_trials) {
+    srch[b] = 0;
+    steps[b] = 0.0;
+    return;
+  }
+  steps[b] *= 0.5;
+  atomicAdd(st.cnt + 1, 1ull);
+}
+
+// End of the line search: the accepted step becomes the state update's
+// step; an instance that took none keeps its state, so every later
+// iteration would repeat this one: it stops here, its history continued at
+// the same norm (the host loop's stall rule).
+__device__ __forceinline__ void nt_accept(int B, NtState st, int* act, double* steps) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  if (!act[b]) {
+    steps[b] = 0.0;
+    return;
+  }
+  const double tk = st.taken[b];
+  steps[b] = tk;
+  if (tk == 0.0) {
+    act[b] = 0;
+    const int it = *st.it;
+    const int end = st.max_iter < st.hl ? st.max_iter : st.hl;
+    for (int k = it + 1; k < end; ++k) st.hist[(size_t)b * st.hl + k] = st.nrm[b];
+  }
+}
+
 }  // namespace colloc
 
 // ------------------------------------------------------------ entry points
 extern "C" {
 __global__ void colloc_fill_u64(unsigned long long* p, unsigned long long v, int n) { colloc::fill_u64(p, v, n); }
+
+__global__ __launch_bounds__(256) void colloc_nt_check(int B, colloc::NtState st, const unsigned long long* slot,
+    int* act, int* srch, double* steps) {
+  colloc::nt_check(B, st, slot, act, srch, steps);
+}
+__global__ __launch_bounds__(colloc::NT_SCAN) void colloc_nt_list_scan(colloc::Batch bt, colloc::NtState st) {
+  colloc::nt_list_scan(bt, st);
+}
+__global__ __launch_bounds__(256) void colloc_nt_list_fill(int nint, colloc::Batch bt, colloc::NtState st, int* list) {
+  colloc::nt_list_fill(nint, bt, st, list);
+}
+__global__ __launch_bounds__(256) void colloc_nt_line_search(int B, colloc::NtState st,
+    const unsigned long long* slot, int* srch, double* steps) {
+  colloc::nt_line_search(B, st, slot, srch, steps);
+}
+__global__ __launch_bounds__(256) void colloc_nt_accept(int B, colloc::NtState st, int* act, double* steps) {
+  colloc::nt_accept(B, st, act, steps);
+}
+__global__ void colloc_nt_tick(colloc::NtState st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *st.it += 1;
+}
 
 __global__ __launch_bounds__(256) void colloc_residual_full(int nint, colloc::Batch bt, const double* t,
     const double* y, const double* ydot, const double* ztil, const double* p, const double* c, double alpha,
@@ -1976,18 +2160,19 @@ __global__ __launch_bounds__(256) void colloc_jacobian_static32(long long nst, f
 
 __global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const int* iinst, const int* ilist,
     const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
-    const int* only) {
-  colloc::condense<Cfg::CW, double>(nint, iinst, ilist, t, jac, fa, fb, rows, rec, status, only);
+    const int* only, const int* nlistp) {
+  colloc::condense<Cfg::CW, double>(nint, iinst, ilist, t, jac, fa, fb, rows, rec, status, only, nlistp);
 }
 // the float64 Gauss-Jordan fallback of the FP32 direction (float32 stage Jacobians)
 __global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense32(int nint, const int* iinst, const int* ilist,
     const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
-    const int* only) {
-  colloc::condense<Cfg::CW, float>(nint, iinst, ilist, t, jac, fa, fb, rows, rec, status, only);
+    const int* only, const int* nlistp) {
+  colloc::condense<Cfg::CW, float>(nint, iinst, ilist, t, jac, fa, fb, rows, rec, status, only, nlistp);
 }
 __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nint, colloc::Batch bt,
     const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
     int* fallback, unsigned long long* counters) {
+  if (colloc::beyond_list(bt, nint)) return;
   if constexpr (colloc::NS_OK) {
     colloc::condense_ns<double>(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
   } else if (threadIdx.x == 0) {
@@ -2004,6 +2189,7 @@ __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nin
 __global__ __launch_bounds__(colloc::MF_THREADS) void colloc_condense_mf32(int nint, colloc::Batch bt,
     const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
     int* fallback, unsigned long long* counters) {
+  if (colloc::beyond_list(bt, nint)) return;
   if constexpr (colloc::MF_OK) {
     colloc::condense_mf32(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
   } else if (threadIdx.x == 0) {
@@ -2025,6 +2211,7 @@ __device__ int colloc_caps[4] = {colloc::MF_OK ? 1 : 0, colloc::CR_THREADS, coll
 __global__ __launch_bounds__(colloc::NS_THREADS, 4) void colloc_condense_ns32(int nint, colloc::Batch bt,
     const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
     int* fallback, unsigned long long* counters) {
+  if (colloc::beyond_list(bt, nint)) return;
   if constexpr (colloc::ns_ok<float>()) {
     colloc::condense_ns<float>(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
   } else if (threadIdx.x == 0) {
@@ -2057,11 +2244,13 @@ __global__ __launch_bounds__(64) void colloc_cr_back(int level, colloc::Batch bt
 
 __global__ __launch_bounds__(256) void colloc_recover(int nint, colloc::Batch bt, const double* rec,
     const double* dy, const double* dp, double* dydot, double* dz) {
+  if (colloc::beyond_list(bt, nint)) return;
   colloc::recover<double>(nint, bt, rec, dy, dp, dydot, dz);
 }
 // condensation records stored in float32 (the FP32 direction)
 __global__ __launch_bounds__(256) void colloc_recover32(int nint, colloc::Batch bt, const float* rec,
     const double* dy, const double* dp, double* dydot, double* dz) {
+  if (colloc::beyond_list(bt, nint)) return;
   colloc::recover<float>(nint, bt, rec, dy, dp, dydot, dz);
 }
 

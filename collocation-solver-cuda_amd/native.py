@@ -46,6 +46,7 @@ SIGNATURES = {
     "cs_set_babd_substitution": (c_int, [c_void_p, c_int]),
     "cs_set_direction_precision": (c_int, [c_void_p, c_int]),
     "cs_set_condense_kernel": (c_int, [c_void_p, c_int]),
+    "cs_set_newton_graph": (c_int, [c_void_p, c_int]),
     "cs_newton": (c_int, [c_void_p, c_double, c_double, c_int, P_I, P_D, P_D, c_int]),
     "cs_newton_progress": (c_int, [c_void_p, c_double, c_double, c_int, c_int, c_double, P_I, P_D, P_D, c_int]),
     "cs_newton_stall": (c_int, [c_void_p, c_double, c_double, c_int, c_int, c_int, c_double, P_I, P_D, P_D, c_int]),

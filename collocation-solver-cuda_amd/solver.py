@@ -274,6 +274,11 @@ class CollocationSolver:
         convergence test stay float64)."""
         check(lib().cs_set_direction_precision(self.h, int(bits)))
 
+    def set_newton_graph(self, on):
+        """Run the Newton loops device resident, one HIP-graph replay per
+        iteration (the same results as the host-driven loop)."""
+        check(lib().cs_set_newton_graph(self.h, 1 if on else 0))
+
     def set_condense_kernel(self, kind):
         """Series condensation kernel of the FP32 direction: 0 matrix cores
         (default, where the stage blocks fit), 1 vector ALU."""

@@ -176,6 +176,17 @@ int cs_solve_babd(cs_handle* h, double* delta_y, double* delta_p);
  * Jacobians widened to float64.  No counterpart in the reference. */
 int cs_set_direction_precision(cs_handle* h, int bits);
 
+/* Newton loops (cs_newton, cs_newton_progress, cs_newton_stall) run device
+ * resident when on != 0: the per-instance decisions of solver:109-174 (the
+ * convergence test, the progress / stall tests, the line search's step
+ * choice and acceptance) are made by kernels, the launch list of the
+ * iterating instances' intervals is built on the device, and each iteration
+ * is a replay of HIP graphs captured once per mesh and settings (the host
+ * reads three counters per iteration instead of synchronising for every
+ * residual).  The results are those of the host-driven loop (on = 0, the
+ * default).  No counterpart in the reference. */
+int cs_set_newton_graph(cs_handle* h, int on);
+
 /* Kernel of the series condensation in the FP32 direction: 0 (default) the
  * products of the series on matrix cores (v_mfma_f32_16x16x4f32, one wave
  * per stage block) where the problem's stage blocks fit (ny <= 48 and
