@@ -55,8 +55,12 @@ struct Batch {
   int nlist;
   const int* nlistp;
 };
-// Per-instance state of the device-resident Newton loop (struct NtState of
-// the kernels).
+// Per-instance state of the device-resident Newton loop and its settings
+// (structs NtState and NtParams of the kernels).
+struct NtParams {
+  double tol, min_ratio;
+  int hl, window, patience, max_iter, max_trials, pad;
+};
 struct NtState {
   double* nrm;
   double* taken;
@@ -69,8 +73,7 @@ struct NtState {
   unsigned long long* cnt;
   int* nlist;
   int* lofs;
-  int hl, window, patience, max_iter, max_trials;
-  double tol, min_ratio;
+  const NtParams* par;
 };
 // A captured Newton phase and the timed kernel groups inside it: group
 // (kernel id, first stamp slot, last stamp slot) of the graph's slots.
@@ -104,6 +107,9 @@ const char* kFnNames[F_COUNT] = {
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
+// workgroups of the Gauss-Jordan condensation inside a captured Newton
+// iteration (it walks the queue the series kernel filled; a few per CU)
+constexpr int kGaussJordanGrid = 1024;
 
 // Per-instance residual-norm slots of cs_compute_f_q_noalloc: a fixed device
 // array of this library (64 KiB on every GPU it runs on, held until the
@@ -176,7 +182,7 @@ struct cs_handle {
   int* noff = nullptr;       // node offsets (B + 1)
   int* act = nullptr;        // instances still iterating
   int* srch = nullptr;       // instances still line-searching
-  int* fbk = nullptr;        // per interval: 1 if the series condensation left it to Gauss-Jordan
+  int* fbk = nullptr;        // the intervals the series condensation left to Gauss-Jordan (a queue)
   int* alist = nullptr;      // the current launch list of intervals (one of the two buffers below)
   int nalist = 0;            // its length; 0: launch over all intervals
   int* alist_buf = nullptr;  // intervals of the instances still iterating
@@ -207,6 +213,8 @@ struct cs_handle {
   double* nt_taken = nullptr;
   int* nt_ints = nullptr;               // trials | done | gave | lofs (bmax each) | it | nlist
   unsigned long long* nt_cnt = nullptr; // [0] iterating, [1] searching, [2] listed intervals, [3] spare
+  NtParams* nt_par = nullptr;           // the settings of the current solve (device)
+  NtParams nt_par_host = {};            // their host copy (the source of the upload)
   double* nt_hist = nullptr;
   size_t nt_hist_cap = 0;
   double* nt_ring = nullptr;
@@ -225,7 +233,8 @@ struct cs_handle {
   unsigned long long cond_terms = 0;      // series terms summed over intervals since creation
   unsigned long long cond_intervals = 0;  // intervals condensed since creation
   // [0, bmax): residual norms; [bmax, 2 bmax): segment-residual maxima;
-  // [2 bmax, 2 bmax + 2): condensation counters
+  // [2 bmax, 2 bmax + 3): condensation counters (queue length, series
+  // terms, intervals queued for Gauss-Jordan)
   unsigned long long* scal = nullptr;
   int* status = nullptr;
   unsigned long long* scratch_slots = nullptr;   // this device's `scratch` (cs_compute_f_q_noalloc)
@@ -434,19 +443,19 @@ int jacobian_static(cs_handle* h) {
 }
 
 // Condensation: the series kernel handles every interval whose stage system
-// converges as a Neumann series (fine meshes) and counts the others; when
-// there are any, the Gauss-Jordan kernel redoes them (a block per interval,
-// the converged ones exit at once).
+// converges as a Neumann series (fine meshes) and queues the others; the
+// Gauss-Jordan kernel redoes the queued ones.
 int condense(cs_handle* h, const int* mask, long long ncond) {
   ProfScope ps(h, CS_KERNEL_CONDENSE);
   h->cond_intervals += (unsigned long long)ncond;
   int nint = h->nint;
-  unsigned long long* counters = h->scal + 2 * h->bmax;   // [intervals left to Gauss-Jordan, series terms]
-  // captured in a graph: the counters add up over the whole Newton solve
-  // and the Gauss-Jordan kernel is always launched (its workgroups return
-  // for the intervals the series condensed)
+  // [queue length, series terms, intervals queued]
+  unsigned long long* counters = h->scal + 2 * h->bmax;
+  // captured in a graph: the terms and queued intervals add up over the
+  // whole Newton solve, and the Gauss-Jordan kernel is always launched, on
+  // a fixed grid that walks whatever the queue holds
   const bool in_graph = h->capturing || h->nalistp != nullptr;
-  if (!in_graph) TRY(fill_u64(h, counters, 2));
+  TRY(fill_u64(h, counters, in_graph ? 1 : 3));
   Batch bt = h->batch_listed(mask);
   void* args_ns[] = {&nint, &bt, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &h->fbk, &counters};
   // the arithmetic follows the stored type of the stage Jacobians, and so
@@ -457,20 +466,18 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
     TRY(launch(h, F_COND_MF32, (unsigned)h->launch_intervals(), (unsigned)(h->m - 1) * 64, args_ns));
   else
     TRY(launch(h, f32 ? F_COND_NS32 : F_COND_NS, (unsigned)h->launch_intervals(), kNsThreads, args_ns));
+  unsigned grid = (unsigned)std::min(h->launch_intervals(), kGaussJordanGrid);
   if (!in_graph) {
-    unsigned long long cnt[2] = {0, 0};
+    unsigned long long cnt[3] = {0, 0, 0};
     HIPCHK(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    h->cond_fallback += cnt[0];
+    h->cond_fallback += cnt[2];
     h->cond_terms += cnt[1];
     if (cnt[0] == 0) return CS_OK;
+    grid = (unsigned)cnt[0];
   }
-  const int* only = h->fbk;
-  const int* ilist = bt.ilist;
-  const int* nlistp = bt.nlistp;
-  void* args[] = {&nint, &h->iinst, &ilist, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &only,
-                  &nlistp};
-  return launch(h, f32 ? F_COND32 : F_COND, (unsigned)h->launch_intervals(), (unsigned)(h->cw * 64), args);
+  void* args[] = {&h->iinst, &h->fbk, &counters, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status};
+  return launch(h, f32 ? F_COND32 : F_COND, grid, (unsigned)(h->cw * 64), args);
 }
 
 // Cyclic-reduction schedule of the current mesh (host side, once per mesh):
@@ -750,6 +757,7 @@ int cs_create(const cs_dims* d, cs_handle** out) {
   if (r == CS_OK) {
     hipError_t e = hipMalloc((void**)&h->nt_cnt, 4 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&h->stamps, 3 * kStampSlots * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&h->nt_par, sizeof(NtParams));
     int dev = 0, khz = 0;
     if (e == hipSuccess) e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
@@ -761,7 +769,7 @@ int cs_create(const cs_dims* d, cs_handle** out) {
     h->g_acc.base = 2 * kStampSlots;
   }
   if (r == CS_OK) {
-    hipError_t e = hipMalloc((void**)&h->scal, (2 * bm + 2) * sizeof(unsigned long long));
+    hipError_t e = hipMalloc((void**)&h->scal, (2 * bm + 3) * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&h->status, sizeof(int));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(h->status, 0, sizeof(int));
@@ -789,6 +797,7 @@ void cs_destroy(cs_handle* h) {
   for (Graph* g : {&h->g_dir, &h->g_trial, &h->g_acc})
     if (g->exec) hipGraphExecDestroy(g->exec);
   if (h->stamps) hipFree(h->stamps);
+  if (h->nt_par) hipFree(h->nt_par);
   double* bufs[] = {h->t, h->state, h->trial, h->dir, h->c, h->ytil, h->fa, h->fb, h->rbc, h->jac, h->Dr,
                     h->rows0, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps, h->nt_nrm, h->nt_taken,
                     h->nt_hist, h->nt_ring};
@@ -1182,13 +1191,16 @@ NtState nt_state(cs_handle* h, double tol, int max_iter, int patience, double mi
   st.hist = h->nt_hist;
   st.ring = h->nt_ring;
   st.cnt = h->nt_cnt;
-  st.hl = hl;
-  st.window = window;
-  st.patience = patience;
-  st.max_iter = max_iter;
-  st.max_trials = 20;   // solver:64
-  st.tol = tol;
-  st.min_ratio = min_ratio;
+  st.par = h->nt_par;
+  NtParams& P = h->nt_par_host;
+  P.hl = hl;
+  P.window = window;
+  P.patience = patience;
+  P.max_iter = max_iter;
+  P.max_trials = 20;   // solver:64
+  P.pad = 0;
+  P.tol = tol;
+  P.min_ratio = min_ratio;
   return st;
 }
 
@@ -1290,14 +1302,17 @@ int newton_graph(cs_handle* h, double alpha, double tol, int max_iter, int* iter
     }
     HIPCHK(hipMemcpyAsync(h->nt_ints, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
     TRY(upload_ints(h, h->act, std::vector<int>(B, 1)));
+    HIPCHK(hipMemcpyAsync(h->nt_par, &h->nt_par_host, sizeof(NtParams), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));   // the host vectors above are temporaries
   }
   h->jac_bits = h->dir_bits;
   TRY(jacobian_static(h));
-  TRY(fill_u64(h, h->scal + 2 * h->bmax, 2));   // condensation counters, summed over the solve
+  TRY(fill_u64(h, h->scal + 2 * h->bmax, 3));   // condensation counters, summed over the solve
   TRY(nt_list(h, st));
   // (re)capture when the mesh or any setting baked into the graphs changed
-  const std::vector<double> key = {(double)h->mesh_version, (double)B, (double)h->N, alpha, tol, (double)max_iter,
-                                   (double)patience, min_ratio, (double)window, (double)hl, (double)h->dir_bits,
+  // (the solve's tolerance, budget and progress test are read from device
+  // memory, so they are not part of it)
+  const std::vector<double> key = {(double)h->mesh_version, (double)B, (double)h->N, alpha, (double)h->dir_bits,
                                    (double)h->cond_kind, (double)h->prof, (double)h->subst_spans,
                                    (double)(size_t)h->nt_hist, (double)(size_t)h->nt_ring};
   h->alist = h->dlist;
@@ -1347,10 +1362,10 @@ int newton_graph(cs_handle* h, double alpha, double tol, int max_iter, int* iter
   // results, as newton() reports them
   std::vector<int> ints(4 * (size_t)h->bmax + 2);
   std::vector<double> nrm(B);
-  std::vector<unsigned long long> ctr(2);
+  std::vector<unsigned long long> ctr(3);
   HIPCHK(hipMemcpyAsync(ints.data(), h->nt_ints, ints.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipMemcpyAsync(nrm.data(), h->nt_nrm, B * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(ctr.data(), h->scal + 2 * h->bmax, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(ctr.data(), h->scal + 2 * h->bmax, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                         h->stream));
   std::vector<double> dh;
   if (hist && hist_len > 0) {
@@ -1358,7 +1373,7 @@ int newton_graph(cs_handle* h, double alpha, double tol, int max_iter, int* iter
     HIPCHK(hipMemcpyAsync(dh.data(), h->nt_hist, dh.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   }
   HIPCHK(hipStreamSynchronize(h->stream));
-  h->cond_fallback += ctr[0];
+  h->cond_fallback += ctr[2];
   h->cond_terms += ctr[1];
   const int last_all = std::min(max_iter, hist_len) - 1;
   for (int b = 0; b < B; ++b) {

@@ -790,17 +790,9 @@ __device__ __forceinline__ int stage_eliminate_mf(int i, const float* __restrict
 //   parameter columns), b = -f_b + U_{M-1}[:, NVA-1].  Only the NPA parameters
 //   the stage functions depend on get columns (g_p = f_p = 0 for the rest).
 template <int CW, class TJ>
-__device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst, const int* __restrict__ ilist,
-                         const double* __restrict__ t, const TJ* __restrict__ jac,
-                         const double* __restrict__ fa, const double* __restrict__ fb,
-                         double* __restrict__ rows, TJ* __restrict__ rec, int* __restrict__ status,
-                         const int* __restrict__ only, const int* __restrict__ nlistp) {
-  // workgroup k condenses interval ilist[k] (k without a list; *nlistp, when
-  // given, is the list's length); only != nullptr: just the intervals the
-  // series kernel left (only[i] != 0)
-  if (ilist && nlistp && (int)blockIdx.x >= *nlistp) return;
-  const int i = ilist ? ilist[blockIdx.x] : blockIdx.x;
-  if (only && !only[i]) return;
+__device__ __forceinline__ void condense_one(int i, const int* __restrict__ iinst, const double* __restrict__ t,
+                         const TJ* __restrict__ jac, const double* __restrict__ fa, const double* __restrict__ fb,
+                         double* __restrict__ rows, TJ* __restrict__ rec, int* __restrict__ status) {
   using Solver = GJB<NS, NS + NVA, CW, 16>;
   constexpr int NT = CW * WAVE;
   // sT (phases 1-2) and the elimination's panel / pivot-row buffers (phase 3)
@@ -908,6 +900,31 @@ __device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst
   if (tid == 0 && sbad) atomicOr(status, sbad);
 }
 
+// The intervals of the work queue (queue[0 .. *nq)), the workgroups taking
+// every gridDim.x-th: a launch of any size covers them.
+template <int CW, class TJ>
+__device__ __forceinline__ void condense(const int* __restrict__ iinst, const int* __restrict__ queue,
+                         const unsigned long long* __restrict__ nq, const double* __restrict__ t,
+                         const TJ* __restrict__ jac, const double* __restrict__ fa,
+                         const double* __restrict__ fb, double* __restrict__ rows, TJ* __restrict__ rec,
+                         int* __restrict__ status) {
+  const int n = (int)*nq;
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    if (k != (int)blockIdx.x) __syncthreads();   // the previous interval's LDS reads are done
+    condense_one<CW, TJ>(queue[k], iinst, t, jac, fa, fb, rows, rec, status);
+  }
+}
+
+// The work queue of the Gauss-Jordan condensation: the series kernels append
+// the intervals they leave (queue[counters[0]++] = i; counters[2] counts
+// them too, summed over a whole solve where counters[0] is reset per
+// launch); colloc_condense walks the queue.
+__device__ __forceinline__ void queue_gauss_jordan(int i, int* queue, unsigned long long* counters) {
+  const unsigned long long q = atomicAdd(counters, 1ull);
+  queue[q] = i;
+  atomicAdd(counters + 2, 1ull);
+}
+
 // --------------------------------------------------- series condensation
 // The stage system S U = R of condense, S = I - h (A^ x I) diag(M_k) with
 // A^ = A[1:, 1:] and M_k = T_k[:, :NY], solved by the fixed-point iteration
@@ -917,7 +934,7 @@ __device__ __forceinline__ void condense(int nint, const int* __restrict__ iinst
 // term changes U by at most 2^-50 of its largest entry (a few units of FP64
 // rounding: each term recomputes U from R, so the change does not go to 0);
 // an interval that has not converged after NS_MAX_TERMS terms (or produced a
-// non-finite value) is marked in fallback[] and re-done by condense.
+// non-finite value) is queued for condense (queue_gauss_jordan).
 // Work item (k, r, h) of the workgroup owns row r of stage block k and the
 // columns [h NS_CPT, (h + 1) NS_CPT) of U.  U and the products Y_k = M_k U_k
 // pass through one LDS buffer whose rows are padded to NVAP columns, so every
@@ -965,10 +982,7 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = wave_id();
   const int ib = bt.iinst[i];
-  if (skipped(bt, ib)) {   // inactive instance: nothing to condense, nothing for Gauss-Jordan
-    if (tid == 0) fallback[i] = 0;
-    return;
-  }
+  if (skipped(bt, ib)) return;   // inactive instance: nothing to condense, nothing for Gauss-Jordan
   if (tid == 0) sbad = 0;
   const R h = (R)(t[i + ib + 1] - t[i + ib]);
   R* recU = rec + (size_t)i * REC;
@@ -1074,8 +1088,7 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
     __syncthreads();   // sred is rewritten by the next term
   }
   if (tid == 0) {
-    fallback[i] = converged ? 0 : 1;
-    if (!converged) atomicAdd(counters, 1ull);    // counters[0]: intervals left to Gauss-Jordan
+    if (!converged) queue_gauss_jordan(i, fallback, counters);
     atomicAdd(counters + 1, (unsigned long long)terms);   // counters[1]: series terms
   }
   if (bad) atomicOr(&sbad, bad);
@@ -1155,10 +1168,7 @@ __device__ __forceinline__ void condense_mf32(int nint, Batch bt, const double* 
   const int tid = threadIdx.x;
   const int lane = lane_id(), k = wave_id();
   const int ib = bt.iinst[i];
-  if (skipped(bt, ib)) {
-    if (tid == 0) fallback[i] = 0;
-    return;
-  }
+  if (skipped(bt, ib)) return;
   if (tid == 0) sbad = 0;
   const float h = (float)(t[i + ib + 1] - t[i + ib]);
   float* recU = rec + (size_t)i * REC;
@@ -1263,8 +1273,7 @@ __device__ __forceinline__ void condense_mf32(int nint, Batch bt, const double* 
     }
   }
   if (tid == 0) {
-    fallback[i] = converged ? 0 : 1;
-    if (!converged) atomicAdd(counters, 1ull);
+    if (!converged) queue_gauss_jordan(i, fallback, counters);
     atomicAdd(counters + 1, (unsigned long long)terms);
   }
   if (bad) atomicOr(&sbad, bad);
@@ -1963,7 +1972,12 @@ __device__ __forceinline__ void recover_z(int nnodes, Batch bt, const double* __
 // stopped it (-1: not); hist[b][hl]: norm history; ring[b][window + 1]: the
 // last norms for the progress test; *it: the iteration; cnt: [0] instances
 // still iterating after the check, [1] still searching after a trial, [2]
-// intervals in the launch list.
+// intervals in the launch list; *par: the solve's settings (in device
+// memory, so that a captured iteration serves any of them).
+struct NtParams {
+  double tol, min_ratio;
+  int hl, window, patience, max_iter, max_trials, pad;
+};
 struct NtState {
   double* nrm;
   double* taken;
@@ -1976,8 +1990,7 @@ struct NtState {
   unsigned long long* cnt;
   int* nlist;
   int* lofs;
-  int hl, window, patience, max_iter, max_trials;
-  double tol, min_ratio;
+  const NtParams* par;
 };
 
 // After the residual of the iterate (norms in slot[b]): record it, stop the
@@ -1994,16 +2007,17 @@ __device__ __forceinline__ void nt_check(int B, NtState st, const unsigned long 
     return;
   }
   const int it = *st.it;
+  const NtParams P = *st.par;
   const double cur = __longlong_as_double((long long)slot[b]);
   st.nrm[b] = cur;
-  if (it < st.hl) st.hist[(size_t)b * st.hl + it] = cur;
-  const int rs = st.patience > 0 ? st.window + 1 : 1;
+  if (it < P.hl) st.hist[(size_t)b * P.hl + it] = cur;
+  const int rs = P.patience > 0 ? P.window + 1 : 1;
   bool on = true;
-  if (cur < st.tol) {
+  if (cur < P.tol) {
     on = false;
     st.done[b] = it;
-  } else if (st.patience > 0 && it >= st.patience && it >= st.window &&
-             !(cur * st.min_ratio <= st.ring[(size_t)b * rs + (it - st.window) % rs])) {
+  } else if (P.patience > 0 && it >= P.patience && it >= P.window &&
+             !(cur * P.min_ratio <= st.ring[(size_t)b * rs + (it - P.window) % rs])) {
     on = false;
     st.gave[b] = it;
   }
@@ -2073,7 +2087,7 @@ __device__ __forceinline__ void nt_line_search(int B, NtState st, const unsigned
   }
   const int tr = st.trials[b] + 1;
   st.trials[b] = tr;
-  if (tr >= st.max_trials) {
+  if (tr >= st.par->max_trials) {
     srch[b] = 0;
     steps[b] = 0.0;
     return;
@@ -2098,8 +2112,9 @@ __device__ __forceinline__ void nt_accept(int B, NtState st, int* act, double* s
   if (tk == 0.0) {
     act[b] = 0;
     const int it = *st.it;
-    const int end = st.max_iter < st.hl ? st.max_iter : st.hl;
-    for (int k = it + 1; k < end; ++k) st.hist[(size_t)b * st.hl + k] = st.nrm[b];
+    const int hl = st.par->hl, mi = st.par->max_iter;
+    const int end = mi < hl ? mi : hl;
+    for (int k = it + 1; k < end; ++k) st.hist[(size_t)b * hl + k] = st.nrm[b];
   }
 }
 
@@ -2158,16 +2173,16 @@ __global__ __launch_bounds__(256) void colloc_jacobian_static32(long long nst, f
   colloc::jacobian_static<float>(nst, jac);
 }
 
-__global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(int nint, const int* iinst, const int* ilist,
-    const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
-    const int* only, const int* nlistp) {
-  colloc::condense<Cfg::CW, double>(nint, iinst, ilist, t, jac, fa, fb, rows, rec, status, only, nlistp);
+__global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(const int* iinst, const int* queue,
+    const unsigned long long* nq, const double* t, const double* jac, const double* fa, const double* fb,
+    double* rows, double* rec, int* status) {
+  colloc::condense<Cfg::CW, double>(iinst, queue, nq, t, jac, fa, fb, rows, rec, status);
 }
 // the float64 Gauss-Jordan fallback of the FP32 direction (float32 stage Jacobians)
-__global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense32(int nint, const int* iinst, const int* ilist,
-    const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
-    const int* only, const int* nlistp) {
-  colloc::condense<Cfg::CW, float>(nint, iinst, ilist, t, jac, fa, fb, rows, rec, status, only, nlistp);
+__global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense32(const int* iinst, const int* queue,
+    const unsigned long long* nq, const double* t, const float* jac, const double* fa, const double* fb,
+    double* rows, float* rec, int* status) {
+  colloc::condense<Cfg::CW, float>(iinst, queue, nq, t, jac, fa, fb, rows, rec, status);
 }
 __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nint, colloc::Batch bt,
     const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
@@ -2177,9 +2192,7 @@ __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nin
     colloc::condense_ns<double>(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
   } else if (threadIdx.x == 0) {
     const int i = colloc::listed_interval(bt, blockIdx.x);
-    const bool skip = bt.act && !bt.act[bt.iinst[i]];
-    fallback[i] = skip ? 0 : 1;
-    if (!skip) atomicAdd(counters, 1ull);
+    if (!(bt.act && !bt.act[bt.iinst[i]])) colloc::queue_gauss_jordan(i, fallback, counters);
   }
 }
 
@@ -2194,9 +2207,7 @@ __global__ __launch_bounds__(colloc::MF_THREADS) void colloc_condense_mf32(int n
     colloc::condense_mf32(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
   } else if (threadIdx.x == 0) {
     const int i = colloc::listed_interval(bt, blockIdx.x);
-    const bool skip = bt.act && !bt.act[bt.iinst[i]];
-    fallback[i] = skip ? 0 : 1;
-    if (!skip) atomicAdd(counters, 1ull);
+    if (!(bt.act && !bt.act[bt.iinst[i]])) colloc::queue_gauss_jordan(i, fallback, counters);
   }
 }
 // capabilities of this problem's kernels: [0] colloc_condense_mf32 runs the
@@ -2216,9 +2227,7 @@ __global__ __launch_bounds__(colloc::NS_THREADS, 4) void colloc_condense_ns32(in
     colloc::condense_ns<float>(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
   } else if (threadIdx.x == 0) {
     const int i = colloc::listed_interval(bt, blockIdx.x);
-    const bool skip = bt.act && !bt.act[bt.iinst[i]];
-    fallback[i] = skip ? 0 : 1;
-    if (!skip) atomicAdd(counters, 1ull);
+    if (!(bt.act && !bt.act[bt.iinst[i]])) colloc::queue_gauss_jordan(i, fallback, counters);
   }
 }
 
