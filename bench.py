@@ -262,7 +262,7 @@ def main():
     ap.add_argument("--fallback-stall", default="10,5,2",
                     help="batch, learned: the plain fallback's stall test start,window,ratio; 0: off")
     ap.add_argument("--graph", type=int, choices=(0, 1), default=0,
-                    help="batch: Newton loops device resident, each iteration a HIP-graph replay (same results)")
+                    help="Newton loops device resident, each iteration a HIP-graph replay (same results)")
     ap.add_argument("--cpu-sample", type=int, default=640, help="Nint of the CPU baseline sample (0: skip)")
     args = ap.parse_args()
     check_gpus(args)
@@ -509,6 +509,7 @@ def single_workload(args):
     inst, m = problems.config(1, nint=args.nint)
     bvp = solver.BvpDae(inst.ocp_text())
     S = solver.CollocationSolver(bvp, m, n_max=bvp.N)
+    S.set_newton_graph(bool(args.graph))
     S.set_mesh(bvp.T0)
     S.form_initial_input(bvp.Y0, bvp.Z0, bvp.P0)
     schedule = inst.homotopy_schedule()
@@ -563,7 +564,9 @@ def single_workload(args):
         "dtype": "f64",
         "data": "synthetic (seeded OCP family, generated in-process)",
         "config": {"workload": "plain FP64 single-instance solve, tanh OCP n_x=20 n_u=10 (ny=40 nz=10 np=40), "
-                               "Nint=%d, m=%d, all-ones guess + homotopy S: 0->0.5" % (nint, m),
+                               "Nint=%d, m=%d, all-ones guess + homotopy S: 0->0.5%s"
+                               % (nint, m, "; Newton loop device resident (HIP-graph replay per iteration)"
+                                  if args.graph else ""),
                    "nint": nint, "m": m, "ny": ny, "nz": nz, "np": np_, "np_in_stage_functions": npa,
                    "tolerance": tol,
                    "converged_steps": converged, "newton_iterations": results[-1][1]},
