@@ -261,6 +261,9 @@ def main():
                          "final one")
     ap.add_argument("--fallback-stall", default="10,5,2",
                     help="batch, learned: the plain fallback's stall test start,window,ratio; 0: off")
+    ap.add_argument("--refine", type=int, default=0,
+                    help="batch: iterative refinement steps per Newton direction (float64 linear residual with the "
+                         "stored Jacobians, correction from the same direction kernels)")
     ap.add_argument("--graph", type=int, choices=(0, 1), default=0,
                     help="Newton loops device resident, each iteration a HIP-graph replay (same results)")
     ap.add_argument("--cpu-sample", type=int, default=640, help="Nint of the CPU baseline sample (0: skip)")
@@ -296,6 +299,7 @@ def batch_workload(args):
     S.set_condense_kernel(args.condense_kernel)
     S.set_babd_substitution(args.babd_spans)
     S.set_newton_graph(bool(args.graph))
+    S.set_refinement(args.refine)
     ts_all = [np.linspace(0.0, inst.T, inst.nodes) for inst in insts]
     cons_all = np.array([inst.constant_values() for inst in insts])
     hom_all = [np.array([inst.constant_values(s=0.0) for inst in insts])]
@@ -456,6 +460,8 @@ def batch_workload(args):
                  if args.precision == 32 else "; float64 throughout")
     form_txt += ("; BABD pairs by substitution while a pair spans <= 1/%d of its instance, by Householder QR above"
                  % args.babd_spans if args.babd_spans > 0 else "; BABD pairs by Householder QR on every level")
+    if args.refine:
+        form_txt += "; %d iterative refinement step(s) per Newton direction" % args.refine
     if args.graph:
         form_txt += "; Newton loops device resident, each iteration a HIP-graph replay"
     line = {

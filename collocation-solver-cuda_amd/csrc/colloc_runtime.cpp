@@ -94,7 +94,8 @@ struct CRSeg {
 enum Fn {
   F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_JAC32, F_COND, F_COND32, F_COND_NS, F_COND_NS32, F_CR_PAIR, F_CR_SUBST,
   F_CR_FINAL, F_CR_BACK, F_RECOVER, F_RECOVER32, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_JAC_STATIC, F_JAC_STATIC32,
-  F_AXPY_LISTED, F_COND_MF32, F_NT_CHECK, F_NT_SCAN, F_NT_FILL, F_NT_LS, F_NT_ACCEPT, F_NT_TICK, F_COUNT
+  F_AXPY_LISTED, F_COND_MF32, F_NT_CHECK, F_NT_SCAN, F_NT_FILL, F_NT_LS, F_NT_ACCEPT, F_NT_TICK, F_LINRES, F_LINRES32,
+  F_STATE_ADD, F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
   "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian", "colloc_jacobian32",
@@ -103,7 +104,8 @@ const char* kFnNames[F_COUNT] = {
   "colloc_cr_final", "colloc_cr_back", "colloc_recover", "colloc_recover32", "colloc_axpy4", "colloc_init_input",
   "colloc_segres", "colloc_recover_z", "colloc_jacobian_static", "colloc_jacobian_static32",
   "colloc_axpy_listed", "colloc_condense_mf32", "colloc_nt_check", "colloc_nt_list_scan", "colloc_nt_list_fill",
-  "colloc_nt_line_search", "colloc_nt_accept", "colloc_nt_tick"};
+  "colloc_nt_line_search", "colloc_nt_accept", "colloc_nt_tick", "colloc_linres", "colloc_linres32",
+  "colloc_state_add"};
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
@@ -207,6 +209,7 @@ struct cs_handle {
   int jac_parts = 1;                    // threads per stage of the Jacobian kernel (from the module)
   // device-resident Newton loop (cs_set_newton_graph)
   bool use_graph = false;
+  int refine = 0;                       // refinement steps per Newton direction (cs_set_refinement)
   const int* nalistp = nullptr;         // set: the launch list alist is built on the device, its length here
   int* dlist = nullptr;                 // the device-built launch list (intervals of the iterating instances)
   double* nt_nrm = nullptr;
@@ -628,6 +631,42 @@ int axpy_state(cs_handle* h, const double* x, const double* d, double* out) {
   Batch bt = h->batch(nullptr);
   void* args[] = {&s, &bt, &x, &d, &h->steps, &out};
   return launch(h, F_AXPY, (unsigned)((total + 255) / 256), 256, args);
+}
+
+// rho = F + J delta for the direction in h->dir (see colloc_linres): per-
+// instance max |rho| into slot[B]; write: rho replaces f_a, f_b, r_bc.
+int linear_residual(cs_handle* h, const int* mask, unsigned long long* slot, bool write) {
+  TRY(fill_u64(h, slot, h->B));
+  int nint = h->nint;
+  Batch bt = h->batch_listed(mask);
+  const double* dy = h->dir + h->st_y;
+  const double* dyd = h->dir + h->st_yd;
+  const double* dz = h->dir + h->st_z;
+  const double* dp = h->dir + h->st_p;
+  int wr = write ? 1 : 0;
+  void* args[] = {&nint, &bt, &h->t, &h->jac, &dy, &dyd, &dz, &dp, &h->fa, &h->fb, &h->rbc, &h->Dr, &slot, &wr};
+  const long long rows = (long long)h->launch_intervals() * (h->m * (h->ny + h->nz) + h->ny) +
+                         (long long)h->B * (h->ny + h->np);
+  return launch(h, h->jac_bits == 32 ? F_LINRES32 : F_LINRES, (unsigned)((rows + 255) / 256), 256, args);
+}
+
+int state_add(cs_handle* h, const double* a, const double* b, double* out) {
+  long long n = (long long)h->st_p + (long long)h->B * h->np;
+  void* args[] = {&n, &a, &b, &out};
+  return launch(h, F_STATE_ADD, (unsigned)((n + 255) / 256), 256, args);
+}
+
+// Iterative refinement of the Newton direction: rho = F + J delta in
+// float64 (with the stored Jacobians), the correction e from the same
+// direction kernels with rho in place of F (J e = -rho), delta <- delta + e.
+// The trial state holds delta meanwhile (the line search writes it later).
+int refine_direction(cs_handle* h, const int* mask, long long ncond) {
+  TRY(linear_residual(h, mask, h->scal, true));
+  TRY(state_add(h, h->dir, nullptr, h->trial));
+  TRY(condense(h, mask, ncond));
+  TRY(babd(h, mask));
+  TRY(recover(h, mask));
+  return state_add(h, h->trial, h->dir, h->dir);
 }
 
 int copy_out(cs_handle* h, double* host, const double* dev, size_t n) {
@@ -1118,6 +1157,7 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
     TRY(condense(h, h->act, ncond));
     TRY(babd(h, h->act));
     TRY(recover(h, h->act));
+    for (int r = 0; r < h->refine; ++r) TRY(refine_direction(h, h->act, ncond));
     for (int b = 0; b < B; ++b) {
       search[b] = active[b];
       step[b] = active[b] ? 1.0 : 0.0;
@@ -1313,7 +1353,7 @@ int newton_graph(cs_handle* h, double alpha, double tol, int max_iter, int* iter
   // (the solve's tolerance, budget and progress test are read from device
   // memory, so they are not part of it)
   const std::vector<double> key = {(double)h->mesh_version, (double)B, (double)h->N, alpha, (double)h->dir_bits,
-                                   (double)h->cond_kind, (double)h->prof, (double)h->subst_spans,
+                                   (double)h->cond_kind, (double)h->prof, (double)h->subst_spans, (double)h->refine,
                                    (double)(size_t)h->nt_hist, (double)(size_t)h->nt_ring};
   h->alist = h->dlist;
   h->nalist = 0;
@@ -1337,6 +1377,7 @@ int newton_graph(cs_handle* h, double alpha, double tol, int max_iter, int* iter
       TRY(condense(h, h->act, 0));
       TRY(babd(h, h->act));
       TRY(recover(h, h->act));
+      for (int r = 0; r < h->refine; ++r) TRY(refine_direction(h, h->act, 0));
       return nt_trial(h, st, alpha);
     }));
     TRY(nt_capture(h, h->g_trial, [&]() -> int { return nt_trial(h, st, alpha); }));
@@ -1355,7 +1396,7 @@ int newton_graph(cs_handle* h, double alpha, double tol, int max_iter, int* iter
   for (int it = 0; it < max_iter; ++it) {
     TRY(nt_replay(h, h->g_dir, cnt));
     if (cnt[0] == 0) break;
-    h->cond_intervals += cnt[2];
+    h->cond_intervals += cnt[2] * (1ull + (unsigned long long)h->refine);
     while (cnt[1] > 0) TRY(nt_replay(h, h->g_trial, cnt));
     HIPCHK(hipGraphLaunch(h->g_acc.exec, h->stream));
   }
@@ -1586,6 +1627,39 @@ int cs_set_newton_graph(cs_handle* h, int on) {
   if (!h) return fail(CS_E_ARG, "null handle");
   h->use_graph = on != 0;
   return CS_OK;
+}
+
+int cs_set_refinement(cs_handle* h, int steps) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (steps < 0 || steps > 8) return fail(CS_E_ARG, "refinement steps %d (0..8)", steps);
+  h->refine = steps;
+  return CS_OK;
+}
+
+int cs_get_direction(cs_handle* h, double* delta_y, double* delta_y_dot, double* delta_z_tilde, double* delta_p) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  TRY(need_mesh(h));
+  const size_t nint = h->nint;
+  TRY(copy_out(h, delta_y, h->dir + h->st_y, (size_t)h->N * h->ny));
+  TRY(copy_out(h, delta_y_dot, h->dir + h->st_yd, nint * h->m * h->ny));
+  TRY(copy_out(h, delta_z_tilde, h->dir + h->st_z, nint * h->m * h->nz));
+  TRY(copy_out(h, delta_p, h->dir + h->st_p, (size_t)h->B * h->np));
+  return sync(h);
+}
+
+int cs_refine_direction(cs_handle* h, int steps) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (steps < 0 || steps > 8) return fail(CS_E_ARG, "refinement steps %d (0..8)", steps);
+  TRY(need_mesh(h));
+  for (int r = 0; r < steps; ++r) TRY(refine_direction(h, nullptr, h->nint));
+  return sync(h);
+}
+
+int cs_linear_residual(cs_handle* h, double* norm) {
+  if (!h || !norm) return fail(CS_E_ARG, "null argument");
+  TRY(need_mesh(h));
+  TRY(linear_residual(h, nullptr, h->scal, false));
+  return read_slots(h, h->scal, norm);
 }
 
 int cs_set_condense_kernel(cs_handle* h, int kind) {

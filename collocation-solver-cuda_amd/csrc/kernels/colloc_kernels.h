@@ -1795,6 +1795,99 @@ __device__ __forceinline__ void axpy_listed(Seg4 s, Batch bt, const double* x, c
   }
 }
 
+// -------------------------------------------------------- linear residual
+// rho = F + J delta of the Newton system J delta = -F for a direction delta
+// = (dy, dydot, dz, dp), in float64, with the stored stage Jacobians (float32
+// for the FP32 direction) and Dr: the linearisation of compute_f_q
+// (solver:478-559) and of r at the current state,
+//   stage j of interval i:  rho_a = f_a + [Df; Dg] [dyt_j; dz_j; dp] - [dydot_j; 0],
+//                           dyt_j = dy_i + h sum_k A_jk dydot_k
+//   continuity:             rho_b = f_b + dy_{i+1} - dy_i - h sum_k b_k dydot_k
+//   boundary:               rho_r = r_bc + Dr [dy_0; dy_N; dp].
+// One thread per row: the listed intervals' M (NY+NZ) stage rows, then their
+// NY continuity rows, then the B instances' NY+NP boundary rows.  WRITE:
+// rho replaces f_a, f_b, r_bc (the right-hand side of a refinement solve);
+// norm[b] takes max |rho| over instance b's rows.
+template <class TJ, bool WRITE>
+__device__ __forceinline__ void linres(int nint, Batch bt, const double* __restrict__ t, const TJ* __restrict__ jac,
+                                       const double* __restrict__ dy, const double* __restrict__ dyd,
+                                       const double* __restrict__ dz, const double* __restrict__ dp,
+                                       double* __restrict__ fa, double* __restrict__ fb, double* __restrict__ rbc,
+                                       const double* __restrict__ Dr, unsigned long long* __restrict__ norm) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nli = launch_intervals(bt, nint);
+  const long long nsr = (long long)nli * M * NYZ;   // stage rows
+  const long long ncr = (long long)nli * NY;        // continuity rows
+  double v = 0.0;
+  int b = -1;
+  if (gid < nsr) {
+    const int k = (int)(gid / NYZ), r = (int)(gid - (long long)k * NYZ);
+    const int i = listed_interval(bt, k / M), j = k - (k / M) * M;
+    b = bt.iinst[i];
+    if (skipped(bt, b)) {
+      b = -1;
+    } else {
+      const int st = i * M + j;
+      const int nd = i + b;
+      const double h = t[nd + 1] - t[nd];
+      const TJ* row = jac + (size_t)st * JSTAGE + (size_t)r * NJC;
+      const double* ydi = dyd + (size_t)i * M * NY;
+      double s = fa[(size_t)st * NYZ + r];
+      for (int c = 0; c < NY; ++c) {
+        double q = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < M; ++kk) q = fma(c_rkA[j * M + kk], ydi[kk * NY + c], q);
+        s = fma((double)row[c], fma(h, q, dy[(size_t)nd * NY + c]), s);
+      }
+      for (int c = 0; c < NZ; ++c) s = fma((double)row[NY + c], dz[(size_t)st * NZ + c], s);
+      for (int c = 0; c < NP; ++c) s = fma((double)row[NY + NZ + c], dp[(size_t)b * NP + c], s);
+      if (r < NY) s -= ydi[j * NY + r];
+      if (WRITE) fa[(size_t)st * NYZ + r] = s;
+      v = s;
+    }
+  } else if (gid < nsr + ncr) {
+    const int k = (int)((gid - nsr) / NY), l = (int)(gid - nsr - (long long)k * NY);
+    const int i = listed_interval(bt, k);
+    b = bt.iinst[i];
+    if (skipped(bt, b)) {
+      b = -1;
+    } else {
+      const int nd = i + b;
+      const double h = t[nd + 1] - t[nd];
+      double q = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < M; ++kk) q = fma(c_rkb[kk], dyd[((size_t)i * M + kk) * NY + l], q);
+      const double s = fb[(size_t)i * NY + l] + dy[(size_t)(nd + 1) * NY + l] - dy[(size_t)nd * NY + l] - h * q;
+      if (WRITE) fb[(size_t)i * NY + l] = s;
+      v = s;
+    }
+  } else if (gid < nsr + ncr + (long long)bt.B * NRB) {
+    const int k = (int)((gid - nsr - ncr) / NRB), rr = (int)(gid - nsr - ncr - (long long)k * NRB);
+    b = k;
+    if (skipped(bt, b)) {
+      b = -1;
+    } else {
+      const double* D = Dr + (size_t)b * DRSZ + (size_t)rr * (2 * NY + NP);
+      const double* y0 = dy + (size_t)bt.noff[b] * NY;
+      const double* y1 = dy + (size_t)(bt.noff[b + 1] - 1) * NY;
+      double s = rbc[(size_t)b * NRB + rr];
+      for (int c = 0; c < NY; ++c) s = fma(D[c], y0[c], s);
+      for (int c = 0; c < NY; ++c) s = fma(D[NY + c], y1[c], s);
+      for (int c = 0; c < NP; ++c) s = fma(D[2 * NY + c], dp[(size_t)b * NP + c], s);
+      if (WRITE) rbc[(size_t)b * NRB + rr] = s;
+      v = s;
+    }
+  }
+  seg_max_to(fabs(v), b, norm);
+}
+
+// out = a + b over n state entries (b == nullptr: out = a).
+__device__ __forceinline__ void state_add(long long n, const double* __restrict__ a, const double* __restrict__ b,
+                                          double* __restrict__ out) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) out[e] = b ? a[e] + b[e] : a[e];
+}
+
 // -------------------------------------------------------- initial input
 // form_initial_input_kernel (solver:316-356).
 __device__ __forceinline__ void init_input(int nint, Batch bt, const double* __restrict__ y0,
@@ -2261,6 +2354,23 @@ __global__ __launch_bounds__(256) void colloc_recover32(int nint, colloc::Batch 
     const double* dy, const double* dp, double* dydot, double* dz) {
   if (colloc::beyond_list(bt, nint)) return;
   colloc::recover<float>(nint, bt, rec, dy, dp, dydot, dz);
+}
+
+__global__ __launch_bounds__(256) void colloc_linres(int nint, colloc::Batch bt, const double* t, const double* jac,
+    const double* dy, const double* dyd, const double* dz, const double* dp, double* fa, double* fb, double* rbc,
+    const double* Dr, unsigned long long* norm, int write) {
+  if (write) colloc::linres<double, true>(nint, bt, t, jac, dy, dyd, dz, dp, fa, fb, rbc, Dr, norm);
+  else colloc::linres<double, false>(nint, bt, t, jac, dy, dyd, dz, dp, fa, fb, rbc, Dr, norm);
+}
+// stage Jacobians stored in float32 (the FP32 direction)
+__global__ __launch_bounds__(256) void colloc_linres32(int nint, colloc::Batch bt, const double* t, const float* jac,
+    const double* dy, const double* dyd, const double* dz, const double* dp, double* fa, double* fb, double* rbc,
+    const double* Dr, unsigned long long* norm, int write) {
+  if (write) colloc::linres<float, true>(nint, bt, t, jac, dy, dyd, dz, dp, fa, fb, rbc, Dr, norm);
+  else colloc::linres<float, false>(nint, bt, t, jac, dy, dyd, dz, dp, fa, fb, rbc, Dr, norm);
+}
+__global__ __launch_bounds__(256) void colloc_state_add(long long n, const double* a, const double* b, double* out) {
+  colloc::state_add(n, a, b, out);
 }
 
 __global__ __launch_bounds__(256) void colloc_axpy4(colloc::Seg4 s, colloc::Batch bt, const double* x,

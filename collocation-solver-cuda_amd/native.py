@@ -47,6 +47,10 @@ SIGNATURES = {
     "cs_set_direction_precision": (c_int, [c_void_p, c_int]),
     "cs_set_condense_kernel": (c_int, [c_void_p, c_int]),
     "cs_set_newton_graph": (c_int, [c_void_p, c_int]),
+    "cs_set_refinement": (c_int, [c_void_p, c_int]),
+    "cs_linear_residual": (c_int, [c_void_p, P_D]),
+    "cs_refine_direction": (c_int, [c_void_p, c_int]),
+    "cs_get_direction": (c_int, [c_void_p, P_D, P_D, P_D, P_D]),
     "cs_newton": (c_int, [c_void_p, c_double, c_double, c_int, P_I, P_D, P_D, c_int]),
     "cs_newton_progress": (c_int, [c_void_p, c_double, c_double, c_int, c_int, c_double, P_I, P_D, P_D, c_int]),
     "cs_newton_stall": (c_int, [c_void_p, c_double, c_double, c_int, c_int, c_int, c_double, P_I, P_D, P_D, c_int]),

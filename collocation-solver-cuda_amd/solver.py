@@ -279,6 +279,34 @@ class CollocationSolver:
         iteration (the same results as the host-driven loop)."""
         check(lib().cs_set_newton_graph(self.h, 1 if on else 0))
 
+    def set_refinement(self, steps):
+        """Iterative refinement steps per Newton direction (0: none): the
+        float64 linear residual rho = F + J delta and a correction solve with
+        the same direction kernels, delta <- delta + e."""
+        check(lib().cs_set_refinement(self.h, int(steps)))
+
+    def linear_residual(self):
+        """max |F + J delta| per solve for the current direction (the last
+        solve_babd / recover_delta_k) at the last compute_f_q and
+        construct_jacobian."""
+        out = np.zeros(self.B)
+        check(lib().cs_linear_residual(self.h, ptr(out)))
+        return self._per_solve(out)
+
+    def refine_direction(self, steps=1):
+        """Refinement steps on the current stage-level direction; f_a, f_b,
+        r_bc then hold the last linear residual (recompute with compute_f_q)."""
+        check(lib().cs_refine_direction(self.h, int(steps)))
+
+    def get_direction(self):
+        """The current Newton direction (delta_y, delta_y_dot, delta_z_tilde,
+        delta_p) as it stands, nothing recomputed."""
+        nst = (self.N - self.B) * self.m
+        dy = np.empty((self.N, self.ny)); dyd = np.empty((nst, self.ny)); dz = np.empty((nst, self.nz))
+        dp = np.empty(self.np) if self.B == 1 else np.empty((self.B, self.np))
+        check(lib().cs_get_direction(self.h, ptr(dy), ptr(dyd), ptr(dz), ptr(dp)))
+        return dy, dyd, dz, dp
+
     def set_condense_kernel(self, kind):
         """Series condensation kernel of the FP32 direction: 0 matrix cores
         (default, where the stage blocks fit), 1 vector ALU."""

@@ -187,6 +187,37 @@ int cs_set_direction_precision(cs_handle* h, int bits);
  * default).  No counterpart in the reference. */
 int cs_set_newton_graph(cs_handle* h, int on);
 
+/* Iterative refinement of the Newton direction: after the direction delta
+ * (any precision), `steps` times: rho = F + J delta in float64 with the
+ * stored stage Jacobians and Dr, the correction e from the same direction
+ * kernels with rho as the right-hand side (J e = -rho), delta <- delta + e.
+ * For the FP32 direction this takes delta towards the float64 solution of
+ * the stored system.  0 (the default) to 8.  The north star's "FP32
+ * direction with iterative refinement"; no counterpart in the reference. */
+int cs_set_refinement(cs_handle* h, int steps);
+
+/* max |rho| per instance (B doubles) of rho = F + J delta, the linearised
+ * collocation system (the stages of compute_f_q_parallel, solver:478-559,
+ * and r at the boundary) at the current state, for the direction the last
+ * solve_babd / recover_delta_k left, with the Jacobians of the last
+ * construct_jacobian and f_a, f_b, r_bc of the last compute_f_q.  Nothing
+ * is overwritten.  No counterpart in the reference (a check of the
+ * direction's accuracy). */
+int cs_linear_residual(cs_handle* h, double* norm_out);
+
+/* `steps` refinement steps (as cs_set_refinement) on the current direction
+ * of the stage functions (after solve_babd / recover_delta_k, with the
+ * f_a, f_b, r_bc and Jacobians it was computed from).  f_a, f_b and r_bc
+ * are left holding the last linear residual: call compute_f_q again before
+ * using them.  No counterpart in the reference. */
+int cs_refine_direction(cs_handle* h, int steps);
+
+/* The current Newton direction as it stands (after solve_babd,
+ * recover_delta_k or cs_refine_direction), without recomputing any part:
+ * delta_y [N][ny], delta_y_dot and delta_z_tilde [Nint*m][.], delta_p
+ * [B][np]; null pointers skip a part.  No counterpart in the reference. */
+int cs_get_direction(cs_handle* h, double* delta_y, double* delta_y_dot, double* delta_z_tilde, double* delta_p);
+
 /* Kernel of the series condensation in the FP32 direction: 0 (default) the
  * products of the series on matrix cores (v_mfma_f32_16x16x4f32, one wave
  * per stage block) where the problem's stage blocks fit (ny <= 48 and
