@@ -261,6 +261,9 @@ def main():
                          "final one")
     ap.add_argument("--fallback-stall", default="10,5,2",
                     help="batch, learned: the plain fallback's stall test start,window,ratio; 0: off")
+    ap.add_argument("--babd-precision", type=int, choices=(32, 64), default=64,
+                    help="batch: stored type of the cyclic reduction's rows and records (32: float32 levels, "
+                         "float64 boundary solve and back substitution)")
     ap.add_argument("--refine", type=int, default=0,
                     help="batch: iterative refinement steps per Newton direction (float64 linear residual with the "
                          "stored Jacobians, correction from the same direction kernels)")
@@ -300,6 +303,7 @@ def batch_workload(args):
     S.set_babd_substitution(args.babd_spans)
     S.set_newton_graph(bool(args.graph))
     S.set_refinement(args.refine)
+    S.set_babd_precision(args.babd_precision)
     ts_all = [np.linspace(0.0, inst.T, inst.nodes) for inst in insts]
     cons_all = np.array([inst.constant_values() for inst in insts])
     hom_all = [np.array([inst.constant_values(s=0.0) for inst in insts])]
@@ -401,6 +405,8 @@ def batch_workload(args):
     if args.form == "learned" and args.precision == 32 and args.fp64_pass:
         # the same learned + batched solve with the float64 direction
         S.set_direction_precision(64)
+        S.set_babd_precision(64)
+        S.set_refinement(0)
         t0 = time.perf_counter()
         c64, it64, b64 = learned_pass()
         lt = time.perf_counter() - t0
@@ -409,6 +415,8 @@ def batch_workload(args):
     if args.form == "learned" and args.plain_pass:
         # the plain batched form (float64 direction) on the same instances, for the split of the gain
         S.set_direction_precision(64)
+        S.set_babd_precision(64)
+        S.set_refinement(0)
         t0 = time.perf_counter()
         pconv, pits, _ = plain_pass()
         pt = time.perf_counter() - t0
@@ -460,6 +468,8 @@ def batch_workload(args):
                  if args.precision == 32 else "; float64 throughout")
     form_txt += ("; BABD pairs by substitution while a pair spans <= 1/%d of its instance, by Householder QR above"
                  % args.babd_spans if args.babd_spans > 0 else "; BABD pairs by Householder QR on every level")
+    if args.babd_precision == 32:
+        form_txt += "; cyclic reduction with float32 rows and records"
     if args.refine:
         form_txt += "; %d iterative refinement step(s) per Newton direction" % args.refine
     if args.graph:

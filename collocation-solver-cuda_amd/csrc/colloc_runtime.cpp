@@ -95,7 +95,8 @@ enum Fn {
   F_FILL, F_RES_FULL, F_RES_NORM, F_JAC, F_JAC32, F_COND, F_COND32, F_COND_NS, F_COND_NS32, F_CR_PAIR, F_CR_SUBST,
   F_CR_FINAL, F_CR_BACK, F_RECOVER, F_RECOVER32, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_JAC_STATIC, F_JAC_STATIC32,
   F_AXPY_LISTED, F_COND_MF32, F_NT_CHECK, F_NT_SCAN, F_NT_FILL, F_NT_LS, F_NT_ACCEPT, F_NT_TICK, F_LINRES, F_LINRES32,
-  F_STATE_ADD, F_COUNT
+  F_STATE_ADD, F_CR_PAIR_DF, F_CR_PAIR_FF, F_CR_SUBST_DF, F_CR_SUBST_FF, F_CR_FINAL_F, F_CR_BACK_FD, F_CR_BACK_FF,
+  F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
   "colloc_fill_u64", "colloc_residual_full", "colloc_residual_norm", "colloc_jacobian", "colloc_jacobian32",
@@ -105,7 +106,8 @@ const char* kFnNames[F_COUNT] = {
   "colloc_segres", "colloc_recover_z", "colloc_jacobian_static", "colloc_jacobian_static32",
   "colloc_axpy_listed", "colloc_condense_mf32", "colloc_nt_check", "colloc_nt_list_scan", "colloc_nt_list_fill",
   "colloc_nt_line_search", "colloc_nt_accept", "colloc_nt_tick", "colloc_linres", "colloc_linres32",
-  "colloc_state_add"};
+  "colloc_state_add", "colloc_cr_pair_df", "colloc_cr_pair_ff", "colloc_cr_subst_df", "colloc_cr_subst_ff",
+  "colloc_cr_final_f", "colloc_cr_back_fd", "colloc_cr_back_ff"};
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
@@ -210,6 +212,7 @@ struct cs_handle {
   // device-resident Newton loop (cs_set_newton_graph)
   bool use_graph = false;
   int refine = 0;                       // refinement steps per Newton direction (cs_set_refinement)
+  int babd_bits = 64;                   // stored type of the cyclic reduction's rows and records (cs_set_babd_precision)
   const int* nalistp = nullptr;         // set: the launch list alist is built on the device, its length here
   int* dlist = nullptr;                 // the device-built launch list (intervals of the iterating instances)
   double* nt_nrm = nullptr;
@@ -565,6 +568,11 @@ int build_cr_schedule(cs_handle* h) {
   return CS_OK;
 }
 
+// The FP32 cyclic reduction (babd_bits 32) stores every level's output rows
+// and records as float at the same element offsets as the float64 one, in
+// the same buffers: each float access lies at or below the byte address of
+// the float64 access to the same element.  Level 0 reads the condensation's
+// float64 rows; cr_final and cr_back read the float32 levels.
 int babd(cs_handle* h, const int* mask) {
   ProfScope ps(h, CS_KERNEL_BABD);
   double* dy = h->dir + h->st_y;
@@ -573,6 +581,7 @@ int babd(cs_handle* h, const int* mask) {
   const size_t rowsz = (size_t)h->ny * h->crw;
   const unsigned pair_block = (unsigned)h->pair_threads;   // 4 lanes per record column
   const int nlev = (int)h->lev_nout.size();
+  const bool f32 = h->babd_bits == 32;
   for (int l = 0; l < nlev; ++l) {
     const int* oinst = h->cr_oinst + h->lev_obase[l];
     const CRSeg* seg = h->cr_seg + h->lev_sbase[l];
@@ -580,20 +589,23 @@ int babd(cs_handle* h, const int* mask) {
     double* rin = h->rows0 + h->lev_rbase[l] * rowsz;
     double* rout = h->rows0 + h->lev_rbase[l + 1] * rowsz;
     void* args[] = {&oinst, &seg, &act, &rin, &rout, &h->crrec};
-    if (h->lev_subst[l]) TRY(launch(h, F_CR_SUBST, (unsigned)h->lev_nout[l], kSubstThreads, args));
-    if (h->lev_qr[l]) TRY(launch(h, F_CR_PAIR, (unsigned)h->lev_nout[l], pair_block, args));
+    const Fn fs = !f32 ? F_CR_SUBST : (l == 0 ? F_CR_SUBST_DF : F_CR_SUBST_FF);
+    const Fn fp = !f32 ? F_CR_PAIR : (l == 0 ? F_CR_PAIR_DF : F_CR_PAIR_FF);
+    if (h->lev_subst[l]) TRY(launch(h, fs, (unsigned)h->lev_nout[l], kSubstThreads, args));
+    if (h->lev_qr[l]) TRY(launch(h, fp, (unsigned)h->lev_nout[l], pair_block, args));
   }
   {
     double* rin = h->rows0 + h->lev_rbase[nlev] * rowsz;
     void* args[] = {&bt, &rin, &h->Dr, &h->rbc, &dy, &dp, &h->status};
-    TRY(launch(h, F_CR_FINAL, (unsigned)h->B, (unsigned)(h->fw * 64), args));
+    TRY(launch(h, f32 && nlev > 0 ? F_CR_FINAL_F : F_CR_FINAL, (unsigned)h->B, (unsigned)(h->fw * 64), args));
   }
   for (int l = nlev - 1; l >= 0; --l) {
     const int* oinst = h->cr_oinst + h->lev_obase[l];
     const CRSeg* seg = h->cr_seg + h->lev_sbase[l];
     double* rin = h->rows0 + h->lev_rbase[l] * rowsz;
     void* args[] = {&l, &bt, &oinst, &seg, &h->crrec, &rin, &dy, &dp};
-    TRY(launch(h, F_CR_BACK, (unsigned)h->lev_nout[l], 64, args));
+    const Fn fb = !f32 ? F_CR_BACK : (l == 0 ? F_CR_BACK_FD : F_CR_BACK_FF);
+    TRY(launch(h, fb, (unsigned)h->lev_nout[l], 64, args));
   }
   return CS_OK;
 }
@@ -1354,6 +1366,7 @@ int newton_graph(cs_handle* h, double alpha, double tol, int max_iter, int* iter
   // memory, so they are not part of it)
   const std::vector<double> key = {(double)h->mesh_version, (double)B, (double)h->N, alpha, (double)h->dir_bits,
                                    (double)h->cond_kind, (double)h->prof, (double)h->subst_spans, (double)h->refine,
+                                   (double)h->babd_bits,
                                    (double)(size_t)h->nt_hist, (double)(size_t)h->nt_ring};
   h->alist = h->dlist;
   h->nalist = 0;
@@ -1626,6 +1639,13 @@ int cs_set_direction_precision(cs_handle* h, int bits) {
 int cs_set_newton_graph(cs_handle* h, int on) {
   if (!h) return fail(CS_E_ARG, "null handle");
   h->use_graph = on != 0;
+  return CS_OK;
+}
+
+int cs_set_babd_precision(cs_handle* h, int bits) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  if (bits != 32 && bits != 64) return fail(CS_E_ARG, "BABD precision %d (32 or 64)", bits);
+  h->babd_bits = bits;
   return CS_OK;
 }
 

@@ -1326,10 +1326,13 @@ __device__ __forceinline__ void condense_mf32(int nint, Batch bt, const double* 
 // is broadcast through LDS, where rows above the current pivot are kept zero
 // so every loop has a fixed trip count.  An odd last row is carried.
 // Workgroup g makes output row g of the level: oinst[g] is its instance,
-// seg[oinst[g]] the instance's place in this level's buffers.
+// seg[oinst[g]] the instance's place in this level's buffers.  TI / TO: the
+// stored type of the input rows / of the output rows and records (float for
+// the FP32 cyclic reduction; the arithmetic is float64 either way).
+template <class TI, class TO>
 __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
-                                        const int* __restrict__ act, const double* __restrict__ rows_in,
-                                        double* __restrict__ rows_out, double* __restrict__ recs) {
+                                        const int* __restrict__ act, const TI* __restrict__ rows_in,
+                                        TO* __restrict__ rows_out, TO* __restrict__ recs) {
   constexpr int R2 = 2 * NY;
   __shared__ double sv[2][CR_PARTS * CR_ROWS];
   __shared__ double sbeta[2];
@@ -1345,9 +1348,9 @@ __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRS
   const int npairs = nrows >> 1;
   const int tid = threadIdx.x;
   if (k == npairs) {  // carry the odd last row
-    const double* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
-    double* dst = rows_out + (size_t)npairs * NY * CRW;
-    for (int e = tid; e < NY * CRW; e += blockDim.x) dst[e] = src[e];
+    const TI* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
+    TO* dst = rows_out + (size_t)npairs * NY * CRW;
+    for (int e = tid; e < NY * CRW; e += blockDim.x) dst[e] = (TO)src[e];
     return;
   }
   for (int e = tid; e < 2 * CR_PARTS * CR_ROWS; e += blockDim.x) (&sv[0][0])[e] = 0.0;
@@ -1358,8 +1361,8 @@ __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRS
   // the carried column's place in the record ([R | Gp | Ep | Jp | d]; Jp
   // absent when HZ) and in the output row (column - NY)
   const int pcol = cidx < 3 * NY + NPH ? cidx : RECW - 1;
-  const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
-  const double* rb = ra + NY * CRW;
+  const TI* ra = rows_in + (size_t)(2 * k) * NY * CRW;
+  const TI* rb = ra + NY * CRW;
   // column layout: [x_M (NY) | x_L (NY) | x_R (NY) | p (NP) | rhs]
   double a[CR_ROWS];
 #pragma unroll
@@ -1368,7 +1371,7 @@ __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRS
     double v = 0.0;
     if (live && r < R2) {
       const bool top = r < NY;
-      const double* src = top ? ra + r * CRW : rb + (r - NY) * CRW;
+      const TI* src = top ? ra + r * CRW : rb + (r - NY) * CRW;
       if (cidx < NY) v = top ? src[NY + cidx] : src[cidx];
       else if (cidx < 2 * NY) v = top ? src[cidx - NY] : 0.0;
       else if (cidx < 3 * NY) v = top ? 0.0 : src[cidx - NY];
@@ -1439,13 +1442,13 @@ __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRS
     }
   });
   if (live) {
-    double* rec = recs + (size_t)k * NY * RECW;
-    double* out = rows_out + (size_t)k * NY * CRW;
+    TO* rec = recs + (size_t)k * NY * RECW;
+    TO* out = rows_out + (size_t)k * NY * CRW;
 #pragma unroll
     for (int i = 0; i < CR_ROWS; ++i) {
       const int r = part * CR_ROWS + i;
-      if (r < NY) rec[r * RECW + pcol] = a[i];
-      else if (r < R2 && cidx >= NY) out[(r - NY) * CRW + pcol - NY] = a[i];
+      if (r < NY) rec[r * RECW + pcol] = (TO)a[i];
+      else if (r < R2 && cidx >= NY) out[(r - NY) * CRW + pcol - NY] = (TO)a[i];
     }
   }
 }
@@ -1467,12 +1470,15 @@ constexpr int CS_CG = (CS_XC + 3) / 4;                // column groups of 4
 constexpr int CS_RG = (NY + 3) / 4;                   // row groups of 4
 constexpr int CS_THREADS = 256;
 static_assert(CS_RG * CS_CG <= 4 * CS_THREADS, "cr_subst tiles");
+// TI / TO: stored type of the input / output rows; the product is computed
+// in TO (float32 arithmetic for the FP32 cyclic reduction).
+template <class TI, class TO>
 __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
-                                         const int* __restrict__ act, const double* __restrict__ rows_in,
-                                         double* __restrict__ rows_out, double* __restrict__ recs) {
+                                         const int* __restrict__ act, const TI* __restrict__ rows_in,
+                                         TO* __restrict__ rows_out, TO* __restrict__ recs) {
   constexpr int XP = CS_CG * 4;                         // padded columns
-  __shared__ double sG[CS_RG * 4][NY];                  // G_b (rows padded to a multiple of 4)
-  __shared__ double sX[NY][XP];                         // [G_a | H_a | b_a]
+  __shared__ TO sG[CS_RG * 4][NY];                      // G_b (rows padded to a multiple of 4)
+  __shared__ TO sX[NY][XP];                             // [G_a | H_a | b_a]
   const int ib = oinst[blockIdx.x];
   if (act && !act[ib]) return;
   const CRSeg sg = seg[ib];
@@ -1488,28 +1494,28 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
   // it is written (as I) only where the next level is not one (subst == 1)
   const bool wc = sg.subst == 1;
   if (k == npairs) {  // carry the odd last row
-    const double* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
-    double* dst = rows_out + (size_t)npairs * NY * CRW;
+    const TI* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
+    TO* dst = rows_out + (size_t)npairs * NY * CRW;
     for (int e = tid; e < NY * CRW; e += blockDim.x) {
       const int r = e / CRW, c = e - (e / CRW) * CRW;
-      if (c < NY || c == CRW - 1 || (!HZ && c >= 2 * NY)) dst[e] = src[e];
-      else if (wc && c < 2 * NY) dst[e] = (c - NY == r) ? 1.0 : 0.0;
+      if (c < NY || c == CRW - 1 || (!HZ && c >= 2 * NY)) dst[e] = (TO)src[e];
+      else if (wc && c < 2 * NY) dst[e] = (c - NY == r) ? (TO)1 : (TO)0;
     }
     return;
   }
-  const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
-  const double* rb = ra + NY * CRW;
-  double* out = rows_out + (size_t)k * NY * CRW;
+  const TI* ra = rows_in + (size_t)(2 * k) * NY * CRW;
+  const TI* rb = ra + NY * CRW;
+  TO* out = rows_out + (size_t)k * NY * CRW;
   for (int e = tid; e < CS_RG * 4 * NY; e += blockDim.x) {
     const int r = e / NY, c = e - (e / NY) * NY;
-    (&sG[0][0])[e] = r < NY ? rb[r * CRW + c] : 0.0;
+    (&sG[0][0])[e] = r < NY ? (TO)rb[r * CRW + c] : (TO)0;
   }
   for (int e = tid; e < NY * XP; e += blockDim.x) {
     const int r = e / XP, c = e - (e / XP) * XP;
-    double v = 0.0;
-    if (c < NY) v = ra[r * CRW + c];                       // G_a
-    else if (c < NY + NPH) v = ra[r * CRW + NY + c];       // H_a (after C_a)
-    else if (c == NY + NPH) v = ra[r * CRW + CRW - 1];     // b_a
+    TO v = 0;
+    if (c < NY) v = (TO)ra[r * CRW + c];                       // G_a
+    else if (c < NY + NPH) v = (TO)ra[r * CRW + NY + c];       // H_a (after C_a)
+    else if (c == NY + NPH) v = (TO)ra[r * CRW + CRW - 1];     // b_a
     sX[r][c] = v;
   }
   // no record: back substitution reads row a itself (x_M = b_a - G_a x_L - H_a p),
@@ -1517,14 +1523,14 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
   __syncthreads();
   for (int t = tid; t < CS_RG * CS_CG; t += blockDim.x) {
     const int rg = t / CS_CG, cg = t - (t / CS_CG) * CS_CG;
-    double acc[4][4];
+    TO acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+      for (int c = 0; c < 4; ++c) acc[a][c] = 0;
 #pragma unroll 4
     for (int q = 0; q < NY; ++q) {
-      double g[4], x[4];
+      TO g[4], x[4];
 #pragma unroll
       for (int a = 0; a < 4; ++a) g[a] = sG[rg * 4 + a][q];
 #pragma unroll
@@ -1541,16 +1547,16 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int col = cg * 4 + c;
-        if (col < NY) out[r * CRW + col] = -acc[a][c];                                   // G'
-        else if (col < NY + NPH) out[r * CRW + NY + col] = rb[r * CRW + NY + col] - acc[a][c];  // H'
-        else if (col == NY + NPH) out[r * CRW + CRW - 1] = rb[r * CRW + CRW - 1] - acc[a][c];  // b'
+        if (col < NY) out[r * CRW + col] = -acc[a][c];                                         // G'
+        else if (col < NY + NPH) out[r * CRW + NY + col] = (TO)rb[r * CRW + NY + col] - acc[a][c];  // H'
+        else if (col == NY + NPH) out[r * CRW + CRW - 1] = (TO)rb[r * CRW + CRW - 1] - acc[a][c];  // b'
       }
     }
   }
   if (wc) {
     for (int e = tid; e < NY * NY; e += blockDim.x) {      // C' = C_b = I
       const int r = e / NY, c = e - (e / NY) * NY;
-      out[r * CRW + NY + c] = (r == c) ? 1.0 : 0.0;
+      out[r * CRW + NY + c] = (r == c) ? (TO)1 : (TO)0;
     }
   }
 }
@@ -1559,8 +1565,8 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
 // boundary conditions B_0 x_0 + B_n x_N + V_n p = -r_bc: unknowns
 // [x_N | x_0 | p] as in qr_decomposition's final block (solve_babd_system.py:364-381).
 // Workgroup b solves instance b, whose last row is row b of `rows`.
-template <int FW>
-__device__ __forceinline__ void cr_final(Batch bt, const double* __restrict__ rows, const double* __restrict__ Dr,
+template <int FW, class TRW>
+__device__ __forceinline__ void cr_final(Batch bt, const TRW* __restrict__ rows, const double* __restrict__ Dr,
                          const double* __restrict__ rbc, double* __restrict__ dy, double* __restrict__ dp,
                          int* __restrict__ status) {
   using Solver = GJ<NFIN, NFIN + 1, FW>;
@@ -1570,7 +1576,7 @@ __device__ __forceinline__ void cr_final(Batch bt, const double* __restrict__ ro
   __shared__ int invperm[Solver::RQ * WAVE];
   const int ib = blockIdx.x;
   if (skipped(bt, ib)) return;
-  const double* row = rows + (size_t)ib * NY * CRW;
+  const TRW* row = rows + (size_t)ib * NY * CRW;
   Dr += (size_t)ib * DRSZ;
   rbc += (size_t)ib * NRB;
   double* x0 = dy + (size_t)bt.noff[ib] * NY;
@@ -1587,10 +1593,10 @@ __device__ __forceinline__ void cr_final(Batch bt, const double* __restrict__ ro
       const int cc = Solver::col(cq);
       double v = 0.0;
       if (r < NY) {
-        if (cc < NY) v = row[r * CRW + NY + cc];                 // C
-        else if (cc < 2 * NY) v = row[r * CRW + cc - NY];        // G
-        else if (cc < NFIN) v = HZ ? 0.0 : row[r * CRW + cc];    // H
-        else if (cc == NFIN) v = row[r * CRW + CRW - 1];         // b
+        if (cc < NY) v = (double)row[r * CRW + NY + cc];                 // C
+        else if (cc < 2 * NY) v = (double)row[r * CRW + cc - NY];        // G
+        else if (cc < NFIN) v = HZ ? 0.0 : (double)row[r * CRW + cc];    // H
+        else if (cc == NFIN) v = (double)row[r * CRW + CRW - 1];         // b
       } else if (r < NFIN) {
         const int rr = r - NY;
         if (cc < NY) v = Dr[rr * DRW + NY + cc];                 // B_n
@@ -1626,10 +1632,12 @@ __device__ __forceinline__ void cr_final(Batch bt, const double* __restrict__ ro
 //   x_M = R^-1 (d - Gp x_L - Ep x_R - Jp p)   (as partition_backward_substitution_kernel,
 // solve_babd_system.py:537-572, with the triangular solve done directly).
 // Launched with the level's output-row grid of cr_pair; carried rows have
-// no record and return.
+// no record and return.  TC / TRW: stored type of the records / of the
+// level's input rows (arithmetic in float64).
+template <class TC, class TRW>
 __device__ __forceinline__ void cr_back(int level, Batch bt, const int* __restrict__ oinst,
-                        const CRSeg* __restrict__ seg, const double* __restrict__ recs,
-                        const double* __restrict__ rows_in, double* __restrict__ dy,
+                        const CRSeg* __restrict__ seg, const TC* __restrict__ recs,
+                        const TRW* __restrict__ rows_in, double* __restrict__ dy,
                         const double* __restrict__ dp) {
   const int ib = oinst[blockIdx.x];
   if (skipped(bt, ib)) return;
@@ -1648,28 +1656,28 @@ __device__ __forceinline__ void cr_back(int level, Batch bt, const int* __restri
   if (sg.subst) {
     // substitution level: row a of the pair, [G_a | I | H_a | b_a] (this
     // level's input, kept in place), gives x_M = b_a - G_a x_L - H_a p
-    const double* ra = rows_in + (size_t)(sg.in_off + 2 * k) * NY * CRW + (lane < NY ? lane : 0) * CRW;
-    double rhs = ra[CRW - 1];
+    const TRW* ra = rows_in + (size_t)(sg.in_off + 2 * k) * NY * CRW + (lane < NY ? lane : 0) * CRW;
+    double rhs = (double)ra[CRW - 1];
 #pragma unroll 8
-    for (int c = 0; c < NY; ++c) rhs = fma(-ra[c], xl[c], rhs);
+    for (int c = 0; c < NY; ++c) rhs = fma(-(double)ra[c], xl[c], rhs);
 #pragma unroll 8
-    for (int c = 0; c < NPH; ++c) rhs = fma(-ra[2 * NY + c], dp[c], rhs);
+    for (int c = 0; c < NPH; ++c) rhs = fma(-(double)ra[2 * NY + c], dp[c], rhs);
     if (lane < NY) dy[(size_t)mid * NY + lane] = rhs;
     return;
   }
-  const double* rec = recs + (size_t)(sg.rec_off + k) * NY * RECW;
+  const TC* rec = recs + (size_t)(sg.rec_off + k) * NY * RECW;
   // lane r holds row r of R and the right-hand side d - Gp x_L - Ep x_R - Jp p
-  const double* rw = rec + (lane < NY ? lane : 0) * RECW;
+  const TC* rw = rec + (lane < NY ? lane : 0) * RECW;
   double Rr[NY];
 #pragma unroll
-  for (int c = 0; c < NY; ++c) Rr[c] = rw[c];
-  double rhs = rw[RECW - 1];
+  for (int c = 0; c < NY; ++c) Rr[c] = (double)rw[c];
+  double rhs = (double)rw[RECW - 1];
 #pragma unroll 8
-  for (int c = 0; c < NY; ++c) rhs = fma(-rw[NY + c], xl[c], rhs);
+  for (int c = 0; c < NY; ++c) rhs = fma(-(double)rw[NY + c], xl[c], rhs);
 #pragma unroll 8
-  for (int c = 0; c < NY; ++c) rhs = fma(-rw[2 * NY + c], xr[c], rhs);
+  for (int c = 0; c < NY; ++c) rhs = fma(-(double)rw[2 * NY + c], xr[c], rhs);
 #pragma unroll 8
-  for (int c = 0; c < NPH; ++c) rhs = fma(-rw[3 * NY + c], dp[c], rhs);
+  for (int c = 0; c < NPH; ++c) rhs = fma(-(double)rw[3 * NY + c], dp[c], rhs);
   double x = 0.0;
 #pragma unroll
   for (int r = NY - 1; r >= 0; --r) {
@@ -2326,22 +2334,53 @@ __global__ __launch_bounds__(colloc::NS_THREADS, 4) void colloc_condense_ns32(in
 
 __global__ __launch_bounds__(colloc::CR_THREADS) void colloc_cr_pair(const int* oinst, const colloc::CRSeg* seg,
     const int* act, const double* rows_in, double* rows_out, double* recs) {
-  colloc::cr_pair(oinst, seg, act, rows_in, rows_out, recs);
+  colloc::cr_pair<double, double>(oinst, seg, act, rows_in, rows_out, recs);
+}
+// the FP32 cyclic reduction: its first level (float64 rows of the
+// condensation in, float32 out) and the others (float32 in and out)
+__global__ __launch_bounds__(colloc::CR_THREADS) void colloc_cr_pair_df(const int* oinst, const colloc::CRSeg* seg,
+    const int* act, const double* rows_in, float* rows_out, float* recs) {
+  colloc::cr_pair<double, float>(oinst, seg, act, rows_in, rows_out, recs);
+}
+__global__ __launch_bounds__(colloc::CR_THREADS) void colloc_cr_pair_ff(const int* oinst, const colloc::CRSeg* seg,
+    const int* act, const float* rows_in, float* rows_out, float* recs) {
+  colloc::cr_pair<float, float>(oinst, seg, act, rows_in, rows_out, recs);
 }
 
 __global__ __launch_bounds__(colloc::CS_THREADS) void colloc_cr_subst(const int* oinst, const colloc::CRSeg* seg,
     const int* act, const double* rows_in, double* rows_out, double* recs) {
-  colloc::cr_subst(oinst, seg, act, rows_in, rows_out, recs);
+  colloc::cr_subst<double, double>(oinst, seg, act, rows_in, rows_out, recs);
+}
+__global__ __launch_bounds__(colloc::CS_THREADS) void colloc_cr_subst_df(const int* oinst, const colloc::CRSeg* seg,
+    const int* act, const double* rows_in, float* rows_out, float* recs) {
+  colloc::cr_subst<double, float>(oinst, seg, act, rows_in, rows_out, recs);
+}
+__global__ __launch_bounds__(colloc::CS_THREADS) void colloc_cr_subst_ff(const int* oinst, const colloc::CRSeg* seg,
+    const int* act, const float* rows_in, float* rows_out, float* recs) {
+  colloc::cr_subst<float, float>(oinst, seg, act, rows_in, rows_out, recs);
 }
 
 __global__ __launch_bounds__(Cfg::FW * 64) void colloc_cr_final(colloc::Batch bt, const double* rows,
     const double* Dr, const double* rbc, double* dy, double* dp, int* status) {
-  colloc::cr_final<Cfg::FW>(bt, rows, Dr, rbc, dy, dp, status);
+  colloc::cr_final<Cfg::FW, double>(bt, rows, Dr, rbc, dy, dp, status);
+}
+__global__ __launch_bounds__(Cfg::FW * 64) void colloc_cr_final_f(colloc::Batch bt, const float* rows,
+    const double* Dr, const double* rbc, double* dy, double* dp, int* status) {
+  colloc::cr_final<Cfg::FW, float>(bt, rows, Dr, rbc, dy, dp, status);
 }
 
 __global__ __launch_bounds__(64) void colloc_cr_back(int level, colloc::Batch bt, const int* oinst,
     const colloc::CRSeg* seg, const double* recs, const double* rows_in, double* dy, const double* dp) {
-  colloc::cr_back(level, bt, oinst, seg, recs, rows_in, dy, dp);
+  colloc::cr_back<double, double>(level, bt, oinst, seg, recs, rows_in, dy, dp);
+}
+// FP32 cyclic reduction: float32 records with the first level's float64 input rows, or float32 both
+__global__ __launch_bounds__(64) void colloc_cr_back_fd(int level, colloc::Batch bt, const int* oinst,
+    const colloc::CRSeg* seg, const float* recs, const double* rows_in, double* dy, const double* dp) {
+  colloc::cr_back<float, double>(level, bt, oinst, seg, recs, rows_in, dy, dp);
+}
+__global__ __launch_bounds__(64) void colloc_cr_back_ff(int level, colloc::Batch bt, const int* oinst,
+    const colloc::CRSeg* seg, const float* recs, const float* rows_in, double* dy, const double* dp) {
+  colloc::cr_back<float, float>(level, bt, oinst, seg, recs, rows_in, dy, dp);
 }
 
 __global__ __launch_bounds__(256) void colloc_recover(int nint, colloc::Batch bt, const double* rec,

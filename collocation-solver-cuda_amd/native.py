@@ -48,6 +48,7 @@ SIGNATURES = {
     "cs_set_condense_kernel": (c_int, [c_void_p, c_int]),
     "cs_set_newton_graph": (c_int, [c_void_p, c_int]),
     "cs_set_refinement": (c_int, [c_void_p, c_int]),
+    "cs_set_babd_precision": (c_int, [c_void_p, c_int]),
     "cs_linear_residual": (c_int, [c_void_p, P_D]),
     "cs_refine_direction": (c_int, [c_void_p, c_int]),
     "cs_get_direction": (c_int, [c_void_p, P_D, P_D, P_D, P_D]),

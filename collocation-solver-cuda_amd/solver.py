@@ -279,6 +279,12 @@ class CollocationSolver:
         iteration (the same results as the host-driven loop)."""
         check(lib().cs_set_newton_graph(self.h, 1 if on else 0))
 
+    def set_babd_precision(self, bits):
+        """Stored type of the cyclic reduction's rows and records: 64
+        (default) or 32 (float32 levels, float64 boundary solve and back
+        substitution)."""
+        check(lib().cs_set_babd_precision(self.h, int(bits)))
+
     def set_refinement(self, steps):
         """Iterative refinement steps per Newton direction (0: none): the
         float64 linear residual rho = F + J delta and a correction solve with

@@ -196,6 +196,16 @@ int cs_set_newton_graph(cs_handle* h, int on);
  * direction with iterative refinement"; no counterpart in the reference. */
 int cs_set_refinement(cs_handle* h, int steps);
 
+/* Stored type of the cyclic reduction's rows and records (the partitioned
+ * QR of solve_babd_system.py:93-258 and its back substitution,
+ * :495-572): 64 (default) or 32.  With 32 every level after the first
+ * stores float32 rows and records and the substitution pairing computes in
+ * float32; the Householder pairing, the boundary-block solve (the reduced
+ * system of qr_decomposition, :338-440) and the back substitution compute
+ * in float64.  Part of the FP32 direction (with cs_set_direction_precision
+ * and cs_set_refinement); no counterpart in the reference. */
+int cs_set_babd_precision(cs_handle* h, int bits);
+
 /* max |rho| per instance (B doubles) of rho = F + J delta, the linearised
  * collocation system (the stages of compute_f_q_parallel, solver:478-559,
  * and r at the boundary) at the current state, for the direction the last
