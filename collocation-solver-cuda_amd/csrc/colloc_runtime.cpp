@@ -209,6 +209,8 @@ struct cs_handle {
   int pair_threads = 0;                 // block size of colloc_cr_pair (from the module)
   int h_zero = 0;                       // the module's BABD rows carry no H (identically zero)
   int jac_parts = 1;                    // threads per stage of the Jacobian kernel (from the module)
+  int tab_g = 0;                        // table-form callbacks: points per workgroup (0: straight-line, a thread each)
+  int large = 0;                        // the module's large-block condensation and cyclic reduction (ny > 64)
   // device-resident Newton loop (cs_set_newton_graph)
   bool use_graph = false;
   int refine = 0;                       // refinement steps per Newton direction (cs_set_refinement)
@@ -392,6 +394,13 @@ int upload_doubles(cs_handle* h, double* dev, const std::vector<double>& v) {
   return CS_OK;
 }
 
+// Workgroups of a table-form callback kernel over the stages of nlaunch
+// intervals and the B instances' boundary conditions, tab_g points each.
+unsigned point_groups(cs_handle* h, int nlaunch) {
+  const long long g = h->tab_g;
+  return (unsigned)(((long long)nlaunch * h->m + g - 1) / g + (h->B + g - 1) / g);
+}
+
 // Residual of a packed state for the instances in mask (null: all); writes
 // f_a/f_b/y_tilde/r_bc when full; per-instance norms to norms[B].
 int residual_to(cs_handle* h, const double* st, double alpha, bool full, const int* mask,
@@ -410,7 +419,8 @@ int residual_to(cs_handle* h, const double* st, double alpha, bool full, const i
   void* args[] = {&nint, &bt, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &h->ytil, &h->fa, &h->fb, &h->rbc, &slot};
   const int nlaunch = h->launch_intervals();
   unsigned nthreads = (unsigned)(nlaunch * h->m + h->B);
-  return launch(h, full ? F_RES_FULL : F_RES_NORM, (nthreads + 255) / 256, 256, args);
+  return launch(h, full ? F_RES_FULL : F_RES_NORM, h->tab_g ? point_groups(h, nlaunch) : (nthreads + 255) / 256, 256,
+                args);
 }
 
 int residual(cs_handle* h, const double* st, double alpha, bool full, const int* mask, double* norms) {
@@ -432,7 +442,9 @@ int jacobian(cs_handle* h, double alpha, const int* mask) {
   // the FP32 direction stores the stage Jacobians in float32 (in the same buffer)
   h->jac_bits = h->dir_bits;
   TRY(jacobian_static(h));
-  return launch(h, h->jac_bits == 32 ? F_JAC32 : F_JAC, (nthreads + 255) / 256 * (unsigned)h->jac_parts, 256, args);
+  const unsigned grid = h->tab_g ? point_groups(h, h->launch_intervals())
+                                 : (nthreads + 255) / 256 * (unsigned)h->jac_parts;
+  return launch(h, h->jac_bits == 32 ? F_JAC32 : F_JAC, grid, 256, args);
 }
 
 int jacobian_static(cs_handle* h) {
@@ -907,12 +919,15 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
   h->cw = meta[5];
   h->fw = meta[6];
   HIPCHK(hipModuleGetGlobal(&gp, &gsz, h->module, "colloc_caps"));
-  int caps[4] = {};
+  int caps[8] = {};
+  if (gsz < sizeof caps) return fail(CS_E_SHAPE, "module capabilities hold %zu bytes, expected %zu", gsz, sizeof caps);
   HIPCHK(hipMemcpy(caps, (void*)gp, sizeof caps, hipMemcpyDeviceToHost));
   h->mf_ok = caps[0];
   h->pair_threads = caps[1];
   h->h_zero = caps[2];
   h->jac_parts = caps[3] > 0 ? caps[3] : 1;
+  h->tab_g = caps[4];
+  h->large = caps[5];
   // the static stage-Jacobian entries are the new module's constants: the
   // buffer holds the previous module's until they are written again
   h->jac_static_bits = 0;
@@ -1060,7 +1075,8 @@ int init_from_saved(cs_handle* h) {
   double* zt = h->zt();
   Batch bt = h->batch(nullptr);
   void* args[] = {&nint, &bt, &y, &zb, &p, &h->c, &yd, &zt};
-  TRY(launch(h, F_INIT, (unsigned)((nint * h->m + 255) / 256), 256, args));
+  TRY(launch(h, F_INIT, h->tab_g ? (unsigned)((nint * h->m + h->tab_g - 1) / h->tab_g)
+                                  : (unsigned)((nint * h->m + 255) / 256), 256, args));
   return sync(h);
 }
 
@@ -1590,7 +1606,8 @@ int cs_segment_residual_ex(cs_handle* h, double alpha, double tol, int residual_
   Batch bt = h->batch(nullptr);
   void* args[] = {&nint, &bt, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &tol, &residual_type, &st, &rel,
                   &h->segres, &slot};
-  TRY(launch(h, F_SEGRES, (unsigned)((nint + h->B + 255) / 256), 256, args));
+  TRY(launch(h, F_SEGRES, h->tab_g ? (unsigned)(nint + (h->B + h->tab_g - 1) / h->tab_g)
+                                    : (unsigned)((nint + h->B + 255) / 256), 256, args));
   std::vector<double> mr(h->B);
   TRY(read_slots(h, slot, mr.data()));
   if (max_residual) std::memcpy(max_residual, mr.data(), mr.size() * sizeof(double));

@@ -156,4 +156,28 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* slot,
   }
 }
 
+// Tables of a problem program in the polynomial form (large problems): the
+// values of nb base atoms (y | z | p | instance constants | alpha) at a
+// point, the nodes (nonlinear functions of polynomials, atoms nb ..
+// natoms-1), the monomials (products of atom powers; monomial 0 is 1) and
+// the polynomials, in sliced ELL slabs of 64 rows (term t of row q of slab s
+// at slab_term_off[s] + 64 t + q).  Node rows are slabs
+// [node_slab_off[L], node_slab_off[L + 1]) for the nodes of level L + 1
+// (slab row q = node index); output rows are the slabs from out0 on (slab
+// row q = slab_rows[64 s + q], the output's destination, -1: padding).
+struct TabProg {
+  int nb, natoms, nmono, nlev, nslab, out0;
+  const int* mono_fac_off;
+  const int* mono_fac_atom;
+  const signed char* mono_fac_pow;
+  const int* mono_lev_off;
+  const int* node_slab_off;
+  const unsigned char* node_kind;
+  const double* node_r;
+  const int* slab_term_off;
+  const int* slab_rows;
+  const double* coef;
+  const int* mono;
+};
+
 }  // namespace colloc

@@ -58,7 +58,9 @@ __host__ __device__ constexpr bool p_active(int j) {
   return false;
 }
 static_assert(CR_PARTS == 4, "cr_pair sums over the four 16-lane rows of a wave");
+#if !COLLOC_LARGE
 static_assert(NY <= 64, "cyclic-reduction kernels hold one row per lane");
+#endif
 
 constexpr int DRSZ = (NY + NP) * (2 * NY + NP);   // one instance's Dr
 constexpr int NRB = NY + NP;                       // one instance's r_bc
@@ -128,6 +130,7 @@ __device__ __forceinline__ void seg_max_to(double v, int b, unsigned long long* 
   if (b >= 0 && b != b0) atomicMax(slot + b, (unsigned long long)__double_as_longlong(fabs(v)));
 }
 
+#if !COLLOC_TABLE_FORM
 // ---------------------------------------------------------------- residual
 // One thread per (interval, stage); thread Nint*M + b evaluates r of
 // instance b.  Restates collocation_update_kernel + compute_f_q_kernel1/2
@@ -301,6 +304,8 @@ __device__ __forceinline__ void jacobian(int nint, Batch bt, const double* __res
   }
 }
 
+#endif  // !COLLOC_TABLE_FORM
+
 // The static entries of every stage Jacobian block (prob::jac_static_mask /
 // jac_static_val: entries that depend on nothing but the problem's
 // compile-time constants) for stages [0, nst) of the buffer, written once per
@@ -315,6 +320,7 @@ __device__ __forceinline__ void jacobian_static(long long nst, TJ* __restrict__ 
   }
 }
 
+#if !COLLOC_LARGE
 // Stage eliminations of one interval (the first phase of both condensation
 // kernels): wave j eliminates the algebraic unknowns of stage j, computing
 // in R (double, or float for the FP32 direction).  Writes T_j to sT (LDS,
@@ -1688,6 +1694,8 @@ __device__ __forceinline__ void cr_back(int level, Batch bt, const int* __restri
   if (lane < NY) dy[(size_t)mid * NY + lane] = x;
 }
 
+#endif  // !COLLOC_LARGE
+
 // ---------------------------------------------------------------- recover
 // One workgroup per interval (recover_delta_k_kernel, solver:1619-1650):
 //   u_0 = dy_i, u_j = U_j v,  dydot_j = T_j [u_j; dp; 1],  dz_j = -G_j [u_j; dp; 1].
@@ -1896,6 +1904,7 @@ __device__ __forceinline__ void state_add(long long n, const double* __restrict_
   if (e < n) out[e] = b ? a[e] + b[e] : a[e];
 }
 
+#if !COLLOC_TABLE_FORM
 // -------------------------------------------------------- initial input
 // form_initial_input_kernel (solver:316-356).
 __device__ __forceinline__ void init_input(int nint, Batch bt, const double* __restrict__ y0,
@@ -2044,6 +2053,8 @@ __device__ __forceinline__ void segres(int nint, Batch bt, const double* __restr
   }
   seg_max_to(val, ib, maxres);
 }
+
+#endif  // !COLLOC_TABLE_FORM
 
 // ---------------------------------------------------------- recover z
 // recover_solution_kernel (solver:2049-2075).
@@ -2221,6 +2232,13 @@ __device__ __forceinline__ void nt_accept(int B, NtState st, int* act, double* s
 
 }  // namespace colloc
 
+#if COLLOC_TABLE_FORM
+#include "colloc_tables.h"
+#endif
+#if COLLOC_LARGE
+#include "colloc_large.h"
+#endif
+
 // ------------------------------------------------------------ entry points
 extern "C" {
 __global__ void colloc_fill_u64(unsigned long long* p, unsigned long long v, int n) { colloc::fill_u64(p, v, n); }
@@ -2246,26 +2264,35 @@ __global__ void colloc_nt_tick(colloc::NtState st) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *st.it += 1;
 }
 
+// The table form launches one workgroup per TAB_G points (stages, then
+// instances), the straight-line form one thread per point.
+#if COLLOC_TABLE_FORM
+#define COLLOC_CB(name) name##_tab
+constexpr int kTabPoints = colloc::TAB_G;
+#else
+#define COLLOC_CB(name) name
+constexpr int kTabPoints = 0;
+#endif
 __global__ __launch_bounds__(256) void colloc_residual_full(int nint, colloc::Batch bt, const double* t,
     const double* y, const double* ydot, const double* ztil, const double* p, const double* c, double alpha,
     double* ytil, double* fa, double* fb, double* rbc, unsigned long long* norm) {
-  colloc::residual<true>(nint, bt, t, y, ydot, ztil, p, c, alpha, ytil, fa, fb, rbc, norm);
+  colloc::COLLOC_CB(residual)<true>(nint, bt, t, y, ydot, ztil, p, c, alpha, ytil, fa, fb, rbc, norm);
 }
 
 __global__ __launch_bounds__(256) void colloc_residual_norm(int nint, colloc::Batch bt, const double* t,
     const double* y, const double* ydot, const double* ztil, const double* p, const double* c, double alpha,
     double* ytil, double* fa, double* fb, double* rbc, unsigned long long* norm) {
-  colloc::residual<false>(nint, bt, t, y, ydot, ztil, p, c, alpha, ytil, fa, fb, rbc, norm);
+  colloc::COLLOC_CB(residual)<false>(nint, bt, t, y, ydot, ztil, p, c, alpha, ytil, fa, fb, rbc, norm);
 }
 
 __global__ __launch_bounds__(256) void colloc_jacobian(int nint, colloc::Batch bt, const double* y,
     const double* ytil, const double* ztil, const double* p, const double* c, double alpha, double* jac, double* Dr) {
-  colloc::jacobian<double>(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
+  colloc::COLLOC_CB(jacobian)<double>(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
 }
 // stage Jacobians stored in float32 (the FP32 direction)
 __global__ __launch_bounds__(256) void colloc_jacobian32(int nint, colloc::Batch bt, const double* y,
     const double* ytil, const double* ztil, const double* p, const double* c, double alpha, float* jac, double* Dr) {
-  colloc::jacobian<float>(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
+  colloc::COLLOC_CB(jacobian)<float>(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
 }
 __global__ __launch_bounds__(256) void colloc_jacobian_static(long long nst, double* jac) {
   colloc::jacobian_static<double>(nst, jac);
@@ -2274,6 +2301,7 @@ __global__ __launch_bounds__(256) void colloc_jacobian_static32(long long nst, f
   colloc::jacobian_static<float>(nst, jac);
 }
 
+#if !COLLOC_LARGE
 __global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense(const int* iinst, const int* queue,
     const unsigned long long* nq, const double* t, const double* jac, const double* fa, const double* fb,
     double* rows, double* rec, int* status) {
@@ -2316,7 +2344,11 @@ __global__ __launch_bounds__(colloc::MF_THREADS) void colloc_condense_mf32(int n
 // [1] the block size of colloc_cr_pair; [2] the BABD rows carry no H block
 // (no stage function depends on a parameter: H is identically zero);
 // [3] workgroups per 256 stages of colloc_jacobian (threads per stage)
-__device__ int colloc_caps[4] = {colloc::MF_OK ? 1 : 0, colloc::CR_THREADS, colloc::HZ ? 1 : 0, prob::JAC_PARTS};
+// [4] points per workgroup of the table-form callback kernels (0: one
+// thread per point, straight-line callbacks); [5] the large-block kernels
+// (colloc_large.h) replace the register-resident ones
+__device__ int colloc_caps[8] = {colloc::MF_OK ? 1 : 0, colloc::CR_THREADS, colloc::HZ ? 1 : 0, prob::JAC_PARTS,
+                                 kTabPoints, 0, 0, 0};
 
 // The FP32-direction condensation: two 512-thread workgroups per CU (half the
 // LDS and registers of the float64 kernel).
@@ -2382,6 +2414,7 @@ __global__ __launch_bounds__(64) void colloc_cr_back_ff(int level, colloc::Batch
     const colloc::CRSeg* seg, const float* recs, const float* rows_in, double* dy, const double* dp) {
   colloc::cr_back<float, float>(level, bt, oinst, seg, recs, rows_in, dy, dp);
 }
+#endif  // !COLLOC_LARGE
 
 __global__ __launch_bounds__(256) void colloc_recover(int nint, colloc::Batch bt, const double* rec,
     const double* dy, const double* dp, double* dydot, double* dz) {
@@ -2424,13 +2457,13 @@ __global__ __launch_bounds__(64) void colloc_axpy_listed(colloc::Seg4 s, colloc:
 
 __global__ __launch_bounds__(256) void colloc_init_input(int nint, colloc::Batch bt, const double* y0,
     const double* z0, const double* p, const double* c, double* ydot, double* ztil) {
-  colloc::init_input(nint, bt, y0, z0, p, c, ydot, ztil);
+  colloc::COLLOC_CB(init_input)(nint, bt, y0, z0, p, c, ydot, ztil);
 }
 
 __global__ __launch_bounds__(256) void colloc_segres(int nint, colloc::Batch bt, const double* t, const double* y,
     const double* ydot, const double* ztil, const double* p, const double* c, double alpha, double tol,
     int rtype, int scale_time, int relative, double* residual, unsigned long long* maxres) {
-  colloc::segres(nint, bt, t, y, ydot, ztil, p, c, alpha, tol, rtype, scale_time, relative, residual, maxres);
+  colloc::COLLOC_CB(segres)(nint, bt, t, y, ydot, ztil, p, c, alpha, tol, rtype, scale_time, relative, residual, maxres);
 }
 
 __global__ __launch_bounds__(256) void colloc_recover_z(int nnodes, colloc::Batch bt, const double* ztil, double* z) {

@@ -597,21 +597,15 @@ def active_p_columns(sym):
     return out
 
 
-def build_unit(sym, m, tables):
-    """Complete hiprtc translation unit: callbacks, sizes, tables, kernels.
+LARGE_NY = 64   # the register-resident condensation and cyclic-reduction kernels hold ny <= 64
 
-    tables: object with lobatto A, b, c and gauss t, w plus the Lagrange basis
-    L(tau) and its integral I(tau) at the Gauss points and L(0), L(1)."""
-    ny, nz, np_ = sym.size_y, sym.size_z, sym.size_p
-    nc = len(sym.instance_constants)
-    pact = active_p_columns(sym)
-    cw, fw = choose_waves(ny, nz, np_, m, npa=len(pact))
-    parts = ['#include "colloc_common.h"', emit_problem(sym)]
-    parts.append("struct Cfg {\n  static constexpr int NY = %d, NZ = %d, NP = %d, NC = %d, M = %d, CW = %d, FW = %d;\n"
-                 "  // parameters the stage functions depend on (the other H columns are zero)\n"
-                 "  static constexpr int NPA = %d;\n  static constexpr int PACT[%d] = {%s};\n};"
-                 % (ny, nz, np_, nc, m, cw, fw, len(pact), max(len(pact), 1),
-                    ", ".join(str(j) for j in pact) if pact else "0"))
+
+def _unit_tail(ny, nz, np_, nc, m, cw, fw, pact, tables):
+    parts = ["struct Cfg {\n  static constexpr int NY = %d, NZ = %d, NP = %d, NC = %d, M = %d, CW = %d, FW = %d;\n"
+             "  // parameters the stage functions depend on (the other H columns are zero)\n"
+             "  static constexpr int NPA = %d;\n  static constexpr int PACT[%d] = {%s};\n};"
+             % (ny, nz, np_, nc, m, cw, fw, len(pact), max(len(pact), 1),
+                ", ".join(str(j) for j in pact) if pact else "0")]
     parts.append(_carray("c_rkA", tables.A.ravel()))
     parts.append(_carray("c_rkb", tables.b))
     parts.append(_carray("c_rkc", tables.c))
@@ -623,8 +617,137 @@ def build_unit(sym, m, tables):
     parts.append('extern "C" __device__ int colloc_meta[8] = {%d, %d, %d, %d, %d, %d, %d, %d};'
                  % (ny, nz, np_, m, nc, cw, fw, len(pact)))
     parts.append('#include "colloc_kernels.h"')
+    return parts
+
+
+def build_unit(sym, m, tables):
+    """Complete hiprtc translation unit: callbacks, sizes, tables, kernels.
+
+    tables: object with lobatto A, b, c and gauss t, w plus the Lagrange basis
+    L(tau) and its integral I(tau) at the Gauss points and L(0), L(1)."""
+    ny, nz, np_ = sym.size_y, sym.size_z, sym.size_p
+    nc = len(sym.instance_constants)
+    pact = active_p_columns(sym)
+    cw, fw = choose_waves(ny, nz, np_, m, npa=len(pact))
+    parts = ["#define COLLOC_TABLE_FORM 0", "#define COLLOC_LARGE 0", '#include "colloc_common.h"',
+             emit_problem(sym)]
+    parts += _unit_tail(ny, nz, np_, nc, m, cw, fw, pact, tables)
     return "\n".join(parts) + "\n"
 
 
 def unit_hash(text):
     return hashlib.sha256(text.encode()).hexdigest()[:24]
+
+
+# ------------------------------------------------- polynomial (table) form
+# Problems whose straight-line callbacks would be too large for the device
+# compiler (and for the instruction cache) are emitted as tables of their
+# polynomial form (polyform.py), walked by the evaluator of colloc_tables.h.
+TABLE_TERMS = 100000   # Jacobian + f + g terms beyond which the table form is used
+
+
+def _int_table(decl, vals, per_line=32):
+    vals = [int(v) for v in vals] or [0]
+    return _table(decl, [str(v) for v in vals], per_line)
+
+
+def _dbl_table(decl, vals, per_line=8):
+    vals = [float(v) for v in vals] or [0.0]
+    return _table(decl, [repr(v) for v in vals], per_line)
+
+
+def emit_program(name, prog):
+    """C tables of one polyform.Program and a function returning its TabProg."""
+    out = [
+        _int_table("__device__ const int %s_mono_fac_off" % name, prog.mono_fac_off),
+        _int_table("__device__ const int %s_mono_fac_atom" % name, prog.mono_fac_atom),
+        _int_table("__device__ const signed char %s_mono_fac_pow" % name, prog.mono_fac_pow),
+        _int_table("__device__ const int %s_mono_lev_off" % name, prog.mono_lev_off),
+        _int_table("__device__ const int %s_node_slab_off" % name, prog.node_slab_off),
+        _int_table("__device__ const unsigned char %s_node_kind" % name, prog.node_kind),
+        _dbl_table("__device__ const double %s_node_r" % name, prog.node_r),
+        _int_table("__device__ const int %s_slab_term_off" % name, prog.slab_term_off),
+        _int_table("__device__ const int %s_slab_rows" % name, [v for row in prog.slab_rows for v in row]),
+        _dbl_table("__device__ const double %s_coef" % name, prog.coef),
+        _int_table("__device__ const int %s_mono" % name, prog.mono),
+    ]
+    nb = prog.natoms - prog.nnode
+    out.append("__device__ __forceinline__ colloc::TabProg %s_prog() {\n"
+               "  return colloc::TabProg{%d, %d, %d, %d, %d, %d, %s_mono_fac_off, %s_mono_fac_atom, "
+               "%s_mono_fac_pow, %s_mono_lev_off, %s_node_slab_off, %s_node_kind, %s_node_r, %s_slab_term_off, "
+               "%s_slab_rows, %s_coef, %s_mono};\n}"
+               % ((name, nb, prog.natoms, prog.nmono, prog.nlev, prog.nslab, prog.out_slab_first) + (name,) * 11))
+    out.append("constexpr int %s_NATOMS = %d, %s_NMONO = %d;" % (name.upper(), prog.natoms, name.upper(),
+                                                                prog.nmono))
+    return "\n".join(out)
+
+
+def table_programs(P):
+    """The programs of a BvpDaePoly: fg (f then g: destinations
+    0 .. ny+nz-1), jac (the state-dependent entries of the stage block
+    [Df; Dg]: destination = entry index), ri / rf (the initial / final
+    boundary rows: destination = row of r), dri / drf (their rows of Dr:
+    destination = entry of Dr, columns [y0 | y1 | p]).  Returns (programs,
+    static mask, static values) of the stage block."""
+    from . import polyform
+    ny, nz, np_ = P.size_y, P.size_z, P.size_p
+    fg = polyform.Program(P.engine_fg, P.f_polys + P.g_polys, list(range(ny + nz)))
+    block = [e for row in P.Df_polys for e in row] + [e for row in P.Dg_polys for e in row]
+    sval = [P.static_value(e) for e in block]
+    dyn = [k for k, v in enumerate(sval) if v is None]
+    jac = polyform.Program(P.engine_fg, [block[k] for k in dyn], dyn)
+    ndr = 2 * ny + np_
+    nri = len(P.ri_polys)
+    ri = polyform.Program(P.engine_ri, list(P.ri_polys), list(range(nri)))
+    rf = polyform.Program(P.engine_rf, list(P.rf_polys), [nri + i for i in range(len(P.rf_polys))])
+
+    def dr_program(engine, rows, first, initial):
+        outs, dest = [], []
+        for i, row in enumerate(rows):
+            for j in range(ndr):
+                if j < ny:
+                    e = row[j] if initial else {}
+                elif j < 2 * ny:
+                    e = {} if initial else row[j - ny]
+                else:
+                    e = row[ny + j - 2 * ny]
+                outs.append(e)
+                dest.append((first + i) * ndr + j)
+        return polyform.Program(engine, outs, dest)
+
+    dri = dr_program(P.engine_ri, P.Dri_polys, 0, True)
+    drf = dr_program(P.engine_rf, P.Drf_polys, nri, False)
+    mask = [0 if v is None else 1 for v in sval]
+    vals = [0.0 if v is None else v for v in sval]
+    return {"fg": fg, "jac": jac, "ri": ri, "rf": rf, "dri": dri, "drf": drf}, mask, vals
+
+
+def emit_problem_tables(P):
+    """HIP source of namespace prob for the polynomial (table) form."""
+    progs, mask, vals = table_programs(P)
+    out = ["// Generated BVP-DAE callbacks in polynomial table form (the _abvp_* contract of "
+           "OCP2ABVP.py:203-327, evaluated by colloc_tables.h).",
+           "namespace prob {",
+           "constexpr bool TABLE_FORM = true;",
+           "constexpr int TB_NB = %d;   // base atoms: y | z | p | instance constants | alpha" % P.nb]
+    for name, prog in progs.items():
+        out.append(emit_program(name, prog))
+    out.append(_int_table("__device__ const unsigned char jac_static_mask", mask))
+    out.append(_dbl_table("__device__ const double jac_static_val", vals))
+    out.append("constexpr int JAC_DYNAMIC = %d;   // dynamic entries per stage block" % mask.count(0))
+    out.append("constexpr int JAC_PARTS = 1;")
+    out.append("}  // namespace prob")
+    return "\n".join(out) + "\n"
+
+
+def build_unit_tables(P, m, tables):
+    """hiprtc translation unit of a problem in the polynomial (table) form."""
+    ny, nz, np_ = P.size_y, P.size_z, P.size_p
+    nc = P.n_inst
+    pact = P.active_p_columns()
+    large = ny > LARGE_NY
+    cw, fw = (0, 0) if large else choose_waves(ny, nz, np_, m, npa=len(pact))
+    parts = ["#define COLLOC_TABLE_FORM 1", "#define COLLOC_LARGE %d" % (1 if large else 0),
+             '#include "colloc_common.h"', emit_problem_tables(P)]
+    parts += _unit_tail(ny, nz, np_, nc, m, cw, fw, pact, tables)
+    return "\n".join(parts) + "\n"

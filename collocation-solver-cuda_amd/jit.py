@@ -10,14 +10,16 @@ import hashlib
 import json
 import os
 
-from . import build, emitter, native, tables
+from . import build, emitter, native, polyform, tables
 from .ocp import _statements, parse_ocp
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE_DIR = os.path.join(HERE, "_jit")
-_SOURCES = ["emitter.py", "tables.py", "ocp.py", "jit.py",
+_SOURCES = ["emitter.py", "polyform.py", "tables.py", "ocp.py", "jit.py",
             os.path.join("csrc", "kernels", "colloc_common.h"),
             os.path.join("csrc", "kernels", "colloc_gj.h"),
+            os.path.join("csrc", "kernels", "colloc_tables.h"),
+            os.path.join("csrc", "kernels", "colloc_large.h"),
             os.path.join("csrc", "kernels", "colloc_kernels.h")]
 
 
@@ -55,24 +57,53 @@ def code_text(ocp_text):
     return ";\n".join(keep)
 
 
-def cache_key(ocp_text, m):
+def cache_key(ocp_text, m, form="auto"):
     h = hashlib.sha256()
     h.update(code_text(ocp_text).encode())
     h.update(b"\0m=%d\0" % m)
+    if form != "auto":
+        h.update(b"\0form=%s\0" % form.encode())
     h.update(_sources_digest().encode())
     return h.hexdigest()[:32]
 
 
-def code_object(ocp_text, m, verbose=False):
-    """gfx950 code object (bytes) of the problem's kernels, compiling on a cache miss."""
-    key = cache_key(ocp_text, m)
+def choose_form(ocp, form="auto"):
+    """"tables" (the polynomial table form, polyform.py) or "code" (straight-
+    line callbacks from sympy) for a problem: tables when ny exceeds what the
+    register-resident kernels hold or the callbacks would exceed
+    emitter.TABLE_TERMS terms, unless form forces one.  Returns (form,
+    BvpDaePoly or None)."""
+    if form == "code":
+        return "code", None
+    ny = sizes_of(ocp)[0]
+    try:
+        P = polyform.BvpDaePoly(ocp)
+    except polyform.Unsupported:
+        if form == "tables" or ny > emitter.LARGE_NY:
+            raise
+        return "code", None
+    if form == "tables" or ny > emitter.LARGE_NY or P.term_count() > emitter.TABLE_TERMS:
+        return "tables", P
+    return "code", None
+
+
+def code_object(ocp_text, m, verbose=False, form="auto"):
+    """gfx950 code object (bytes) of the problem's kernels, compiling on a
+    cache miss.  form: "auto", "code" or "tables" (see choose_form)."""
+    key = cache_key(ocp_text, m, form)
     path = os.path.join(CACHE_DIR, key + ".hsaco")
     if os.path.exists(path):
         with open(path, "rb") as f:
             return f.read()
     ocp = parse_ocp(ocp_text)
-    sym = emitter.BvpDaeSymbolic(ocp)
-    unit = emitter.build_unit(sym, m, tables.collocation_tables(m))
+    kind, P = choose_form(ocp, form)
+    if kind == "tables":
+        unit = emitter.build_unit_tables(P, m, tables.collocation_tables(m))
+        active = P.active_p_columns()
+    else:
+        sym = emitter.BvpDaeSymbolic(ocp)
+        unit = emitter.build_unit(sym, m, tables.collocation_tables(m))
+        active = emitter.active_p_columns(sym)
     code, log = native.jit_compile(unit, "colloc_%s.hip" % key, build.KERNEL_DIR)
     if verbose and log.strip():
         print(log)
@@ -82,16 +113,17 @@ def code_object(ocp_text, m, verbose=False):
         f.write(code)
     os.replace(tmp, path)
     with open(os.path.join(CACHE_DIR, key + ".json"), "w") as f:
-        json.dump({"m": m, "sizes": sizes_of(ocp), "bytes": len(code),
-                   "active_params": emitter.active_p_columns(sym)}, f)
+        json.dump({"m": m, "sizes": sizes_of(ocp), "bytes": len(code), "form": kind,
+                   "active_params": active}, f)
     return code
 
 
-def unit_info(ocp_text, m):
-    """Metadata of the problem's code object: sizes and the parameters whose
-    condensed columns are kept (compiles the code object on a cache miss)."""
-    path = os.path.join(CACHE_DIR, cache_key(ocp_text, m) + ".json")
+def unit_info(ocp_text, m, form="auto"):
+    """Metadata of the problem's code object: sizes, form and the parameters
+    whose condensed columns are kept (compiles the code object on a cache
+    miss)."""
+    path = os.path.join(CACHE_DIR, cache_key(ocp_text, m, form) + ".json")
     if not os.path.exists(path):
-        code_object(ocp_text, m)
+        code_object(ocp_text, m, form=form)
     with open(path) as f:
         return json.load(f)

@@ -104,9 +104,10 @@ class CollocationSolver:
     values and the arrays are the instances' arrays concatenated.
     """
 
-    def __init__(self, bvp, m, n_max=None, batch=1):
+    def __init__(self, bvp, m, n_max=None, batch=1, form="auto"):
         self.bvp = bvp
         self.m = m
+        self.form = form   # device code of the callbacks: "auto", "code" or "tables" (jit.choose_form)
         self.ny, self.nz, self.np = bvp.size_y, bvp.size_z, bvp.size_p
         self.n_max = int(n_max or max(bvp.maximum_nodes, bvp.N))
         self.batch = int(batch)
@@ -125,7 +126,7 @@ class CollocationSolver:
         generated callbacks and the kernels specialised to them); the mesh and
         state are kept.  The role of importing a generated bvp_problem module
         (solver:10-17)."""
-        code = jit.code_object(bvp.ocp_text, self.m)
+        code = jit.code_object(bvp.ocp_text, self.m, form=self.form)
         buf = native.ctypes.create_string_buffer(code, len(code))
         check(lib().cs_load_module(self.h, buf, len(code)))
         self.bvp = bvp
@@ -207,7 +208,8 @@ class CollocationSolver:
 
     # -- stages --------------------------------------------------------------
     def form_initial_input(self, y0, z0, p0):
-        """form_initial_input_parallel (solver:287): returns y, y_dot, z_tilde."""
+        """form_initial_input_parallel (solver:287): the handle's state becomes
+        y0, the initial y_dot and z_tilde, and p0 (read it with get_state)."""
         y0 = np.ascontiguousarray(y0, dtype=np.float64)
         z0 = np.ascontiguousarray(z0, dtype=np.float64).reshape(self.N, self.nz)
         p0 = np.ascontiguousarray(p0, dtype=np.float64)
