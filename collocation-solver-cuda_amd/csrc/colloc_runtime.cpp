@@ -96,6 +96,8 @@ enum Fn {
   F_CR_FINAL, F_CR_BACK, F_RECOVER, F_RECOVER32, F_AXPY, F_INIT, F_SEGRES, F_RECZ, F_JAC_STATIC, F_JAC_STATIC32,
   F_AXPY_LISTED, F_COND_MF32, F_NT_CHECK, F_NT_SCAN, F_NT_FILL, F_NT_LS, F_NT_ACCEPT, F_NT_TICK, F_LINRES, F_LINRES32,
   F_STATE_ADD, F_CR_PAIR_DF, F_CR_PAIR_FF, F_CR_SUBST_DF, F_CR_SUBST_FF, F_CR_FINAL_F, F_CR_BACK_FD, F_CR_BACK_FF,
+  F_L_STAGE, F_L_STAGE32, F_L_SINIT, F_L_SINIT32, F_L_TERM, F_L_TERM32, F_L_SOUT, F_L_SOUT32, F_L_CR_SUBST,
+  F_L_CR_PAIR, F_L_CR_FINAL, F_L_CR_BACK,
   F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
@@ -107,7 +109,21 @@ const char* kFnNames[F_COUNT] = {
   "colloc_axpy_listed", "colloc_condense_mf32", "colloc_nt_check", "colloc_nt_list_scan", "colloc_nt_list_fill",
   "colloc_nt_line_search", "colloc_nt_accept", "colloc_nt_tick", "colloc_linres", "colloc_linres32",
   "colloc_state_add", "colloc_cr_pair_df", "colloc_cr_pair_ff", "colloc_cr_subst_df", "colloc_cr_subst_ff",
-  "colloc_cr_final_f", "colloc_cr_back_fd", "colloc_cr_back_ff"};
+  "colloc_cr_final_f", "colloc_cr_back_fd", "colloc_cr_back_ff",
+  "colloc_l_stage_elim", "colloc_l_stage_elim32", "colloc_l_series_init", "colloc_l_series_init32",
+  "colloc_l_series_term", "colloc_l_series_term32", "colloc_l_series_out", "colloc_l_series_out32",
+  "colloc_l_cr_subst", "colloc_l_cr_pair", "colloc_l_cr_final", "colloc_l_cr_back"};
+// the large-block kernels (ny > 64) take the place of the register-resident
+// condensation and cyclic reduction: a module has one set or the other
+bool large_kernel(int f) { return f >= F_L_STAGE; }
+bool small_kernel(int f) {
+  return (f >= F_COND && f <= F_CR_BACK) || f == F_COND_MF32 || (f >= F_CR_PAIR_DF && f <= F_CR_BACK_FF);
+}
+constexpr int kLargeMaxTerms = 48;        // L_MAX_TERMS of the large-block series
+constexpr int kLargeSlot = 8;             // L_SLOT: u64 per interval of its convergence slots
+constexpr unsigned kLargeThreads = 256;   // L_THREADS
+constexpr unsigned kLargeQrThreads = 1024;  // L_QR_THREADS
+constexpr unsigned kLargeFinThreads = 1024; // L_FIN_THREADS
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
@@ -211,6 +227,15 @@ struct cs_handle {
   int jac_parts = 1;                    // threads per stage of the Jacobian kernel (from the module)
   int tab_g = 0;                        // table-form callbacks: points per workgroup (0: straight-line, a thread each)
   int large = 0;                        // the module's large-block condensation and cyclic reduction (ny > 64)
+  int l_tiles = 0, l_stiles = 0;        // large-block: workgroups per interval of a series term, per pair of a
+                                        // substitution product
+  double* l_ubuf = nullptr;             // large-block series: the second U buffer and the right-hand sides R
+  double* l_rbuf = nullptr;
+  unsigned long long* l_slots = nullptr;  // per interval: convergence slots (kLargeSlot u64)
+  unsigned long long* l_active = nullptr; // per term: intervals that computed it
+  double* l_work = nullptr;             // large-block Householder pairs' work matrices
+  size_t l_work_cap = 0;                // in doubles
+  double* l_fin = nullptr;              // large-block boundary systems (one per instance)
   // device-resident Newton loop (cs_set_newton_graph)
   bool use_graph = false;
   int refine = 0;                       // refinement steps per Newton direction (cs_set_refinement)
@@ -343,6 +368,7 @@ void drain_prof(cs_handle* h) {
 
 int launch(cs_handle* h, Fn f, unsigned grid, unsigned block, void** args) {
   if (!h->module) return fail(CS_E_NOMODULE, "no problem module loaded");
+  if (!h->fn[f]) return fail(CS_E_UNSUPPORTED, "kernel %s is not in this problem's module", kFnNames[f]);
   if (grid == 0) return CS_OK;
   HIPCHK(hipModuleLaunchKernel(h->fn[f], grid, 1, 1, block, 1, 1, 0, h->stream, args, nullptr));
   return CS_OK;
@@ -463,7 +489,10 @@ int jacobian_static(cs_handle* h) {
 // Condensation: the series kernel handles every interval whose stage system
 // converges as a Neumann series (fine meshes) and queues the others; the
 // Gauss-Jordan kernel redoes the queued ones.
+int condense_large(cs_handle* h, const int* mask, long long ncond);
+
 int condense(cs_handle* h, const int* mask, long long ncond) {
+  if (h->large) return condense_large(h, mask, ncond);
   ProfScope ps(h, CS_KERNEL_CONDENSE);
   h->cond_intervals += (unsigned long long)ncond;
   int nint = h->nint;
@@ -585,7 +614,10 @@ int build_cr_schedule(cs_handle* h) {
 // the same buffers: each float access lies at or below the byte address of
 // the float64 access to the same element.  Level 0 reads the condensation's
 // float64 rows; cr_final and cr_back read the float32 levels.
+int babd_large(cs_handle* h, const int* mask);
+
 int babd(cs_handle* h, const int* mask) {
+  if (h->large) return babd_large(h, mask);
   ProfScope ps(h, CS_KERNEL_BABD);
   double* dy = h->dir + h->st_y;
   double* dp = h->dir + h->st_p;
@@ -618,6 +650,119 @@ int babd(cs_handle* h, const int* mask) {
     void* args[] = {&l, &bt, &oinst, &seg, &h->crrec, &rin, &dy, &dp};
     const Fn fb = !f32 ? F_CR_BACK : (l == 0 ? F_CR_BACK_FD : F_CR_BACK_FF);
     TRY(launch(h, fb, (unsigned)h->lev_nout[l], 64, args));
+  }
+  return CS_OK;
+}
+
+// Large-block condensation (ny > 64, colloc_large.h): the stage
+// eliminations, R, then series terms until every listed interval has
+// converged (the host reads, after each term, how many intervals computed
+// it), then the records and BABD rows.  An interval whose series has not
+// converged after kLargeMaxTerms terms is an error: the large-block path
+// has no Gauss-Jordan fallback.
+int condense_large(cs_handle* h, const int* mask, long long ncond) {
+  ProfScope ps(h, CS_KERNEL_CONDENSE);
+  if (h->capturing || h->nalistp) return fail(CS_E_UNSUPPORTED, "the large-block condensation is host-driven");
+  h->cond_intervals += (unsigned long long)ncond;
+  int nint = h->nint;
+  unsigned long long* counters = h->scal + 2 * h->bmax;
+  TRY(fill_u64(h, counters, 3));
+  TRY(fill_u64(h, h->l_active, kLargeMaxTerms + 1));
+  Batch bt = h->batch_listed(mask);
+  const bool f32 = h->jac_bits == 32;
+  h->rec_bits = h->jac_bits;
+  const unsigned nl = (unsigned)h->launch_intervals();
+  {
+    void* args[] = {&nint, &bt, &h->jac, &h->fa, &h->rec, &h->status};
+    TRY(launch(h, f32 ? F_L_STAGE32 : F_L_STAGE, nl * (unsigned)h->m, kLargeThreads, args));
+  }
+  {
+    void* args[] = {&nint, &bt, &h->t, &h->rec, &h->l_rbuf, &h->l_slots};
+    TRY(launch(h, f32 ? F_L_SINIT32 : F_L_SINIT, nl * (unsigned)(h->m - 1), kLargeThreads, args));
+  }
+  const long long rstride = (long long)h->recn;
+  const long long ustride = (long long)h->ns * (h->ny + h->npa + 1);
+  int nterms = 0;
+  for (int term = 1; term <= kLargeMaxTerms; ++term) {
+    // U(term - 1) is in the record for even term - 1, in l_ubuf for odd
+    void* uold = (term - 1) % 2 == 0 ? (void*)h->rec : (void*)h->l_ubuf;
+    void* unew = term % 2 == 0 ? (void*)h->rec : (void*)h->l_ubuf;
+    long long so = (term - 1) % 2 == 0 ? rstride : ustride;
+    long long sn = term % 2 == 0 ? rstride : ustride;
+    void* args[] = {&nint, &bt, &h->t, &h->rec, &h->l_rbuf, &uold, &so, &unew, &sn, &h->l_slots, &h->l_active,
+                    &term};
+    TRY(launch(h, f32 ? F_L_TERM32 : F_L_TERM, nl * (unsigned)h->l_tiles, kLargeThreads, args));
+    nterms = term;
+    if (term >= 2) {
+      unsigned long long act = 0;
+      HIPCHK(hipMemcpyAsync(&act, h->l_active + term, sizeof act, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(hipStreamSynchronize(h->stream));
+      if (act == 0) break;
+    }
+  }
+  {
+    void* args[] = {&nint, &bt, &h->fb, &h->rows0, &h->rec, &h->l_ubuf, &h->l_slots, &nterms, &counters};
+    TRY(launch(h, f32 ? F_L_SOUT32 : F_L_SOUT, nl, kLargeThreads, args));
+  }
+  unsigned long long cnt[3] = {0, 0, 0};
+  HIPCHK(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->cond_terms += cnt[1];
+  if (cnt[2] != 0)
+    return fail(CS_E_SINGULAR, "large-block condensation: the series of %llu interval(s) did not converge in %d terms "
+                "(the mesh is too coarse for the series)", cnt[2], kLargeMaxTerms);
+  return CS_OK;
+}
+
+// Large-block cyclic reduction (float64): substitution products on matrix
+// cores, Householder pairs on work matrices in device memory, the boundary
+// systems by Gauss-Jordan in device memory, back substitution per record.
+int babd_large(cs_handle* h, const int* mask) {
+  ProfScope ps(h, CS_KERNEL_BABD);
+  if (h->babd_bits != 64) return fail(CS_E_UNSUPPORTED, "the large-block cyclic reduction is float64 only");
+  double* dy = h->dir + h->st_y;
+  double* dp = h->dir + h->st_p;
+  Batch bt = h->batch(mask);
+  const size_t rowsz = (size_t)h->ny * h->crw;
+  const int nlev = (int)h->lev_nout.size();
+  // the Householder pairs' work matrices: one per output row of the widest level that pairs by QR
+  size_t need = 0;
+  for (int l = 0; l < nlev; ++l)
+    if (h->lev_qr[l]) need = std::max(need, (size_t)h->lev_nout[l]);
+  need *= (size_t)(2 * h->ny) * (size_t)(3 * h->ny + (h->h_zero ? 0 : h->np) + 1);
+  if (need > h->l_work_cap) {
+    if (h->l_work) hipFree(h->l_work);
+    h->l_work = nullptr;
+    h->l_work_cap = 0;
+    TRY(dmalloc(&h->l_work, need));
+    h->l_work_cap = need;
+  }
+  for (int l = 0; l < nlev; ++l) {
+    const int* oinst = h->cr_oinst + h->lev_obase[l];
+    const CRSeg* seg = h->cr_seg + h->lev_sbase[l];
+    const int* act = mask;
+    double* rin = h->rows0 + h->lev_rbase[l] * rowsz;
+    double* rout = h->rows0 + h->lev_rbase[l + 1] * rowsz;
+    if (h->lev_subst[l]) {
+      void* args[] = {&oinst, &seg, &act, &rin, &rout};
+      TRY(launch(h, F_L_CR_SUBST, (unsigned)h->lev_nout[l] * (unsigned)h->l_stiles, kLargeThreads, args));
+    }
+    if (h->lev_qr[l]) {
+      void* args[] = {&oinst, &seg, &act, &rin, &rout, &h->crrec, &h->l_work};
+      TRY(launch(h, F_L_CR_PAIR, (unsigned)h->lev_nout[l], kLargeQrThreads, args));
+    }
+  }
+  {
+    double* rin = h->rows0 + h->lev_rbase[nlev] * rowsz;
+    void* args[] = {&bt, &rin, &h->Dr, &h->rbc, &dy, &dp, &h->status, &h->l_fin};
+    TRY(launch(h, F_L_CR_FINAL, (unsigned)h->B, kLargeFinThreads, args));
+  }
+  for (int l = nlev - 1; l >= 0; --l) {
+    const int* oinst = h->cr_oinst + h->lev_obase[l];
+    const CRSeg* seg = h->cr_seg + h->lev_sbase[l];
+    double* rin = h->rows0 + h->lev_rbase[l] * rowsz;
+    void* args[] = {&l, &bt, &oinst, &seg, &h->crrec, &rin, &dy, &dp};
+    TRY(launch(h, F_L_CR_BACK, (unsigned)h->lev_nout[l], kLargeThreads, args));
   }
   return CS_OK;
 }
@@ -763,9 +908,6 @@ int cs_create(const cs_dims* d, cs_handle** out) {
       d->n_max < 2 * bmax)
     return fail(CS_E_SHAPE, "invalid dims ny=%d nz=%d np=%d m=%d n_max=%d nc=%d batch=%d", d->ny, d->nz, d->np,
                 d->m, d->n_max, d->nc, d->batch);
-  // the cyclic-reduction kernels hold one BABD row per lane of a wave
-  if (d->ny > 64)
-    return fail(CS_E_UNSUPPORTED, "ny=%d: the register-resident kernels support ny <= 64", d->ny);
   cs_handle* h = new cs_handle();
   h->ny = d->ny; h->nz = d->nz; h->np = d->np; h->m = d->m; h->n_max = d->n_max; h->nc = d->nc; h->bmax = bmax;
   const int ny = h->ny, nz = h->nz, np = h->np, m = h->m;
@@ -863,7 +1005,9 @@ void cs_destroy(cs_handle* h) {
   if (h->nt_par) hipFree(h->nt_par);
   double* bufs[] = {h->t, h->state, h->trial, h->dir, h->c, h->ytil, h->fa, h->fb, h->rbc, h->jac, h->Dr,
                     h->rows0, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps, h->nt_nrm, h->nt_taken,
-                    h->nt_hist, h->nt_ring};
+                    h->nt_hist, h->nt_ring, h->l_ubuf, h->l_rbuf, h->l_work, h->l_fin};
+  if (h->l_slots) hipFree(h->l_slots);
+  if (h->l_active) hipFree(h->l_active);
   for (double* b : bufs)
     if (b) hipFree(b);
   int* ibufs[] = {h->iinst, h->ninst, h->noff, h->act, h->srch, h->fbk, h->alist_buf, h->slist_buf, h->cr_oinst,
@@ -903,7 +1047,12 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
     h->module = nullptr;
   }
   HIPCHK(hipModuleLoadData(&h->module, code));
+  // which of the two kernel sets the module has
+  hipFunction_t probe = nullptr;
+  const bool mod_large = hipModuleGetFunction(&probe, h->module, kFnNames[F_L_STAGE]) == hipSuccess;
   for (int f = 0; f < F_COUNT; ++f) {
+    h->fn[f] = nullptr;
+    if ((mod_large && small_kernel(f)) || (!mod_large && large_kernel(f))) continue;
     hipError_t e = hipModuleGetFunction(&h->fn[f], h->module, kFnNames[f]);
     if (e != hipSuccess) return fail(CS_E_HIP, "kernel %s missing: %s", kFnNames[f], hipGetErrorString(e));
   }
@@ -928,6 +1077,18 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
   h->jac_parts = caps[3] > 0 ? caps[3] : 1;
   h->tab_g = caps[4];
   h->large = caps[5];
+  h->l_tiles = caps[6];
+  h->l_stiles = caps[7];
+  if (h->large) {
+    // the large-block series' buffers, once per handle
+    const size_t nint = (size_t)(h->n_max - 1), un = (size_t)h->ns * (h->ny + meta[7] + 1);
+    if (!h->l_ubuf) TRY(dmalloc(&h->l_ubuf, nint * un));
+    if (!h->l_rbuf) TRY(dmalloc(&h->l_rbuf, nint * un));
+    if (!h->l_slots) HIPCHK(hipMalloc((void**)&h->l_slots, nint * kLargeSlot * sizeof(unsigned long long)));
+    if (!h->l_active) HIPCHK(hipMalloc((void**)&h->l_active, (kLargeMaxTerms + 1) * sizeof(unsigned long long)));
+    const size_t nfin = (size_t)(2 * h->ny + h->np);
+    if (!h->l_fin) TRY(dmalloc(&h->l_fin, (size_t)h->bmax * nfin * (nfin + 1)));
+  }
   // the static stage-Jacobian entries are the new module's constants: the
   // buffer holds the previous module's until they are written again
   h->jac_static_bits = 0;
@@ -1655,6 +1816,7 @@ int cs_set_direction_precision(cs_handle* h, int bits) {
 
 int cs_set_newton_graph(cs_handle* h, int on) {
   if (!h) return fail(CS_E_ARG, "null handle");
+  if (on && h->large) return fail(CS_E_UNSUPPORTED, "the large-block kernels run the host-driven Newton loop");
   h->use_graph = on != 0;
   return CS_OK;
 }
@@ -1662,6 +1824,7 @@ int cs_set_newton_graph(cs_handle* h, int on) {
 int cs_set_babd_precision(cs_handle* h, int bits) {
   if (!h) return fail(CS_E_ARG, "null handle");
   if (bits != 32 && bits != 64) return fail(CS_E_ARG, "BABD precision %d (32 or 64)", bits);
+  if (bits == 32 && h->large) return fail(CS_E_UNSUPPORTED, "the large-block cyclic reduction is float64 only");
   h->babd_bits = bits;
   return CS_OK;
 }
