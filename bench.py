@@ -100,6 +100,9 @@ CONDENSE_KERNELS = {
     "colloc_condense_mf32": ("mfma", FP32_PEAK_TFLOPS),
     "colloc_condense_ns32": ("valu", FP32_PEAK_TFLOPS / 2),
     "colloc_condense_ns": ("valu", FP64_PEAK_TFLOPS),
+    # the large-block series (ny > 64): products on matrix cores in 64 x 64 tiles
+    "colloc_l_series_term32": ("mfma", FP32_PEAK_TFLOPS),
+    "colloc_l_series_term": ("mfma", FP64_PEAK_TFLOPS),
 }
 
 
@@ -223,7 +226,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: 2 batch passes / 5 solves)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("batch", "single"), default="batch")
+    ap.add_argument("--workload", choices=("batch", "single", "precision"), default="batch",
+                    help="batch: the metric's workload; single: one n_x=20 instance; precision: the n_x=64 "
+                         "instance with the float64 and the FP32 Newton direction")
     ap.add_argument("--nint", type=int, default=None, help="single: override Nint (default 4096)")
     ap.add_argument("--instances", type=int, default=1024, help="batch: instances of the family")
     ap.add_argument("--chunk-nodes", type=int, default=320000, help="batch: most nodes solved in one chunk")
@@ -279,6 +284,8 @@ def main():
         __graft_entry__.build()
     if args.workload == "batch":
         line = batch_workload(args)
+    elif args.workload == "precision":
+        line = precision_workload(args)
     else:
         line = single_workload(args)
     print(json.dumps(line), flush=True)
@@ -590,6 +597,98 @@ def single_workload(args):
     }
     if args.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(inst, m, args.cpu_sample, nint)
+    S.close()
+    return line
+
+
+def precision_workload(args):
+    """The precision form on the dense instance: tanh OCP n_x=64 n_u=32 (ny=128,
+    nz=32, np=128), Nint=4096, m=4, plain all-ones guess + homotopy S: 0 ->
+    0.5, solved with the float64 Newton direction and with the FP32 one
+    (float32 Jacobians and condensation; residuals, BABD, convergence test in
+    float64; --refine steps of iterative refinement), timed alternately."""
+    from colloc_amd import jit, problems, solver
+    steps = args.steps if args.steps is not None else 3
+    inst, m = problems.config(2, nint=args.nint)
+    bvp = solver.BvpDae(inst.ocp_text())
+    S = solver.CollocationSolver(bvp, m, n_max=bvp.N)
+    S.set_babd_substitution(args.babd_spans)
+    S.set_mesh(bvp.T0)
+    S.form_initial_input(bvp.Y0, bvp.Z0, bvp.P0)
+    schedule = inst.homotopy_schedule()
+    final_c = bvp.constants
+    tol = bvp.tolerance
+
+    def one_solve(bits):
+        S.set_direction_precision(bits)
+        S.set_refinement(args.refine if bits == 32 else 0)
+        S.restart()
+        its = []
+        for cv in schedule:
+            S.set_constants(cv)
+            it, nrm, _ = S.newton(0.1, tol, bvp.maximum_newton_iterations)
+            its.append(it)
+        S.set_constants(final_c)
+        it, nrm, _ = S.newton(0.1, tol, bvp.maximum_newton_iterations)
+        its.append(it)
+        _, mx = S.segment_residual(0.1, tol)
+        ok = nrm < tol and mx <= 1.0 and it < bvp.maximum_newton_iterations - 1
+        return ok, its, nrm, mx
+
+    forms = {}
+    sols = {}
+    for bits in (64, 32):
+        for _ in range(args.warmup):
+            one_solve(bits)
+        S.profile(True)
+        st0 = S.condense_stats()
+        t0 = time.perf_counter()
+        results = [one_solve(bits) for _ in range(steps)]
+        elapsed = time.perf_counter() - t0
+        st1 = S.condense_stats()
+        prof = {k: S.profile_get(k) for k in ("residual", "jacobian", "condense", "babd", "recover")}
+        S.profile(False)
+        y, _, _, p = S.get_state()
+        sols[bits] = (y, p)
+        forms[bits] = dict(results=results, elapsed=elapsed, st=(st0, st1), prof=prof)
+        print("direction fp%d: %.3f s per solve, Newton iterations %s, kernel totals %s"
+              % (bits, elapsed / steps, results[-1][1], prof), file=sys.stderr)
+    nint = bvp.N - 1
+    ny, nz, np_ = bvp.size_y, bvp.size_z, bvp.size_p
+    npa = len(jit.unit_info(bvp.ocp_text, m)["active_params"])
+    f32 = forms[32]
+    conv32 = sum(1 for r in f32["results"] if r[0])
+    value = conv32 / f32["elapsed"]
+    roof = condense_roofline(S, f32["st"][0], f32["st"][1], f32["prof"]["condense"][0], f32["prof"]["condense"][1],
+                             ny, nz, np_, npa, m, None, "colloc_l_series_term32")
+    roof["kernel"] = "large-block condensation (colloc_l_stage_elim32 + colloc_l_series_term32 on matrix cores)"
+    roof["arithmetic"] = "fp32"
+    scale = max(np.max(np.abs(sols[64][0])), 1e-300)
+    line = {
+        "metric": "converged BVP solves/s @ residual <=1e-8, FP32-direction precision form, 1 GPU",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": f32["elapsed"] / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (residuals, BABD, convergence) + f32 (Jacobians, condensation)",
+        "data": "synthetic (seeded OCP family, generated in-process)",
+        "config": {"workload": "precision form: tanh OCP n_x=64 n_u=32 (ny=128 nz=32 np=128), Nint=%d, m=%d, "
+                               "all-ones guess + homotopy S: 0->0.5; FP32 Newton direction (%d refinement steps) "
+                               "against the float64 one; callbacks in polynomial table form, large-block kernels"
+                               % (nint, m, args.refine),
+                   "nint": nint, "m": m, "ny": ny, "nz": nz, "np": np_, "tolerance": tol,
+                   "fp32_converged_steps": conv32, "fp32_newton_iterations": f32["results"][-1][1],
+                   "fp64_solves_per_s": sum(1 for r in forms[64]["results"] if r[0]) / forms[64]["elapsed"],
+                   "fp64_newton_iterations": forms[64]["results"][-1][1],
+                   "fp32_vs_fp64_solution_rel_diff": float(np.max(np.abs(sols[32][0] - sols[64][0])) / scale),
+                   "kernel_ms_fp32": f32["prof"], "kernel_ms_fp64": forms[64]["prof"]},
+        "roofline": roof,
+    }
     S.close()
     return line
 

@@ -298,6 +298,7 @@ def standard_units():
     yield "tanh4_m5", small, 5
     yield "tanh4_seed2_m5", reseeded(small, 2), 5
     yield "tanh20_m5", config(1)[0], 5
+    yield "tanh64_m4", config(2)[0], 4
     yield "param_osc_m4", TextProblem(PARAM_OSC_OCP), 4
     for name, text in CONSTRAINED_EXAMPLES.items():
         yield "%s_m4" % name, TextProblem(text), 4

@@ -81,7 +81,11 @@ void cs_free(void* ptr);
 
 /* Create a handle owning all device workspace for problems of the given
  * sizes, and one HIP stream.  Replaces the per-call cuda.device_array
- * allocations of every stage wrapper (e.g. solver:294-307, 411-445). */
+ * allocations of every stage wrapper (e.g. solver:294-307, 411-445).  Any
+ * ny >= 1: modules of problems with ny <= 64 carry the register-resident
+ * condensation and cyclic reduction, larger ones the large-block kernels
+ * (float64 cyclic reduction, host-driven Newton loop; an interval whose
+ * series condensation does not converge is reported as CS_E_SINGULAR). */
 int cs_create(const cs_dims* dims, cs_handle** out);
 void cs_destroy(cs_handle* h);
 
@@ -184,7 +188,8 @@ int cs_set_direction_precision(cs_handle* h, int bits);
  * is a replay of HIP graphs captured once per mesh and settings (the host
  * reads three counters per iteration instead of synchronising for every
  * residual).  The results are those of the host-driven loop (on = 0, the
- * default).  No counterpart in the reference. */
+ * default).  CS_E_UNSUPPORTED for a module of large-block kernels (ny > 64).
+ * No counterpart in the reference. */
 int cs_set_newton_graph(cs_handle* h, int on);
 
 /* Iterative refinement of the Newton direction: after the direction delta
@@ -203,7 +208,8 @@ int cs_set_refinement(cs_handle* h, int steps);
  * float32; the Householder pairing, the boundary-block solve (the reduced
  * system of qr_decomposition, :338-440) and the back substitution compute
  * in float64.  Part of the FP32 direction (with cs_set_direction_precision
- * and cs_set_refinement); no counterpart in the reference. */
+ * and cs_set_refinement); 32 is CS_E_UNSUPPORTED for a module of large-block
+ * kernels (ny > 64).  No counterpart in the reference. */
 int cs_set_babd_precision(cs_handle* h, int bits);
 
 /* max |rho| per instance (B doubles) of rho = F + J delta, the linearised
