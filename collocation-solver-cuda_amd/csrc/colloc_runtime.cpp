@@ -122,8 +122,8 @@ bool small_kernel(int f) {
 constexpr int kLargeMaxTerms = 48;        // L_MAX_TERMS of the large-block series
 constexpr int kLargeSlot = 8;             // L_SLOT: u64 per interval of its convergence slots
 constexpr unsigned kLargeThreads = 256;   // L_THREADS
-constexpr unsigned kLargeQrThreads = 1024;  // L_QR_THREADS
-constexpr unsigned kLargeFinThreads = 1024; // L_FIN_THREADS
+constexpr unsigned kLargeQrThreads = 512;   // L_QR_THREADS
+constexpr unsigned kLargeFinThreads = 512;  // L_FIN_THREADS
 constexpr unsigned kSubstThreads = 256;   // CS_THREADS of the substitution pairing kernel
 constexpr int kDefaultSubstSpans = 64;    // substitution while a pair spans <= 1/64 of the instance's intervals
 constexpr unsigned kNsThreads = 512;   // NS_THREADS of the series condensation kernel
@@ -229,8 +229,7 @@ struct cs_handle {
   int large = 0;                        // the module's large-block condensation and cyclic reduction (ny > 64)
   int l_tiles = 0, l_stiles = 0;        // large-block: workgroups per interval of a series term, per pair of a
                                         // substitution product
-  double* l_ubuf = nullptr;             // large-block series: the second U buffer and the right-hand sides R
-  double* l_rbuf = nullptr;
+  double* l_ubuf = nullptr;             // large-block series: the second U buffer
   unsigned long long* l_slots = nullptr;  // per interval: convergence slots (kLargeSlot u64)
   unsigned long long* l_active = nullptr; // per term: intervals that computed it
   double* l_work = nullptr;             // large-block Householder pairs' work matrices
@@ -677,7 +676,7 @@ int condense_large(cs_handle* h, const int* mask, long long ncond) {
     TRY(launch(h, f32 ? F_L_STAGE32 : F_L_STAGE, nl * (unsigned)h->m, kLargeThreads, args));
   }
   {
-    void* args[] = {&nint, &bt, &h->t, &h->rec, &h->l_rbuf, &h->l_slots};
+    void* args[] = {&nint, &bt, &h->t, &h->rec, &h->l_slots};
     TRY(launch(h, f32 ? F_L_SINIT32 : F_L_SINIT, nl * (unsigned)(h->m - 1), kLargeThreads, args));
   }
   const long long rstride = (long long)h->recn;
@@ -689,8 +688,7 @@ int condense_large(cs_handle* h, const int* mask, long long ncond) {
     void* unew = term % 2 == 0 ? (void*)h->rec : (void*)h->l_ubuf;
     long long so = (term - 1) % 2 == 0 ? rstride : ustride;
     long long sn = term % 2 == 0 ? rstride : ustride;
-    void* args[] = {&nint, &bt, &h->t, &h->rec, &h->l_rbuf, &uold, &so, &unew, &sn, &h->l_slots, &h->l_active,
-                    &term};
+    void* args[] = {&nint, &bt, &h->t, &h->rec, &uold, &so, &unew, &sn, &h->l_slots, &h->l_active, &term};
     TRY(launch(h, f32 ? F_L_TERM32 : F_L_TERM, nl * (unsigned)h->l_tiles, kLargeThreads, args));
     nterms = term;
     if (term >= 2) {
@@ -1005,7 +1003,7 @@ void cs_destroy(cs_handle* h) {
   if (h->nt_par) hipFree(h->nt_par);
   double* bufs[] = {h->t, h->state, h->trial, h->dir, h->c, h->ytil, h->fa, h->fb, h->rbc, h->jac, h->Dr,
                     h->rows0, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps, h->nt_nrm, h->nt_taken,
-                    h->nt_hist, h->nt_ring, h->l_ubuf, h->l_rbuf, h->l_work, h->l_fin};
+                    h->nt_hist, h->nt_ring, h->l_ubuf, h->l_work, h->l_fin};
   if (h->l_slots) hipFree(h->l_slots);
   if (h->l_active) hipFree(h->l_active);
   for (double* b : bufs)
@@ -1083,7 +1081,6 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
     // the large-block series' buffers, once per handle
     const size_t nint = (size_t)(h->n_max - 1), un = (size_t)h->ns * (h->ny + meta[7] + 1);
     if (!h->l_ubuf) TRY(dmalloc(&h->l_ubuf, nint * un));
-    if (!h->l_rbuf) TRY(dmalloc(&h->l_rbuf, nint * un));
     if (!h->l_slots) HIPCHK(hipMalloc((void**)&h->l_slots, nint * kLargeSlot * sizeof(unsigned long long)));
     if (!h->l_active) HIPCHK(hipMalloc((void**)&h->l_active, (kLargeMaxTerms + 1) * sizeof(unsigned long long)));
     const size_t nfin = (size_t)(2 * h->ny + h->np);

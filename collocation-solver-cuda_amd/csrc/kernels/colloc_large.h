@@ -38,8 +38,8 @@ constexpr int CS_XC = NY + NPH + 1;              // columns of [G_a | H_a | b_a]
 constexpr int L_SCT = (CS_XC + L_TILE - 1) / L_TILE;
 constexpr int L_STILES = L_NRT * L_SCT;
 constexpr int L_WC = 3 * NY + NPH + 1;           // columns of a Householder pair's work matrix
-constexpr int L_QR_THREADS = 1024;
-constexpr int L_FIN_THREADS = 1024;
+constexpr int L_QR_THREADS = 512;
+constexpr int L_FIN_THREADS = 512;
 
 template <class R> __device__ __forceinline__ R l_tol();
 template <> __device__ __forceinline__ double l_tol<double>() { return 0x1p-50; }
@@ -73,27 +73,51 @@ constexpr int L_SB = L_KC * (L_TILE + 1);
 
 // acc += A B for a 64 x 64 output tile, A (64 x K) and B (K x 64) read
 // through A(r, k) and B(k, c) (which return 0 outside the matrices), staged
-// in LDS in chunks of L_KC.  Wave w owns rows 32 (w / 2) .. +32 and columns
-// 32 (w % 2) .. +32 of the tile as 2 x 2 matrix-core tiles of 16 x 16:
-// element v of acc[ti][tj] in lane l is row 32 (w / 2) + 16 ti +
-// MfmaOp<T>::row(l, v), column 32 (w % 2) + 16 tj + l % 16.  Every thread of
-// the (4-wave) workgroup calls it.
+// in LDS in chunks of L_KC; the next chunk's global loads are issued before
+// the current chunk's products, so their latency overlaps the matrix cores.
+// Wave w owns rows 32 (w / 2) .. +32 and columns 32 (w % 2) .. +32 of the
+// tile as 2 x 2 matrix-core tiles of 16 x 16: element v of acc[ti][tj] in
+// lane l is row 32 (w / 2) + 16 ti + MfmaOp<T>::row(l, v), column
+// 32 (w % 2) + 16 tj + l % 16.  Every thread of the workgroup of L_THREADS
+// calls it.
+constexpr int L_FA = L_TILE * L_KC / L_THREADS;   // A elements a thread stages per chunk
+constexpr int L_FB = L_KC * L_TILE / L_THREADS;   // B elements
+static_assert(L_FA * L_THREADS == L_TILE * L_KC && L_FB * L_THREADS == L_KC * L_TILE, "tile_gemm staging");
 template <class T, class FA, class FB>
 __device__ __forceinline__ void tile_gemm(int K, FA&& A, FB&& B, T* sA, T* sB,
                                           typename MfmaOp<T>::V (&acc)[2][2]) {
   const int tid = threadIdx.x, lane = lane_id(), w = wave_id();
   const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
+  T ra[L_FA], rb[L_FB];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < L_FA; ++it) {
+      const int e = tid + it * L_THREADS;
+      const int r = e / L_KC, k = e - (e / L_KC) * L_KC;
+      ra[it] = (k0 + k < K) ? A(r, k0 + k) : (T)0;
+    }
+#pragma unroll
+    for (int it = 0; it < L_FB; ++it) {
+      const int e = tid + it * L_THREADS;
+      const int k = e / L_TILE, c = e - (e / L_TILE) * L_TILE;
+      rb[it] = (k0 + k < K) ? B(k0 + k, c) : (T)0;
+    }
+  };
+  fetch(0);
   for (int k0 = 0; k0 < K; k0 += L_KC) {
     __syncthreads();   // the previous chunk's reads are done
-    for (int e = tid; e < L_TILE * L_KC; e += blockDim.x) {
-      const int r = e / L_KC, k = e - (e / L_KC) * L_KC;
-      sA[r * (L_KC + 1) + k] = (k0 + k < K) ? A(r, k0 + k) : (T)0;
+#pragma unroll
+    for (int it = 0; it < L_FA; ++it) {
+      const int e = tid + it * L_THREADS;
+      sA[(e / L_KC) * (L_KC + 1) + (e - (e / L_KC) * L_KC)] = ra[it];
     }
-    for (int e = tid; e < L_KC * L_TILE; e += blockDim.x) {
-      const int k = e / L_TILE, c = e - (e / L_TILE) * L_TILE;
-      sB[k * (L_TILE + 1) + c] = (k0 + k < K) ? B(k0 + k, c) : (T)0;
+#pragma unroll
+    for (int it = 0; it < L_FB; ++it) {
+      const int e = tid + it * L_THREADS;
+      sB[(e / L_TILE) * (L_TILE + 1) + (e - (e / L_TILE) * L_TILE)] = rb[it];
     }
     __syncthreads();
+    if (k0 + L_KC < K) fetch(k0 + L_KC);
 #pragma unroll
     for (int s = 0; s < L_KC / 4; ++s) {
       T a[2], b[2];
@@ -133,95 +157,157 @@ __device__ __forceinline__ void l_block_max2(double d, double x, unsigned long l
 
 __device__ __forceinline__ double slot_d(unsigned long long v) { return __longlong_as_double((long long)v); }
 
+// XCD-aware workgroup order: the hardware deals consecutive workgroups to
+// the 8 XCDs in turn, each with its own L2.  Workgroup g becomes logical
+// work unit L(g) such that the units an XCD runs are consecutive, so the
+// tiles of one interval (or pair), which read the same operand blocks, share
+// that XCD's L2.  A bijection of [0, G).
+constexpr int L_XCDS = 8;
+__device__ __forceinline__ int xcd_unit(int g, int G) {
+  const int x = g % L_XCDS, slot = g / L_XCDS;
+  const int q = G / L_XCDS, rem = G - q * L_XCDS;
+  const int base = x * q + (x < rem ? x : rem);
+  return base + slot;
+}
+
 // ------------------------------------------------------ stage eliminations
-// One workgroup per (listed interval, stage j), computing in R: [g_z | g_y
-// g_pa f_a,g] (NZ rows) in LDS, Gauss-Jordan with partial pivoting (the
-// largest |entry| of the column, first on ties), then T_j row by row.
-// Writes G_j and T_j to the record (type R); status bit 1 on a zero pivot.
-constexpr int L_GZW = NZ + NVA + 1;   // padded row of the algebraic block
+// One workgroup per (listed interval, stage j), computing in R.  The
+// algebraic block: every wave holds [g_z | its quarter of g_y g_pa f_a,g]
+// in registers (lane = column, the NZ rows per lane) and runs the same
+// Gauss-Jordan with partial pivoting (the largest |entry| of the column,
+// the first on ties; the pivot column broadcast by v_readlane), so the four
+// waves take identical pivot decisions with no exchange; G_j goes to LDS and
+// the record.  Then T_j = [h_y | h_pa | f_a,h] - h_z G_j on matrix cores.
+// Status bit 1 on a zero pivot.
+constexpr int L_GZW = NVA + 1;                            // padded row of G_j in LDS
+constexpr int L_SE_CPW = (NVA + 3) / 4;                   // right-hand-side columns per wave
+constexpr int L_SE_SL = (NZ + L_SE_CPW + WAVE - 1) / WAVE; // register slices of a wave's columns
 template <class R>
 __device__ __forceinline__ void l_stage_elim(int nint, Batch bt, const R* __restrict__ jac,
                                              const double* __restrict__ fa, R* __restrict__ rec,
                                              int* __restrict__ status) {
-  __shared__ R gz[NZ1 * L_GZW];
-  __shared__ R hz[NY * NZ1];
-  __shared__ R fcol[NZ1];
-  __shared__ int spiv;
+  __shared__ R gz[NZ1 * L_GZW];   // G_j
+  __shared__ __attribute__((aligned(16))) R sA[L_SA];
+  __shared__ __attribute__((aligned(16))) R sB[L_SB];
   const int item = blockIdx.x / M, j = blockIdx.x - (blockIdx.x / M) * M;
   if (item >= launch_intervals(bt, nint)) return;
   const int i = listed_interval(bt, item);
   const int ib = bt.iinst[i];
   if (skipped(bt, ib)) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = lane_id(), w = wave_id();
   const R* Dfj = jac + ((size_t)i * M + j) * JSTAGE;
   const R* Dgj = Dfj + NY * NJC;
   const double* faj = fa + (size_t)i * M * NYZ + j * NYZ;
   R* recT = rec + (size_t)i * REC + NS * NVA;
   R* recG = recT + M * NY * NVA;
-  for (int e = tid; e < NZ * (NZ + NVA); e += blockDim.x) {
-    const int r = e / (NZ + NVA), cc = e - (e / (NZ + NVA)) * (NZ + NVA);
-    R v;
-    if (cc < NZ) v = Dgj[r * NJC + NY + cc];
-    else {
-      const int q = cc - NZ;
-      v = q < NY ? Dgj[r * NJC + q] : (q < NY + NPA ? Dgj[r * NJC + NY + NZ + pact(q - NY)] : (R)faj[NY + r]);
+  int bad = 0;
+  if (NZ > 0) {
+    // column of slice sl of this lane: g_z column (< NZ) or right-hand side q
+    R gc[L_SE_SL][NZ1];
+    int qcol[L_SE_SL];
+#pragma unroll
+    for (int sl = 0; sl < L_SE_SL; ++sl) {
+      const int cc = lane + WAVE * sl;
+      const int q = cc < NZ ? -1 : w * L_SE_CPW + (cc - NZ);
+      const bool isq = cc >= NZ && cc - NZ < L_SE_CPW && q < NVA;
+      qcol[sl] = isq ? q : -2;
+#pragma unroll
+      for (int r = 0; r < NZ; ++r) {
+        R v = (R)0;
+        if (cc < NZ) v = Dgj[r * NJC + NY + cc];
+        else if (isq) v = q < NY ? Dgj[r * NJC + q] : (q < NY + NPA ? Dgj[r * NJC + NY + NZ + pact(q - NY)] : (R)faj[NY + r]);
+        gc[sl][r] = v;
+      }
     }
-    gz[r * L_GZW + cc] = v;
-  }
-  for (int e = tid; e < NY * NZ; e += blockDim.x) {
-    const int r = e / NZ1, z = e - (e / NZ1) * NZ1;
-    hz[e] = Dfj[r * NJC + NY + z];
+    // every step at compile time (static_for), so that the register arrays
+    // are indexed by constants
+    static_for<0, NZ>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      constexpr int ks = k / WAVE, kl = k % WAVE;
+      // pivot: the lane holding column k finds the largest |entry| at or below
+      // row k (the first on ties); every lane learns the row
+      int prl = k;
+      {
+        R best = fabs(gc[ks][k]);
+#pragma unroll
+        for (int r = k + 1; r < NZ; ++r)
+          if (fabs(gc[ks][r]) > best) { best = fabs(gc[ks][r]); prl = r; }
+      }
+      const int pr = __builtin_amdgcn_readlane(prl, kl);
+#pragma unroll
+      for (int r = k + 1; r < NZ; ++r) {
+        const bool sw = r == pr;
+#pragma unroll
+        for (int sl = 0; sl < L_SE_SL; ++sl) {
+          const R t1 = gc[sl][r];
+          gc[sl][r] = sw ? gc[sl][k] : gc[sl][r];
+          gc[sl][k] = sw ? t1 : gc[sl][k];
+        }
+      }
+      const R cpr = readlane_r(gc[ks][k], kl);
+      if (!(fabs(cpr) > (R)0)) bad = 1;
+      const R inv = (R)1 / cpr;
+      R pk[L_SE_SL];
+#pragma unroll
+      for (int sl = 0; sl < L_SE_SL; ++sl) pk[sl] = gc[sl][k] * inv;
+      // eliminate column k from the other rows, eight multipliers at a time
+#pragma unroll
+      for (int r0 = 0; r0 < NZ; r0 += 8) {
+        R cr[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cr[q] = (r0 + q < NZ) ? readlane_r(gc[ks][r0 + q < NZ ? r0 + q : 0], kl) : (R)0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int r = r0 + q;
+          if (r >= NZ || r == k) continue;
+#pragma unroll
+          for (int sl = 0; sl < L_SE_SL; ++sl) gc[sl][r] = fma(-cr[q], pk[sl], gc[sl][r]);
+        }
+      }
+#pragma unroll
+      for (int sl = 0; sl < L_SE_SL; ++sl) gc[sl][k] = pk[sl];
+    });
+    // [I | G_j]: this wave's columns of G_j
+#pragma unroll
+    for (int sl = 0; sl < L_SE_SL; ++sl) {
+      const int q = qcol[sl];
+      if (q < 0) continue;
+#pragma unroll
+      for (int r = 0; r < NZ; ++r) {
+        gz[r * L_GZW + q] = gc[sl][r];
+        recG[(j * NZ + r) * NVA + q] = gc[sl][r];
+      }
+    }
   }
   __syncthreads();
-  int bad = 0;
-  for (int k = 0; k < NZ; ++k) {
-    if (tid < WAVE) {
-      // pivot: the largest |g_z[r][k]|, r >= k, the first on ties
-      R best = (R)-1;
-      int pr = k;
-      for (int r = k + tid; r < NZ; r += WAVE) {
-        const R a = fabs(gz[r * L_GZW + k]);
-        if (a > best) { best = a; pr = r; }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const R ob = __shfl_xor(best, off);
-        const int op = __shfl_xor(pr, off);
-        if (ob > best || (ob == best && op < pr)) { best = ob; pr = op; }
-      }
-      if (tid == 0) spiv = pr;
-    }
-    __syncthreads();
-    const int pr = spiv;
-    if (pr != k)
-      for (int c = tid; c < NZ + NVA; c += blockDim.x) {
-        const R t0 = gz[k * L_GZW + c];
-        gz[k * L_GZW + c] = gz[pr * L_GZW + c];
-        gz[pr * L_GZW + c] = t0;
-      }
-    __syncthreads();
-    const R piv = gz[k * L_GZW + k];
-    if (!(fabs(piv) > (R)0)) bad = 1;
-    const R inv = (R)1 / piv;
-    if (tid < NZ) fcol[tid] = gz[tid * L_GZW + k];
-    __syncthreads();
-    for (int c = tid; c < NZ + NVA; c += blockDim.x) gz[k * L_GZW + c] *= inv;
-    __syncthreads();
-    for (int e = tid; e < NZ * (NZ + NVA); e += blockDim.x) {
-      const int r = e / (NZ + NVA), c = e - (e / (NZ + NVA)) * (NZ + NVA);
-      if (r != k) gz[r * L_GZW + c] = fma(-fcol[r], gz[k * L_GZW + c], gz[r * L_GZW + c]);
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < NZ * NVA; e += blockDim.x) {
-    const int r = e / NVA, q = e - (e / NVA) * NVA;
-    recG[(j * NZ + r) * NVA + q] = gz[r * L_GZW + NZ + q];
-  }
-  // T_j = [h_y | h_pa | f_a,h] - h_z G_j
-  for (int e = tid; e < NY * NVA; e += blockDim.x) {
-    const int r = e / NVA, q = e - (e / NVA) * NVA;
-    R v = q < NY ? Dfj[r * NJC + q] : (q < NY + NPA ? Dfj[r * NJC + NY + NZ + pact(q - NY)] : (R)faj[r]);
-    for (int z = 0; z < NZ; ++z) v = fma(-hz[r * NZ1 + z], gz[z * L_GZW + NZ + q], v);
-    recT[(j * NY + r) * NVA + q] = v;
+  // T_j = [h_y | h_pa | f_a,h] - h_z G_j: the product on matrix cores in
+  // 64 x 64 tiles (K = NZ)
+  for (int tile = 0; tile < L_TILES; ++tile) {
+    const int rbase = (tile / L_NCT) * L_TILE, cbase = (tile - (tile / L_NCT) * L_NCT) * L_TILE;
+    typename MfmaOp<R>::V acc[2][2];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[ti][tj][v] = (R)0;
+    if (NZ > 0)
+      tile_gemm<R>(NZ,
+          [&](int r, int z) { return rbase + r < NY ? Dfj[(rbase + r) * NJC + NY + z] : (R)0; },
+          [&](int z, int c) { return cbase + c < NVA ? gz[z * L_GZW + cbase + c] : (R)0; },
+          sA, sB, acc);
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = rbase + 32 * (w >> 1) + 16 * ti + MfmaOp<R>::row(lane, v);
+          const int q = cbase + 32 * (w & 1) + 16 * tj + (lane & 15);
+          if (r >= NY || q >= NVA) continue;
+          const R hb = q < NY ? Dfj[r * NJC + q] : (q < NY + NPA ? Dfj[r * NJC + NY + NZ + pact(q - NY)] : (R)faj[r]);
+          recT[(j * NY + r) * NVA + q] = hb - acc[ti][tj][v];
+        }
   }
   if (bad && tid == 0) atomicOr(status, 1);
 }
@@ -229,11 +315,11 @@ __device__ __forceinline__ void l_stage_elim(int nint, Batch bt, const R* __rest
 // ---------------------------------------------------------------- series
 // R_k (k = 0 .. NSK-1, stage k+1) of the listed intervals:
 //   R_k = [I | 0] + h A[k+1][0] T_0 + sum_{j>=1} h A[k+1][j] [0 | T_j[:, NY:]],
-// written to rbuf and to the record's U block (the series' first iterate).
+// written to the record's U block (the series' first iterate).
 // One workgroup per (listed interval, stage block); slots cleared.
 template <class R>
 __device__ __forceinline__ void l_series_init(int nint, Batch bt, const double* __restrict__ t, R* __restrict__ rec,
-                                              R* __restrict__ rbuf, unsigned long long* __restrict__ slots) {
+                                              unsigned long long* __restrict__ slots) {
   const int item = blockIdx.x / NSK, k = blockIdx.x - (blockIdx.x / NSK) * NSK;
   if (item >= launch_intervals(bt, nint)) return;
   const int i = listed_interval(bt, item);
@@ -242,7 +328,6 @@ __device__ __forceinline__ void l_series_init(int nint, Batch bt, const double* 
   const R h = (R)(t[i + ib + 1] - t[i + ib]);
   R* recU = rec + (size_t)i * REC;
   const R* recT = recU + NS * NVA;
-  R* rb = rbuf + (size_t)i * NS * NVA;
   const int js = k + 1;
   for (int e = threadIdx.x; e < NY * NVA; e += blockDim.x) {
     const int r = e / NVA, c = e - (e / NVA) * NVA;
@@ -252,7 +337,6 @@ __device__ __forceinline__ void l_series_init(int nint, Batch bt, const double* 
     } else {
       for (int jj = 1; jj < M; ++jj) v += h * (R)c_rkA[js * M + jj] * recT[(jj * NY + r) * NVA + c];
     }
-    rb[(k * NY + r) * NVA + c] = v;
     recU[(k * NY + r) * NVA + c] = v;
   }
   if (k == 0 && threadIdx.x < L_SLOT) slots[(size_t)i * L_SLOT + threadIdx.x] = 0;
@@ -270,14 +354,15 @@ __device__ __forceinline__ bool l_converged(const unsigned long long* s, int set
 // term | -]; active[term]: intervals that computed this term.
 template <class R>
 __device__ __forceinline__ void l_series_term(int nint, Batch bt, const double* __restrict__ t,
-                                              const R* __restrict__ rec, const R* __restrict__ rbuf,
+                                              const R* __restrict__ rec,
                                               const R* __restrict__ uold, long long ustride_old,
                                               R* __restrict__ unew, long long ustride_new,
                                               unsigned long long* __restrict__ slots,
                                               unsigned long long* __restrict__ active, int term) {
   __shared__ __attribute__((aligned(16))) R sA[L_SA];
   __shared__ __attribute__((aligned(16))) R sB[L_SB];
-  const int item = blockIdx.x / L_TILES, tile = blockIdx.x - (blockIdx.x / L_TILES) * L_TILES;
+  const int unit = xcd_unit(blockIdx.x, gridDim.x);
+  const int item = unit / L_TILES, tile = unit - (unit / L_TILES) * L_TILES;
   if (item >= launch_intervals(bt, nint)) return;
   const int i = listed_interval(bt, item);
   const int ib = bt.iinst[i];
@@ -318,32 +403,36 @@ __device__ __forceinline__ void l_series_term(int nint, Batch bt, const double* 
   }
   const R h = (R)(t[i + ib + 1] - t[i + ib]);
   const int lane = lane_id(), w = wave_id();
-  const R* rb = rbuf + (size_t)i * NS * NVA;
   R* un = unew + (size_t)i * ustride_new;
   double dmax = 0.0, xmax = 0.0;
+  // R_k = [I | 0] + h A[k+1][0] T_0 + sum_{j>=1} h A[k+1][j] [0 | T_j[:, NY:]],
+  // formed here from T_0 (one read for the NSK blocks) instead of stored
 #pragma unroll
-  for (int k = 0; k < NSK; ++k) {
-    R wk[NSK];
+  for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-    for (int q = 0; q < NSK; ++q) wk[q] = h * (R)c_rkA[(k + 1) * M + q + 1];
+    for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
+      for (int v = 0; v < 4; ++v) {
+        const int row = rbase + 32 * (w >> 1) + 16 * ti + MfmaOp<R>::row(lane, v);
+        const int col = cbase + 32 * (w & 1) + 16 * tj + (lane & 15);
+        if (row >= NY || col >= NVA) continue;
+        const R t0 = recT[row * NVA + col];
 #pragma unroll
-      for (int tj = 0; tj < 2; ++tj)
+        for (int k = 0; k < NSK; ++k) {
+          R x = h * (R)c_rkA[(k + 1) * M] * t0;
+          if (col < NY) {
+            if (col == row) x += (R)1;
+          } else {
+            for (int jj = 1; jj < M; ++jj) x += h * (R)c_rkA[(k + 1) * M + jj] * recT[(jj * NY + row) * NVA + col];
+          }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = rbase + 32 * (w >> 1) + 16 * ti + MfmaOp<R>::row(lane, v);
-          const int col = cbase + 32 * (w & 1) + 16 * tj + (lane & 15);
-          if (row >= NY || col >= NVA) continue;
+          for (int q = 0; q < NSK; ++q) x = fma(h * (R)c_rkA[(k + 1) * M + q + 1], acc[q][ti][tj][v], x);
           const size_t e = (size_t)(k * NY + row) * NVA + col;
-          R x = rb[e];
-#pragma unroll
-          for (int q = 0; q < NSK; ++q) x = fma(wk[q], acc[q][ti][tj][v], x);
           dmax = nmax(fabs((double)x - (double)uo[e]), dmax);
           xmax = nmax(fabs((double)x), xmax);
           un[e] = x;
         }
-  }
+      }
   l_block_max2(dmax, xmax, s + 2 * (term % 3), s + 2 * (term % 3) + 1);
 }
 
@@ -407,7 +496,8 @@ __device__ __forceinline__ void l_cr_subst(const int* __restrict__ oinst, const 
                                            double* __restrict__ rows_out) {
   __shared__ __attribute__((aligned(16))) double sA[L_SA];
   __shared__ __attribute__((aligned(16))) double sB[L_SB];
-  const int wg = blockIdx.x / L_STILES, tile = blockIdx.x - (blockIdx.x / L_STILES) * L_STILES;
+  const int unit = xcd_unit(blockIdx.x, gridDim.x);
+  const int wg = unit / L_STILES, tile = unit - (unit / L_STILES) * L_STILES;
   const int ib = oinst[wg];
   if (act && !act[ib]) return;
   const CRSeg sg = seg[ib];
@@ -482,7 +572,7 @@ __device__ __forceinline__ void l_cr_subst(const int* __restrict__ oinst, const 
 // (global memory), one workgroup of L_QR_THREADS; thread (column slot,
 // row quarter) applies each reflector to a column's quarter, the quarters'
 // dot products summed through LDS.
-__device__ __forceinline__ void l_cr_pair(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
+__device__ __forceinline__ void l_cr_pair_global(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
                                           const int* __restrict__ act, const double* __restrict__ rows_in,
                                           double* __restrict__ rows_out, double* __restrict__ recs,
                                           double* __restrict__ work) {
@@ -586,27 +676,221 @@ __device__ __forceinline__ void l_cr_pair(const int* __restrict__ oinst, const C
   }
 }
 
-// Boundary system (see cr_final) by Gauss-Jordan with partial pivoting on
-// the (NFIN) x (NFIN + 1) matrix in work (global memory), one workgroup of
+// Householder pairing by panels (ny <= 192): the pair's 2NY x L_WC block
+// [x_M | x_L | x_R | p | rhs] (work, device memory) is reduced L_PW
+// columns at a time.  Each panel is loaded to LDS and factored there (the
+// reference's qr step per column, matrix_factorization_cuda.py:5-82: g =
+// -sign(v_k) |v|, beta = -2 / |v|^2, stop at |v|^2 < eps), its reflectors
+// kept in LDS; then the panel's reflectors are applied in order to every
+// column after it, each column held in registers by four lanes (one per
+// 16-lane row of a wave, NY / 2 rows each), the dot products summed across
+// the four by lane exchanges.  The trailing columns are read and written
+// once per panel instead of twice per reflector.
+constexpr int L_PW = 16;                          // panel width
+constexpr int L_PARTS = 8;                        // lanes per trailing column (one per 8-lane group of a wave)
+constexpr int L_RPL = (2 * NY + L_PARTS - 1) / L_PARTS;   // rows per lane of a trailing column
+template <int RPL>
+__device__ __forceinline__ void l_cr_pair_panels(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
+                                                 const int* __restrict__ act, const double* __restrict__ rows_in,
+                                                 double* __restrict__ rows_out, double* __restrict__ recs,
+                                                 double* __restrict__ work) {
+  constexpr int R2 = 2 * NY;
+  constexpr int NW = L_QR_THREADS / WAVE;
+  __shared__ double Pn[R2 * (L_PW + 1)];      // the panel, padded rows
+  __shared__ double V[L_PW * R2];             // its reflectors (zero above their pivot row)
+  __shared__ double Bt[L_PW];                 // and their betas
+  __shared__ double red[NW];
+  __shared__ double sdot[L_PW];
+  __shared__ int snref, sstop;
+  const int ib = oinst[blockIdx.x];
+  if (act && !act[ib]) return;
+  const CRSeg sg = seg[ib];
+  if (sg.subst) return;
+  const int k = blockIdx.x - sg.out_off;
+  const int nrows = sg.nrows;
+  rows_in += (size_t)sg.in_off * NY * CRW;
+  rows_out += (size_t)sg.out_off * NY * CRW;
+  recs += (size_t)sg.rec_off * NY * RECW;
+  const int npairs = nrows >> 1;
+  const int tid = threadIdx.x, lane = lane_id(), w = wave_id();
+  if (k == npairs) {
+    const double* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
+    double* dst = rows_out + (size_t)npairs * NY * CRW;
+    for (int e = tid; e < NY * CRW; e += blockDim.x) dst[e] = src[e];
+    return;
+  }
+  double* W = work + (size_t)blockIdx.x * R2 * L_WC;
+  const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
+  const double* rb = ra + NY * CRW;
+  for (int e = tid; e < R2 * L_WC; e += blockDim.x) {
+    const int r = e / L_WC, c = e - (e / L_WC) * L_WC;
+    const bool top = r < NY;
+    const double* src = top ? ra + (size_t)r * CRW : rb + (size_t)(r - NY) * CRW;
+    const int pcol = c < 3 * NY + NPH ? c : RECW - 1;
+    double v;
+    if (c < NY) v = top ? src[NY + c] : src[c];
+    else if (c < 2 * NY) v = top ? src[c - NY] : 0.0;
+    else if (c < 3 * NY) v = top ? 0.0 : src[c - NY];
+    else v = src[pcol - NY];
+    W[e] = v;
+  }
+  __syncthreads();
+  const double eps = 2.220446049250313e-16;
+  const int part = lane >> 3, c8 = lane & 7;
+  for (int kk0 = 0; kk0 < NY; kk0 += L_PW) {
+    const int npw = min(L_PW, NY - kk0);
+    for (int e = tid; e < R2 * L_PW; e += blockDim.x) {
+      const int r = e / L_PW, j = e - (e / L_PW) * L_PW;
+      Pn[r * (L_PW + 1) + j] = j < npw ? W[(size_t)r * L_WC + kk0 + j] : 0.0;
+    }
+    if (tid == 0) { snref = 0; sstop = 0; }
+    __syncthreads();
+    // factor the panel: column j's reflector, then its application to the
+    // panel's later columns (wave w takes columns j + 1 + w, + NW, ...)
+    for (int j = 0; j < npw; ++j) {
+      const int kk = kk0 + j;
+      double part2 = 0.0;
+      for (int r = kk + tid; r < R2; r += blockDim.x) {
+        const double a = Pn[r * (L_PW + 1) + j];
+        part2 = fma(a, a, part2);
+      }
+      part2 = wave_sum(part2);
+      if (lane == 0) red[w] = part2;
+      __syncthreads();
+      if (tid == 0) {
+        double tf = 0.0;
+        for (int q = 0; q < NW; ++q) tf += red[q];
+        const double akk = Pn[kk * (L_PW + 1) + j];
+        const double nrm = sqrt(tf);
+        const double g = akk >= 0.0 ? -nrm : nrm;
+        const double v0 = akk - g;
+        const double vs = tf - akk * akk + v0 * v0;
+        if (vs < eps) {
+          sstop = 1;   // the reference stops reflecting here
+        } else {
+          Bt[j] = -2.0 / vs;
+          sdot[0] = g;
+          sdot[1] = v0;
+          snref = j + 1;
+        }
+      }
+      __syncthreads();
+      if (sstop) break;
+      const double g = sdot[0], v0 = sdot[1], beta = Bt[j];
+      for (int r = tid; r < R2; r += blockDim.x)
+        V[j * R2 + r] = r == kk ? v0 : (r > kk ? Pn[r * (L_PW + 1) + j] : 0.0);
+      __syncthreads();
+      for (int jj = j + 1 + w; jj < npw; jj += NW) {
+        double d = 0.0;
+        for (int r = kk + lane; r < R2; r += WAVE) d = fma(V[j * R2 + r], Pn[r * (L_PW + 1) + jj], d);
+        d = wave_sum(d) * beta;
+        for (int r = kk + lane; r < R2; r += WAVE) Pn[r * (L_PW + 1) + jj] = fma(d, V[j * R2 + r], Pn[r * (L_PW + 1) + jj]);
+      }
+      for (int r = kk + tid; r < R2; r += blockDim.x) Pn[r * (L_PW + 1) + j] = r == kk ? g : 0.0;
+      __syncthreads();
+    }
+    const int nref = snref;
+    const bool stop = sstop != 0;
+    // the factored panel columns back (those reflected; a stopped column and
+    // the ones after it keep their values)
+    for (int e = tid; e < R2 * L_PW; e += blockDim.x) {
+      const int r = e / L_PW, j = e - (e / L_PW) * L_PW;
+      if (j < nref) W[(size_t)r * L_WC + kk0 + j] = Pn[r * (L_PW + 1) + j];
+      else if (stop && j < npw) {
+        // not reflected: left as it is in W (columns >= the stopped one get no
+        // further reflectors either)
+      }
+    }
+    // the panel's reflectors on the trailing columns: the panel's unreflected
+    // columns (after a stop) and every column after the panel
+    const int cfirst = stop ? kk0 + nref : kk0 + npw;
+    if (nref > 0) {
+      for (int c0 = cfirst; c0 < L_WC; c0 += NW * 8) {
+        const int c = c0 + w * 8 + c8;
+        const bool live = c < L_WC;
+        double a[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+          const int r = part * RPL + i;
+          a[i] = (live && r < R2) ? W[(size_t)r * L_WC + c] : 0.0;
+        }
+        for (int j = 0; j < nref; ++j) {
+          const double* vj = V + j * R2 + part * RPL;
+          double dd[4] = {0.0, 0.0, 0.0, 0.0};
+          static_for<0, RPL>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            if (part * RPL + i < R2) dd[i & 3] = fma(vj[i], a[i], dd[i & 3]);
+          });
+          double d = (dd[0] + dd[1]) + (dd[2] + dd[3]);
+          d += __shfl_xor(d, 8);
+          d += __shfl_xor(d, 16);
+          d += __shfl_xor(d, 32);
+          d *= Bt[j];
+          static_for<0, RPL>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            if (part * RPL + i < R2) a[i] = fma(d, vj[i], a[i]);
+          });
+        }
+        if (live) {
+#pragma unroll
+          for (int i = 0; i < RPL; ++i) {
+            const int r = part * RPL + i;
+            if (r < R2) W[(size_t)r * L_WC + c] = a[i];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (stop) break;
+  }
+  double* rec = recs + (size_t)k * NY * RECW;
+  double* out = rows_out + (size_t)k * NY * CRW;
+  for (int e = tid; e < R2 * L_WC; e += blockDim.x) {
+    const int r = e / L_WC, c = e - (e / L_WC) * L_WC;
+    const int pcol = c < 3 * NY + NPH ? c : RECW - 1;
+    if (r < NY) rec[(size_t)r * RECW + pcol] = W[e];
+    else if (c >= NY) out[(size_t)(r - NY) * CRW + pcol - NY] = W[e];
+  }
+}
+
+__device__ __forceinline__ void l_cr_pair(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
+                                          const int* __restrict__ act, const double* __restrict__ rows_in,
+                                          double* __restrict__ rows_out, double* __restrict__ recs,
+                                          double* __restrict__ work) {
+  if constexpr (L_RPL <= 48) l_cr_pair_panels<L_RPL>(oinst, seg, act, rows_in, rows_out, recs, work);
+  else l_cr_pair_global(oinst, seg, act, rows_in, rows_out, recs, work);
+}
+
+// Boundary system (see cr_final): [C G H | b] of the last row and
+// [B_n B_0 V_n | -r_bc] of the boundary conditions, NFIN x (NFIN + 1) in
+// work (device memory), solved by right-looking blocked LU with partial
+// pivoting (the largest |entry| of the column at or below the diagonal, the
+// first on ties; the row exchanges of a panel applied to the columns after
+// it, so the right-hand side column receives L^-1 with the trailing
+// updates), then blocked back substitution.  One workgroup of
 // L_FIN_THREADS per instance; status bit 4 on a zero pivot.
+constexpr int L_PB = 16;                              // panel width
+constexpr int L_FC = NFIN + 1;                        // columns of the augmented system
 __device__ __forceinline__ void l_cr_final(Batch bt, const double* __restrict__ rows, const double* __restrict__ Dr,
                                            const double* __restrict__ rbc, double* __restrict__ dy,
                                            double* __restrict__ dp, int* __restrict__ status,
                                            double* __restrict__ work) {
-  constexpr int FC = NFIN + 1;
+  constexpr int FC = L_FC;
   constexpr int DRW = 2 * NY + NP;
-  __shared__ double fcol[NFIN];
-  __shared__ double prow[FC];
+  __shared__ double P[NFIN * L_PB];        // the panel (rows c0 .. NFIN-1 used), then L21
+  __shared__ double S[L_PB * FC];          // the panel rows of the columns after the panel (U12)
+  __shared__ double xs[NFIN];
   __shared__ double rv[L_FIN_THREADS / WAVE];
   __shared__ int ri[L_FIN_THREADS / WAVE];
+  __shared__ int piv[L_PB];
   const int ib = blockIdx.x;
   if (skipped(bt, ib)) return;
   const double* row = rows + (size_t)ib * NY * CRW;
   Dr += (size_t)ib * DRSZ;
   rbc += (size_t)ib * NRB;
   double* F = work + (size_t)ib * NFIN * FC;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < NFIN * FC; e += blockDim.x) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  for (int e = tid; e < NFIN * FC; e += nthr) {
     const int r = e / FC, cc = e - (e / FC) * FC;
     double v = 0.0;
     if (r < NY) {
@@ -625,52 +909,145 @@ __device__ __forceinline__ void l_cr_final(Batch bt, const double* __restrict__ 
   }
   __syncthreads();
   int bad = 0;
-  for (int k = 0; k < NFIN; ++k) {
-    // pivot: the largest |F[r][k]|, r >= k (the first on ties)
-    double best = -1.0;
-    int pr = NFIN;
-    for (int r = k + tid; r < NFIN; r += blockDim.x) {
-      const double a = fabs(F[(size_t)r * FC + k]);
-      if (a > best || (a == best && r < pr)) { best = a; pr = r; }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ob = __shfl_xor(best, off);
-      const int op = __shfl_xor(pr, off);
-      if (ob > best || (ob == best && op < pr)) { best = ob; pr = op; }
-    }
-    if (lane_id() == 0) { rv[wave_id()] = best; ri[wave_id()] = pr; }
-    __syncthreads();
-    if (tid == 0) {
-      for (int w = 1; w < (int)blockDim.x / WAVE; ++w)
-        if (rv[w] > rv[0] || (rv[w] == rv[0] && ri[w] < ri[0])) { rv[0] = rv[w]; ri[0] = ri[w]; }
+  for (int c0 = 0; c0 < NFIN; c0 += L_PB) {
+    const int nb = min(L_PB, NFIN - c0);
+    const int ce = c0 + nb;   // first column after the panel
+    for (int e = tid; e < (NFIN - c0) * L_PB; e += nthr) {
+      const int r = c0 + e / L_PB, j = e - (e / L_PB) * L_PB;
+      P[r * L_PB + j] = j < nb ? F[(size_t)r * FC + c0 + j] : 0.0;
     }
     __syncthreads();
-    pr = ri[0];
-    if (pr != k)
-      for (int c = tid; c < FC; c += blockDim.x) {
-        const double t0 = F[(size_t)k * FC + c];
-        F[(size_t)k * FC + c] = F[(size_t)pr * FC + c];
-        F[(size_t)pr * FC + c] = t0;
+    // unblocked LU of the panel
+    for (int j = 0; j < nb; ++j) {
+      const int k = c0 + j;
+      double best = -1.0;
+      int pr = NFIN;
+      for (int r = k + tid; r < NFIN; r += nthr) {
+        const double a = fabs(P[r * L_PB + j]);
+        if (a > best || (a == best && r < pr)) { best = a; pr = r; }
       }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int op = __shfl_xor(pr, off);
+        if (ob > best || (ob == best && op < pr)) { best = ob; pr = op; }
+      }
+      if (lane_id() == 0) { rv[wave_id()] = best; ri[wave_id()] = pr; }
+      __syncthreads();
+      if (tid == 0) {
+        for (int w = 1; w < nthr / WAVE; ++w)
+          if (rv[w] > rv[0] || (rv[w] == rv[0] && ri[w] < ri[0])) { rv[0] = rv[w]; ri[0] = ri[w]; }
+        piv[j] = ri[0];
+      }
+      __syncthreads();
+      pr = piv[j];
+      if (pr != k && tid < L_PB) {
+        const double t0 = P[k * L_PB + tid];
+        P[k * L_PB + tid] = P[pr * L_PB + tid];
+        P[pr * L_PB + tid] = t0;
+      }
+      __syncthreads();
+      const double pv = P[k * L_PB + j];
+      if (!(fabs(pv) > 0.0)) bad = 1;
+      const double inv = 1.0 / pv;
+      for (int e = tid; e < (NFIN - k - 1) * L_PB; e += nthr) {
+        const int r = k + 1 + e / L_PB, jj = e - (e / L_PB) * L_PB;
+        if (jj <= j || jj >= nb) continue;
+        // multiplier of row r (recomputed per entry: the column is read before it is scaled)
+        P[r * L_PB + jj] = fma(-(P[r * L_PB + j] * inv), P[k * L_PB + jj], P[r * L_PB + jj]);
+      }
+      __syncthreads();
+      for (int r = k + 1 + tid; r < NFIN; r += nthr) P[r * L_PB + j] *= inv;
+      __syncthreads();
+    }
+    // the panel's row exchanges on the columns after it, in order
+    for (int j = 0; j < nb; ++j) {
+      const int k = c0 + j, pr = piv[j];
+      if (pr != k)
+        for (int c = ce + tid; c < FC; c += nthr) {
+          const double t0 = F[(size_t)k * FC + c];
+          F[(size_t)k * FC + c] = F[(size_t)pr * FC + c];
+          F[(size_t)pr * FC + c] = t0;
+        }
+      __syncthreads();
+    }
+    // U11 back to F (the back substitution reads it); U12 = L11^-1 F[c0:ce, ce:]
+    for (int e = tid; e < nb * L_PB; e += nthr) {
+      const int j = e / L_PB, jj = e - (e / L_PB) * L_PB;
+      if (jj >= j && jj < nb) F[(size_t)(c0 + j) * FC + c0 + jj] = P[(c0 + j) * L_PB + jj];
+    }
+    for (int c = ce + tid; c < FC; c += nthr) {
+      // column c of U12 by forward substitution with the unit lower L11, in LDS
+      for (int j = 0; j < L_PB; ++j) {
+        double v = 0.0;
+        if (j < nb) {
+          v = F[(size_t)(c0 + j) * FC + c];
+          for (int q = 0; q < j; ++q) v = fma(-P[(c0 + j) * L_PB + q], S[q * FC + c], v);
+          F[(size_t)(c0 + j) * FC + c] = v;
+        }
+        S[j * FC + c] = v;
+      }
+    }
     __syncthreads();
-    const double piv = F[(size_t)k * FC + k];
-    if (!(fabs(piv) > 0.0)) bad = 1;
-    const double inv = 1.0 / piv;
-    for (int c = tid; c < FC; c += blockDim.x) prow[c] = F[(size_t)k * FC + c] * inv;
-    for (int r = tid; r < NFIN; r += blockDim.x) fcol[r] = F[(size_t)r * FC + k];
+    // trailing update F[ce:, ce:] -= L21 U12, 4 x 4 register tiles
+    constexpr int TR = 4, TCL = 4;
+    const int nr = NFIN - ce, nc = FC - ce;
+    const int tr = (nr + TR - 1) / TR, tc = (nc + TCL - 1) / TCL;
+    for (int t = tid; t < tr * tc; t += nthr) {
+      const int rt = t / tc, ct = t - (t / tc) * tc;
+      const int r0 = ce + rt * TR, cb = ce + ct * TCL;
+      double acc[TR][TCL];
+#pragma unroll
+      for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int q = 0; q < TCL; ++q) acc[a][q] = 0.0;
+#pragma nounroll
+      for (int j = 0; j < L_PB; ++j) {
+        double l[TR], u[TCL];
+#pragma unroll
+        for (int a = 0; a < TR; ++a) l[a] = r0 + a < NFIN ? P[(r0 + a) * L_PB + j] : 0.0;
+#pragma unroll
+        for (int q = 0; q < TCL; ++q) u[q] = cb + q < FC ? S[j * FC + cb + q] : 0.0;
+#pragma unroll
+        for (int a = 0; a < TR; ++a)
+#pragma unroll
+          for (int q = 0; q < TCL; ++q) acc[a][q] = fma(l[a], u[q], acc[a][q]);
+      }
+#pragma unroll
+      for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int q = 0; q < TCL; ++q)
+          if (r0 + a < NFIN && cb + q < FC) F[(size_t)(r0 + a) * FC + cb + q] -= acc[a][q];
+    }
     __syncthreads();
-    for (int e = tid; e < NFIN * FC; e += blockDim.x) {
-      const int r = e / FC, c = e - (e / FC) * FC;
-      if (c < k) continue;
-      F[e] = r == k ? prow[c] : fma(-fcol[r], prow[c], F[e]);
+  }
+  // back substitution U x = F[:, NFIN], a block of L_PB rows at a time
+  for (int r = tid; r < NFIN; r += nthr) xs[r] = F[(size_t)r * FC + NFIN];
+  __syncthreads();
+  for (int b0 = ((NFIN - 1) / L_PB) * L_PB; b0 >= 0; b0 -= L_PB) {
+    const int be = min(b0 + L_PB, NFIN);
+    if (tid < WAVE) {   // the diagonal block, one wave: lane j holds row b0 + j
+      const int r = b0 + tid;
+      double v = r < be ? xs[r] : 0.0;
+      for (int c = be - 1; c >= b0; --c) {
+        const double xc = __shfl(v, c - b0) / F[(size_t)c * FC + c];
+        if (r == c) v = xc;
+        else if (r < c) v = fma(-F[(size_t)r * FC + c], xc, v);
+      }
+      if (r < be) xs[r] = v;
+    }
+    __syncthreads();
+    for (int r = tid; r < b0; r += nthr) {
+      double v = xs[r];
+      for (int c = b0; c < be; ++c) v = fma(-F[(size_t)r * FC + c], xs[c], v);
+      xs[r] = v;
     }
     __syncthreads();
   }
   double* x0 = dy + (size_t)bt.noff[ib] * NY;
   double* xn = dy + (size_t)(bt.noff[ib + 1] - 1) * NY;
   dp += (size_t)ib * NP;
-  for (int r = tid; r < NFIN; r += blockDim.x) {
-    const double v = F[(size_t)r * FC + NFIN];
+  for (int r = tid; r < NFIN; r += nthr) {
+    const double v = xs[r];
     if (r < NY) xn[r] = v;
     else if (r < 2 * NY) x0[r - NY] = v;
     else dp[r - 2 * NY] = v;
@@ -736,31 +1113,34 @@ __device__ __forceinline__ void l_cr_back(int level, Batch bt, const int* __rest
 
 // ------------------------------------------------------------ entry points
 extern "C" {
-__global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_stage_elim(int nint, colloc::Batch bt,
+// the register-resident Gauss-Jordan wants the whole register file of its SIMD
+__global__ __launch_bounds__(colloc::L_THREADS) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void colloc_l_stage_elim(int nint, colloc::Batch bt,
     const double* jac, const double* fa, double* rec, int* status) {
   colloc::l_stage_elim<double>(nint, bt, jac, fa, rec, status);
 }
-__global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_stage_elim32(int nint, colloc::Batch bt,
+__global__ __launch_bounds__(colloc::L_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void colloc_l_stage_elim32(int nint, colloc::Batch bt,
     const float* jac, const double* fa, float* rec, int* status) {
   colloc::l_stage_elim<float>(nint, bt, jac, fa, rec, status);
 }
 __global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_series_init(int nint, colloc::Batch bt,
-    const double* t, double* rec, double* rbuf, unsigned long long* slots) {
-  colloc::l_series_init<double>(nint, bt, t, rec, rbuf, slots);
+    const double* t, double* rec, unsigned long long* slots) {
+  colloc::l_series_init<double>(nint, bt, t, rec, slots);
 }
 __global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_series_init32(int nint, colloc::Batch bt,
-    const double* t, float* rec, float* rbuf, unsigned long long* slots) {
-  colloc::l_series_init<float>(nint, bt, t, rec, rbuf, slots);
+    const double* t, float* rec, unsigned long long* slots) {
+  colloc::l_series_init<float>(nint, bt, t, rec, slots);
 }
 __global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_series_term(int nint, colloc::Batch bt,
-    const double* t, const double* rec, const double* rbuf, const double* uold, long long so, double* unew,
-    long long sn, unsigned long long* slots, unsigned long long* active, int term) {
-  colloc::l_series_term<double>(nint, bt, t, rec, rbuf, uold, so, unew, sn, slots, active, term);
+    const double* t, const double* rec, const double* uold, long long so, double* unew, long long sn,
+    unsigned long long* slots, unsigned long long* active, int term) {
+  colloc::l_series_term<double>(nint, bt, t, rec, uold, so, unew, sn, slots, active, term);
 }
 __global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_series_term32(int nint, colloc::Batch bt,
-    const double* t, const float* rec, const float* rbuf, const float* uold, long long so, float* unew,
-    long long sn, unsigned long long* slots, unsigned long long* active, int term) {
-  colloc::l_series_term<float>(nint, bt, t, rec, rbuf, uold, so, unew, sn, slots, active, term);
+    const double* t, const float* rec, const float* uold, long long so, float* unew, long long sn,
+    unsigned long long* slots, unsigned long long* active, int term) {
+  colloc::l_series_term<float>(nint, bt, t, rec, uold, so, unew, sn, slots, active, term);
 }
 __global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_series_out(int nint, colloc::Batch bt,
     const double* fb, double* rows, double* rec, const double* ubuf, const unsigned long long* slots, int nterms,
