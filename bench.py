@@ -275,6 +275,9 @@ def main():
     ap.add_argument("--graph", type=int, choices=(0, 1), default=0,
                     help="Newton loops device resident, each iteration a HIP-graph replay (same results)")
     ap.add_argument("--cpu-sample", type=int, default=640, help="Nint of the CPU baseline sample (0: skip)")
+    ap.add_argument("--autotune", type=int, choices=(0, 1), default=1,
+                    help="batch: choose the condensation launch, the residual workgroup and the BABD pairing by "
+                         "measurement (autotune.py) after the warmup, outside the timed region")
     args = ap.parse_args()
     check_gpus(args)
     load_package()
@@ -289,6 +292,13 @@ def main():
     else:
         line = single_workload(args)
     print(json.dumps(line), flush=True)
+
+
+def S_spans(S, tune_rec, default):
+    """The BABD pairing the batch runs with (the autotuner's choice, if it ran)."""
+    if tune_rec and "babd_spans" in tune_rec["choices"]:
+        return tune_rec["choices"]["babd_spans"]
+    return default
 
 
 def batch_workload(args):
@@ -389,6 +399,12 @@ def batch_workload(args):
                "fallback_instances": 0}
     for _ in range(args.warmup):
         one_pass()
+    tune_rec = None
+    if args.autotune:
+        # on the handle's last chunk (mesh and state as the warmup left them)
+        from colloc_amd import autotune
+        tune_rec = autotune.tune(S, log=lambda s: print("autotune: " + s, file=sys.stderr, flush=True))
+    spans_used = S_spans(S, tune_rec, args.babd_spans)
     phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0, "warm_converged": 0, "learned_converged": 0,
                "fallback_instances": 0}
     S.profile(True)
@@ -474,7 +490,7 @@ def batch_workload(args):
     form_txt += ("; Newton direction condensed in float32 (residuals, BABD, convergence test in float64)"
                  if args.precision == 32 else "; float64 throughout")
     form_txt += ("; BABD pairs by substitution while a pair spans <= 1/%d of its instance, by Householder QR above"
-                 % args.babd_spans if args.babd_spans > 0 else "; BABD pairs by Householder QR on every level")
+                 % spans_used if spans_used > 0 else "; BABD pairs by Householder QR on every level")
     if args.babd_precision == 32:
         form_txt += "; cyclic reduction with float32 rows and records"
     if args.refine:
@@ -516,6 +532,8 @@ def batch_workload(args):
                    "looped_sample_converged": loop_ok,
                    "vs_looped_plain": value / loop_rate if loop_rate > 0 else None,
                    "model": model_info,
+                   "autotune": ({"choices": tune_rec["choices"], "timings_ms": tune_rec["timings_ms"],
+                                 "cached": tune_rec["cached"]} if tune_rec else None),
                    "pcie_included": "per chunk: mesh (and for the plain form the all-ones guess) copied in, "
                                     "network input copied in (inside the timed step)", **extra},
         "roofline": roof,

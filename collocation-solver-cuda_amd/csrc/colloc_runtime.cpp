@@ -97,7 +97,7 @@ enum Fn {
   F_AXPY_LISTED, F_COND_MF32, F_NT_CHECK, F_NT_SCAN, F_NT_FILL, F_NT_LS, F_NT_ACCEPT, F_NT_TICK, F_LINRES, F_LINRES32,
   F_STATE_ADD, F_CR_PAIR_DF, F_CR_PAIR_FF, F_CR_SUBST_DF, F_CR_SUBST_FF, F_CR_FINAL_F, F_CR_BACK_FD, F_CR_BACK_FF,
   F_L_STAGE, F_L_STAGE32, F_L_SINIT, F_L_SINIT32, F_L_TERM, F_L_TERM32, F_L_SOUT, F_L_SOUT32, F_L_CR_SUBST,
-  F_L_CR_PAIR, F_L_CR_FINAL, F_L_CR_BACK,
+  F_L_CR_PAIR, F_L_CR_FINAL, F_L_CR_BACK, F_COND_NS_Q, F_COND_NS32_Q, F_COND_MF32_Q,
   F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
@@ -112,12 +112,14 @@ const char* kFnNames[F_COUNT] = {
   "colloc_cr_final_f", "colloc_cr_back_fd", "colloc_cr_back_ff",
   "colloc_l_stage_elim", "colloc_l_stage_elim32", "colloc_l_series_init", "colloc_l_series_init32",
   "colloc_l_series_term", "colloc_l_series_term32", "colloc_l_series_out", "colloc_l_series_out32",
-  "colloc_l_cr_subst", "colloc_l_cr_pair", "colloc_l_cr_final", "colloc_l_cr_back"};
+  "colloc_l_cr_subst", "colloc_l_cr_pair", "colloc_l_cr_final", "colloc_l_cr_back", "colloc_condense_ns_q",
+  "colloc_condense_ns32_q", "colloc_condense_mf32_q"};
 // the large-block kernels (ny > 64) take the place of the register-resident
 // condensation and cyclic reduction: a module has one set or the other
-bool large_kernel(int f) { return f >= F_L_STAGE; }
+bool large_kernel(int f) { return f >= F_L_STAGE && f <= F_L_CR_BACK; }
 bool small_kernel(int f) {
-  return (f >= F_COND && f <= F_CR_BACK) || f == F_COND_MF32 || (f >= F_CR_PAIR_DF && f <= F_CR_BACK_FF);
+  return (f >= F_COND && f <= F_CR_BACK) || f == F_COND_MF32 || (f >= F_CR_PAIR_DF && f <= F_CR_BACK_FF) ||
+         f >= F_COND_NS_Q;
 }
 constexpr int kLargeMaxTerms = 48;        // L_MAX_TERMS of the large-block series
 constexpr int kLargeSlot = 8;             // L_SLOT: u64 per interval of its convergence slots
@@ -222,6 +224,9 @@ struct cs_handle {
   int rec_bits = 64;                    // stored type of the last condensation records (64 or 32)
   int mf_ok = 0;                        // the module's series condensation on matrix cores fits its sizes
   int cond_kind = 0;                    // see cs_set_condense_kernel
+  int cond_grid = 0;                    // series condensation: 0 one workgroup per interval, else a persistent
+                                        // grid of this many workgroups (cs_set_launch)
+  int res_block = 256;                  // workgroup size of the straight-line residual kernel (cs_set_launch)
   int pair_threads = 0;                 // block size of colloc_cr_pair (from the module)
   int h_zero = 0;                       // the module's BABD rows carry no H (identically zero)
   int jac_parts = 1;                    // threads per stage of the Jacobian kernel (from the module)
@@ -265,8 +270,8 @@ struct cs_handle {
   unsigned long long cond_terms = 0;      // series terms summed over intervals since creation
   unsigned long long cond_intervals = 0;  // intervals condensed since creation
   // [0, bmax): residual norms; [bmax, 2 bmax): segment-residual maxima;
-  // [2 bmax, 2 bmax + 3): condensation counters (queue length, series
-  // terms, intervals queued for Gauss-Jordan)
+  // [2 bmax, 2 bmax + 4): condensation counters (queue length, series
+  // terms, intervals queued for Gauss-Jordan, the persistent grid's ticket)
   unsigned long long* scal = nullptr;
   int* status = nullptr;
   unsigned long long* scratch_slots = nullptr;   // this device's `scratch` (cs_compute_f_q_noalloc)
@@ -444,8 +449,9 @@ int residual_to(cs_handle* h, const double* st, double alpha, bool full, const i
   void* args[] = {&nint, &bt, &h->t, &y, &yd, &zt, &p, &h->c, &alpha, &h->ytil, &h->fa, &h->fb, &h->rbc, &slot};
   const int nlaunch = h->launch_intervals();
   unsigned nthreads = (unsigned)(nlaunch * h->m + h->B);
-  return launch(h, full ? F_RES_FULL : F_RES_NORM, h->tab_g ? point_groups(h, nlaunch) : (nthreads + 255) / 256, 256,
-                args);
+  const unsigned rb = (unsigned)h->res_block;
+  return launch(h, full ? F_RES_FULL : F_RES_NORM, h->tab_g ? point_groups(h, nlaunch) : (nthreads + rb - 1) / rb,
+                h->tab_g ? 256u : rb, args);
 }
 
 int residual(cs_handle* h, const double* st, double alpha, bool full, const int* mask, double* norms) {
@@ -502,16 +508,24 @@ int condense(cs_handle* h, const int* mask, long long ncond) {
   // a fixed grid that walks whatever the queue holds
   const bool in_graph = h->capturing || h->nalistp != nullptr;
   TRY(fill_u64(h, counters, in_graph ? 1 : 3));
+  // the persistent grid's ticket (counters[3]), cleared for every launch
+  int persist = h->cond_grid > 0 ? 1 : 0;
+  if (persist) TRY(fill_u64(h, counters + 3, 1));
   Batch bt = h->batch_listed(mask);
   void* args_ns[] = {&nint, &bt, &h->t, &h->jac, &h->fa, &h->fb, &h->rows0, &h->rec, &h->status, &h->fbk, &counters};
   // the arithmetic follows the stored type of the stage Jacobians, and so
   // does the records' type
   const bool f32 = h->jac_bits == 32;
   h->rec_bits = h->jac_bits;
+  // one workgroup per listed interval, or the persistent grid (at most one
+  // workgroup per interval)
+  const unsigned grid_ns = persist ? (unsigned)std::min(h->cond_grid, h->launch_intervals())
+                                   : (unsigned)h->launch_intervals();
   if (f32 && h->mf_ok && h->cond_kind == 0)   // the series products on matrix cores, one wave per stage block
-    TRY(launch(h, F_COND_MF32, (unsigned)h->launch_intervals(), (unsigned)(h->m - 1) * 64, args_ns));
+    TRY(launch(h, persist ? F_COND_MF32_Q : F_COND_MF32, grid_ns, (unsigned)(h->m - 1) * 64, args_ns));
   else
-    TRY(launch(h, f32 ? F_COND_NS32 : F_COND_NS, (unsigned)h->launch_intervals(), kNsThreads, args_ns));
+    TRY(launch(h, f32 ? (persist ? F_COND_NS32_Q : F_COND_NS32) : (persist ? F_COND_NS_Q : F_COND_NS), grid_ns,
+               kNsThreads, args_ns));
   unsigned grid = (unsigned)std::min(h->launch_intervals(), kGaussJordanGrid);
   if (!in_graph) {
     unsigned long long cnt[3] = {0, 0, 0};
@@ -972,7 +986,7 @@ int cs_create(const cs_dims* d, cs_handle** out) {
     h->g_acc.base = 2 * kStampSlots;
   }
   if (r == CS_OK) {
-    hipError_t e = hipMalloc((void**)&h->scal, (2 * bm + 3) * sizeof(unsigned long long));
+    hipError_t e = hipMalloc((void**)&h->scal, (2 * bm + 4) * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&h->status, sizeof(int));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemset(h->status, 0, sizeof(int));
@@ -1048,6 +1062,9 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
   // which of the two kernel sets the module has
   hipFunction_t probe = nullptr;
   const bool mod_large = hipModuleGetFunction(&probe, h->module, kFnNames[F_L_STAGE]) == hipSuccess;
+  // the failed lookup of a small module is the runtime's last error now: clear
+  // it, or the next caller that checks hipGetLastError (torch does) reports it
+  if (!mod_large) (void)hipGetLastError();
   for (int f = 0; f < F_COUNT; ++f) {
     h->fn[f] = nullptr;
     if ((mod_large && small_kernel(f)) || (!mod_large && large_kernel(f))) continue;
@@ -1864,6 +1881,39 @@ int cs_set_condense_kernel(cs_handle* h, int kind) {
   if (kind != 0 && kind != 1) return fail(CS_E_ARG, "condensation kernel %d (0 or 1)", kind);
   h->cond_kind = kind;
   return CS_OK;
+}
+
+int cs_set_launch(cs_handle* h, int which, int value) {
+  if (!h) return fail(CS_E_ARG, "null handle");
+  switch (which) {
+    case CS_LAUNCH_CONDENSE_GRID:
+      if (value < 0) return fail(CS_E_ARG, "condensation grid %d (>= 0)", value);
+      h->cond_grid = value;
+      return CS_OK;
+    case CS_LAUNCH_RESIDUAL_BLOCK:
+      if (value != 64 && value != 128 && value != 256)
+        return fail(CS_E_ARG, "residual workgroup %d (64, 128 or 256)", value);
+      h->res_block = value;
+      return CS_OK;
+    default:
+      return fail(CS_E_ARG, "launch setting %d", which);
+  }
+}
+
+int cs_get_launch(cs_handle* h, int which, int* value) {
+  if (!h || !value) return fail(CS_E_ARG, "null argument");
+  switch (which) {
+    case CS_LAUNCH_CONDENSE_GRID: *value = h->cond_grid; return CS_OK;
+    case CS_LAUNCH_RESIDUAL_BLOCK: *value = h->res_block; return CS_OK;
+    case CS_LAUNCH_COMPUTE_UNITS: {
+      int dev = 0;
+      HIPCHK(hipGetDevice(&dev));
+      HIPCHK(hipDeviceGetAttribute(value, hipDeviceAttributeMultiprocessorCount, dev));
+      return CS_OK;
+    }
+    default:
+      return fail(CS_E_ARG, "launch setting %d", which);
+  }
 }
 
 int cs_set_babd_substitution(cs_handle* h, int min_spans) {

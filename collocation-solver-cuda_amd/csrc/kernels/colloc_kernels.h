@@ -970,7 +970,7 @@ template <> __device__ __forceinline__ float ns_tol<float>() { return 0x1p-21f; 
 // stored in float64); an interval whose float series does not converge is
 // left to the float64 Gauss-Jordan kernel, like a float64 one.
 template <class R>
-__device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __restrict__ t,
+__device__ __forceinline__ void condense_ns(int item, Batch bt, const double* __restrict__ t,
                                             const R* __restrict__ jac,
                                             const double* __restrict__ fa, const double* __restrict__ fb,
                                             double* __restrict__ rows, R* __restrict__ rec,
@@ -984,7 +984,7 @@ __device__ __forceinline__ void condense_ns(int nint, Batch bt, const double* __
   R* sT = smem_ns + NS_BUF;
   __shared__ R sred[2 * NWV];
   __shared__ int sbad;
-  const int i = listed_interval(bt, blockIdx.x);
+  const int i = listed_interval(bt, item);
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = wave_id();
   const int ib = bt.iinst[i];
@@ -1158,7 +1158,7 @@ constexpr int MF_LDS0 = M * NY * NVA + NSK * SE_SCR_MF;   // sT and the stage el
 constexpr int MF_LDS = (MF_LDS0 > NSK * MF_T * WAVE * 4) ? MF_LDS0 : NSK * MF_T * WAVE * 4;
 constexpr bool MF_OK = NSK >= 1 && NSK <= 16 && MF_T * 12 + MF_RT * MF_KS <= 176 && MF_LDS * 4 <= 64 * 1024;
 
-__device__ __forceinline__ void condense_mf32(int nint, Batch bt, const double* __restrict__ t,
+__device__ __forceinline__ void condense_mf32(int item, Batch bt, const double* __restrict__ t,
                                               const float* __restrict__ jac, const double* __restrict__ fa,
                                               const double* __restrict__ fb, double* __restrict__ rows,
                                               float* __restrict__ rec, int* __restrict__ status,
@@ -1170,7 +1170,7 @@ __device__ __forceinline__ void condense_mf32(int nint, Batch bt, const double* 
   __shared__ int sbad;
   float* sT = smem_mf;
   mf_f4* ybuf = reinterpret_cast<mf_f4*>(smem_mf);
-  const int i = listed_interval(bt, blockIdx.x);
+  const int i = listed_interval(bt, item);
   const int tid = threadIdx.x;
   const int lane = lane_id(), k = wave_id();
   const int ib = bt.iinst[i];
@@ -2230,6 +2230,43 @@ __device__ __forceinline__ void nt_accept(int B, NtState st, int* act, double* s
   }
 }
 
+#if !COLLOC_LARGE
+// The series condensation kernels run one work item (a listed interval) per
+// workgroup (persist = 0), or as a persistent grid (persist = 1): each
+// workgroup takes the next item from a ticket counter (*ticket, cleared by
+// the runtime before the launch) until the launch list is exhausted, so a
+// grid sized to the machine serves any list length, including one known
+// only on the device (a captured Newton iteration).  The body of an item
+// ends with its last LDS read before any barrier-free exit, and the next
+// item starts after the two barriers around the ticket.
+template <class F>
+__device__ __forceinline__ void series_items(int nint, const Batch& bt, int persist, unsigned long long* ticket,
+                                             F&& body) {
+  if (!persist) {
+    if (beyond_list(bt, nint)) return;
+    body((int)blockIdx.x);
+    return;
+  }
+  __shared__ int sitem;
+  const int n = launch_intervals(bt, nint);
+  for (;;) {
+    if (threadIdx.x == 0) sitem = (int)atomicAdd(ticket, 1ull);
+    __syncthreads();
+    const int item = sitem;
+    __syncthreads();
+    if (item >= n) break;
+    body(item);
+  }
+}
+// A kernel whose blocks do not fit leaves the item to Gauss-Jordan.
+__device__ __forceinline__ void queue_all(int item, const Batch& bt, int* fallback, unsigned long long* counters) {
+  if (threadIdx.x == 0) {
+    const int i = listed_interval(bt, item);
+    if (!(bt.act && !bt.act[bt.iinst[i]])) queue_gauss_jordan(i, fallback, counters);
+  }
+}
+#endif  // !COLLOC_LARGE
+
 }  // namespace colloc
 
 #if COLLOC_TABLE_FORM
@@ -2313,16 +2350,24 @@ __global__ __launch_bounds__(Cfg::CW * 64) void colloc_condense32(const int* iin
     double* rows, float* rec, int* status) {
   colloc::condense<Cfg::CW, float>(iinst, queue, nq, t, jac, fa, fb, rows, rec, status);
 }
+
 __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns(int nint, colloc::Batch bt,
     const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
     int* fallback, unsigned long long* counters) {
   if (colloc::beyond_list(bt, nint)) return;
-  if constexpr (colloc::NS_OK) {
-    colloc::condense_ns<double>(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
-  } else if (threadIdx.x == 0) {
-    const int i = colloc::listed_interval(bt, blockIdx.x);
-    if (!(bt.act && !bt.act[bt.iinst[i]])) colloc::queue_gauss_jordan(i, fallback, counters);
-  }
+  if constexpr (colloc::NS_OK) colloc::condense_ns<double>(blockIdx.x, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
+  else colloc::queue_all(blockIdx.x, bt, fallback, counters);
+}
+// The persistent-grid forms (a work queue over the launch list) are their
+// own kernels, so that the loop does not cost the one-workgroup-per-interval
+// forms registers.
+__global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns_q(int nint, colloc::Batch bt,
+    const double* t, const double* jac, const double* fa, const double* fb, double* rows, double* rec, int* status,
+    int* fallback, unsigned long long* counters) {
+  colloc::series_items(nint, bt, 1, counters + 3, [&](int item) {
+    if constexpr (colloc::NS_OK) colloc::condense_ns<double>(item, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
+    else colloc::queue_all(item, bt, fallback, counters);
+  });
 }
 
 // The FP32-direction condensation with the series products on matrix cores:
@@ -2332,12 +2377,16 @@ __global__ __launch_bounds__(colloc::MF_THREADS) void colloc_condense_mf32(int n
     const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
     int* fallback, unsigned long long* counters) {
   if (colloc::beyond_list(bt, nint)) return;
-  if constexpr (colloc::MF_OK) {
-    colloc::condense_mf32(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
-  } else if (threadIdx.x == 0) {
-    const int i = colloc::listed_interval(bt, blockIdx.x);
-    if (!(bt.act && !bt.act[bt.iinst[i]])) colloc::queue_gauss_jordan(i, fallback, counters);
-  }
+  if constexpr (colloc::MF_OK) colloc::condense_mf32(blockIdx.x, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
+  else colloc::queue_all(blockIdx.x, bt, fallback, counters);
+}
+__global__ __launch_bounds__(colloc::MF_THREADS) void colloc_condense_mf32_q(int nint, colloc::Batch bt,
+    const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
+    int* fallback, unsigned long long* counters) {
+  colloc::series_items(nint, bt, 1, counters + 3, [&](int item) {
+    if constexpr (colloc::MF_OK) colloc::condense_mf32(item, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
+    else colloc::queue_all(item, bt, fallback, counters);
+  });
 }
 // capabilities of this problem's kernels: [0] colloc_condense_mf32 runs the
 // matrix-core series (else it leaves every interval to Gauss-Jordan);
@@ -2356,12 +2405,16 @@ __global__ __launch_bounds__(colloc::NS_THREADS, 4) void colloc_condense_ns32(in
     const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
     int* fallback, unsigned long long* counters) {
   if (colloc::beyond_list(bt, nint)) return;
-  if constexpr (colloc::ns_ok<float>()) {
-    colloc::condense_ns<float>(nint, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
-  } else if (threadIdx.x == 0) {
-    const int i = colloc::listed_interval(bt, blockIdx.x);
-    if (!(bt.act && !bt.act[bt.iinst[i]])) colloc::queue_gauss_jordan(i, fallback, counters);
-  }
+  if constexpr (colloc::ns_ok<float>()) colloc::condense_ns<float>(blockIdx.x, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
+  else colloc::queue_all(blockIdx.x, bt, fallback, counters);
+}
+__global__ __launch_bounds__(colloc::NS_THREADS, 4) void colloc_condense_ns32_q(int nint, colloc::Batch bt,
+    const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
+    int* fallback, unsigned long long* counters) {
+  colloc::series_items(nint, bt, 1, counters + 3, [&](int item) {
+    if constexpr (colloc::ns_ok<float>()) colloc::condense_ns<float>(item, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
+    else colloc::queue_all(item, bt, fallback, counters);
+  });
 }
 
 __global__ __launch_bounds__(colloc::CR_THREADS) void colloc_cr_pair(const int* oinst, const colloc::CRSeg* seg,

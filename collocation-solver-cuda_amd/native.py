@@ -44,6 +44,8 @@ SIGNATURES = {
     "cs_solve_babd": (c_int, [c_void_p, P_D, P_D]),
     "cs_recover_delta_k": (c_int, [c_void_p, P_D, P_D]),
     "cs_set_babd_substitution": (c_int, [c_void_p, c_int]),
+    "cs_set_launch": (c_int, [c_void_p, c_int, c_int]),
+    "cs_get_launch": (c_int, [c_void_p, c_int, P_I]),
     "cs_set_direction_precision": (c_int, [c_void_p, c_int]),
     "cs_set_condense_kernel": (c_int, [c_void_p, c_int]),
     "cs_set_newton_graph": (c_int, [c_void_p, c_int]),
@@ -78,6 +80,8 @@ class ColloError(RuntimeError):
         super().__init__("colloc error %d: %s" % (code, msg))
         self.code = code
 
+
+CS_LAUNCH_CONDENSE_GRID, CS_LAUNCH_RESIDUAL_BLOCK, CS_LAUNCH_COMPUTE_UNITS = 0, 1, 2
 
 _lib = None
 

@@ -289,6 +289,19 @@ class TextProblem:
         return self.text
 
 
+def form_units():
+    """Problems whose device code the tests build in an explicit callback
+    form ("code": straight-line, "tables": polynomial tables), to compare
+    the two: (label, OCP text, m, form)."""
+    small, _ = config(1, nx=4, nu=2, nint=32)
+    items = [("tanh4", small.ocp_text(), 5),
+             ("tanh4_batch", batch_family(1, nx=4, nu=2, nint_range=(16, 64))[0].ocp_text(), 5),
+             ("trolley", CONSTRAINED_EXAMPLES["trolley"], 4), ("param_osc", PARAM_OSC_OCP, 4)]
+    for label, text, m in items:
+        for form in ("code", "tables"):
+            yield "%s_%s_m%d" % (label, form, m), text, m, form
+
+
 def standard_units():
     """Problems whose device code the build step compiles ahead of time:
     (label, FamilyInstance, m).  Mesh size does not change the generated

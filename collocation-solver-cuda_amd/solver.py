@@ -108,6 +108,7 @@ class CollocationSolver:
         self.bvp = bvp
         self.m = m
         self.form = form   # device code of the callbacks: "auto", "code" or "tables" (jit.choose_form)
+        self.direction_precision = 64
         self.ny, self.nz, self.np = bvp.size_y, bvp.size_z, bvp.size_p
         self.n_max = int(n_max or max(bvp.maximum_nodes, bvp.N))
         self.batch = int(batch)
@@ -270,11 +271,23 @@ class CollocationSolver:
         (0: Householder everywhere, as the reference)."""
         check(lib().cs_set_babd_substitution(self.h, int(min_spans)))
 
+    def set_launch(self, which, value):
+        """A launch shape that does not change results (native.CS_LAUNCH_*):
+        the series condensation's persistent grid (0: one workgroup per
+        interval) or the residual kernel's workgroup size."""
+        check(lib().cs_set_launch(self.h, int(which), int(value)))
+
+    def get_launch(self, which):
+        v = np.zeros(1, dtype=np.int32)
+        check(lib().cs_get_launch(self.h, int(which), native.iptr(v)))
+        return int(v[0])
+
     def set_direction_precision(self, bits):
         """Condensation arithmetic of the Newton direction: 64 (default) or
         32 (an inexact-Newton direction; residuals, BABD solve and the
         convergence test stay float64)."""
         check(lib().cs_set_direction_precision(self.h, int(bits)))
+        self.direction_precision = int(bits)
 
     def set_newton_graph(self, on):
         """Run the Newton loops device resident, one HIP-graph replay per

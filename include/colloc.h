@@ -242,6 +242,22 @@ int cs_get_direction(cs_handle* h, double* delta_y, double* delta_y_dot, double*
  * (solver:887-1001) to the same tolerance.  No counterpart in the reference. */
 int cs_set_condense_kernel(cs_handle* h, int kind);
 
+/* Launch shapes that do not change results (no counterpart in the
+ * reference; chosen by the autotuner, autotune.py):
+ *   CS_LAUNCH_CONDENSE_GRID: 0 (default) launches the series condensation
+ *     with one workgroup per interval; n > 0 launches a persistent grid of
+ *     n workgroups that take the intervals of the launch list from a work
+ *     queue (a ticket counter), so a captured Newton iteration needs no grid
+ *     sized for every interval;
+ *   CS_LAUNCH_RESIDUAL_BLOCK: workgroup size (64, 128, 256) of the residual
+ *     kernel of straight-line callbacks (one thread per stage);
+ *   CS_LAUNCH_COMPUTE_UNITS (get only): the device's compute units. */
+#define CS_LAUNCH_CONDENSE_GRID 0
+#define CS_LAUNCH_RESIDUAL_BLOCK 1
+#define CS_LAUNCH_COMPUTE_UNITS 2
+int cs_set_launch(cs_handle* h, int which, int value);
+int cs_get_launch(cs_handle* h, int which, int* value);
+
 /* Pairing rule of the BABD solve's cyclic reduction.  Every block row that
  * the condensation produces has C_i = I; while a pair of rows spans at most
  * 1/min_spans of its instance's intervals, the middle node is eliminated by

@@ -4,8 +4,9 @@ The reference turns an OCP into a BVP-DAE by forming the Hamiltonian
 H = L + lambda'f and emitting f = [H_lambda; -H_x], g = H_u and the boundary
 conditions r = [Gamma; lambda + Gamma_x' kappa_i; Psi; lambda - phi_x - Psi_x' kappa_f]
 (OCP2ABVP.py:41-182).  For the two benchmark families those expressions are
-derived here by hand, so that they pin both the oracle's generic sympy path
-(``oracle.abvp``) and the product's emitted HIP device code.
+derived here by hand, so that they pin the product's emitted device code
+(straight-line and polynomial-table forms) and the symbolic problem of
+tests/symbolic_problem.py.
 
 Family ``tanh``:  x' = A x + B u + s * tanh(C x),
                   cost int (q x'x + c_u u'u) dt,  x(0) = x0,  x(T) = 0.
