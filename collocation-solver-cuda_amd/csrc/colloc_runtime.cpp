@@ -122,7 +122,7 @@ bool small_kernel(int f) {
          f >= F_COND_NS_Q;
 }
 constexpr int kLargeMaxTerms = 48;        // L_MAX_TERMS of the large-block series
-constexpr int kLargeSlot = 8;             // L_SLOT: u64 per interval of its convergence slots
+constexpr int kLargeSlot = 10;            // L_SLOT: u64 per interval of its convergence slots
 constexpr unsigned kLargeThreads = 256;   // L_THREADS
 constexpr unsigned kLargeQrThreads = 512;   // L_QR_THREADS
 constexpr unsigned kLargeFinThreads = 512;  // L_FIN_THREADS

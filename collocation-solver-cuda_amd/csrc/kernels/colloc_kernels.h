@@ -964,6 +964,21 @@ constexpr bool NS_OK = ns_ok<double>();
 template <class R> __device__ __forceinline__ R ns_tol();
 template <> __device__ __forceinline__ double ns_tol<double>() { return 0x1p-50; }
 template <> __device__ __forceinline__ float ns_tol<float>() { return 0x1p-21f; }
+// The series' stopping rule after a term: dm is the term's largest change,
+// xm the largest entry of the new iterate, dprev the previous term's change
+// (0 before the second term).  Stop when the change is within the
+// tolerance; in float32 also when the observed contraction q = dm / dprev
+// is below 1/2 and bounds what the remaining terms would add (at most
+// dm q / (1 - q) <= 2 dm q) by a quarter of the tolerance: the iterate is
+// then as accurate as the rule asks, one term before its change is.  The
+// float64 series keeps the change test alone.
+template <class R>
+__device__ __forceinline__ bool ns_stop(R dm, R dprev, R xm) {
+  if (dm <= ns_tol<R>() * xm) return true;
+  if constexpr (sizeof(R) == 4)
+    return dprev > (R)0 && dm < (R)0.5 * dprev && (R)8 * dm * dm <= ns_tol<R>() * xm * dprev;
+  return false;
+}
 
 // R = double: the float64 condensation.  R = float: the FP32 direction (the
 // stage eliminations and the series in float, the records and the BABD row
@@ -1029,6 +1044,7 @@ __device__ __forceinline__ void condense_ns(int item, Batch bt, const double* __
   __syncthreads();
   bool converged = false;
   int terms = 0;
+  R dprev = (R)0;
   for (int term = 0; term < NS_MAX_TERMS; ++term) {
     ++terms;
     // Y = M_{k+1} U_k on this item's columns.  The row of M is re-read from
@@ -1087,10 +1103,11 @@ __device__ __forceinline__ void condense_ns(int item, Batch bt, const double* __
       dm = nmax(sred[w], dm);
       xm = nmax(sred[NWV + w], xm);
     }
-    if (dm <= ns_tol<R>() * xm) {
+    if (ns_stop<R>(dm, dprev, xm)) {
       converged = true;
       break;
     }
+    dprev = dm;
     __syncthreads();   // sred is rewritten by the next term
   }
   if (tid == 0) {
@@ -1157,6 +1174,12 @@ constexpr int MF_THREADS = NSK * WAVE;
 constexpr int MF_LDS0 = M * NY * NVA + NSK * SE_SCR_MF;   // sT and the stage eliminations' scratch
 constexpr int MF_LDS = (MF_LDS0 > NSK * MF_T * WAVE * 4) ? MF_LDS0 : NSK * MF_T * WAVE * 4;
 constexpr bool MF_OK = NSK >= 1 && NSK <= 16 && MF_T * 12 + MF_RT * MF_KS <= 176 && MF_LDS * 4 <= 64 * 1024;
+// Three workgroups fit a CU's LDS; three waves per SIMD leave 168 registers
+// a lane.  Where the tiles' registers (12 per accumulator tile, one per
+// A operand) leave room for addressing below that, the kernel is held to it
+// (19.5 -> 16.8 ms for the n_x = 20 family's 298 663-interval launch);
+// otherwise the compiler's choice (two waves per SIMD) stands.
+constexpr int MF_WAVES = MF_T * 12 + MF_RT * MF_KS <= 140 ? 3 : 1;
 
 __device__ __forceinline__ void condense_mf32(int item, Batch bt, const double* __restrict__ t,
                                               const float* __restrict__ jac, const double* __restrict__ fa,
@@ -1226,6 +1249,7 @@ __device__ __forceinline__ void condense_mf32(int item, Batch bt, const double* 
 
   bool converged = false;
   int terms = 0;
+  float dprev = 0.0f;
   for (int term = 0; term < NS_MAX_TERMS; ++term) {
     ++terms;
     mf_f4 y[MF_T];
@@ -1273,10 +1297,11 @@ __device__ __forceinline__ void condense_mf32(int item, Batch bt, const double* 
       dm = nmax(sred[w], dm);
       xm = nmax(sred[NSK + w], xm);
     }
-    if (dm <= ns_tol<float>() * xm) {
+    if (ns_stop<float>(dm, dprev, xm)) {
       converged = true;
       break;
     }
+    dprev = dm;
   }
   if (tid == 0) {
     if (!converged) queue_gauss_jordan(i, fallback, counters);
@@ -1469,22 +1494,32 @@ __device__ __forceinline__ void cr_pair(const int* __restrict__ oinst, const CRS
 // (NY x NY times NY x (NY+NP+1)) per pair instead of NY dependent reflector
 // steps.  Used only on the finest levels (a pair spans a small fraction of
 // the horizon, so the transition products stay near the identity); the
-// coarse levels keep the Householder step.  Thread (rg, cg) of the
-// workgroup computes a 4 x 4 tile of the product from LDS.
+// coarse levels keep the Householder step.
+// Every thread issues all its loads of G_b, [G_a | H_a | b_a] and [H_b | b_b]
+// before the first store to LDS (fixed trip counts); thread (rg, cg) then computes a
+// CS_TR x CS_TC tile of the product from LDS, the tiles go back to LDS,
+// and the output row is written row by row, with consecutive threads on
+// consecutive columns.
 constexpr int CS_XC = NY + NPH + 1;                   // columns of [G_a | H_a | b_a] (no H_a when HZ)
-constexpr int CS_CG = (CS_XC + 3) / 4;                // column groups of 4
-constexpr int CS_RG = (NY + 3) / 4;                   // row groups of 4
+constexpr int CS_TR = 2, CS_TC = 4;                   // a thread's product tile
+constexpr int CS_RG = (NY + CS_TR - 1) / CS_TR;       // row groups
+constexpr int CS_CG = (CS_XC + CS_TC - 1) / CS_TC;    // column groups
 constexpr int CS_THREADS = 256;
-static_assert(CS_RG * CS_CG <= 4 * CS_THREADS, "cr_subst tiles");
+constexpr int CS_GR = CS_RG * CS_TR;                  // padded rows of G_b
+constexpr int CS_XP = CS_CG * CS_TC;                  // padded columns of [G_a | H_a | b_a]
+constexpr int CS_LG = (NY * NY + CS_THREADS - 1) / CS_THREADS;      // loads per thread: G_b
+constexpr int CS_LX = (NY * CS_XC + CS_THREADS - 1) / CS_THREADS;   // [G_a | H_a | b_a]
+constexpr int CS_RC = CS_XC - NY;                                    // columns of [H_b | b_b]
+constexpr int CS_LB = (NY * CS_RC + CS_THREADS - 1) / CS_THREADS;   // [H_b | b_b]
 // TI / TO: stored type of the input / output rows; the product is computed
 // in TO (float32 arithmetic for the FP32 cyclic reduction).
 template <class TI, class TO>
 __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
                                          const int* __restrict__ act, const TI* __restrict__ rows_in,
                                          TO* __restrict__ rows_out, TO* __restrict__ recs) {
-  constexpr int XP = CS_CG * 4;                         // padded columns
-  __shared__ TO sG[CS_RG * 4][NY];                      // G_b (rows padded to a multiple of 4)
-  __shared__ TO sX[NY][XP];                             // [G_a | H_a | b_a]
+  __shared__ __attribute__((aligned(16))) TO sGT[NY][CS_GR];   // G_b transposed (padded rows zero)
+  __shared__ __attribute__((aligned(16))) TO sX[NY][CS_XP];    // [G_a | H_a | b_a], then the product
+  __shared__ TO sR[NY][CS_RC];                                 // [H_b | b_b]
   const int ib = oinst[blockIdx.x];
   if (act && !act[ib]) return;
   const CRSeg sg = seg[ib];
@@ -1502,7 +1537,7 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
   if (k == npairs) {  // carry the odd last row
     const TI* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
     TO* dst = rows_out + (size_t)npairs * NY * CRW;
-    for (int e = tid; e < NY * CRW; e += blockDim.x) {
+    for (int e = tid; e < NY * CRW; e += CS_THREADS) {
       const int r = e / CRW, c = e - (e / CRW) * CRW;
       if (c < NY || c == CRW - 1 || (!HZ && c >= 2 * NY)) dst[e] = (TO)src[e];
       else if (wc && c < 2 * NY) dst[e] = (c - NY == r) ? (TO)1 : (TO)0;
@@ -1512,55 +1547,105 @@ __device__ __forceinline__ void cr_subst(const int* __restrict__ oinst, const CR
   const TI* ra = rows_in + (size_t)(2 * k) * NY * CRW;
   const TI* rb = ra + NY * CRW;
   TO* out = rows_out + (size_t)k * NY * CRW;
-  for (int e = tid; e < CS_RG * 4 * NY; e += blockDim.x) {
+  // column c of [G_a | H_a | b_a] in a row: G_a at c, H_a after C_a, b_a last
+  auto xcol = [](int c) { return c < NY ? c : (c < NY + NPH ? NY + c : CRW - 1); };
+  TI vg[CS_LG], vx[CS_LX], vb[CS_LB];
+#pragma unroll
+  for (int q = 0; q < CS_LG; ++q) {
+    const int e = tid + q * CS_THREADS;
     const int r = e / NY, c = e - (e / NY) * NY;
-    (&sG[0][0])[e] = r < NY ? (TO)rb[r * CRW + c] : (TO)0;
+    vg[q] = e < NY * NY ? rb[r * CRW + c] : (TI)0;
   }
-  for (int e = tid; e < NY * XP; e += blockDim.x) {
-    const int r = e / XP, c = e - (e / XP) * XP;
-    TO v = 0;
-    if (c < NY) v = (TO)ra[r * CRW + c];                       // G_a
-    else if (c < NY + NPH) v = (TO)ra[r * CRW + NY + c];       // H_a (after C_a)
-    else if (c == NY + NPH) v = (TO)ra[r * CRW + CRW - 1];     // b_a
-    sX[r][c] = v;
+#pragma unroll
+  for (int q = 0; q < CS_LX; ++q) {
+    const int e = tid + q * CS_THREADS;
+    const int r = e / CS_XC, c = e - (e / CS_XC) * CS_XC;
+    vx[q] = e < NY * CS_XC ? ra[r * CRW + xcol(c)] : (TI)0;
   }
+#pragma unroll
+  for (int q = 0; q < CS_LB; ++q) {
+    const int e = tid + q * CS_THREADS;
+    const int r = e / CS_RC, c = NY + e - (e / CS_RC) * CS_RC;
+    vb[q] = e < NY * CS_RC ? rb[r * CRW + xcol(c)] : (TI)0;
+  }
+#pragma unroll
+  for (int q = 0; q < CS_LG; ++q) {
+    const int e = tid + q * CS_THREADS;
+    const int r = e / NY, c = e - (e / NY) * NY;
+    if (e < NY * NY) sGT[c][r] = (TO)vg[q];
+  }
+#pragma unroll
+  for (int q = 0; q < CS_LX; ++q) {
+    const int e = tid + q * CS_THREADS;
+    const int r = e / CS_XC, c = e - (e / CS_XC) * CS_XC;
+    if (e < NY * CS_XC) sX[r][c] = (TO)vx[q];
+  }
+#pragma unroll
+  for (int q = 0; q < CS_LB; ++q) {
+    const int e = tid + q * CS_THREADS;
+    if (e < NY * CS_RC) sR[e / CS_RC][e - (e / CS_RC) * CS_RC] = (TO)vb[q];
+  }
+  // the padding: rows NY.. of G_b, columns CS_XC.. of [G_a | H_a | b_a]
+  if constexpr (CS_GR > NY)
+    for (int e = tid; e < NY * (CS_GR - NY); e += CS_THREADS) sGT[e / (CS_GR - NY)][NY + e % (CS_GR - NY)] = (TO)0;
+  if constexpr (CS_XP > CS_XC)
+    for (int e = tid; e < NY * (CS_XP - CS_XC); e += CS_THREADS)
+      sX[e / (CS_XP - CS_XC)][CS_XC + e % (CS_XP - CS_XC)] = (TO)0;
   // no record: back substitution reads row a itself (x_M = b_a - G_a x_L - H_a p),
   // which stays in place in this level's input rows
   __syncthreads();
-  for (int t = tid; t < CS_RG * CS_CG; t += blockDim.x) {
-    const int rg = t / CS_CG, cg = t - (t / CS_CG) * CS_CG;
-    TO acc[4][4];
+  constexpr int NT = CS_RG * CS_CG;
+  constexpr int NTL = (NT + CS_THREADS - 1) / CS_THREADS;
+  TO acc[NTL][CS_TR][CS_TC];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+  for (int u = 0; u < NTL; ++u) {
+    const int t = tid + u * CS_THREADS;
+    const int rg = t < NT ? t / CS_CG : 0, cg = t < NT ? t - (t / CS_CG) * CS_CG : 0;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[a][c] = 0;
+    for (int a = 0; a < CS_TR; ++a)
+#pragma unroll
+      for (int c = 0; c < CS_TC; ++c) acc[u][a][c] = 0;
 #pragma unroll 4
     for (int q = 0; q < NY; ++q) {
-      TO g[4], x[4];
+      TO g[CS_TR], x[CS_TC];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) g[a] = sG[rg * 4 + a][q];
+      for (int a = 0; a < CS_TR; ++a) g[a] = sGT[q][rg * CS_TR + a];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) x[c] = sX[q][cg * 4 + c];
+      for (int c = 0; c < CS_TC; ++c) x[c] = sX[q][cg * CS_TC + c];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < CS_TR; ++a)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] = fma(g[a], x[c], acc[a][c]);
+        for (int c = 0; c < CS_TC; ++c) acc[u][a][c] = fma(g[a], x[c], acc[u][a][c]);
     }
+  }
+  __syncthreads();   // every thread has read sX
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int r = rg * 4 + a;
-      if (r >= NY) continue;
+  for (int u = 0; u < NTL; ++u) {
+    const int t = tid + u * CS_THREADS;
+    if (t >= NT) continue;
+    const int rg = t / CS_CG, cg = t - (t / CS_CG) * CS_CG;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int col = cg * 4 + c;
-        if (col < NY) out[r * CRW + col] = -acc[a][c];                                         // G'
-        else if (col < NY + NPH) out[r * CRW + NY + col] = (TO)rb[r * CRW + NY + col] - acc[a][c];  // H'
-        else if (col == NY + NPH) out[r * CRW + CRW - 1] = (TO)rb[r * CRW + CRW - 1] - acc[a][c];  // b'
+    for (int a = 0; a < CS_TR; ++a) {
+      const int r = rg * CS_TR + a;
+      if (r < NY) {
+#pragma unroll
+        for (int c = 0; c < CS_TC; ++c) sX[r][cg * CS_TC + c] = acc[u][a][c];
       }
     }
   }
+  __syncthreads();
+  // [G' | H' | b'] = [-P_G | H_b - P_H | b_b - P_b], P = G_b [G_a | H_a | b_a]
+  for (int e = tid; e < NY * CS_XC; e += CS_THREADS) {
+    const int r = e / CS_XC, c = e - (e / CS_XC) * CS_XC;
+    const TO v = sX[r][c];
+    if (c < NY) {
+      out[r * CRW + c] = -v;                                   // G'
+    } else {
+      out[r * CRW + xcol(c)] = sR[r][c - NY] - v;              // H', b'
+    }
+  }
   if (wc) {
-    for (int e = tid; e < NY * NY; e += blockDim.x) {      // C' = C_b = I
+    for (int e = tid; e < NY * NY; e += CS_THREADS) {      // C' = C_b = I
       const int r = e / NY, c = e - (e / NY) * NY;
       out[r * CRW + NY + c] = (r == c) ? (TO)1 : (TO)0;
     }
@@ -2373,7 +2458,8 @@ __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns_q(int n
 // The FP32-direction condensation with the series products on matrix cores:
 // one wave per stage block (the runtime launches it when colloc_caps[0] says
 // the problem's blocks fit, else colloc_condense_ns32).
-__global__ __launch_bounds__(colloc::MF_THREADS) void colloc_condense_mf32(int nint, colloc::Batch bt,
+__global__ __launch_bounds__(colloc::MF_THREADS) __attribute__((amdgpu_waves_per_eu(colloc::MF_WAVES)))
+void colloc_condense_mf32(int nint, colloc::Batch bt,
     const double* t, const float* jac, const double* fa, const double* fb, double* rows, float* rec, int* status,
     int* fallback, unsigned long long* counters) {
   if (colloc::beyond_list(bt, nint)) return;

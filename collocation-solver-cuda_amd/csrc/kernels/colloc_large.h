@@ -33,7 +33,7 @@ constexpr int L_NRT = (NY + L_TILE - 1) / L_TILE;
 constexpr int L_NCT = (NVA + L_TILE - 1) / L_TILE;
 constexpr int L_TILES = L_NRT * L_NCT;
 constexpr int L_MAX_TERMS = 48;
-constexpr int L_SLOT = 8;                        // u64 per interval: 3 x (change, size), the converged term
+constexpr int L_SLOT = 10;                       // u64 per interval: 4 x (change, size), the converged term, -
 constexpr int CS_XC = NY + NPH + 1;              // columns of [G_a | H_a | b_a]
 constexpr int L_SCT = (CS_XC + L_TILE - 1) / L_TILE;
 constexpr int L_STILES = L_NRT * L_SCT;
@@ -342,16 +342,31 @@ __device__ __forceinline__ void l_series_init(int nint, Batch bt, const double* 
   if (k == 0 && threadIdx.x < L_SLOT) slots[(size_t)i * L_SLOT + threadIdx.x] = 0;
 }
 
-__device__ __forceinline__ bool l_converged(const unsigned long long* s, int set, double tol) {
+// The stopping rule of ns_stop (colloc_kernels.h) on the slots after term
+// `term`: its change within the tolerance, or, in float32 from the second
+// term on, a contraction against the previous term's change that bounds the
+// remaining terms by a quarter of the tolerance.
+template <class R>
+__device__ __forceinline__ bool l_stop(const unsigned long long* s, int term, double tol) {
+  const int set = term % 4;
   const double d = slot_d(s[2 * set]), x = slot_d(s[2 * set + 1]);
-  return d <= tol * x;
+  if (d <= tol * x) return true;
+  if constexpr (sizeof(R) == 4) {
+    if (term >= 2) {
+      const double dp = slot_d(s[2 * ((term - 1) % 4)]);
+      return dp > 0.0 && d < 0.5 * dp && 8.0 * d * d <= tol * x * dp;
+    }
+  }
+  return false;
 }
 
 // Term `term` (>= 1) of the series for tile (rt, ct) of the listed interval:
 // Unew_k = R_k + sum_q h A[k+1][q+1] M_{q+1} Uold_q on rows rt, columns ct.
 // Uold / Unew: the U blocks (stride ustride_old / ustride_new per interval);
-// slots[i]: [3 x (largest change, largest entry) by term mod 3 | converged
-// term | -]; active[term]: intervals that computed this term.
+// slots[i]: [4 x (largest change, largest entry) by term mod 4 | converged
+// term | -]; a term reads the sets of the two terms before it, writes its
+// own and clears the next term's; active[term]: intervals that computed
+// this term.
 template <class R>
 __device__ __forceinline__ void l_series_term(int nint, Batch bt, const double* __restrict__ t,
                                               const R* __restrict__ rec,
@@ -369,12 +384,12 @@ __device__ __forceinline__ void l_series_term(int nint, Batch bt, const double* 
   if (skipped(bt, ib)) return;
   unsigned long long* s = slots + (size_t)i * L_SLOT;
   const double tol = (double)l_tol<R>();
-  if (term >= 2 && l_converged(s, (term - 1) % 3, tol)) {
-    if (tile == 0 && threadIdx.x == 0 && s[6] == 0) s[6] = (unsigned long long)(term - 1);
+  if (term >= 2 && l_stop<R>(s, term - 1, tol)) {
+    if (tile == 0 && threadIdx.x == 0 && s[8] == 0) s[8] = (unsigned long long)(term - 1);
     return;
   }
   if (tile == 0 && threadIdx.x == 0) {
-    const int nx = (term + 1) % 3;
+    const int nx = (term + 1) % 4;
     s[2 * nx] = 0;
     s[2 * nx + 1] = 0;
     atomicAdd(active + term, 1ull);
@@ -433,7 +448,7 @@ __device__ __forceinline__ void l_series_term(int nint, Batch bt, const double* 
           un[e] = x;
         }
       }
-  l_block_max2(dmax, xmax, s + 2 * (term % 3), s + 2 * (term % 3) + 1);
+  l_block_max2(dmax, xmax, s + 2 * (term % 4), s + 2 * (term % 4) + 1);
 }
 
 // After `nterms` terms: the term at which each listed interval converged
@@ -453,8 +468,8 @@ __device__ __forceinline__ void l_series_out(int nint, Batch bt, const double* _
   if (skipped(bt, ib)) return;
   const unsigned long long* s = slots + (size_t)i * L_SLOT;
   if (threadIdx.x == 0) {
-    int tc = (int)s[6];
-    if (tc == 0 && nterms >= 1 && l_converged(s, nterms % 3, (double)l_tol<R>())) tc = nterms;
+    int tc = (int)s[8];
+    if (tc == 0 && nterms >= 1 && l_stop<R>(s, nterms, (double)l_tol<R>())) tc = nterms;
     sterm = tc;
     if (tc == 0) atomicAdd(counters + 2, 1ull);
     else atomicAdd(counters + 1, (unsigned long long)tc);
