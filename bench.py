@@ -655,9 +655,16 @@ def precision_workload(args):
 
     forms = {}
     sols = {}
+    tuned = {}
     for bits in (64, 32):
         for _ in range(args.warmup):
             one_solve(bits)
+        if args.autotune:
+            # the pairing rule on the solved state, outside the timed region
+            from colloc_amd import autotune
+            S.set_direction_precision(bits)
+            rec = autotune.tune(S, log=lambda s: print("autotune fp%d: %s" % (bits, s), file=sys.stderr, flush=True))
+            tuned[bits] = {"choices": rec["choices"], "timings_ms": rec["timings_ms"], "cached": rec["cached"]}
         S.profile(True)
         st0 = S.condense_stats()
         t0 = time.perf_counter()
@@ -704,7 +711,8 @@ def precision_workload(args):
                    "fp64_solves_per_s": sum(1 for r in forms[64]["results"] if r[0]) / forms[64]["elapsed"],
                    "fp64_newton_iterations": forms[64]["results"][-1][1],
                    "fp32_vs_fp64_solution_rel_diff": float(np.max(np.abs(sols[32][0] - sols[64][0])) / scale),
-                   "kernel_ms_fp32": f32["prof"], "kernel_ms_fp64": forms[64]["prof"]},
+                   "kernel_ms_fp32": f32["prof"], "kernel_ms_fp64": forms[64]["prof"],
+                   "autotune": {"fp%d" % b: v for b, v in tuned.items()} or None},
         "roofline": roof,
     }
     S.close()

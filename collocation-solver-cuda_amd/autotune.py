@@ -6,9 +6,11 @@ chosen here by measurement on the handle's current mesh and state:
 
 * the series condensation's launch: one workgroup per interval, or a
   persistent grid of k workgroups per compute unit taking intervals from a
-  work queue (cs_set_launch CS_LAUNCH_CONDENSE_GRID);
+  work queue (cs_set_launch CS_LAUNCH_CONDENSE_GRID; not for ny > 64, whose
+  large-block series has no such launch);
 * the residual kernel's workgroup size (CS_LAUNCH_RESIDUAL_BLOCK; the
-  straight-line callbacks, one thread per stage);
+  straight-line callbacks, one thread per stage; the table form's
+  evaluator has a fixed workgroup);
 * the cyclic reduction's pairing rule (cs_set_babd_substitution: the
   substitution levels' span limit, the "QR tile shape" of the block
   reduction), among the candidates whose Newton direction agrees with the
@@ -29,7 +31,7 @@ import os
 
 import numpy as np
 
-from . import jit, native
+from . import emitter, jit, native
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE_DIR = os.path.join(HERE, "_tune")
@@ -72,7 +74,7 @@ def _rel(a, b):
 
 
 def tune(S, alpha=0.1, reps=3, grid_factors=(2, 3, 4, 8), residual_blocks=(64, 128, 256),
-         spans=(0, 8, 16, 32, 64), direction_tol=1e-7, use_cache=True, log=None):
+         spans=(0, 2, 4, 8, 16, 32, 64), direction_tol=1e-7, use_cache=True, log=None):
     """Measure the candidates on S's current mesh and state, apply the
     fastest of each and return {"choices": ..., "timings_ms": ...}.  The
     state is left as it was (only the direction buffers change)."""
@@ -90,6 +92,10 @@ def tune(S, alpha=0.1, reps=3, grid_factors=(2, 3, 4, 8), residual_blocks=(64, 1
     choices = {}
     S.compute_f_q(alpha, outputs=False)
     S.construct_jacobian(alpha, outputs=False)
+    if S.ny > emitter.LARGE_NY:
+        grid_factors = ()
+    if jit.unit_info(S.bvp.ocp_text, S.m, S.form)["form"] == "tables":
+        residual_blocks = ()
     # condensation launch (results identical for every grid)
     cand = [0] + [k * cus for k in grid_factors]
     tc = {}
@@ -107,10 +113,11 @@ def tune(S, alpha=0.1, reps=3, grid_factors=(2, 3, 4, 8), residual_blocks=(64, 1
         S.set_launch(native.CS_LAUNCH_RESIDUAL_BLOCK, b)
         tr[b] = _timed(S, "residual", lambda: S.compute_f_q(alpha, outputs=False), reps)
         say("residual block %d: %.3f ms" % (b, tr[b]))
-    best = min(tr, key=tr.get)
-    S.set_launch(native.CS_LAUNCH_RESIDUAL_BLOCK, best)
-    choices["residual_block"] = best
-    timings["residual_block"] = {str(k): v for k, v in tr.items()}
+    if tr:
+        best = min(tr, key=tr.get)
+        S.set_launch(native.CS_LAUNCH_RESIDUAL_BLOCK, best)
+        choices["residual_block"] = best
+        timings["residual_block"] = {str(k): v for k, v in tr.items()}
     # pairing rule, among the directions that agree with Householder-only
     if spans:
         S.set_babd_substitution(0)

@@ -97,7 +97,7 @@ enum Fn {
   F_AXPY_LISTED, F_COND_MF32, F_NT_CHECK, F_NT_SCAN, F_NT_FILL, F_NT_LS, F_NT_ACCEPT, F_NT_TICK, F_LINRES, F_LINRES32,
   F_STATE_ADD, F_CR_PAIR_DF, F_CR_PAIR_FF, F_CR_SUBST_DF, F_CR_SUBST_FF, F_CR_FINAL_F, F_CR_BACK_FD, F_CR_BACK_FF,
   F_L_STAGE, F_L_STAGE32, F_L_SINIT, F_L_SINIT32, F_L_TERM, F_L_TERM32, F_L_SOUT, F_L_SOUT32, F_L_CR_SUBST,
-  F_L_CR_PAIR, F_L_CR_FINAL, F_L_CR_BACK, F_COND_NS_Q, F_COND_NS32_Q, F_COND_MF32_Q,
+  F_L_CR_PAIR, F_L_CR_FINAL, F_L_CR_BACK, F_COND_NS_Q, F_COND_NS32_Q, F_COND_MF32_Q, F_ROWS_C,
   F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
@@ -113,7 +113,7 @@ const char* kFnNames[F_COUNT] = {
   "colloc_l_stage_elim", "colloc_l_stage_elim32", "colloc_l_series_init", "colloc_l_series_init32",
   "colloc_l_series_term", "colloc_l_series_term32", "colloc_l_series_out", "colloc_l_series_out32",
   "colloc_l_cr_subst", "colloc_l_cr_pair", "colloc_l_cr_final", "colloc_l_cr_back", "colloc_condense_ns_q",
-  "colloc_condense_ns32_q", "colloc_condense_mf32_q"};
+  "colloc_condense_ns32_q", "colloc_condense_mf32_q", "colloc_rows_constant"};
 // the large-block kernels (ny > 64) take the place of the register-resident
 // condensation and cyclic reduction: a module has one set or the other
 bool large_kernel(int f) { return f >= F_L_STAGE && f <= F_L_CR_BACK; }
@@ -193,6 +193,7 @@ struct cs_handle {
   double* Dr = nullptr;
   double* rows0 = nullptr;   // BABD rows of every cyclic-reduction level, level after level
   size_t rows_cap = 0;       // rows0's capacity in rows
+  bool rows_c_ok = false;    // level-0 rows hold C = I (and zero inactive H columns) for this mesh
   double* rec = nullptr;
   double* crrec = nullptr;
   double* segres = nullptr;
@@ -496,8 +497,23 @@ int jacobian_static(cs_handle* h) {
 // Gauss-Jordan kernel redoes the queued ones.
 int condense_large(cs_handle* h, const int* mask, long long ncond);
 
+// C = I and the zero inactive H columns of the level-0 BABD rows
+// (colloc_rows_constant): written once per mesh and module, outside any
+// captured iteration; the register-resident condensation kernels write only
+// G, the active H columns and b.  The large-block kernels write whole rows.
+int fill_rows_constant(cs_handle* h) {
+  h->rows_c_ok = false;
+  if (!h->module || h->large || h->nint <= 0 || !h->fn[F_ROWS_C]) return CS_OK;
+  int nint = h->nint;
+  void* args[] = {&nint, &h->rows0};
+  TRY(launch(h, F_ROWS_C, (unsigned)nint, 256, args));
+  h->rows_c_ok = true;
+  return CS_OK;
+}
+
 int condense(cs_handle* h, const int* mask, long long ncond) {
   if (h->large) return condense_large(h, mask, ncond);
+  if (!h->rows_c_ok) TRY(fill_rows_constant(h));
   ProfScope ps(h, CS_KERNEL_CONDENSE);
   h->cond_intervals += (unsigned long long)ncond;
   int nint = h->nint;
@@ -1118,7 +1134,8 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
     h->recn = recn;
   }
   h->npa = npa;
-  return CS_OK;
+  // the new module's constant row blocks (its active parameters may differ)
+  return fill_rows_constant(h);
 }
 
 int cs_set_mesh(cs_handle* h, const double* t, int N) {
@@ -1233,7 +1250,8 @@ int set_batch_mesh(cs_handle* h, int B, const int* nodes, const double* t) {
   TRY(upload_ints(h, h->iinst, iinst));
   TRY(upload_ints(h, h->ninst, ninst));
   TRY(sync(h));
-  return build_cr_schedule(h);
+  TRY(build_cr_schedule(h));
+  return fill_rows_constant(h);
 }
 
 int init_from_saved(cs_handle* h) {

@@ -608,71 +608,95 @@ __device__ __forceinline__ int stage_eliminate_all(int i, const TJ* __restrict__
 constexpr int SE_KZ = (NZ + 3) / 4;                 // k-steps over the algebraic unknowns
 constexpr int SE_GW = ((NVA + 15) / 16) * 16;       // padded columns of the LDS copy of G_j
 constexpr int SE_SCR_MF = NYZ + SE_KZ * 4 * SE_GW;
-template <int RL, class TR>
-__device__ __forceinline__ int stage_rows_mf(int i, int j, int r0, int r1, bool write_g, const float* __restrict__ jac,
-                                             const double* __restrict__ fa, float* sT, TR* __restrict__ recT,
-                                             TR* __restrict__ recG, float* scr) {
-  typedef float f4 __attribute__((ext_vector_type(4)));
-  constexpr int NCZ = NZ + NVA;
-  constexpr int SL = (NCZ + WAVE - 1) / WAVE;
-  constexpr int RTL = (RL + 15) / 16;
-  constexpr int CT = SE_GW / 16;
+// The loads of stage_rows_mf: the g rows (lanes = columns), h_z as A
+// operands, [h_y | h_p] in the accumulator layout, f_a.  Split from the
+// arithmetic so that a wave can have two stages' loads in flight.
+typedef float se_f4 __attribute__((ext_vector_type(4)));
+template <int RL>
+struct StageLoadMF {
+  static constexpr int SL = (NZ + NVA + WAVE - 1) / WAVE;
+  static constexpr int RTL = (RL + 15) / 16;
+  static constexpr int CT = SE_GW / 16;
+  static constexpr int FV = (NYZ + WAVE - 1) / WAVE;
+  float gc[SL][NZ1];
+  float az[RTL][SE_KZ];
+  se_f4 hv[RTL][CT];
+  float fv[FV];
+};
+
+__device__ __forceinline__ int se_src(int cc) {   // Jacobian column of [g_z | g_y | g_pa | -] column cc
+  const int q = cc - NZ;
+  return cc < NZ ? NY + cc : (q < NY ? q : (q < NY + NPA ? NY + NZ + pact(q - NY < NPA ? q - NY : 0) : 0));
+}
+
+template <int RL>
+__device__ __forceinline__ void stage_load_mf(int i, int j, int r0, int r1, const float* __restrict__ jac,
+                                              const double* __restrict__ fa, StageLoadMF<RL>& L) {
+  typedef StageLoadMF<RL> SL_;
   const int lane = lane_id(), g = lane >> 4, c16 = lane & 15;
   const float* Dfj = jac + ((size_t)i * M + j) * JSTAGE;
   const float* Dgj = Dfj + NY * NJC;
   const double* faj = fa + (size_t)i * M * NYZ + j * NYZ;
-  float* sfa = scr;             // f_a of the stage
-  float* sG = scr + NYZ;        // G_j, SE_KZ * 4 rows of SE_GW
-  int src[SL];
-  bool rhs[SL];
-#pragma unroll
-  for (int sl = 0; sl < SL; ++sl) {
-    const int cc = lane + WAVE * sl, q = cc - NZ;
-    src[sl] = cc < NZ ? NY + cc : (q < NY ? q : (q < NY + NPA ? NY + NZ + pact(q - NY < NPA ? q - NY : 0) : 0));
-    rhs[sl] = q == NY + NPA;
-  }
-  // every load first: g rows, h_z as A operands, [h_y | h_p] in the accumulator layout, f_a
-  float gc[SL][NZ1];
 #pragma unroll
   for (int r = 0; r < NZ; ++r)
 #pragma unroll
-    for (int sl = 0; sl < SL; ++sl) gc[sl][r] = Dgj[r * NJC + src[sl]];
-  float az[RTL][SE_KZ];
+    for (int sl = 0; sl < SL_::SL; ++sl) L.gc[sl][r] = Dgj[r * NJC + se_src(lane + WAVE * sl)];
 #pragma unroll
-  for (int ri = 0; ri < RTL; ++ri)
+  for (int ri = 0; ri < SL_::RTL; ++ri)
 #pragma unroll
     for (int ks = 0; ks < SE_KZ; ++ks) {
       const int row = r0 + 16 * ri + c16, z = 4 * ks + g;
       const float v = Dfj[(row < NY ? row : NY - 1) * NJC + NY + (z < NZ ? z : 0)];
-      az[ri][ks] = (row < r1 && z < NZ) ? v : 0.0f;
+      L.az[ri][ks] = (row < r1 && z < NZ) ? v : 0.0f;
     }
-  f4 hv[RTL][CT];
 #pragma unroll
-  for (int ri = 0; ri < RTL; ++ri)
+  for (int ri = 0; ri < SL_::RTL; ++ri)
 #pragma unroll
-    for (int ci = 0; ci < CT; ++ci) {
+    for (int ci = 0; ci < SL_::CT; ++ci) {
       const int q = 16 * ci + c16;
       const int sq = q < NY ? q : (q < NY + NPA ? NY + NZ + pact(q - NY < NPA ? q - NY : 0) : 0);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = r0 + 16 * ri + 4 * g + v;
-        hv[ri][ci][v] = Dfj[(row < NY ? row : NY - 1) * NJC + sq];
+        L.hv[ri][ci][v] = Dfj[(row < NY ? row : NY - 1) * NJC + sq];
       }
     }
-  float fv[(NYZ + WAVE - 1) / WAVE];
 #pragma unroll
-  for (int k = 0; k < (NYZ + WAVE - 1) / WAVE; ++k) {
+  for (int k = 0; k < SL_::FV; ++k) {
     const int e = lane + WAVE * k;
-    fv[k] = (float)faj[e < NYZ ? e : 0];
+    L.fv[k] = (float)faj[e < NYZ ? e : 0];
   }
+}
+
+// stage_rows for float32 with T_j = [h_y | h_p | f_a,h] - h_z G_j on matrix
+// cores, from the loads of stage_load_mf: h_z rows as A operands (row
+// 16 ri + l % 16, column 4 ks + l / 16), G_j as B operands from the wave's
+// LDS copy, and [h_y | h_p | f_a,h] in the accumulator layout (lane l,
+// value v of tile (ri, ci): row r0 + 16 ri + 4 (l / 16) + v, column
+// 16 ci + l % 16).  The g_z elimination is stage_rows'.  scr: SE_SCR_MF
+// floats of LDS.
+template <int RL, class TR>
+__device__ __forceinline__ int stage_compute_mf(int i, int j, int r0, int r1, bool write_g, StageLoadMF<RL>& L,
+                                                float* sT, TR* __restrict__ recT, TR* __restrict__ recG, float* scr) {
+  typedef StageLoadMF<RL> SL_;
+  constexpr int SL = SL_::SL;
+  constexpr int RTL = SL_::RTL;
+  constexpr int CT = SL_::CT;
+  const int lane = lane_id(), g = lane >> 4, c16 = lane & 15;
+  float* sfa = scr;             // f_a of the stage
+  float* sG = scr + NYZ;        // G_j, SE_KZ * 4 rows of SE_GW
+  auto& gc = L.gc;
+  bool rhs[SL];
 #pragma unroll
-  for (int k = 0; k < (NYZ + WAVE - 1) / WAVE; ++k)
-    if (lane + WAVE * k < NYZ) sfa[lane + WAVE * k] = fv[k];
+  for (int sl = 0; sl < SL; ++sl) rhs[sl] = lane + WAVE * sl - NZ == NY + NPA;
+#pragma unroll
+  for (int k = 0; k < SL_::FV; ++k)
+    if (lane + WAVE * k < NYZ) sfa[lane + WAVE * k] = L.fv[k];
 #pragma unroll
   for (int sl = 0; sl < SL; ++sl)
 #pragma unroll
     for (int r = 0; r < NZ; ++r)
-      if (rhs[sl]) gc[sl][r] = readlane_r(fv[(NY + r) / WAVE], (NY + r) % WAVE);
+      if (rhs[sl]) gc[sl][r] = readlane_r(L.fv[(NY + r) / WAVE], (NY + r) % WAVE);
   // Gauss-Jordan on [g_z | ...] with partial pivoting (lanes = columns)
   int bad = 0;
 #pragma unroll
@@ -734,15 +758,15 @@ __device__ __forceinline__ int stage_rows_mf(int i, int j, int r0, int r1, bool 
   for (int ri = 0; ri < RTL; ++ri)
 #pragma unroll
     for (int ci = 0; ci < CT; ++ci) {
-      f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+      se_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int ks = 0; ks < SE_KZ; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(az[ri][ks], bg[ks][ci], acc, 0, 0, 0);
+      for (int ks = 0; ks < SE_KZ; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(L.az[ri][ks], bg[ks][ci], acc, 0, 0, 0);
       const int q = 16 * ci + c16;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int row = r0 + 16 * ri + 4 * g + v;
         if (row >= r1 || q >= NVA) continue;
-        const float h = (q == NY + NPA) ? sfa[row] : hv[ri][ci][v];
+        const float h = (q == NY + NPA) ? sfa[row] : L.hv[ri][ci][v];
         const float tv = h - acc[v];
         sT[(j * NY + row) * NVA + q] = tv;
         recT[(j * NY + row) * NVA + q] = (TR)tv;
@@ -752,7 +776,9 @@ __device__ __forceinline__ int stage_rows_mf(int i, int j, int r0, int r1, bool 
 }
 
 // stage_eliminate_all for the matrix-core kernel (float32): the same
-// assignment of stages and row ranges to waves.  scr: NW * SE_SCR_MF floats.
+// assignment of stages and row ranges to waves.  With one stage per wave
+// and one stage split by rows (m = NW + 1), both stages' loads are issued
+// before either is computed.  scr: NW * SE_SCR_MF floats.
 template <int NW, class TR>
 __device__ __forceinline__ int stage_eliminate_mf(int i, const float* __restrict__ jac, const double* __restrict__ fa,
                                                   float* sT, TR* __restrict__ recT, TR* __restrict__ recG, float* scr) {
@@ -760,18 +786,38 @@ __device__ __forceinline__ int stage_eliminate_mf(int i, const float* __restrict
   float* my = scr + wave * SE_SCR_MF;
   int bad = 0;
   if constexpr (NW >= M) {
-    if (wave < M) bad |= stage_rows_mf<NY, TR>(i, wave, 0, NY, true, jac, fa, sT, recT, recG, my);
+    if (wave < M) {
+      StageLoadMF<NY> L;
+      stage_load_mf<NY>(i, wave, 0, NY, jac, fa, L);
+      bad |= stage_compute_mf<NY, TR>(i, wave, 0, NY, true, L, sT, recT, recG, my);
+    }
   } else {
     constexpr int FULL = (M / NW) * NW;
     constexpr int RS = (NY + NW - 1) / NW;
-    for (int j = wave; j < FULL; j += NW)
-      bad |= stage_rows_mf<NY, TR>(i, j, 0, NY, true, jac, fa, sT, recT, recG, my);
-    for (int j = FULL; j < M; ++j) {
-      const int r0 = wave * RS < NY ? wave * RS : NY;
-      const int r1 = r0 + RS < NY ? r0 + RS : NY;
+    const int r0 = wave * RS < NY ? wave * RS : NY;
+    const int r1 = r0 + RS < NY ? r0 + RS : NY;
+    if constexpr (FULL == NW && M - FULL == 1) {
+      StageLoadMF<NY> L;
+      StageLoadMF<RS> E;
+      stage_load_mf<NY>(i, wave, 0, NY, jac, fa, L);
+      stage_load_mf<RS>(i, FULL, r0, r1, jac, fa, E);
+      bad |= stage_compute_mf<NY, TR>(i, wave, 0, NY, true, L, sT, recT, recG, my);
       __builtin_amdgcn_wave_barrier();
       asm volatile("" ::: "memory");   // the previous stage's scratch reads are done
-      bad |= stage_rows_mf<RS, TR>(i, j, r0, r1, wave == 0, jac, fa, sT, recT, recG, my);
+      bad |= stage_compute_mf<RS, TR>(i, FULL, r0, r1, wave == 0, E, sT, recT, recG, my);
+    } else {
+      for (int j = wave; j < FULL; j += NW) {
+        StageLoadMF<NY> L;
+        stage_load_mf<NY>(i, j, 0, NY, jac, fa, L);
+        bad |= stage_compute_mf<NY, TR>(i, j, 0, NY, true, L, sT, recT, recG, my);
+      }
+      for (int j = FULL; j < M; ++j) {
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");   // the previous stage's scratch reads are done
+        StageLoadMF<RS> E;
+        stage_load_mf<RS>(i, j, r0, r1, jac, fa, E);
+        bad |= stage_compute_mf<RS, TR>(i, j, r0, r1, wave == 0, E, sT, recT, recG, my);
+      }
     }
   }
   return bad;
@@ -893,16 +939,7 @@ __device__ __forceinline__ void condense_one(int i, const int* __restrict__ iins
       }
     }
   }
-  for (int e = tid; e < NY * NY; e += NT) {
-    const int r = e / NY, q = e - (e / NY) * NY;
-    rowi[r * CRW + NY + q] = (r == q) ? 1.0 : 0.0;
-  }
-  if (!HZ && NPA < NP) {   // H columns of parameters the stage functions do not depend on
-    for (int e = tid; e < NY * NP; e += NT) {
-      const int r = e / NP, j = e - (e / NP) * NP;
-      if (!p_active(j)) rowi[r * CRW + 2 * NY + j] = 0.0;
-    }
-  }
+  // C = I and the inactive H columns: colloc_rows_constant, once per mesh
   if (tid == 0 && sbad) atomicOr(status, sbad);
 }
 
@@ -1118,7 +1155,7 @@ __device__ __forceinline__ void condense_ns(int item, Batch bt, const double* __
   __syncthreads();
   if (tid == 0 && sbad) atomicOr(status, sbad);
   if (!converged) return;
-  // outputs: U, the BABD row from the last stage block, C = I, zero H columns
+  // outputs: U and the BABD row [G | H_active | b] from the last stage block
   double* rowi = rows + (size_t)i * NY * CRW;
   if (act) {
 #pragma unroll
@@ -1134,16 +1171,7 @@ __device__ __forceinline__ void condense_ns(int item, Batch bt, const double* __
       }
     }
   }
-  for (int e = tid; e < NY * NY; e += NS_THREADS) {
-    const int a = e / NY, q = e - (e / NY) * NY;
-    rowi[a * CRW + NY + q] = (a == q) ? 1.0 : 0.0;
-  }
-  if (!HZ && NPA < NP) {
-    for (int e = tid; e < NY * NP; e += NS_THREADS) {
-      const int a = e / NP, j = e - (e / NP) * NP;
-      if (!p_active(j)) rowi[a * CRW + 2 * NY + j] = 0.0;
-    }
-  }
+  // C = I and the inactive H columns: colloc_rows_constant, once per mesh
 }
 
 // ------------------------------------- series condensation on matrix cores
@@ -1311,7 +1339,7 @@ __device__ __forceinline__ void condense_mf32(int item, Batch bt, const double* 
   __syncthreads();
   if (tid == 0 && sbad) atomicOr(status, sbad);
   if (!converged) return;
-  // outputs: U_k, the BABD row from the last stage block, C = I, zero H columns
+  // outputs: U_k and the BABD row [G | H_active | b] from the last stage block
   double* rowi = rows + (size_t)i * NY * CRW;
   const bool last = k == NSK - 1;
 #pragma unroll
@@ -1331,16 +1359,7 @@ __device__ __forceinline__ void condense_mf32(int item, Batch bt, const double* 
           else rowi[row * CRW + CRW - 1] = dv - fb[(size_t)i * NY + row];
         }
       }
-  for (int e = tid; e < NY * NY; e += MF_THREADS) {
-    const int a = e / NY, q = e - (e / NY) * NY;
-    rowi[a * CRW + NY + q] = (a == q) ? 1.0 : 0.0;
-  }
-  if (!HZ && NPA < NP) {
-    for (int e = tid; e < NY * NP; e += MF_THREADS) {
-      const int a = e / NP, j = e - (e / NP) * NP;
-      if (!p_active(j)) rowi[a * CRW + 2 * NY + j] = 0.0;
-    }
-  }
+  // C = I and the inactive H columns: colloc_rows_constant, once per mesh
 }
 
 // ------------------------------------------------------ cyclic reduction
@@ -2453,6 +2472,28 @@ __global__ __launch_bounds__(colloc::NS_THREADS) void colloc_condense_ns_q(int n
     if constexpr (colloc::NS_OK) colloc::condense_ns<double>(item, bt, t, jac, fa, fb, rows, rec, status, fallback, counters);
     else colloc::queue_all(item, bt, fallback, counters);
   });
+}
+
+// The constant blocks of the condensation's BABD rows (the cyclic
+// reduction's level-0 rows): C = I and, where the stage functions depend on
+// only some parameters, the zero H columns of the others.  No later level
+// writes into level 0's rows, so the runtime writes these once per mesh and
+// the condensation kernels write only G, the active H columns and b.
+// One workgroup per interval.
+__global__ __launch_bounds__(256) void colloc_rows_constant(int nint, double* rows) {
+  const int i = blockIdx.x;
+  if (i >= nint) return;
+  double* rowi = rows + (size_t)i * colloc::NY * colloc::CRW;
+  for (int e = threadIdx.x; e < colloc::NY * colloc::NY; e += blockDim.x) {
+    const int a = e / colloc::NY, q = e - (e / colloc::NY) * colloc::NY;
+    rowi[a * colloc::CRW + colloc::NY + q] = (a == q) ? 1.0 : 0.0;
+  }
+  if constexpr (!colloc::HZ && colloc::NPA < colloc::NP) {
+    for (int e = threadIdx.x; e < colloc::NY * colloc::NP; e += blockDim.x) {
+      const int a = e / colloc::NP, j = e - (e / colloc::NP) * colloc::NP;
+      if (!colloc::p_active(j)) rowi[a * colloc::CRW + 2 * colloc::NY + j] = 0.0;
+    }
+  }
 }
 
 // The FP32-direction condensation with the series products on matrix cores:
