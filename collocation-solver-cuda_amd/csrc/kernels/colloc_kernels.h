@@ -1280,19 +1280,22 @@ __device__ __forceinline__ void condense_mf32(int item, Batch bt, const double* 
   float dprev = 0.0f;
   for (int term = 0; term < NS_MAX_TERMS; ++term) {
     ++terms;
-    mf_f4 y[MF_T];
+    // one row of output tiles at a time (MF_CT independent accumulation
+    // chains), each stored to the exchange buffer when complete, so only
+    // one row's accumulators are live
 #pragma unroll
-    for (int tt = 0; tt < MF_T; ++tt) y[tt] = mf_f4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ri = 0; ri < MF_RT; ++ri) {
+      mf_f4 y[MF_CT];
 #pragma unroll
-    for (int ks = 0; ks < MF_KS; ++ks)
+      for (int ci = 0; ci < MF_CT; ++ci) y[ci] = mf_f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int ri = 0; ri < MF_RT; ++ri)
+      for (int ks = 0; ks < MF_KS; ++ks)
 #pragma unroll
         for (int ci = 0; ci < MF_CT; ++ci)
-          y[ri * MF_CT + ci] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[ri][ks], u[(ks >> 2) * MF_CT + ci][ks & 3],
-                                                                     y[ri * MF_CT + ci], 0, 0, 0);
+          y[ci] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[ri][ks], u[(ks >> 2) * MF_CT + ci][ks & 3], y[ci], 0, 0, 0);
 #pragma unroll
-    for (int tt = 0; tt < MF_T; ++tt) ybuf[(k * MF_T + tt) * WAVE + lane] = y[tt];
+      for (int ci = 0; ci < MF_CT; ++ci) ybuf[(k * MF_T + ri * MF_CT + ci) * WAVE + lane] = y[ci];
+    }
     __syncthreads();
     // U_k <- R_k + sum_q h A^[k][q] Y_q (padded entries stay zero)
     float dmax = 0.0f, xmax = 0.0f;
