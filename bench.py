@@ -27,6 +27,7 @@ import importlib.util
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -231,7 +232,10 @@ def main():
                          "instance with the float64 and the FP32 Newton direction")
     ap.add_argument("--nint", type=int, default=None, help="single: override Nint (default 4096)")
     ap.add_argument("--instances", type=int, default=1024, help="batch: instances of the family")
-    ap.add_argument("--chunk-nodes", type=int, default=320000, help="batch: most nodes solved in one chunk")
+    ap.add_argument("--chunk-nodes", type=int, default=320000,
+                    help="batch: most nodes held at once (divided among the --streams handles)")
+    ap.add_argument("--streams", type=int, default=3,
+                    help="batch: handles solving chunks concurrently, one host thread and HIP stream each")
     ap.add_argument("--loop-sample", type=int, default=64,
                     help="batch: instances solved one by one for the looped plain rate")
     ap.add_argument("--form", choices=("learned", "plain"), default="learned",
@@ -312,15 +316,54 @@ def batch_workload(args):
     ny, nz, np_ = bvp.size_y, bvp.size_z, bvp.size_p
     tol, max_iter = bvp.tolerance, bvp.maximum_newton_iterations
     nodes = [inst.nodes for inst in insts]
-    chunks = solver.plan_chunks(nodes, args.chunk_nodes, len(insts))
-    S = solver.CollocationSolver(bvp, m, n_max=max(sum(nodes[i] for i in ch) for ch in chunks),
-                                 batch=max(len(ch) for ch in chunks))
-    S.set_direction_precision(args.precision)
-    S.set_condense_kernel(args.condense_kernel)
-    S.set_babd_substitution(args.babd_spans)
-    S.set_newton_graph(bool(args.graph))
-    S.set_refinement(args.refine)
-    S.set_babd_precision(args.babd_precision)
+    # --streams handles solve chunks concurrently, each driven by a host
+    # thread of its own on its own HIP stream (the library releases the
+    # interpreter inside its calls); the chunks shrink so the handles
+    # together hold what one handle of --chunk-nodes would
+    nstreams = max(1, args.streams)
+    chunk_nodes = args.chunk_nodes // nstreams
+    chunks = solver.plan_chunks(nodes, chunk_nodes, len(insts))
+    n_max = max(sum(nodes[i] for i in ch) for ch in chunks)
+    b_max = max(len(ch) for ch in chunks)
+    handles = [solver.CollocationSolver(bvp, m, n_max=n_max, batch=b_max) for _ in range(nstreams)]
+    S = handles[0]
+
+    def each(fn):
+        for S_ in handles:
+            fn(S_)
+    each(lambda S_: S_.set_direction_precision(args.precision))
+    each(lambda S_: S_.set_condense_kernel(args.condense_kernel))
+    each(lambda S_: S_.set_babd_substitution(args.babd_spans))
+    each(lambda S_: S_.set_newton_graph(bool(args.graph)))
+    each(lambda S_: S_.set_refinement(args.refine))
+    each(lambda S_: S_.set_babd_precision(args.babd_precision))
+
+    def run_chunks(chs, work):
+        """work(handle, chunk) for every chunk: chunk k on handle k mod
+        --streams, each handle's chunks in order on a thread of its own.
+        Returns [(chunk, result)] in the order of chs."""
+        groups = [list(range(k, len(chs), nstreams)) for k in range(nstreams)]
+        out = [None] * len(chs)
+        if nstreams == 1:
+            for k, ch in enumerate(chs):
+                out[k] = (ch, work(S, ch))
+            return out
+        errors = []
+
+        def body(h, ks):
+            try:
+                for k in ks:
+                    out[k] = (chs[k], work(handles[h], chs[k]))
+            except BaseException as e:   # re-raised on the main thread
+                errors.append(e)
+        threads = [threading.Thread(target=body, args=(h, groups[h])) for h in range(nstreams)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        if errors:
+            raise errors[0]
+        return out
     ts_all = [np.linspace(0.0, inst.T, inst.nodes) for inst in insts]
     cons_all = np.array([inst.constant_values() for inst in insts])
     hom_all = [np.array([inst.constant_values(s=0.0) for inst in insts])]
@@ -341,12 +384,14 @@ def batch_workload(args):
         hom = hom_all if hom is None else hom
         conv = np.zeros(len(idx), dtype=bool)
         its = np.zeros((len(hom) + 1, len(idx)), dtype=int)
-        for ch in solver.plan_chunks([nodes[i] for i in idx], S.n_max, S.batch):
+
+        def work(S_, ch):
             sel = [idx[k] for k in ch]
             Nt = sum(nodes[i] for i in sel)
-            S.set_batch_mesh([ts_all[i] for i in sel])
-            S.form_initial_input(np.ones((Nt, ny)), np.ones((Nt, nz)), np.ones((len(sel), np_)))
-            r = solver.newton_stages(S, cons_all[sel], [h[sel] for h in hom], 0.1, tol, max_iter, stall=stall)
+            S_.set_batch_mesh([ts_all[i] for i in sel])
+            S_.form_initial_input(np.ones((Nt, ny)), np.ones((Nt, nz)), np.ones((len(sel), np_)))
+            return solver.newton_stages(S_, cons_all[sel], [h[sel] for h in hom], 0.1, tol, max_iter, stall=stall)
+        for ch, r in run_chunks(solver.plan_chunks([nodes[i] for i in idx], S.n_max, S.batch), work):
             conv[ch] = r["converged"]
             its[:, ch] = r["iterations"]
         return conv, its
@@ -370,10 +415,12 @@ def batch_workload(args):
         its = np.zeros(len(insts), dtype=int)
         newton_ok = np.zeros(len(insts), dtype=bool)
         t_warm = time.perf_counter()
-        for ch in chunks:
-            r = learned.warm_solve(ws, S, [ts_all[i] for i in ch], feats_all[ch], cons_all[ch], hom_all[0][ch],
-                                   tol=tol, max_iter=max_iter, max_iter_warm=args.warm_cap,
-                                   patience=args.warm_patience, min_ratio=args.warm_ratio, cont=warm_cont)
+
+        def work(S_, ch):
+            return learned.warm_solve(ws, S_, [ts_all[i] for i in ch], feats_all[ch], cons_all[ch], hom_all[0][ch],
+                                      tol=tol, max_iter=max_iter, max_iter_warm=args.warm_cap,
+                                      patience=args.warm_patience, min_ratio=args.warm_ratio, cont=warm_cont)
+        for ch, r in run_chunks(chunks, work):
             conv[ch] = r["converged"]
             its[ch] = r["iterations"].sum(axis=0) + r["lq_iterations"]
             newton_ok[ch] = r["norm"] < tol
@@ -401,20 +448,24 @@ def batch_workload(args):
         one_pass()
     tune_rec = None
     if args.autotune:
-        # on the handle's last chunk (mesh and state as the warmup left them)
+        # on each handle's last chunk (mesh and state as the warmup left them)
         from colloc_amd import autotune
-        tune_rec = autotune.tune(S, log=lambda s: print("autotune: " + s, file=sys.stderr, flush=True))
+        for S_ in handles:
+            tune_rec = autotune.tune(S_, log=lambda s: print("autotune: " + s, file=sys.stderr, flush=True))
     spans_used = S_spans(S, tune_rec, args.babd_spans)
     phase_s = {"warm": 0.0, "fallback": 0.0, "fallback_converged": 0, "warm_converged": 0, "learned_converged": 0,
                "fallback_instances": 0}
-    S.profile(True)
-    st0 = S.condense_stats()
+    each(lambda S_: S_.profile(True))
+    st0 = tuple(int(x) for x in np.sum([S_.condense_stats() for S_ in handles], axis=0))
     t0 = time.perf_counter()
     results = [one_pass() for _ in range(steps)]
     elapsed = time.perf_counter() - t0
-    st1 = S.condense_stats()
-    prof = {k: S.profile_get(k) for k in ("residual", "jacobian", "condense", "babd", "recover")}
-    S.profile(False)
+    st1 = tuple(int(x) for x in np.sum([S_.condense_stats() for S_ in handles], axis=0))
+    prof = {}
+    for k in ("residual", "jacobian", "condense", "babd", "recover"):
+        v = [S_.profile_get(k) for S_ in handles]
+        prof[k] = (sum(x[0] for x in v), sum(x[1] for x in v))
+    each(lambda S_: S_.profile(False))
     phases = {k: v / steps for k, v in phase_s.items()}
     converged = int(sum(r[0].sum() for r in results))
     value = converged / elapsed
@@ -424,12 +475,16 @@ def batch_workload(args):
     roof = condense_roofline(S, st0, st1, prof["condense"][0], prof["condense"][1], ny, nz, np_, npa, m,
                              measured_traffic("batch_%s" % args.form, kname), kname)
     roof["arithmetic"] = "fp%d" % args.precision
+    if nstreams > 1:
+        roof["concurrent_streams"] = nstreams
+        roof["note"] = ("launch durations are HIP-event times on each handle's stream while %d handles run "
+                        "concurrently: a launch shares the GPU with the other streams' kernels" % nstreams)
     extra = {}
     if args.form == "learned" and args.precision == 32 and args.fp64_pass:
         # the same learned + batched solve with the float64 direction
-        S.set_direction_precision(64)
-        S.set_babd_precision(64)
-        S.set_refinement(0)
+        each(lambda S_: S_.set_direction_precision(64))
+        each(lambda S_: S_.set_babd_precision(64))
+        each(lambda S_: S_.set_refinement(0))
         t0 = time.perf_counter()
         c64, it64, b64 = learned_pass()
         lt = time.perf_counter() - t0
@@ -437,15 +492,15 @@ def batch_workload(args):
                       "learned_fp64_unconverged_after_newton": b64})
     if args.form == "learned" and args.plain_pass:
         # the plain batched form (float64 direction) on the same instances, for the split of the gain
-        S.set_direction_precision(64)
-        S.set_babd_precision(64)
-        S.set_refinement(0)
+        each(lambda S_: S_.set_direction_precision(64))
+        each(lambda S_: S_.set_babd_precision(64))
+        each(lambda S_: S_.set_refinement(0))
         t0 = time.perf_counter()
         pconv, pits, _ = plain_pass()
         pt = time.perf_counter() - t0
         extra.update({"plain_batched_solves_per_s": pconv.sum() / pt, "plain_batched_converged": int(pconv.sum()),
                       "plain_newton_iterations_mean": pits.sum(axis=0).mean()})
-    S.close()
+    each(lambda S_: S_.close())
 
     # the plain single-instance solve looped over a sample of the same instances
     S1 = solver.CollocationSolver(bvp, m, n_max=max(nodes), batch=1)
@@ -514,9 +569,12 @@ def batch_workload(args):
         "data": "synthetic (seeded OCP family, generated in-process)",
         "config": {"workload": "batched family: %d instances of the tanh OCP n_x=20 n_u=10 (ny=40 nz=10 np=40), "
                                "shared A,B,C (seed 1), x0~U[-1,1]^20, T~U[0.5,2], Nint~U{256..2048}, m=%d; %s; "
-                               "%d chunk(s) of <= %d nodes" % (len(insts), m, form_txt, len(chunks), args.chunk_nodes),
+                               "%d chunk(s) of <= %d nodes%s" % (len(insts), m, form_txt, len(chunks), chunk_nodes,
+                                                                 "" if nstreams == 1 else
+                                                                 ", solved %d at a time (one handle, host thread "
+                                                                 "and HIP stream each)" % nstreams),
                    "form": args.form, "direction_precision": args.precision, "instances": len(insts), "mean_nint": mean_nint, "m": m, "tolerance": tol,
-                   "converged_per_pass": int(conv.sum()), "chunks": len(chunks),
+                   "converged_per_pass": int(conv.sum()), "chunks": len(chunks), "streams": nstreams,
                    "newton_iterations_mean": float(it_mean),
                    "warm_stage_converged_per_pass": phases["warm_converged"] if args.form == "learned" else None,
                    "unconverged_after_newton_per_pass": n_back if args.form == "learned" else None,

@@ -34,6 +34,7 @@ must reach the same solution.
 """
 import ctypes
 import os
+import threading
 import time
 
 import numpy as np
@@ -305,6 +306,9 @@ class WarmStart:
         self.base = base
         self.mean = torch.tensor(np.asarray(mean, dtype=np.float32), device="cuda")
         self.sd = torch.tensor(np.asarray(sd, dtype=np.float32), device="cuda")
+        # the network's activation buffers are shared: one guess at a time
+        # when several host threads drive handles
+        self._lock = threading.Lock()
 
     def save(self, path):
         st = self.net.state()
@@ -332,7 +336,13 @@ class WarmStart:
         """Initial guess of every instance of solv's current batch (its meshes
         set) from the network, written with cs_form_initial_input_device.
         With base "lq" the solver's current state must be the instances' LQ
-        solution on these meshes: the network's output is added to it."""
+        solution on these meshes: the network's output is added to it.
+        Safe to call from several threads (one at a time: the network's
+        buffers are shared; the work ends with a stream synchronisation)."""
+        with self._lock:
+            return self._guess_on_meshes(solv, feats)
+
+    def _guess_on_meshes(self, solv, feats):
         torch = _torch()
         B = solv.B
         G = self.predict_grid(feats)
