@@ -270,9 +270,10 @@ def main():
                          "final one")
     ap.add_argument("--fallback-stall", default="10,5,2",
                     help="batch, learned: the plain fallback's stall test start,window,ratio; 0: off")
-    ap.add_argument("--babd-precision", type=int, choices=(32, 64), default=64,
-                    help="batch: stored type of the cyclic reduction's rows and records (32: float32 levels, "
-                         "float64 boundary solve and back substitution)")
+    ap.add_argument("--babd-precision", type=int, choices=(32, 64), default=None,
+                    help="stored type of the cyclic reduction's rows and records of the FP32-direction solves "
+                         "(32: float32 levels, float64 boundary solve and back substitution); default 64 for "
+                         "the batch workload, 32 for the precision workload")
     ap.add_argument("--refine", type=int, default=0,
                     help="batch: iterative refinement steps per Newton direction (float64 linear residual with the "
                          "stored Jacobians, correction from the same direction kernels)")
@@ -289,6 +290,8 @@ def main():
     if not os.path.exists(b.LIB_PATH):
         import __graft_entry__
         __graft_entry__.build()
+    if args.babd_precision is None:
+        args.babd_precision = 32 if args.workload == "precision" else 64
     if args.workload == "batch":
         line = batch_workload(args)
     elif args.workload == "precision":
@@ -698,6 +701,7 @@ def precision_workload(args):
     def one_solve(bits):
         S.set_direction_precision(bits)
         S.set_refinement(args.refine if bits == 32 else 0)
+        S.set_babd_precision(args.babd_precision if bits == 32 else 64)
         S.restart()
         its = []
         for cv in schedule:
@@ -761,9 +765,11 @@ def precision_workload(args):
         "dtype": "f64 (residuals, BABD, convergence) + f32 (Jacobians, condensation)",
         "data": "synthetic (seeded OCP family, generated in-process)",
         "config": {"workload": "precision form: tanh OCP n_x=64 n_u=32 (ny=128 nz=32 np=128), Nint=%d, m=%d, "
-                               "all-ones guess + homotopy S: 0->0.5; FP32 Newton direction (%d refinement steps) "
-                               "against the float64 one; callbacks in polynomial table form, large-block kernels"
-                               % (nint, m, args.refine),
+                               "all-ones guess + homotopy S: 0->0.5; FP32 Newton direction (float32 Jacobians and "
+                               "condensation, %s cyclic reduction, %d refinement steps) against the float64 one; "
+                               "callbacks in polynomial table form, large-block kernels"
+                               % (nint, m, "float32" if args.babd_precision == 32 else "float64", args.refine),
+                   "babd_precision": args.babd_precision, "refine": args.refine,
                    "nint": nint, "m": m, "ny": ny, "nz": nz, "np": np_, "tolerance": tol,
                    "fp32_converged_steps": conv32, "fp32_newton_iterations": f32["results"][-1][1],
                    "fp64_solves_per_s": sum(1 for r in forms[64]["results"] if r[0]) / forms[64]["elapsed"],

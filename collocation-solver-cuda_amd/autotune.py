@@ -15,14 +15,16 @@ chosen here by measurement on the handle's current mesh and state:
   substitution levels' span limit, the "QR tile shape" of the block
   reduction), among the candidates whose Newton direction agrees with the
   Householder-only one (the reference's orthogonal elimination) to
-  `direction_tol` relative.
+  `direction_tol` relative.  The directions are compared with float64
+  rows and records (a property of the elimination order, not of the
+  storage); the candidates are timed with the handle's own BABD precision.
 
 Each candidate is timed with the library's HIP-event profile
 (cs_profile_get) over `reps` launches of its stage; the launch shapes do
 not change any result (the GPU tests check this), the pairing rule changes
 the direction only at rounding level (checked here).  Choices are cached
-per (problem code, m, direction precision, batch size, total nodes, device
-compute units) in _tune/ beside this file, so a later run with the same
+per (problem code, m, direction precision, BABD precision, batch size,
+total nodes, device compute units) in _tune/ beside this file, so a later run with the same
 shape applies them without measuring.
 """
 import hashlib
@@ -42,7 +44,7 @@ def cache_key(S):
     cus = S.get_launch(native.CS_LAUNCH_COMPUTE_UNITS)
     h = hashlib.sha256()
     h.update(jit.cache_key(S.bvp.ocp_text, S.m, S.form).encode())
-    h.update(b"|%d|%d|%d|%d|%d" % (S.direction_precision, S.B, int(S.N), S.m, cus))
+    h.update(b"|%d|%d|%d|%d|%d|%d" % (S.direction_precision, S.babd_precision, S.B, int(S.N), S.m, cus))
     return h.hexdigest()[:24]
 
 
@@ -120,14 +122,19 @@ def tune(S, alpha=0.1, reps=3, grid_factors=(2, 3, 4, 8), residual_blocks=(64, 1
         timings["residual_block"] = {str(k): v for k, v in tr.items()}
     # pairing rule, among the directions that agree with Householder-only
     if spans:
+        bits = S.babd_precision
+        S.set_babd_precision(64)
         S.set_babd_substitution(0)
         ref = _direction(S, alpha)
-        tb, err = {}, {}
+        err = {}
         for sp in spans:
             S.set_babd_substitution(sp)
-            d = _direction(S, alpha)
-            err[sp] = float(_rel(d, ref))
+            err[sp] = float(_rel(_direction(S, alpha), ref))
+        S.set_babd_precision(bits)   # level 0 reads the condensation's rows, float64 for either
+        tb = {}
+        for sp in spans:
             if err[sp] <= direction_tol:
+                S.set_babd_substitution(sp)
                 tb[sp] = _timed(S, "babd", lambda: S.solve_babd(), reps)
                 say("pairing 1/%d: %.3f ms, direction %.2e" % (sp, tb[sp], err[sp]))
             else:

@@ -98,6 +98,7 @@ enum Fn {
   F_STATE_ADD, F_CR_PAIR_DF, F_CR_PAIR_FF, F_CR_SUBST_DF, F_CR_SUBST_FF, F_CR_FINAL_F, F_CR_BACK_FD, F_CR_BACK_FF,
   F_L_STAGE, F_L_STAGE32, F_L_SINIT, F_L_SINIT32, F_L_TERM, F_L_TERM32, F_L_SOUT, F_L_SOUT32, F_L_CR_SUBST,
   F_L_CR_PAIR, F_L_CR_FINAL, F_L_CR_BACK, F_COND_NS_Q, F_COND_NS32_Q, F_COND_MF32_Q, F_ROWS_C,
+  F_L_CR_SUBST_DF, F_L_CR_SUBST_FF, F_L_CR_PAIR_DF, F_L_CR_PAIR_FF, F_L_CR_FINAL_F, F_L_CR_BACK_FD, F_L_CR_BACK_FF,
   F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
@@ -113,13 +114,15 @@ const char* kFnNames[F_COUNT] = {
   "colloc_l_stage_elim", "colloc_l_stage_elim32", "colloc_l_series_init", "colloc_l_series_init32",
   "colloc_l_series_term", "colloc_l_series_term32", "colloc_l_series_out", "colloc_l_series_out32",
   "colloc_l_cr_subst", "colloc_l_cr_pair", "colloc_l_cr_final", "colloc_l_cr_back", "colloc_condense_ns_q",
-  "colloc_condense_ns32_q", "colloc_condense_mf32_q", "colloc_rows_constant"};
+  "colloc_condense_ns32_q", "colloc_condense_mf32_q", "colloc_rows_constant",
+  "colloc_l_cr_subst_df", "colloc_l_cr_subst_ff", "colloc_l_cr_pair_df", "colloc_l_cr_pair_ff",
+  "colloc_l_cr_final_f", "colloc_l_cr_back_fd", "colloc_l_cr_back_ff"};
 // the large-block kernels (ny > 64) take the place of the register-resident
 // condensation and cyclic reduction: a module has one set or the other
-bool large_kernel(int f) { return f >= F_L_STAGE && f <= F_L_CR_BACK; }
+bool large_kernel(int f) { return (f >= F_L_STAGE && f <= F_L_CR_BACK) || f >= F_L_CR_SUBST_DF; }
 bool small_kernel(int f) {
   return (f >= F_COND && f <= F_CR_BACK) || f == F_COND_MF32 || (f >= F_CR_PAIR_DF && f <= F_CR_BACK_FF) ||
-         f >= F_COND_NS_Q;
+         (f >= F_COND_NS_Q && f <= F_ROWS_C);
 }
 constexpr int kLargeMaxTerms = 48;        // L_MAX_TERMS of the large-block series
 constexpr int kLargeSlot = 10;            // L_SLOT: u64 per interval of its convergence slots
@@ -747,7 +750,7 @@ int condense_large(cs_handle* h, const int* mask, long long ncond) {
 // systems by Gauss-Jordan in device memory, back substitution per record.
 int babd_large(cs_handle* h, const int* mask) {
   ProfScope ps(h, CS_KERNEL_BABD);
-  if (h->babd_bits != 64) return fail(CS_E_UNSUPPORTED, "the large-block cyclic reduction is float64 only");
+  const bool f32 = h->babd_bits == 32;
   double* dy = h->dir + h->st_y;
   double* dp = h->dir + h->st_p;
   Batch bt = h->batch(mask);
@@ -773,24 +776,27 @@ int babd_large(cs_handle* h, const int* mask) {
     double* rout = h->rows0 + h->lev_rbase[l + 1] * rowsz;
     if (h->lev_subst[l]) {
       void* args[] = {&oinst, &seg, &act, &rin, &rout};
-      TRY(launch(h, F_L_CR_SUBST, (unsigned)h->lev_nout[l] * (unsigned)h->l_stiles, kLargeThreads, args));
+      const Fn fs = !f32 ? F_L_CR_SUBST : (l == 0 ? F_L_CR_SUBST_DF : F_L_CR_SUBST_FF);
+      TRY(launch(h, fs, (unsigned)h->lev_nout[l] * (unsigned)h->l_stiles, kLargeThreads, args));
     }
     if (h->lev_qr[l]) {
       void* args[] = {&oinst, &seg, &act, &rin, &rout, &h->crrec, &h->l_work};
-      TRY(launch(h, F_L_CR_PAIR, (unsigned)h->lev_nout[l], kLargeQrThreads, args));
+      const Fn fp = !f32 ? F_L_CR_PAIR : (l == 0 ? F_L_CR_PAIR_DF : F_L_CR_PAIR_FF);
+      TRY(launch(h, fp, (unsigned)h->lev_nout[l], kLargeQrThreads, args));
     }
   }
   {
     double* rin = h->rows0 + h->lev_rbase[nlev] * rowsz;
     void* args[] = {&bt, &rin, &h->Dr, &h->rbc, &dy, &dp, &h->status, &h->l_fin};
-    TRY(launch(h, F_L_CR_FINAL, (unsigned)h->B, kLargeFinThreads, args));
+    TRY(launch(h, f32 && nlev > 0 ? F_L_CR_FINAL_F : F_L_CR_FINAL, (unsigned)h->B, kLargeFinThreads, args));
   }
   for (int l = nlev - 1; l >= 0; --l) {
     const int* oinst = h->cr_oinst + h->lev_obase[l];
     const CRSeg* seg = h->cr_seg + h->lev_sbase[l];
     double* rin = h->rows0 + h->lev_rbase[l] * rowsz;
     void* args[] = {&l, &bt, &oinst, &seg, &h->crrec, &rin, &dy, &dp};
-    TRY(launch(h, F_L_CR_BACK, (unsigned)h->lev_nout[l], kLargeThreads, args));
+    const Fn fb = !f32 ? F_L_CR_BACK : (l == 0 ? F_L_CR_BACK_FD : F_L_CR_BACK_FF);
+    TRY(launch(h, fb, (unsigned)h->lev_nout[l], kLargeThreads, args));
   }
   return CS_OK;
 }
@@ -1856,7 +1862,6 @@ int cs_set_newton_graph(cs_handle* h, int on) {
 int cs_set_babd_precision(cs_handle* h, int bits) {
   if (!h) return fail(CS_E_ARG, "null handle");
   if (bits != 32 && bits != 64) return fail(CS_E_ARG, "BABD precision %d (32 or 64)", bits);
-  if (bits == 32 && h->large) return fail(CS_E_UNSUPPORTED, "the large-block cyclic reduction is float64 only");
   h->babd_bits = bits;
   return CS_OK;
 }

@@ -505,12 +505,15 @@ __device__ __forceinline__ void l_series_out(int nint, Batch bt, const double* _
 // ------------------------------------------------------ cyclic reduction
 // Substitution pairing (see cr_subst): out = [-G_b G_a | C_b | H_b - G_b H_a
 // | b_b - G_b b_a] for tile (rt, ct) of the pair's NY x (NY + NPH + 1)
-// product; workgroup = (output row of the level, tile).
+// product; workgroup = (output row of the level, tile).  TI / TO: stored
+// type of the input / output rows; the product is computed in TO (float32
+// matrix cores for the FP32 cyclic reduction).
+template <class TI, class TO>
 __device__ __forceinline__ void l_cr_subst(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
-                                           const int* __restrict__ act, const double* __restrict__ rows_in,
-                                           double* __restrict__ rows_out) {
-  __shared__ __attribute__((aligned(16))) double sA[L_SA];
-  __shared__ __attribute__((aligned(16))) double sB[L_SB];
+                                           const int* __restrict__ act, const TI* __restrict__ rows_in,
+                                           TO* __restrict__ rows_out) {
+  __shared__ __attribute__((aligned(16))) TO sA[L_SA];
+  __shared__ __attribute__((aligned(16))) TO sB[L_SB];
   const int unit = xcd_unit(blockIdx.x, gridDim.x);
   const int wg = unit / L_STILES, tile = unit - (unit / L_STILES) * L_STILES;
   const int ib = oinst[wg];
@@ -526,36 +529,36 @@ __device__ __forceinline__ void l_cr_subst(const int* __restrict__ oinst, const 
   const int rt = tile / L_SCT, ct = tile - (tile / L_SCT) * L_SCT;
   if (k == npairs) {   // carry the odd last row (rows of tile rt, columns of tile ct's share)
     if (ct != 0) return;
-    const double* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
-    double* dst = rows_out + (size_t)npairs * NY * CRW;
+    const TI* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
+    TO* dst = rows_out + (size_t)npairs * NY * CRW;
     for (int e = threadIdx.x; e < L_TILE * CRW; e += blockDim.x) {
       const int r = rt * L_TILE + e / CRW, c = e - (e / CRW) * CRW;
       if (r >= NY) break;
       const size_t x = (size_t)r * CRW + c;
-      if (c < NY || c == CRW - 1 || (!HZ && c >= 2 * NY)) dst[x] = src[x];
-      else if (wc && c < 2 * NY) dst[x] = (c - NY == r) ? 1.0 : 0.0;
+      if (c < NY || c == CRW - 1 || (!HZ && c >= 2 * NY)) dst[x] = (TO)src[x];
+      else if (wc && c < 2 * NY) dst[x] = (c - NY == r) ? (TO)1 : (TO)0;
     }
     return;
   }
-  const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
-  const double* rb = ra + NY * CRW;
-  double* out = rows_out + (size_t)k * NY * CRW;
+  const TI* ra = rows_in + (size_t)(2 * k) * NY * CRW;
+  const TI* rb = ra + NY * CRW;
+  TO* out = rows_out + (size_t)k * NY * CRW;
   const int rbase = rt * L_TILE, cbase = ct * L_TILE;
-  typename MfmaOp<double>::V acc[2][2];
+  typename MfmaOp<TO>::V acc[2][2];
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[ti][tj][v] = 0.0;
-  tile_gemm<double>(NY,
-      [&](int r, int q) { return rbase + r < NY ? rb[(size_t)(rbase + r) * CRW + q] : 0.0; },
+      for (int v = 0; v < 4; ++v) acc[ti][tj][v] = (TO)0;
+  tile_gemm<TO>(NY,
+      [&](int r, int q) { return rbase + r < NY ? (TO)rb[(size_t)(rbase + r) * CRW + q] : (TO)0; },
       [&](int q, int c) {
         const int col = cbase + c;
-        if (col < NY) return ra[(size_t)q * CRW + col];                 // G_a
-        if (col < NY + NPH) return ra[(size_t)q * CRW + NY + col];      // H_a (after C_a)
-        if (col == NY + NPH) return ra[(size_t)q * CRW + CRW - 1];      // b_a
-        return 0.0;
+        if (col < NY) return (TO)ra[(size_t)q * CRW + col];                 // G_a
+        if (col < NY + NPH) return (TO)ra[(size_t)q * CRW + NY + col];      // H_a (after C_a)
+        if (col == NY + NPH) return (TO)ra[(size_t)q * CRW + CRW - 1];      // b_a
+        return (TO)0;
       },
       sA, sB, acc);
   const int lane = lane_id(), w = wave_id();
@@ -565,18 +568,18 @@ __device__ __forceinline__ void l_cr_subst(const int* __restrict__ oinst, const 
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = rbase + 32 * (w >> 1) + 16 * ti + MfmaOp<double>::row(lane, v);
+        const int r = rbase + 32 * (w >> 1) + 16 * ti + MfmaOp<TO>::row(lane, v);
         const int col = cbase + 32 * (w & 1) + 16 * tj + (lane & 15);
         if (r >= NY) continue;
-        const double a = acc[ti][tj][v];
+        const TO a = acc[ti][tj][v];
         if (col < NY) out[(size_t)r * CRW + col] = -a;
-        else if (col < NY + NPH) out[(size_t)r * CRW + NY + col] = rb[(size_t)r * CRW + NY + col] - a;
-        else if (col == NY + NPH) out[(size_t)r * CRW + CRW - 1] = rb[(size_t)r * CRW + CRW - 1] - a;
+        else if (col < NY + NPH) out[(size_t)r * CRW + NY + col] = (TO)rb[(size_t)r * CRW + NY + col] - a;
+        else if (col == NY + NPH) out[(size_t)r * CRW + CRW - 1] = (TO)rb[(size_t)r * CRW + CRW - 1] - a;
       }
   if (wc && ct == 0) {
     for (int e = threadIdx.x; e < L_TILE * NY; e += blockDim.x) {
       const int r = rbase + e / NY, c = e - (e / NY) * NY;
-      if (r < NY) out[(size_t)r * CRW + NY + c] = (r == c) ? 1.0 : 0.0;
+      if (r < NY) out[(size_t)r * CRW + NY + c] = (r == c) ? (TO)1 : (TO)0;
     }
   }
 }
@@ -586,10 +589,12 @@ __device__ __forceinline__ void l_cr_subst(const int* __restrict__ oinst, const 
 // eps): the pair's 2NY x L_WC block [x_M | x_L | x_R | p | rhs] in work
 // (global memory), one workgroup of L_QR_THREADS; thread (column slot,
 // row quarter) applies each reflector to a column's quarter, the quarters'
-// dot products summed through LDS.
+// dot products summed through LDS.  TI / TO: stored type of the input rows
+// / of the output rows and records (the arithmetic is float64 either way).
+template <class TI, class TO>
 __device__ __forceinline__ void l_cr_pair_global(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
-                                          const int* __restrict__ act, const double* __restrict__ rows_in,
-                                          double* __restrict__ rows_out, double* __restrict__ recs,
+                                          const int* __restrict__ act, const TI* __restrict__ rows_in,
+                                          TO* __restrict__ rows_out, TO* __restrict__ recs,
                                           double* __restrict__ work) {
   constexpr int R2 = 2 * NY;
   constexpr int NQ = 4;                          // row quarters
@@ -611,24 +616,24 @@ __device__ __forceinline__ void l_cr_pair_global(const int* __restrict__ oinst, 
   const int npairs = nrows >> 1;
   const int tid = threadIdx.x;
   if (k == npairs) {
-    const double* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
-    double* dst = rows_out + (size_t)npairs * NY * CRW;
-    for (int e = tid; e < NY * CRW; e += blockDim.x) dst[e] = src[e];
+    const TI* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
+    TO* dst = rows_out + (size_t)npairs * NY * CRW;
+    for (int e = tid; e < NY * CRW; e += blockDim.x) dst[e] = (TO)src[e];
     return;
   }
   double* W = work + (size_t)blockIdx.x * R2 * L_WC;
-  const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
-  const double* rb = ra + NY * CRW;
+  const TI* ra = rows_in + (size_t)(2 * k) * NY * CRW;
+  const TI* rb = ra + NY * CRW;
   for (int e = tid; e < R2 * L_WC; e += blockDim.x) {
     const int r = e / L_WC, c = e - (e / L_WC) * L_WC;
     const bool top = r < NY;
-    const double* src = top ? ra + (size_t)r * CRW : rb + (size_t)(r - NY) * CRW;
+    const TI* src = top ? ra + (size_t)r * CRW : rb + (size_t)(r - NY) * CRW;
     const int pcol = c < 3 * NY + NPH ? c : RECW - 1;
     double v;
-    if (c < NY) v = top ? src[NY + c] : src[c];
-    else if (c < 2 * NY) v = top ? src[c - NY] : 0.0;
-    else if (c < 3 * NY) v = top ? 0.0 : src[c - NY];
-    else v = src[pcol - NY];
+    if (c < NY) v = top ? (double)src[NY + c] : (double)src[c];
+    else if (c < 2 * NY) v = top ? (double)src[c - NY] : 0.0;
+    else if (c < 3 * NY) v = top ? 0.0 : (double)src[c - NY];
+    else v = (double)src[pcol - NY];
     W[e] = v;
   }
   __syncthreads();
@@ -681,13 +686,13 @@ __device__ __forceinline__ void l_cr_pair_global(const int* __restrict__ oinst, 
     }
   }
   __syncthreads();
-  double* rec = recs + (size_t)k * NY * RECW;
-  double* out = rows_out + (size_t)k * NY * CRW;
+  TO* rec = recs + (size_t)k * NY * RECW;
+  TO* out = rows_out + (size_t)k * NY * CRW;
   for (int e = tid; e < R2 * L_WC; e += blockDim.x) {
     const int r = e / L_WC, c = e - (e / L_WC) * L_WC;
     const int pcol = c < 3 * NY + NPH ? c : RECW - 1;
-    if (r < NY) rec[(size_t)r * RECW + pcol] = W[e];
-    else if (c >= NY) out[(size_t)(r - NY) * CRW + pcol - NY] = W[e];
+    if (r < NY) rec[(size_t)r * RECW + pcol] = (TO)W[e];
+    else if (c >= NY) out[(size_t)(r - NY) * CRW + pcol - NY] = (TO)W[e];
   }
 }
 
@@ -704,10 +709,10 @@ __device__ __forceinline__ void l_cr_pair_global(const int* __restrict__ oinst, 
 constexpr int L_PW = 16;                          // panel width
 constexpr int L_PARTS = 8;                        // lanes per trailing column (one per 8-lane group of a wave)
 constexpr int L_RPL = (2 * NY + L_PARTS - 1) / L_PARTS;   // rows per lane of a trailing column
-template <int RPL>
+template <int RPL, class TI, class TO>
 __device__ __forceinline__ void l_cr_pair_panels(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
-                                                 const int* __restrict__ act, const double* __restrict__ rows_in,
-                                                 double* __restrict__ rows_out, double* __restrict__ recs,
+                                                 const int* __restrict__ act, const TI* __restrict__ rows_in,
+                                                 TO* __restrict__ rows_out, TO* __restrict__ recs,
                                                  double* __restrict__ work) {
   constexpr int R2 = 2 * NY;
   constexpr int NW = L_QR_THREADS / WAVE;
@@ -729,24 +734,24 @@ __device__ __forceinline__ void l_cr_pair_panels(const int* __restrict__ oinst, 
   const int npairs = nrows >> 1;
   const int tid = threadIdx.x, lane = lane_id(), w = wave_id();
   if (k == npairs) {
-    const double* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
-    double* dst = rows_out + (size_t)npairs * NY * CRW;
-    for (int e = tid; e < NY * CRW; e += blockDim.x) dst[e] = src[e];
+    const TI* src = rows_in + (size_t)(nrows - 1) * NY * CRW;
+    TO* dst = rows_out + (size_t)npairs * NY * CRW;
+    for (int e = tid; e < NY * CRW; e += blockDim.x) dst[e] = (TO)src[e];
     return;
   }
   double* W = work + (size_t)blockIdx.x * R2 * L_WC;
-  const double* ra = rows_in + (size_t)(2 * k) * NY * CRW;
-  const double* rb = ra + NY * CRW;
+  const TI* ra = rows_in + (size_t)(2 * k) * NY * CRW;
+  const TI* rb = ra + NY * CRW;
   for (int e = tid; e < R2 * L_WC; e += blockDim.x) {
     const int r = e / L_WC, c = e - (e / L_WC) * L_WC;
     const bool top = r < NY;
-    const double* src = top ? ra + (size_t)r * CRW : rb + (size_t)(r - NY) * CRW;
+    const TI* src = top ? ra + (size_t)r * CRW : rb + (size_t)(r - NY) * CRW;
     const int pcol = c < 3 * NY + NPH ? c : RECW - 1;
     double v;
-    if (c < NY) v = top ? src[NY + c] : src[c];
-    else if (c < 2 * NY) v = top ? src[c - NY] : 0.0;
-    else if (c < 3 * NY) v = top ? 0.0 : src[c - NY];
-    else v = src[pcol - NY];
+    if (c < NY) v = top ? (double)src[NY + c] : (double)src[c];
+    else if (c < 2 * NY) v = top ? (double)src[c - NY] : 0.0;
+    else if (c < 3 * NY) v = top ? 0.0 : (double)src[c - NY];
+    else v = (double)src[pcol - NY];
     W[e] = v;
   }
   __syncthreads();
@@ -858,22 +863,23 @@ __device__ __forceinline__ void l_cr_pair_panels(const int* __restrict__ oinst, 
     __syncthreads();
     if (stop) break;
   }
-  double* rec = recs + (size_t)k * NY * RECW;
-  double* out = rows_out + (size_t)k * NY * CRW;
+  TO* rec = recs + (size_t)k * NY * RECW;
+  TO* out = rows_out + (size_t)k * NY * CRW;
   for (int e = tid; e < R2 * L_WC; e += blockDim.x) {
     const int r = e / L_WC, c = e - (e / L_WC) * L_WC;
     const int pcol = c < 3 * NY + NPH ? c : RECW - 1;
-    if (r < NY) rec[(size_t)r * RECW + pcol] = W[e];
-    else if (c >= NY) out[(size_t)(r - NY) * CRW + pcol - NY] = W[e];
+    if (r < NY) rec[(size_t)r * RECW + pcol] = (TO)W[e];
+    else if (c >= NY) out[(size_t)(r - NY) * CRW + pcol - NY] = (TO)W[e];
   }
 }
 
+template <class TI, class TO>
 __device__ __forceinline__ void l_cr_pair(const int* __restrict__ oinst, const CRSeg* __restrict__ seg,
-                                          const int* __restrict__ act, const double* __restrict__ rows_in,
-                                          double* __restrict__ rows_out, double* __restrict__ recs,
+                                          const int* __restrict__ act, const TI* __restrict__ rows_in,
+                                          TO* __restrict__ rows_out, TO* __restrict__ recs,
                                           double* __restrict__ work) {
-  if constexpr (L_RPL <= 48) l_cr_pair_panels<L_RPL>(oinst, seg, act, rows_in, rows_out, recs, work);
-  else l_cr_pair_global(oinst, seg, act, rows_in, rows_out, recs, work);
+  if constexpr (L_RPL <= 48) l_cr_pair_panels<L_RPL, TI, TO>(oinst, seg, act, rows_in, rows_out, recs, work);
+  else l_cr_pair_global<TI, TO>(oinst, seg, act, rows_in, rows_out, recs, work);
 }
 
 // Boundary system (see cr_final): [C G H | b] of the last row and
@@ -883,10 +889,12 @@ __device__ __forceinline__ void l_cr_pair(const int* __restrict__ oinst, const C
 // first on ties; the row exchanges of a panel applied to the columns after
 // it, so the right-hand side column receives L^-1 with the trailing
 // updates), then blocked back substitution.  One workgroup of
-// L_FIN_THREADS per instance; status bit 4 on a zero pivot.
+// L_FIN_THREADS per instance; status bit 4 on a zero pivot.  TI: stored type
+// of the last level's rows.
 constexpr int L_PB = 16;                              // panel width
 constexpr int L_FC = NFIN + 1;                        // columns of the augmented system
-__device__ __forceinline__ void l_cr_final(Batch bt, const double* __restrict__ rows, const double* __restrict__ Dr,
+template <class TI>
+__device__ __forceinline__ void l_cr_final(Batch bt, const TI* __restrict__ rows, const double* __restrict__ Dr,
                                            const double* __restrict__ rbc, double* __restrict__ dy,
                                            double* __restrict__ dp, int* __restrict__ status,
                                            double* __restrict__ work) {
@@ -900,7 +908,7 @@ __device__ __forceinline__ void l_cr_final(Batch bt, const double* __restrict__ 
   __shared__ int piv[L_PB];
   const int ib = blockIdx.x;
   if (skipped(bt, ib)) return;
-  const double* row = rows + (size_t)ib * NY * CRW;
+  const TI* row = rows + (size_t)ib * NY * CRW;
   Dr += (size_t)ib * DRSZ;
   rbc += (size_t)ib * NRB;
   double* F = work + (size_t)ib * NFIN * FC;
@@ -909,10 +917,10 @@ __device__ __forceinline__ void l_cr_final(Batch bt, const double* __restrict__ 
     const int r = e / FC, cc = e - (e / FC) * FC;
     double v = 0.0;
     if (r < NY) {
-      if (cc < NY) v = row[r * CRW + NY + cc];
-      else if (cc < 2 * NY) v = row[r * CRW + cc - NY];
-      else if (cc < NFIN) v = HZ ? 0.0 : row[r * CRW + cc];
-      else v = row[r * CRW + CRW - 1];
+      if (cc < NY) v = (double)row[r * CRW + NY + cc];
+      else if (cc < 2 * NY) v = (double)row[r * CRW + cc - NY];
+      else if (cc < NFIN) v = HZ ? 0.0 : (double)row[r * CRW + cc];
+      else v = (double)row[r * CRW + CRW - 1];
     } else {
       const int rr = r - NY;
       if (cc < NY) v = Dr[rr * DRW + NY + cc];
@@ -1072,10 +1080,12 @@ __device__ __forceinline__ void l_cr_final(Batch bt, const double* __restrict__ 
 
 // Back substitution of one level (see cr_back), one workgroup per output
 // row: the right-hand side per row, then R's triangular solve column by
-// column in LDS.
+// column in LDS.  TR / TI: stored type of the records / of the level's input
+// rows (float64 arithmetic).
+template <class TR, class TI>
 __device__ __forceinline__ void l_cr_back(int level, Batch bt, const int* __restrict__ oinst,
-                                          const CRSeg* __restrict__ seg, const double* __restrict__ recs,
-                                          const double* __restrict__ rows_in, double* __restrict__ dy,
+                                          const CRSeg* __restrict__ seg, const TR* __restrict__ recs,
+                                          const TI* __restrict__ rows_in, double* __restrict__ dy,
                                           const double* __restrict__ dp) {
   __shared__ double rhs[NY];
   __shared__ double xs;
@@ -1094,30 +1104,30 @@ __device__ __forceinline__ void l_cr_back(int level, Batch bt, const int* __rest
   const double* xr = dy + (size_t)right * NY;
   dp += (size_t)ib * NP;
   if (sg.subst) {
-    const double* ra = rows_in + (size_t)(sg.in_off + 2 * k) * NY * CRW;
+    const TI* ra = rows_in + (size_t)(sg.in_off + 2 * k) * NY * CRW;
     for (int r = tid; r < NY; r += blockDim.x) {
-      double v = ra[(size_t)r * CRW + CRW - 1];
-      for (int c = 0; c < NY; ++c) v = fma(-ra[(size_t)r * CRW + c], xl[c], v);
-      for (int c = 0; c < NPH; ++c) v = fma(-ra[(size_t)r * CRW + 2 * NY + c], dp[c], v);
+      double v = (double)ra[(size_t)r * CRW + CRW - 1];
+      for (int c = 0; c < NY; ++c) v = fma(-(double)ra[(size_t)r * CRW + c], xl[c], v);
+      for (int c = 0; c < NPH; ++c) v = fma(-(double)ra[(size_t)r * CRW + 2 * NY + c], dp[c], v);
       dy[(size_t)mid * NY + r] = v;
     }
     return;
   }
-  const double* rec = recs + (size_t)(sg.rec_off + k) * NY * RECW;
+  const TR* rec = recs + (size_t)(sg.rec_off + k) * NY * RECW;
   for (int r = tid; r < NY; r += blockDim.x) {
-    const double* rw = rec + (size_t)r * RECW;
-    double v = rw[RECW - 1];
-    for (int c = 0; c < NY; ++c) v = fma(-rw[NY + c], xl[c], v);
-    for (int c = 0; c < NY; ++c) v = fma(-rw[2 * NY + c], xr[c], v);
-    for (int c = 0; c < NPH; ++c) v = fma(-rw[3 * NY + c], dp[c], v);
+    const TR* rw = rec + (size_t)r * RECW;
+    double v = (double)rw[RECW - 1];
+    for (int c = 0; c < NY; ++c) v = fma(-(double)rw[NY + c], xl[c], v);
+    for (int c = 0; c < NY; ++c) v = fma(-(double)rw[2 * NY + c], xr[c], v);
+    for (int c = 0; c < NPH; ++c) v = fma(-(double)rw[3 * NY + c], dp[c], v);
     rhs[r] = v;
   }
   __syncthreads();
   for (int c = NY - 1; c >= 0; --c) {
-    if (tid == 0) xs = rhs[c] / rec[(size_t)c * RECW + c];
+    if (tid == 0) xs = rhs[c] / (double)rec[(size_t)c * RECW + c];
     __syncthreads();
     const double x = xs;
-    for (int r = tid; r < c; r += blockDim.x) rhs[r] = fma(-rec[(size_t)r * RECW + c], x, rhs[r]);
+    for (int r = tid; r < c; r += blockDim.x) rhs[r] = fma(-(double)rec[(size_t)r * RECW + c], x, rhs[r]);
     if (tid == 0) rhs[c] = x;
     __syncthreads();
   }
@@ -1169,19 +1179,51 @@ __global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_series_out32(int n
 }
 __global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_cr_subst(const int* oinst, const colloc::CRSeg* seg,
     const int* act, const double* rows_in, double* rows_out) {
-  colloc::l_cr_subst(oinst, seg, act, rows_in, rows_out);
+  colloc::l_cr_subst<double, double>(oinst, seg, act, rows_in, rows_out);
 }
 __global__ __launch_bounds__(colloc::L_QR_THREADS) void colloc_l_cr_pair(const int* oinst, const colloc::CRSeg* seg,
     const int* act, const double* rows_in, double* rows_out, double* recs, double* work) {
-  colloc::l_cr_pair(oinst, seg, act, rows_in, rows_out, recs, work);
+  colloc::l_cr_pair<double, double>(oinst, seg, act, rows_in, rows_out, recs, work);
 }
 __global__ __launch_bounds__(colloc::L_FIN_THREADS) void colloc_l_cr_final(colloc::Batch bt, const double* rows,
     const double* Dr, const double* rbc, double* dy, double* dp, int* status, double* work) {
-  colloc::l_cr_final(bt, rows, Dr, rbc, dy, dp, status, work);
+  colloc::l_cr_final<double>(bt, rows, Dr, rbc, dy, dp, status, work);
 }
 __global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_cr_back(int level, colloc::Batch bt, const int* oinst,
     const colloc::CRSeg* seg, const double* recs, const double* rows_in, double* dy, const double* dp) {
-  colloc::l_cr_back(level, bt, oinst, seg, recs, rows_in, dy, dp);
+  colloc::l_cr_back<double, double>(level, bt, oinst, seg, recs, rows_in, dy, dp);
+}
+// the FP32 cyclic reduction: level 0 reads the condensation's float64 rows
+// (_df), the levels after it float32 rows (_ff); records are float32
+__global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_cr_subst_df(const int* oinst,
+    const colloc::CRSeg* seg, const int* act, const double* rows_in, float* rows_out) {
+  colloc::l_cr_subst<double, float>(oinst, seg, act, rows_in, rows_out);
+}
+__global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_cr_subst_ff(const int* oinst,
+    const colloc::CRSeg* seg, const int* act, const float* rows_in, float* rows_out) {
+  colloc::l_cr_subst<float, float>(oinst, seg, act, rows_in, rows_out);
+}
+__global__ __launch_bounds__(colloc::L_QR_THREADS) void colloc_l_cr_pair_df(const int* oinst,
+    const colloc::CRSeg* seg, const int* act, const double* rows_in, float* rows_out, float* recs, double* work) {
+  colloc::l_cr_pair<double, float>(oinst, seg, act, rows_in, rows_out, recs, work);
+}
+__global__ __launch_bounds__(colloc::L_QR_THREADS) void colloc_l_cr_pair_ff(const int* oinst,
+    const colloc::CRSeg* seg, const int* act, const float* rows_in, float* rows_out, float* recs, double* work) {
+  colloc::l_cr_pair<float, float>(oinst, seg, act, rows_in, rows_out, recs, work);
+}
+__global__ __launch_bounds__(colloc::L_FIN_THREADS) void colloc_l_cr_final_f(colloc::Batch bt, const float* rows,
+    const double* Dr, const double* rbc, double* dy, double* dp, int* status, double* work) {
+  colloc::l_cr_final<float>(bt, rows, Dr, rbc, dy, dp, status, work);
+}
+__global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_cr_back_fd(int level, colloc::Batch bt,
+    const int* oinst, const colloc::CRSeg* seg, const float* recs, const double* rows_in, double* dy,
+    const double* dp) {
+  colloc::l_cr_back<float, double>(level, bt, oinst, seg, recs, rows_in, dy, dp);
+}
+__global__ __launch_bounds__(colloc::L_THREADS) void colloc_l_cr_back_ff(int level, colloc::Batch bt,
+    const int* oinst, const colloc::CRSeg* seg, const float* recs, const float* rows_in, double* dy,
+    const double* dp) {
+  colloc::l_cr_back<float, float>(level, bt, oinst, seg, recs, rows_in, dy, dp);
 }
 // [0..3] as the register-resident kernels' (none here), [4] table-form
 // points per workgroup, [5] large-block kernels, [6] tiles per interval of

@@ -109,6 +109,7 @@ class CollocationSolver:
         self.m = m
         self.form = form   # device code of the callbacks: "auto", "code" or "tables" (jit.choose_form)
         self.direction_precision = 64
+        self.babd_precision = 64
         self.ny, self.nz, self.np = bvp.size_y, bvp.size_z, bvp.size_p
         self.n_max = int(n_max or max(bvp.maximum_nodes, bvp.N))
         self.batch = int(batch)
@@ -299,6 +300,7 @@ class CollocationSolver:
         (default) or 32 (float32 levels, float64 boundary solve and back
         substitution)."""
         check(lib().cs_set_babd_precision(self.h, int(bits)))
+        self.babd_precision = int(bits)
 
     def set_refinement(self, steps):
         """Iterative refinement steps per Newton direction (0: none): the

@@ -207,9 +207,10 @@ int cs_set_refinement(cs_handle* h, int steps);
  * stores float32 rows and records and the substitution pairing computes in
  * float32; the Householder pairing, the boundary-block solve (the reduced
  * system of qr_decomposition, :338-440) and the back substitution compute
- * in float64.  Part of the FP32 direction (with cs_set_direction_precision
- * and cs_set_refinement); 32 is CS_E_UNSUPPORTED for a module of large-block
- * kernels (ny > 64).  No counterpart in the reference. */
+ * in float64 (with large-block kernels, ny > 64, the substitution products
+ * run on float32 matrix cores).  Part of the FP32 direction (with
+ * cs_set_direction_precision and cs_set_refinement).  No counterpart in the
+ * reference. */
 int cs_set_babd_precision(cs_handle* h, int bits);
 
 /* max |rho| per instance (B doubles) of rho = F + J delta, the linearised

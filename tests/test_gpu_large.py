@@ -15,7 +15,14 @@ cyclic reduction with work matrices in device memory), against the oracle.
   residual of the GPU solution below the tolerance;
 * the full configuration (Nint = 4096): float64 and FP32 directions both
   converge; the oracle's residual of the float64 solution is below the
-  tolerance, and the FP32-direction solution agrees with it to 1e-7.
+  tolerance, and the FP32-direction solution agrees with it to 1e-7;
+* the FP32 cyclic reduction (float32 rows and records after level 0, the
+  substitution products on float32 matrix cores): with float64 Jacobians the
+  direction within 2e-4 of the oracle's, one refinement step within 1e-7;
+  with the FP32 direction as well within 2e-4, refined within 1e-6;
+  Householder-only and substitution pairings; and the full configuration
+  solved with the FP32 direction, FP32 cyclic reduction and one refinement
+  step to the float64 solution within 1e-7.
 """
 import numpy as np
 import pytest
@@ -175,5 +182,62 @@ def test_large_full_size_fp64_and_fp32():
     say("FP32: %s" % [h["iterations"] for h in r32["history"]])
     S.close()
     assert r32["success"] == 1
+    for k in ("y", "z", "p"):
+        assert rel_err(r32[k], r64[k]) < 1e-7, k
+
+
+@pytest.mark.parametrize("spans", [0, 2])
+def test_large_fp32_cyclic_reduction(large, spans):
+    inst, m, bvp, S, prob, ref = large
+    S.set_babd_substitution(spans)
+    S.set_mesh(ref["t"])
+    S.set_state(ref["y"], ref["yd"], ref["zt"], ref["p"])
+
+    def direction_err():
+        dy, dyd, dz, dp = S.get_direction()
+        return max(rel_err(dy, ref["dy"]), rel_err(dp, ref["dp"]), rel_err(dyd, ref["dyd"]), rel_err(dz, ref["dz"]))
+    try:
+        S.set_babd_precision(32)
+        for bits, bound, bound1 in ((64, 2e-4, 1e-7), (32, 2e-4, 1e-6)):
+            S.set_direction_precision(bits)
+            S.set_refinement(0)
+            say("n_x=64 FP32 cyclic reduction, spans %d, direction fp%d" % (spans, bits))
+            S.compute_f_q(0.1, outputs=False)
+            S.construct_jacobian(0.1, outputs=False)
+            S.reduce_jacobian(outputs=False)
+            S.solve_babd()
+            S.recover_delta_k()
+            e0 = direction_err()
+            S.refine_direction(1)
+            e1 = direction_err()
+            say("direction error %.3g, refined once %.3g" % (e0, e1))
+            assert e0 < bound and e1 < bound1 and e1 < e0
+        assert S.status() == 0
+    finally:
+        S.set_babd_precision(64)
+        S.set_direction_precision(64)
+        S.set_refinement(0)
+
+
+def test_large_full_size_fp32_cyclic_reduction():
+    inst, m = problems.config(2)
+    bvp = solver.BvpDae(inst.ocp_text())
+    hom = inst.homotopy_schedule()
+    S = solver.CollocationSolver(bvp, m, n_max=bvp.N)
+    S.set_babd_substitution(2)
+    say("n_x=64 full size, float64")
+    r64 = solver.collocation_solver_parallel(bvp, m, solver=S, homotopy=hom)
+    S.set_direction_precision(32)
+    S.set_babd_precision(32)
+    S.set_refinement(1)
+    say("FP32 direction, FP32 cyclic reduction, one refinement step")
+    r32 = solver.collocation_solver_parallel(bvp, m, solver=S, homotopy=hom)
+    it64 = [h["iterations"] for h in r64["history"]]
+    it32 = [h["iterations"] for h in r32["history"]]
+    say("iterations float64 %s, FP32 %s" % (it64, it32))
+    assert S.status() == 0
+    S.close()
+    assert r64["success"] == 1 and r32["success"] == 1
+    assert len(it32) == len(it64) and all(b <= a + 2 for a, b in zip(it64, it32))
     for k in ("y", "z", "p"):
         assert rel_err(r32[k], r64[k]) < 1e-7, k
