@@ -99,6 +99,7 @@ enum Fn {
   F_L_STAGE, F_L_STAGE32, F_L_SINIT, F_L_SINIT32, F_L_TERM, F_L_TERM32, F_L_SOUT, F_L_SOUT32, F_L_CR_SUBST,
   F_L_CR_PAIR, F_L_CR_FINAL, F_L_CR_BACK, F_COND_NS_Q, F_COND_NS32_Q, F_COND_MF32_Q, F_ROWS_C,
   F_L_CR_SUBST_DF, F_L_CR_SUBST_FF, F_L_CR_PAIR_DF, F_L_CR_PAIR_FF, F_L_CR_FINAL_F, F_L_CR_BACK_FD, F_L_CR_BACK_FF,
+  F_JAC32_GEMM,
   F_COUNT
 };
 const char* kFnNames[F_COUNT] = {
@@ -116,10 +117,13 @@ const char* kFnNames[F_COUNT] = {
   "colloc_l_cr_subst", "colloc_l_cr_pair", "colloc_l_cr_final", "colloc_l_cr_back", "colloc_condense_ns_q",
   "colloc_condense_ns32_q", "colloc_condense_mf32_q", "colloc_rows_constant",
   "colloc_l_cr_subst_df", "colloc_l_cr_subst_ff", "colloc_l_cr_pair_df", "colloc_l_cr_pair_ff",
-  "colloc_l_cr_final_f", "colloc_l_cr_back_fd", "colloc_l_cr_back_ff"};
+  "colloc_l_cr_final_f", "colloc_l_cr_back_fd", "colloc_l_cr_back_ff", "colloc_jacobian32_gemm"};
 // the large-block kernels (ny > 64) take the place of the register-resident
 // condensation and cyclic reduction: a module has one set or the other
-bool large_kernel(int f) { return (f >= F_L_STAGE && f <= F_L_CR_BACK) || f >= F_L_CR_SUBST_DF; }
+bool large_kernel(int f) { return (f >= F_L_STAGE && f <= F_L_CR_BACK) || (f >= F_L_CR_SUBST_DF && f <= F_L_CR_BACK_FF); }
+// kernels a module may have or not: the matrix-core float32 Jacobian of a
+// straight-line unit whose problem also has the polynomial form
+bool optional_kernel(int f) { return f == F_JAC32_GEMM; }
 bool small_kernel(int f) {
   return (f >= F_COND && f <= F_CR_BACK) || f == F_COND_MF32 || (f >= F_CR_PAIR_DF && f <= F_CR_BACK_FF) ||
          (f >= F_COND_NS_Q && f <= F_ROWS_C);
@@ -197,6 +201,8 @@ struct cs_handle {
   double* rows0 = nullptr;   // BABD rows of every cyclic-reduction level, level after level
   size_t rows_cap = 0;       // rows0's capacity in rows
   bool rows_c_ok = false;    // level-0 rows hold C = I (and zero inactive H columns) for this mesh
+  int jac_gemm = 0;          // stages per workgroup of the module's matrix-core float32 Jacobian (0: none)
+  int jac_gemm_on = 1;       // cs_set_launch CS_LAUNCH_JACOBIAN_GEMM
   double* rec = nullptr;
   double* crrec = nullptr;
   double* segres = nullptr;
@@ -477,6 +483,12 @@ int jacobian(cs_handle* h, double alpha, const int* mask) {
   // the FP32 direction stores the stage Jacobians in float32 (in the same buffer)
   h->jac_bits = h->dir_bits;
   TRY(jacobian_static(h));
+  if (h->jac_bits == 32 && h->jac_gemm > 0 && h->jac_gemm_on) {
+    // matrix-core form: a workgroup per jac_gemm stages, then Dr's threads
+    const long long nst = (long long)h->launch_intervals() * h->m;
+    const unsigned grid = (unsigned)((nst + h->jac_gemm - 1) / h->jac_gemm + (h->B + 255) / 256);
+    return launch(h, F_JAC32_GEMM, grid, 256, args);
+  }
   const unsigned grid = h->tab_g ? point_groups(h, h->launch_intervals())
                                  : (nthreads + 255) / 256 * (unsigned)h->jac_parts;
   return launch(h, h->jac_bits == 32 ? F_JAC32 : F_JAC, grid, 256, args);
@@ -1090,6 +1102,13 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
   for (int f = 0; f < F_COUNT; ++f) {
     h->fn[f] = nullptr;
     if ((mod_large && small_kernel(f)) || (!mod_large && large_kernel(f))) continue;
+    if (optional_kernel(f)) {
+      if (hipModuleGetFunction(&h->fn[f], h->module, kFnNames[f]) != hipSuccess) {
+        h->fn[f] = nullptr;
+        (void)hipGetLastError();   // absent: not an error (see the probe above)
+      }
+      continue;
+    }
     hipError_t e = hipModuleGetFunction(&h->fn[f], h->module, kFnNames[f]);
     if (e != hipSuccess) return fail(CS_E_HIP, "kernel %s missing: %s", kFnNames[f], hipGetErrorString(e));
   }
@@ -1104,6 +1123,11 @@ int cs_load_module(cs_handle* h, const void* code, size_t size) {
                 meta[2], meta[3], meta[4], h->ny, h->nz, h->np, h->m, h->nc);
   h->cw = meta[5];
   h->fw = meta[6];
+  h->jac_gemm = 0;
+  if (h->fn[F_JAC32_GEMM]) {   // its stages per workgroup
+    HIPCHK(hipModuleGetGlobal(&gp, &gsz, h->module, "colloc_jac_gemm"));
+    HIPCHK(hipMemcpy(&h->jac_gemm, (void*)gp, sizeof(int), hipMemcpyDeviceToHost));
+  }
   HIPCHK(hipModuleGetGlobal(&gp, &gsz, h->module, "colloc_caps"));
   int caps[8] = {};
   if (gsz < sizeof caps) return fail(CS_E_SHAPE, "module capabilities hold %zu bytes, expected %zu", gsz, sizeof caps);
@@ -1918,6 +1942,10 @@ int cs_set_launch(cs_handle* h, int which, int value) {
         return fail(CS_E_ARG, "residual workgroup %d (64, 128 or 256)", value);
       h->res_block = value;
       return CS_OK;
+    case CS_LAUNCH_JACOBIAN_GEMM:
+      if (value != 0 && value != 1) return fail(CS_E_ARG, "matrix-core Jacobian switch %d (0 or 1)", value);
+      h->jac_gemm_on = value;
+      return CS_OK;
     default:
       return fail(CS_E_ARG, "launch setting %d", which);
   }
@@ -1928,6 +1956,7 @@ int cs_get_launch(cs_handle* h, int which, int* value) {
   switch (which) {
     case CS_LAUNCH_CONDENSE_GRID: *value = h->cond_grid; return CS_OK;
     case CS_LAUNCH_RESIDUAL_BLOCK: *value = h->res_block; return CS_OK;
+    case CS_LAUNCH_JACOBIAN_GEMM: *value = (h->jac_gemm > 0 && h->jac_gemm_on) ? 1 : 0; return CS_OK;
     case CS_LAUNCH_COMPUTE_UNITS: {
       int dev = 0;
       HIPCHK(hipGetDevice(&dev));

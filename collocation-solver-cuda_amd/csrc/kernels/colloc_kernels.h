@@ -2376,7 +2376,7 @@ __device__ __forceinline__ void queue_all(int item, const Batch& bt, int* fallba
 
 }  // namespace colloc
 
-#if COLLOC_TABLE_FORM
+#if COLLOC_TABLE_FORM || COLLOC_JAC_GEMM
 #include "colloc_tables.h"
 #endif
 #if COLLOC_LARGE
@@ -2438,6 +2438,15 @@ __global__ __launch_bounds__(256) void colloc_jacobian32(int nint, colloc::Batch
     const double* ytil, const double* ztil, const double* p, const double* c, double alpha, float* jac, double* Dr) {
   colloc::COLLOC_CB(jacobian)<float>(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
 }
+#if COLLOC_JAC_GEMM
+// the same on matrix cores (colloc_tables.h jacobian_gemm); colloc_jac_gemm
+// holds its stages per workgroup for the host's grid
+__global__ __launch_bounds__(256) void colloc_jacobian32_gemm(int nint, colloc::Batch bt, const double* y,
+    const double* ytil, const double* ztil, const double* p, const double* c, double alpha, float* jac, double* Dr) {
+  colloc::jacobian_gemm(nint, bt, y, ytil, ztil, p, c, alpha, jac, Dr);
+}
+__device__ int colloc_jac_gemm = prob::JG_POINTS;
+#endif
 __global__ __launch_bounds__(256) void colloc_jacobian_static(long long nst, double* jac) {
   colloc::jacobian_static<double>(nst, jac);
 }

@@ -3,7 +3,8 @@
 // prob::*_prog() at TAB_G points per workgroup, and the kernels that call
 // the callbacks (residual, stage Jacobians, initial input, segment
 // residual), with the same arguments and results as their straight-line
-// counterparts in colloc_kernels.h.
+// counterparts in colloc_kernels.h.  A straight-line unit uses the
+// evaluator for its matrix-core float32 Jacobian (COLLOC_JAC_GEMM).
 //
 // A workgroup holds, in LDS, the atoms of its points as ext[atom][G] (the
 // caller fills the nb base atoms: y | z | p | instance constants | alpha;
@@ -122,6 +123,96 @@ __device__ __forceinline__ void tab_base_rest(T* ext, int g, const double* z, co
   }
 }
 
+#if COLLOC_JAC_GEMM
+// ------------------------------------------- float32 Jacobian on matrix cores
+// For a straight-line unit whose problem also has the polynomial form: the
+// state-dependent entries of a stage block are C mv, C the constant matrix
+// of the jac program's coefficients over its monomials (prob::jg_coef, laid
+// out as v_mfma_f32_16x16x4f32 A operands: row tile, k-step, lane -> row
+// 16 rt + l % 16, column 4 ks + l / 16) and mv the monomials at the stage
+// (tab_prepare, in LDS).  Workgroup k < ceil(Nst / JG_POINTS) takes stages
+// JG_POINTS k .. + JG_POINTS - 1 (every monomial of every stage as B
+// operands in registers) and its waves take the row tiles of C in turn,
+// each row tile's accumulators stored to the rows' destinations
+// (prob::jg_dest) of every stage; the workgroups after them write Dr
+// (straight-line prob::Dr, one thread per instance).  The static entries
+// are the straight-line unit's (jacobian_static).
+constexpr int JG_ST = prob::JG_POINTS / 16;     // 16-stage tiles per workgroup
+static_assert(prob::JG_POINTS % 16 == 0, "stage tiles of 16");
+__device__ __forceinline__ void jacobian_gemm(int nint, Batch bt, const double* __restrict__ y,
+                                              const double* __restrict__ ytil, const double* __restrict__ ztil,
+                                              const double* __restrict__ p, const double* __restrict__ c,
+                                              double alpha, float* __restrict__ jac, double* __restrict__ Dr) {
+  typedef float jg_f4 __attribute__((ext_vector_type(4)));
+  constexpr int G = prob::JG_POINTS;
+  __shared__ __attribute__((aligned(16))) float smem[(prob::JAC_NATOMS + prob::JAC_NMONO) * G];
+  __shared__ long long sst[G];
+  const int nst = launch_intervals(bt, nint) * M;
+  const int nwg = (nst + G - 1) / G;
+  if ((int)blockIdx.x >= nwg) {
+    const int ib = ((int)blockIdx.x - nwg) * (int)blockDim.x + (int)threadIdx.x;
+    if (ib < bt.B && !skipped(bt, ib))
+      prob::Dr(y + (size_t)bt.noff[ib] * NY, y + (size_t)(bt.noff[ib + 1] - 1) * NY, p + (size_t)ib * NP,
+               c + (size_t)ib * NC, Dr + (size_t)ib * DRSZ);
+    return;
+  }
+  float* ext = smem;
+  float* mv = smem + (size_t)prob::JAC_NATOMS * G;
+  const int g = threadIdx.x % G;
+  int st = 0, b = -1;
+  const int gid = blockIdx.x * G + g;
+  if (gid < nst) {
+    const int i = listed_interval(bt, gid / M);
+    st = i * M + (gid - (gid / M) * M);
+    b = bt.iinst[i];
+    if (skipped(bt, b)) b = -1;
+  }
+  const bool valid = b >= 0;
+  for (int a = threadIdx.x / G; a < NY; a += blockDim.x / G)
+    ext[(size_t)a * G + g] = valid ? (float)ytil[(size_t)st * NY + a] : 0.0f;
+  tab_base_rest<float, G>(ext, g, ztil + (size_t)(valid ? st : 0) * NZ, p + (size_t)(valid ? b : 0) * NP,
+                          c + (size_t)(valid ? b : 0) * NC, alpha, valid);
+  if (threadIdx.x < G) sst[threadIdx.x] = valid ? (long long)st : -1;
+  __syncthreads();
+  tab_prepare<float, G>(prob::jac_prog(), ext, mv);   // ends with the monomials in LDS, synchronised
+  const int lane = lane_id(), w = wave_id(), nw = blockDim.x / WAVE;
+  float bm[prob::JG_KS][JG_ST];
+#pragma unroll
+  for (int ks = 0; ks < prob::JG_KS; ++ks)
+#pragma unroll
+    for (int tt = 0; tt < JG_ST; ++tt) {
+      const int k = 4 * ks + (lane >> 4);
+      bm[ks][tt] = k < prob::JAC_NMONO ? mv[(size_t)k * G + 16 * tt + (lane & 15)] : 0.0f;
+    }
+  long long sc[JG_ST];
+#pragma unroll
+  for (int tt = 0; tt < JG_ST; ++tt) sc[tt] = sst[16 * tt + (lane & 15)];
+  for (int rt = w; rt < prob::JG_RT; rt += nw) {
+    const float* A = prob::jg_coef + (size_t)rt * prob::JG_KS * WAVE + lane;
+    float a[prob::JG_KS];
+#pragma unroll
+    for (int ks = 0; ks < prob::JG_KS; ++ks) a[ks] = A[ks * WAVE];
+    jg_f4 acc[JG_ST];
+#pragma unroll
+    for (int tt = 0; tt < JG_ST; ++tt) acc[tt] = jg_f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < prob::JG_KS; ++ks)
+#pragma unroll
+      for (int tt = 0; tt < JG_ST; ++tt) acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], bm[ks][tt], acc[tt], 0, 0, 0);
+    // value v of lane l: row 16 rt + 4 (l / 16) + v, stage 16 tt + l % 16
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int d = prob::jg_dest[16 * rt + 4 * (lane >> 4) + v];
+      if (d < 0) continue;
+#pragma unroll
+      for (int tt = 0; tt < JG_ST; ++tt)
+        if (sc[tt] >= 0) jac[(size_t)sc[tt] * JSTAGE + d] = acc[tt][v];
+    }
+  }
+}
+#endif  // COLLOC_JAC_GEMM
+
+#if COLLOC_TABLE_FORM
 constexpr int TAB_MAX2(int a, int b) { return a > b ? a : b; }
 constexpr int TAB_FG_LDS = (prob::FG_NATOMS + prob::FG_NMONO + NY + NZ) * TAB_G;
 constexpr int TAB_R_LDS = (TAB_MAX2(prob::RI_NATOMS + prob::RI_NMONO, prob::RF_NATOMS + prob::RF_NMONO)) * TAB_G;
@@ -481,5 +572,6 @@ __device__ __forceinline__ void segres_tab(int nint, Batch bt, const double* __r
   residual[nd] = val;
   atomicMax(maxres + ib, (unsigned long long)__double_as_longlong(fabs(val)));
 }
+#endif  // COLLOC_TABLE_FORM
 
 }  // namespace colloc

@@ -620,17 +620,49 @@ def _unit_tail(ny, nz, np_, nc, m, cw, fw, pact, tables):
     return parts
 
 
-def build_unit(sym, m, tables):
+def sym_static_mask(sym):
+    """1 for the stage-block entries of [Df; Dg] that depend only on the
+    problem's compile-time constants (emit_problem's static entries)."""
+    cset = set(c for c in sym.constants if c not in sym.instance_constants)
+    block = [e for row in sym.Df_exprs for e in row] + [e for row in sym.Dg_exprs for e in row]
+    return [1 if not (sp.sympify(e).free_symbols - cset) else 0 for e in block]
+
+
+def jac_gemm_section(sym, poly):
+    """The matrix-core Jacobian's tables for a straight-line unit (the jac
+    program of the problem's polynomial form, its coefficient matrix), or
+    None when the polynomial form's state-dependent entries are not exactly
+    the straight-line form's (then the straight-line Jacobian kernel is the
+    only one)."""
+    if poly is None:
+        return None
+    progs, mask, _ = table_programs(poly)
+    if list(mask) != sym_static_mask(sym) or progs["jac"].nout == 0:
+        return None
+    return "\n".join(["namespace prob {",
+                      "constexpr int TB_NB = %d;   // base atoms: y | z | p | instance constants | alpha" % poly.nb,
+                      emit_program("jac", progs["jac"]), emit_jac_gemm(progs["jac"]),
+                      "}  // namespace prob"])
+
+
+def build_unit(sym, m, tables, poly=None):
     """Complete hiprtc translation unit: callbacks, sizes, tables, kernels.
 
     tables: object with lobatto A, b, c and gauss t, w plus the Lagrange basis
-    L(tau) and its integral I(tau) at the Gauss points and L(0), L(1)."""
+    L(tau) and its integral I(tau) at the Gauss points and L(0), L(1).
+    poly: the problem's polyform.BvpDaePoly, if it has one: its Jacobian
+    program then also evaluates the float32 stage Jacobians on matrix cores
+    (COLLOC_JAC_GEMM)."""
     ny, nz, np_ = sym.size_y, sym.size_z, sym.size_p
     nc = len(sym.instance_constants)
     pact = active_p_columns(sym)
     cw, fw = choose_waves(ny, nz, np_, m, npa=len(pact))
-    parts = ["#define COLLOC_TABLE_FORM 0", "#define COLLOC_LARGE 0", '#include "colloc_common.h"',
+    gemm = jac_gemm_section(sym, poly)
+    parts = ["#define COLLOC_TABLE_FORM 0", "#define COLLOC_LARGE 0",
+             "#define COLLOC_JAC_GEMM %d" % (1 if gemm else 0), '#include "colloc_common.h"',
              emit_problem(sym)]
+    if gemm:
+        parts.append(gemm)
     parts += _unit_tail(ny, nz, np_, nc, m, cw, fw, pact, tables)
     return "\n".join(parts) + "\n"
 
@@ -679,6 +711,51 @@ def emit_program(name, prog):
                % ((name, nb, prog.natoms, prog.nmono, prog.nlev, prog.nslab, prog.out_slab_first) + (name,) * 11))
     out.append("constexpr int %s_NATOMS = %d, %s_NMONO = %d;" % (name.upper(), prog.natoms, name.upper(),
                                                                 prog.nmono))
+    return "\n".join(out)
+
+
+JG_POINTS = 32   # stages per workgroup of the matrix-core Jacobian
+
+
+def jac_gemm_matrix(prog):
+    """The state-dependent stage-Jacobian entries of a jac program as a
+    dense product: entry = sum_k C[row, k] mv[k] over the program's
+    monomials mv.  Returns (C (rows padded to 16, columns to 4), dest per
+    row (-1: padding)), rows ordered by destination."""
+    import numpy as np
+    from . import polyform
+    dests = sorted({r for s in range(prog.out_slab_first, prog.nslab) for r in prog.slab_rows[s] if r >= 0})
+    index = {r: i for i, r in enumerate(dests)}
+    nr = (len(dests) + 15) // 16 * 16
+    nk = (prog.nmono + 3) // 4 * 4
+    C = np.zeros((max(nr, 16), max(nk, 4)))
+    for s in range(prog.out_slab_first, prog.nslab):
+        t0, t1 = prog.slab_term_off[s], prog.slab_term_off[s + 1]
+        c = np.asarray(prog.coef[t0:t1]).reshape(-1, polyform.SLAB)
+        m = np.asarray(prog.mono[t0:t1]).reshape(-1, polyform.SLAB)
+        for q, r in enumerate(prog.slab_rows[s]):
+            if r >= 0:
+                np.add.at(C[index[r]], m[:, q], c[:, q])
+    return C, dests + [-1] * (C.shape[0] - len(dests))
+
+
+def emit_jac_gemm(prog):
+    """Tables of the matrix-core evaluation of the float32 stage Jacobians
+    (colloc_tables.h jacobian_gemm): C as v_mfma_f32_16x16x4f32 A operands
+    (row tile, k-step, lane: row 16 rt + l % 16, column 4 ks + l / 16) and
+    the destination of each row in the stage block."""
+    import numpy as np
+    C, dest = jac_gemm_matrix(prog)
+    rt, ks = C.shape[0] // 16, C.shape[1] // 4
+    lane = np.arange(64)
+    A = np.empty((rt, ks, 64))
+    for r in range(rt):
+        for k in range(ks):
+            A[r, k] = C[16 * r + (lane % 16), 4 * k + lane // 16]
+    out = ["constexpr int JG_RT = %d, JG_KS = %d, JG_POINTS = %d;   // row tiles, k-steps, stages per workgroup"
+           % (rt, ks, JG_POINTS),
+           _table("__device__ const float jg_coef", [repr(float(np.float32(v))) + "f" for v in A.ravel()], 8),
+           _int_table("__device__ const int jg_dest", dest)]
     return "\n".join(out)
 
 
@@ -747,7 +824,7 @@ def build_unit_tables(P, m, tables):
     pact = P.active_p_columns()
     large = ny > LARGE_NY
     cw, fw = (0, 0) if large else choose_waves(ny, nz, np_, m, npa=len(pact))
-    parts = ["#define COLLOC_TABLE_FORM 1", "#define COLLOC_LARGE %d" % (1 if large else 0),
+    parts = ["#define COLLOC_TABLE_FORM 1", "#define COLLOC_LARGE %d" % (1 if large else 0), "#define COLLOC_JAC_GEMM 0",
              '#include "colloc_common.h"', emit_problem_tables(P)]
     parts += _unit_tail(ny, nz, np_, nc, m, cw, fw, pact, tables)
     return "\n".join(parts) + "\n"

@@ -102,7 +102,11 @@ def code_object(ocp_text, m, verbose=False, form="auto"):
         active = P.active_p_columns()
     else:
         sym = emitter.BvpDaeSymbolic(ocp)
-        unit = emitter.build_unit(sym, m, tables.collocation_tables(m))
+        try:   # the polynomial form's Jacobian program for the matrix-core float32 Jacobian
+            poly = P or polyform.BvpDaePoly(ocp)
+        except polyform.Unsupported:
+            poly = None
+        unit = emitter.build_unit(sym, m, tables.collocation_tables(m), poly=poly)
         active = emitter.active_p_columns(sym)
     code, log = native.jit_compile(unit, "colloc_%s.hip" % key, build.KERNEL_DIR)
     if verbose and log.strip():

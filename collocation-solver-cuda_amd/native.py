@@ -81,7 +81,7 @@ class ColloError(RuntimeError):
         self.code = code
 
 
-CS_LAUNCH_CONDENSE_GRID, CS_LAUNCH_RESIDUAL_BLOCK, CS_LAUNCH_COMPUTE_UNITS = 0, 1, 2
+CS_LAUNCH_CONDENSE_GRID, CS_LAUNCH_RESIDUAL_BLOCK, CS_LAUNCH_COMPUTE_UNITS, CS_LAUNCH_JACOBIAN_GEMM = 0, 1, 2, 3
 
 _lib = None
 

@@ -243,8 +243,9 @@ int cs_get_direction(cs_handle* h, double* delta_y, double* delta_y_dot, double*
  * (solver:887-1001) to the same tolerance.  No counterpart in the reference. */
 int cs_set_condense_kernel(cs_handle* h, int kind);
 
-/* Launch shapes that do not change results (no counterpart in the
- * reference; chosen by the autotuner, autotune.py):
+/* Launch shapes and kernel choices (no counterpart in the reference; the
+ * first two chosen by the autotuner, autotune.py, and results identical
+ * for every value):
  *   CS_LAUNCH_CONDENSE_GRID: 0 (default) launches the series condensation
  *     with one workgroup per interval; n > 0 launches a persistent grid of
  *     n workgroups that take the intervals of the launch list from a work
@@ -252,10 +253,17 @@ int cs_set_condense_kernel(cs_handle* h, int kind);
  *     sized for every interval;
  *   CS_LAUNCH_RESIDUAL_BLOCK: workgroup size (64, 128, 256) of the residual
  *     kernel of straight-line callbacks (one thread per stage);
- *   CS_LAUNCH_COMPUTE_UNITS (get only): the device's compute units. */
+ *   CS_LAUNCH_COMPUTE_UNITS (get only): the device's compute units;
+ *   CS_LAUNCH_JACOBIAN_GEMM: 1 (default) evaluates the float32 stage
+ *     Jacobians (the FP32 direction) on matrix cores where the module has
+ *     that kernel (a straight-line unit whose problem also has the
+ *     polynomial form: the entries as a constant coefficient matrix times
+ *     the monomials of the stage), 0 the straight-line kernel; the two agree
+ *     to float32 rounding.  Get: 1 when the matrix-core kernel is in use. */
 #define CS_LAUNCH_CONDENSE_GRID 0
 #define CS_LAUNCH_RESIDUAL_BLOCK 1
 #define CS_LAUNCH_COMPUTE_UNITS 2
+#define CS_LAUNCH_JACOBIAN_GEMM 3
 int cs_set_launch(cs_handle* h, int which, int value);
 int cs_get_launch(cs_handle* h, int which, int* value);
 

@@ -192,3 +192,46 @@ def test_precision_switches_refill_static_jacobian_entries():
         assert zero.any() and np.all(Df[:, zero] == 0)
     assert S.status() == 0
     S.close()
+
+
+@pytest.mark.parametrize("nx,nu", [(20, 10), (4, 2)])
+def test_matrix_core_jacobian_equals_straight_line(nx, nu):
+    """The float32 stage Jacobians on matrix cores (the entries as the jac
+    program's coefficient matrix times the stage's monomials) against the
+    straight-line float32 kernel (itself pinned against the oracle by the
+    tests above): a batch of family members on uneven meshes, seeded
+    states.  The two agree within 2e-6 of the largest entry, with the same
+    zero pattern; Dr (straight-line code in both) is identical; the Newton
+    directions they give agree to the FP32 direction's 1e-4."""
+    from colloc_amd import native
+    insts = problems.batch_family(5, seed=1, nx=nx, nu=nu, nint_range=(40, 120))
+    bvp = solver.BvpDae(insts[0].ocp_text())
+    ts = [np.linspace(0.0, i.T, i.nodes) for i in insts]
+    Nt = sum(len(t) for t in ts)
+    S = solver.CollocationSolver(bvp, 5, n_max=Nt, batch=len(insts))
+    assert S.get_launch(native.CS_LAUNCH_JACOBIAN_GEMM) == 1
+    S.set_batch_mesh(ts)
+    S.set_batch_constants(np.array([i.constant_values() for i in insts]))
+    rng = np.random.default_rng(4)
+    nst = (Nt - len(ts)) * 5
+    S.set_state(rng.normal(scale=0.4, size=(Nt, S.ny)), rng.normal(scale=0.4, size=(nst, S.ny)),
+                rng.normal(scale=0.4, size=(nst, S.nz)), rng.normal(scale=0.4, size=(len(ts), S.np)))
+    S.set_direction_precision(32)
+    S.compute_f_q(0.1, outputs=False)
+    out = {}
+    for on in (1, 0):
+        S.set_launch(native.CS_LAUNCH_JACOBIAN_GEMM, on)
+        say("n_x=%d float32 Jacobian, matrix cores %d" % (nx, on))
+        Df, Dg, Dr = S.construct_jacobian(0.1)
+        S.reduce_jacobian(outputs=False)
+        S.solve_babd()
+        S.recover_delta_k()
+        out[on] = (Df, Dg, Dr) + tuple(S.get_direction())
+    for x, y in zip(out[1][:2], out[0][:2]):
+        assert rel_err(x, y) < 2e-6
+        np.testing.assert_array_equal(x == 0, y == 0)
+    np.testing.assert_array_equal(out[1][2], out[0][2])   # Dr: the same straight-line code
+    for x, y in zip(out[1][3:], out[0][3:]):
+        assert rel_err(x, y) < 1e-4
+    assert S.status() == 0
+    S.close()
