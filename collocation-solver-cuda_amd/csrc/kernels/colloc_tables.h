@@ -127,16 +127,17 @@ __device__ __forceinline__ void tab_base_rest(T* ext, int g, const double* z, co
 // ------------------------------------------- float32 Jacobian on matrix cores
 // For a straight-line unit whose problem also has the polynomial form: the
 // state-dependent entries of a stage block are C mv, C the constant matrix
-// of the jac program's coefficients over its monomials (prob::jg_coef, laid
-// out as v_mfma_f32_16x16x4f32 A operands: row tile, k-step, lane -> row
-// 16 rt + l % 16, column 4 ks + l / 16) and mv the monomials at the stage
-// (tab_prepare, in LDS).  Workgroup k < ceil(Nst / JG_POINTS) takes stages
-// JG_POINTS k .. + JG_POINTS - 1 (every monomial of every stage as B
-// operands in registers) and its waves take the row tiles of C in turn,
-// each row tile's accumulators stored to the rows' destinations
-// (prob::jg_dest) of every stage; the workgroups after them write Dr
-// (straight-line prob::Dr, one thread per instance).  The static entries
-// are the straight-line unit's (jacobian_static).
+// of the jac program's coefficients over its monomials and mv the
+// monomials at the stage (tab_prepare, in LDS).  C's nonzero 16 x 4 blocks
+// are stored row tile by row tile (prob::jg_blk_off, jg_blk_ks) as
+// v_mfma_f32_16x16x4f32 A operands (prob::jg_coef; lane l: row 16 rt +
+// l % 16, column 4 ks + l / 16).  Workgroup k < ceil(Nst / JG_POINTS)
+// takes stages JG_POINTS k .. + JG_POINTS - 1 and its waves take the row
+// tiles in turn, multiplying only the nonzero blocks (the monomials as B
+// operands from LDS), each row tile's accumulators stored to the rows'
+// destinations (prob::jg_dest) of every stage; the workgroups after them
+// write Dr (straight-line prob::Dr, one thread per instance).  The static
+// entries are the straight-line unit's (jacobian_static).
 constexpr int JG_ST = prob::JG_POINTS / 16;     // 16-stage tiles per workgroup
 static_assert(prob::JG_POINTS % 16 == 0, "stage tiles of 16");
 __device__ __forceinline__ void jacobian_gemm(int nint, Batch bt, const double* __restrict__ y,
@@ -176,29 +177,37 @@ __device__ __forceinline__ void jacobian_gemm(int nint, Batch bt, const double* 
   __syncthreads();
   tab_prepare<float, G>(prob::jac_prog(), ext, mv);   // ends with the monomials in LDS, synchronised
   const int lane = lane_id(), w = wave_id(), nw = blockDim.x / WAVE;
-  float bm[prob::JG_KS][JG_ST];
-#pragma unroll
-  for (int ks = 0; ks < prob::JG_KS; ++ks)
-#pragma unroll
-    for (int tt = 0; tt < JG_ST; ++tt) {
-      const int k = 4 * ks + (lane >> 4);
-      bm[ks][tt] = k < prob::JAC_NMONO ? mv[(size_t)k * G + 16 * tt + (lane & 15)] : 0.0f;
-    }
   long long sc[JG_ST];
 #pragma unroll
   for (int tt = 0; tt < JG_ST; ++tt) sc[tt] = sst[16 * tt + (lane & 15)];
   for (int rt = w; rt < prob::JG_RT; rt += nw) {
-    const float* A = prob::jg_coef + (size_t)rt * prob::JG_KS * WAVE + lane;
-    float a[prob::JG_KS];
+    // the row tile's nonzero blocks: A operands and k-steps loaded first,
+    // the B operands (monomials 4 ks .. 4 ks + 3 at the 16 stages of a tile)
+    // from LDS
+    const int j0 = prob::jg_blk_off[rt], nb = prob::jg_blk_off[rt + 1] - j0;
+    float a[prob::JG_KMAX];
+    int kq[prob::JG_KMAX];
 #pragma unroll
-    for (int ks = 0; ks < prob::JG_KS; ++ks) a[ks] = A[ks * WAVE];
+    for (int q = 0; q < prob::JG_KMAX; ++q) {
+      if (q < nb) {
+        a[q] = prob::jg_coef[(size_t)(j0 + q) * WAVE + lane];
+        kq[q] = prob::jg_blk_ks[j0 + q];
+      }
+    }
     jg_f4 acc[JG_ST];
 #pragma unroll
     for (int tt = 0; tt < JG_ST; ++tt) acc[tt] = jg_f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int ks = 0; ks < prob::JG_KS; ++ks)
+    for (int q = 0; q < prob::JG_KMAX; ++q) {
+      if (q < nb) {
+        const int k = 4 * kq[q] + (lane >> 4);
 #pragma unroll
-      for (int tt = 0; tt < JG_ST; ++tt) acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], bm[ks][tt], acc[tt], 0, 0, 0);
+        for (int tt = 0; tt < JG_ST; ++tt) {
+          const float bv = k < prob::JAC_NMONO ? mv[(size_t)k * G + 16 * tt + (lane & 15)] : 0.0f;
+          acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], bv, acc[tt], 0, 0, 0);
+        }
+      }
+    }
     // value v of lane l: row 16 rt + 4 (l / 16) + v, stage 16 tt + l % 16
 #pragma unroll
     for (int v = 0; v < 4; ++v) {

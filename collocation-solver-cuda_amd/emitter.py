@@ -741,19 +741,29 @@ def jac_gemm_matrix(prog):
 
 def emit_jac_gemm(prog):
     """Tables of the matrix-core evaluation of the float32 stage Jacobians
-    (colloc_tables.h jacobian_gemm): C as v_mfma_f32_16x16x4f32 A operands
-    (row tile, k-step, lane: row 16 rt + l % 16, column 4 ks + l / 16) and
-    the destination of each row in the stage block."""
+    (colloc_tables.h jacobian_gemm): the nonzero 16 x 4 blocks of C, row
+    tile by row tile, as v_mfma_f32_16x16x4f32 A operands (lane l: row
+    16 rt + l % 16, column 4 ks + l / 16), each block's k-step, and the
+    destination of each row in the stage block."""
     import numpy as np
     C, dest = jac_gemm_matrix(prog)
     rt, ks = C.shape[0] // 16, C.shape[1] // 4
     lane = np.arange(64)
-    A = np.empty((rt, ks, 64))
+    off, kstep, A = [0], [], []
     for r in range(rt):
         for k in range(ks):
-            A[r, k] = C[16 * r + (lane % 16), 4 * k + lane // 16]
+            blk = C[16 * r:16 * r + 16, 4 * k:4 * k + 4]
+            if np.any(blk != 0):
+                kstep.append(k)
+                A.append(C[16 * r + (lane % 16), 4 * k + lane // 16])
+        off.append(len(kstep))
+    A = np.array(A) if A else np.zeros((1, 64))
     out = ["constexpr int JG_RT = %d, JG_KS = %d, JG_POINTS = %d;   // row tiles, k-steps, stages per workgroup"
            % (rt, ks, JG_POINTS),
+           "constexpr int JG_KMAX = %d;   // nonzero blocks of a row tile, at most" %
+           max(off[r + 1] - off[r] for r in range(rt)),
+           _int_table("__device__ const int jg_blk_off", off),
+           _int_table("__device__ const int jg_blk_ks", kstep),
            _table("__device__ const float jg_coef", [repr(float(np.float32(v))) + "f" for v in A.ravel()], 8),
            _int_table("__device__ const int jg_dest", dest)]
     return "\n".join(out)
