@@ -64,6 +64,7 @@ struct NtParams {
 struct NtState {
   double* nrm;
   double* taken;
+  double* next;
   int* trials;
   int* done;
   int* gave;
@@ -258,6 +259,7 @@ struct cs_handle {
   int* dlist = nullptr;                 // the device-built launch list (intervals of the iterating instances)
   double* nt_nrm = nullptr;
   double* nt_taken = nullptr;
+  double* nt_next = nullptr;            // norm of each instance's accepted trial step
   int* nt_ints = nullptr;               // trials | done | gave | lofs (bmax each) | it | nlist
   unsigned long long* nt_cnt = nullptr; // [0] iterating, [1] searching, [2] listed intervals, [3] spare
   NtParams* nt_par = nullptr;           // the settings of the current solve (device)
@@ -268,7 +270,7 @@ struct cs_handle {
   size_t nt_ring_cap = 0;
   unsigned mesh_version = 0;            // bumped whenever the mesh or the BABD schedule changes
   std::vector<double> graph_key;        // what the captured graphs were made for
-  Graph g_dir, g_trial, g_acc;
+  Graph g_dir0, g_dir, g_trial, g_acc;
   bool capturing = false;               // a Newton graph is being captured
   Graph* cap_prof = nullptr;            // the graph being captured with profiling on
   unsigned long long* stamps = nullptr; // kStampSlots per graph, written by colloc_stamp inside the graphs
@@ -1004,10 +1006,11 @@ int cs_create(const cs_dims* d, cs_handle** out) {
   I(&h->dlist, nint);
   A(&h->nt_nrm, bm);
   A(&h->nt_taken, bm);
+  A(&h->nt_next, bm);
   I(&h->nt_ints, 4 * bm + 2);
   if (r == CS_OK) {
     hipError_t e = hipMalloc((void**)&h->nt_cnt, 4 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc((void**)&h->stamps, 3 * kStampSlots * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&h->stamps, 4 * kStampSlots * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&h->nt_par, sizeof(NtParams));
     int dev = 0, khz = 0;
     if (e == hipSuccess) e = hipGetDevice(&dev);
@@ -1018,6 +1021,7 @@ int cs_create(const cs_dims* d, cs_handle** out) {
     h->g_dir.base = 0;
     h->g_trial.base = kStampSlots;
     h->g_acc.base = 2 * kStampSlots;
+    h->g_dir0.base = 3 * kStampSlots;
   }
   if (r == CS_OK) {
     hipError_t e = hipMalloc((void**)&h->scal, (2 * bm + 4) * sizeof(unsigned long long));
@@ -1045,12 +1049,12 @@ int cs_create(const cs_dims* d, cs_handle** out) {
 void cs_destroy(cs_handle* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
-  for (Graph* g : {&h->g_dir, &h->g_trial, &h->g_acc})
+  for (Graph* g : {&h->g_dir0, &h->g_dir, &h->g_trial, &h->g_acc})
     if (g->exec) hipGraphExecDestroy(g->exec);
   if (h->stamps) hipFree(h->stamps);
   if (h->nt_par) hipFree(h->nt_par);
   double* bufs[] = {h->t, h->state, h->trial, h->dir, h->c, h->ytil, h->fa, h->fb, h->rbc, h->jac, h->Dr,
-                    h->rows0, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps, h->nt_nrm, h->nt_taken,
+                    h->rows0, h->rec, h->crrec, h->segres, h->zbuf, h->y0buf, h->steps, h->nt_nrm, h->nt_taken, h->nt_next,
                     h->nt_hist, h->nt_ring, h->l_ubuf, h->l_work, h->l_fin};
   if (h->l_slots) hipFree(h->l_slots);
   if (h->l_active) hipFree(h->l_active);
@@ -1076,7 +1080,7 @@ void cs_destroy(cs_handle* h) {
 // the moment: they go with the module.
 static void drop_graphs(cs_handle* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
-  for (Graph* g : {&h->g_dir, &h->g_trial, &h->g_acc}) {
+  for (Graph* g : {&h->g_dir0, &h->g_dir, &h->g_trial, &h->g_acc}) {
     if (g->exec) hipGraphExecDestroy(g->exec);
     g->exec = nullptr;
     g->prof.clear();
@@ -1369,7 +1373,7 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
   if (h->use_graph) return newton_graph(h, alpha, tol, max_iter, iter_time, norm, hist, hist_len, patience, min_ratio,
                                         window);
   const int B = h->B;
-  std::vector<double> nrm(B, 0.0), nn(B, 0.0), step(B, 0.0), taken(B, 0.0);
+  std::vector<double> nrm(B, 0.0), nn(B, 0.0), nxt(B, 0.0), step(B, 0.0), taken(B, 0.0);
   // the last window + 1 norms of each instance (ring), for the progress test
   const int ring = patience > 0 ? window + 1 : 1;
   std::vector<double> past((size_t)B * ring, 0.0);
@@ -1377,10 +1381,19 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
   std::vector<int> active(B, 1), search(B, 0), done(B, -1);
   LaunchList lst(h);
   for (int it = 0; it < max_iter; ++it) {
-    TRY(upload_ints(h, h->act, active));
-    TRY(lst.set(active));
+    // The residual of the iterate: evaluated at the start; later iterates
+    // are the accepted trial states, whose residual (norm, and the stage
+    // states, f_a, f_b and r_bc the next direction reads) the line search
+    // evaluated with the same kernel and the same inputs (the accepted
+    // update and the trial step are one axpy with the same step).
     std::vector<double> cur(B, 0.0);
-    TRY(residual(h, h->state, alpha, true, h->act, cur.data()));
+    if (it == 0) {
+      TRY(upload_ints(h, h->act, active));
+      TRY(lst.set(active));
+      TRY(residual(h, h->state, alpha, true, h->act, cur.data()));
+    } else {
+      cur = nxt;
+    }
     int nact = 0;
     for (int b = 0; b < B; ++b) {
       if (!active[b]) continue;
@@ -1420,12 +1433,13 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
       TRY(lst.search(search));
       TRY(upload_doubles(h, h->steps, step));
       TRY(axpy_state(h, h->state, h->dir, h->trial));
-      TRY(residual(h, h->trial, alpha, false, h->srch, nn.data()));
+      TRY(residual(h, h->trial, alpha, true, h->srch, nn.data()));
       int left = 0;
       for (int b = 0; b < B; ++b) {
         if (!search[b]) continue;
         if (nn[b] < nrm[b]) {
           taken[b] = step[b];
+          nxt[b] = nn[b];
           any_taken = true;
           search[b] = 0;
           step[b] = 0.0;
@@ -1460,9 +1474,10 @@ int newton(cs_handle* h, double alpha, double tol, int max_iter, int* iter_time,
 // per-instance decision made by kernels (colloc_nt_*) and the launch list
 // of the iterating instances' intervals built on the device; the work of
 // an iteration is captured once per mesh and settings as three HIP graphs:
-//   direction: residual of the iterate, the convergence / progress check,
-//     the launch list, Jacobian, condensation, BABD, recovery, and the
-//     first trial step of the line search;
+//   direction: residual of the iterate (first iteration; the later ones
+//     start from the accepted trial step's residual, as newton() does), the
+//     convergence / progress check, the launch list, Jacobian, condensation,
+//     BABD, recovery, and the first trial step of the line search;
 //   trial: one more trial step (replayed while instances are searching);
 //   accept: the accepted steps added to the state, the iteration counter.
 // The host reads three counters per replay of the direction graph (and per
@@ -1473,6 +1488,7 @@ NtState nt_state(cs_handle* h, double tol, int max_iter, int patience, double mi
   NtState st;
   st.nrm = h->nt_nrm;
   st.taken = h->nt_taken;
+  st.next = h->nt_next;
   st.trials = h->nt_ints;
   st.done = h->nt_ints + bm;
   st.gave = h->nt_ints + 2 * bm;
@@ -1510,7 +1526,7 @@ int nt_list(cs_handle* h, const NtState& st) {
 // decision (the still-searching count in cnt[1]).
 int nt_trial(cs_handle* h, const NtState& st, double alpha) {
   TRY(axpy_state(h, h->state, h->dir, h->trial));
-  TRY(residual_to(h, h->trial, alpha, false, h->srch, h->scal));
+  TRY(residual_to(h, h->trial, alpha, true, h->srch, h->scal));
   TRY(fill_u64(h, h->nt_cnt + 1, 1));
   NtState s2 = st;
   int B = h->B;
@@ -1616,22 +1632,25 @@ int newton_graph(cs_handle* h, double alpha, double tol, int max_iter, int* iter
   } unlist{h};
   if (key != h->graph_key) {
     h->graph_key.clear();
-    TRY(nt_capture(h, h->g_dir, [&]() -> int {
-      TRY(fill_u64(h, h->nt_cnt, 4));
-      TRY(residual_to(h, h->state, alpha, true, h->act, h->scal));
-      int Bv = B;
-      NtState s2 = st;
-      unsigned long long* slot = h->scal;
-      void* a[] = {&Bv, &s2, &slot, &h->act, &h->srch, &h->steps};
-      TRY(launch(h, F_NT_CHECK, (unsigned)((B + 255) / 256), 256, a));
-      TRY(nt_list(h, st));
-      TRY(jacobian(h, alpha, h->act));
-      TRY(condense(h, h->act, 0));
-      TRY(babd(h, h->act));
-      TRY(recover(h, h->act));
-      for (int r = 0; r < h->refine; ++r) TRY(refine_direction(h, h->act, 0));
-      return nt_trial(h, st, alpha);
-    }));
+    // the first iteration evaluates the residual of the iterate; the later
+    // ones start from the accepted trial's (see newton())
+    for (const bool first : {true, false})
+      TRY(nt_capture(h, first ? h->g_dir0 : h->g_dir, [&]() -> int {
+        TRY(fill_u64(h, h->nt_cnt, 4));
+        if (first) TRY(residual_to(h, h->state, alpha, true, h->act, h->scal));
+        int Bv = B;
+        NtState s2 = st;
+        unsigned long long* slot = first ? h->scal : nullptr;
+        void* a[] = {&Bv, &s2, &slot, &h->act, &h->srch, &h->steps};
+        TRY(launch(h, F_NT_CHECK, (unsigned)((B + 255) / 256), 256, a));
+        TRY(nt_list(h, st));
+        TRY(jacobian(h, alpha, h->act));
+        TRY(condense(h, h->act, 0));
+        TRY(babd(h, h->act));
+        TRY(recover(h, h->act));
+        for (int r = 0; r < h->refine; ++r) TRY(refine_direction(h, h->act, 0));
+        return nt_trial(h, st, alpha);
+      }));
     TRY(nt_capture(h, h->g_trial, [&]() -> int { return nt_trial(h, st, alpha); }));
     TRY(nt_capture(h, h->g_acc, [&]() -> int {
       int Bv = B;
@@ -1646,7 +1665,7 @@ int newton_graph(cs_handle* h, double alpha, double tol, int max_iter, int* iter
   }
   unsigned long long cnt[3] = {0, 0, 0};
   for (int it = 0; it < max_iter; ++it) {
-    TRY(nt_replay(h, h->g_dir, cnt));
+    TRY(nt_replay(h, it == 0 ? h->g_dir0 : h->g_dir, cnt));
     if (cnt[0] == 0) break;
     h->cond_intervals += cnt[2] * (1ull + (unsigned long long)h->refine);
     while (cnt[1] > 0) TRY(nt_replay(h, h->g_trial, cnt));

@@ -2200,6 +2200,7 @@ struct NtParams {
 struct NtState {
   double* nrm;
   double* taken;
+  double* next;                  // norm of the accepted trial step (the next iterate's)
   int* trials;
   int* done;
   int* gave;
@@ -2212,10 +2213,11 @@ struct NtState {
   const NtParams* par;
 };
 
-// After the residual of the iterate (norms in slot[b]): record it, stop the
-// instances that converged (||f_q|| < tol, before the step, solver:116-120)
-// or that the progress test stops, and start the line search of the others
-// at step 1.
+// After the residual of the iterate (norms in slot[b]; slot null: the
+// iterate is the accepted trial of the previous iteration, whose norm the
+// line search kept in next[b]): record it, stop the instances that converged
+// (||f_q|| < tol, before the step, solver:116-120) or that the progress test
+// stops, and start the line search of the others at step 1.
 __device__ __forceinline__ void nt_check(int B, NtState st, const unsigned long long* __restrict__ slot, int* act,
                                          int* srch, double* steps) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2227,7 +2229,7 @@ __device__ __forceinline__ void nt_check(int B, NtState st, const unsigned long 
   }
   const int it = *st.it;
   const NtParams P = *st.par;
-  const double cur = __longlong_as_double((long long)slot[b]);
+  const double cur = slot ? __longlong_as_double((long long)slot[b]) : st.next[b];
   st.nrm[b] = cur;
   if (it < P.hl) st.hist[(size_t)b * P.hl + it] = cur;
   const int rs = P.patience > 0 ? P.window + 1 : 1;
@@ -2300,6 +2302,7 @@ __device__ __forceinline__ void nt_line_search(int B, NtState st, const unsigned
   const double cur = __longlong_as_double((long long)slot[b]);
   if (cur < st.nrm[b]) {
     st.taken[b] = steps[b];
+    st.next[b] = cur;
     srch[b] = 0;
     steps[b] = 0.0;
     return;
