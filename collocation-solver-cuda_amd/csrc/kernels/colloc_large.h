@@ -892,6 +892,7 @@ __device__ __forceinline__ void l_cr_pair(const int* __restrict__ oinst, const C
 // L_FIN_THREADS per instance; status bit 4 on a zero pivot.  TI: stored type
 // of the last level's rows.
 constexpr int L_PB = 16;                              // panel width
+constexpr int L_PP = L_PB + 1;                        // panel row stride in LDS (odd: a column's rows in distinct banks)
 constexpr int L_FC = NFIN + 1;                        // columns of the augmented system
 template <class TI>
 __device__ __forceinline__ void l_cr_final(Batch bt, const TI* __restrict__ rows, const double* __restrict__ Dr,
@@ -900,12 +901,14 @@ __device__ __forceinline__ void l_cr_final(Batch bt, const TI* __restrict__ rows
                                            double* __restrict__ work) {
   constexpr int FC = L_FC;
   constexpr int DRW = 2 * NY + NP;
-  __shared__ double P[NFIN * L_PB];        // the panel (rows c0 .. NFIN-1 used), then L21
+  __shared__ double P[NFIN * L_PP];        // the panel (rows c0 .. NFIN-1 used), then L21
   __shared__ double S[L_PB * FC];          // the panel rows of the columns after the panel (U12)
   __shared__ double xs[NFIN];
   __shared__ double rv[L_FIN_THREADS / WAVE];
   __shared__ int ri[L_FIN_THREADS / WAVE];
   __shared__ int piv[L_PB];
+  __shared__ int trow[2 * L_PB], tsrc[2 * L_PB];
+  __shared__ int ntouch;
   const int ib = blockIdx.x;
   if (skipped(bt, ib)) return;
   const TI* row = rows + (size_t)ib * NY * CRW;
@@ -937,7 +940,7 @@ __device__ __forceinline__ void l_cr_final(Batch bt, const TI* __restrict__ rows
     const int ce = c0 + nb;   // first column after the panel
     for (int e = tid; e < (NFIN - c0) * L_PB; e += nthr) {
       const int r = c0 + e / L_PB, j = e - (e / L_PB) * L_PB;
-      P[r * L_PB + j] = j < nb ? F[(size_t)r * FC + c0 + j] : 0.0;
+      P[r * L_PP + j] = j < nb ? F[(size_t)r * FC + c0 + j] : 0.0;
     }
     __syncthreads();
     // unblocked LU of the panel
@@ -946,7 +949,7 @@ __device__ __forceinline__ void l_cr_final(Batch bt, const TI* __restrict__ rows
       double best = -1.0;
       int pr = NFIN;
       for (int r = k + tid; r < NFIN; r += nthr) {
-        const double a = fabs(P[r * L_PB + j]);
+        const double a = fabs(P[r * L_PP + j]);
         if (a > best || (a == best && r < pr)) { best = a; pr = r; }
       }
       for (int off = 32; off > 0; off >>= 1) {
@@ -964,51 +967,81 @@ __device__ __forceinline__ void l_cr_final(Batch bt, const TI* __restrict__ rows
       __syncthreads();
       pr = piv[j];
       if (pr != k && tid < L_PB) {
-        const double t0 = P[k * L_PB + tid];
-        P[k * L_PB + tid] = P[pr * L_PB + tid];
-        P[pr * L_PB + tid] = t0;
+        const double t0 = P[k * L_PP + tid];
+        P[k * L_PP + tid] = P[pr * L_PP + tid];
+        P[pr * L_PP + tid] = t0;
       }
       __syncthreads();
-      const double pv = P[k * L_PB + j];
+      const double pv = P[k * L_PP + j];
       if (!(fabs(pv) > 0.0)) bad = 1;
       const double inv = 1.0 / pv;
       for (int e = tid; e < (NFIN - k - 1) * L_PB; e += nthr) {
         const int r = k + 1 + e / L_PB, jj = e - (e / L_PB) * L_PB;
         if (jj <= j || jj >= nb) continue;
         // multiplier of row r (recomputed per entry: the column is read before it is scaled)
-        P[r * L_PB + jj] = fma(-(P[r * L_PB + j] * inv), P[k * L_PB + jj], P[r * L_PB + jj]);
+        P[r * L_PP + jj] = fma(-(P[r * L_PP + j] * inv), P[k * L_PP + jj], P[r * L_PP + jj]);
       }
       __syncthreads();
-      for (int r = k + 1 + tid; r < NFIN; r += nthr) P[r * L_PB + j] *= inv;
+      for (int r = k + 1 + tid; r < NFIN; r += nthr) P[r * L_PP + j] *= inv;
       __syncthreads();
     }
-    // the panel's row exchanges on the columns after it, in order
-    for (int j = 0; j < nb; ++j) {
-      const int k = c0 + j, pr = piv[j];
-      if (pr != k)
-        for (int c = ce + tid; c < FC; c += nthr) {
-          const double t0 = F[(size_t)k * FC + c];
-          F[(size_t)k * FC + c] = F[(size_t)pr * FC + c];
-          F[(size_t)pr * FC + c] = t0;
+    // the panel's row exchanges on the columns after it, composed into one
+    // permutation of the rows they touch (the panel's rows, then each pivot
+    // row below the panel once): row trow[t] receives row tsrc[t]'s value
+    if (tid == 0) {
+      int n = nb;
+      for (int j = 0; j < nb; ++j) {
+        trow[j] = c0 + j;
+        tsrc[j] = c0 + j;
+      }
+      for (int j = 0; j < nb; ++j) {
+        const int pr = piv[j];   // >= c0 + j
+        int a = pr - c0;
+        if (pr >= ce) {
+          a = -1;
+          for (int t = nb; t < n; ++t)
+            if (trow[t] == pr) a = t;
+          if (a < 0) {
+            a = n++;
+            trow[a] = pr;
+            tsrc[a] = pr;
+          }
         }
-      __syncthreads();
+        const int t0 = tsrc[j];
+        tsrc[j] = tsrc[a];
+        tsrc[a] = t0;
+      }
+      ntouch = n;
     }
-    // U11 back to F (the back substitution reads it); U12 = L11^-1 F[c0:ce, ce:]
+    // U11 back to F (the back substitution reads it)
     for (int e = tid; e < nb * L_PB; e += nthr) {
       const int j = e / L_PB, jj = e - (e / L_PB) * L_PB;
-      if (jj >= j && jj < nb) F[(size_t)(c0 + j) * FC + c0 + jj] = P[(c0 + j) * L_PB + jj];
+      if (jj >= j && jj < nb) F[(size_t)(c0 + j) * FC + c0 + jj] = P[(c0 + j) * L_PP + jj];
     }
+    __syncthreads();
+    // per column c after the panel: the touched rows read once, U12 =
+    // L11^-1 F[c0:ce, c] by forward substitution in registers, the rows
+    // below the panel that the exchanges touched written back
+    const int nt = ntouch;
     for (int c = ce + tid; c < FC; c += nthr) {
-      // column c of U12 by forward substitution with the unit lower L11, in LDS
+      double v[2 * L_PB];
+#pragma unroll
+      for (int t = 0; t < 2 * L_PB; ++t) v[t] = t < nt ? F[(size_t)tsrc[t] * FC + c] : 0.0;
+#pragma unroll
       for (int j = 0; j < L_PB; ++j) {
-        double v = 0.0;
+        double u = 0.0;
         if (j < nb) {
-          v = F[(size_t)(c0 + j) * FC + c];
-          for (int q = 0; q < j; ++q) v = fma(-P[(c0 + j) * L_PB + q], S[q * FC + c], v);
-          F[(size_t)(c0 + j) * FC + c] = v;
+          u = v[j];
+#pragma unroll
+          for (int q = 0; q < j; ++q) u = fma(-P[(c0 + j) * L_PP + q], v[q], u);
+          v[j] = u;
+          F[(size_t)(c0 + j) * FC + c] = u;
         }
-        S[j * FC + c] = v;
+        S[j * FC + c] = u;
       }
+#pragma unroll
+      for (int t = 0; t < 2 * L_PB; ++t)
+        if (t >= nb && t < nt) F[(size_t)trow[t] * FC + c] = v[t];
     }
     __syncthreads();
     // trailing update F[ce:, ce:] -= L21 U12, 4 x 4 register tiles
@@ -1018,16 +1051,19 @@ __device__ __forceinline__ void l_cr_final(Batch bt, const TI* __restrict__ rows
     for (int t = tid; t < tr * tc; t += nthr) {
       const int rt = t / tc, ct = t - (t / tc) * tc;
       const int r0 = ce + rt * TR, cb = ce + ct * TCL;
-      double acc[TR][TCL];
+      double acc[TR][TCL], fv[TR][TCL];
 #pragma unroll
       for (int a = 0; a < TR; ++a)
 #pragma unroll
-        for (int q = 0; q < TCL; ++q) acc[a][q] = 0.0;
+        for (int q = 0; q < TCL; ++q) {   // the tile's loads in flight during the products
+          fv[a][q] = r0 + a < NFIN && cb + q < FC ? F[(size_t)(r0 + a) * FC + cb + q] : 0.0;
+          acc[a][q] = 0.0;
+        }
 #pragma nounroll
       for (int j = 0; j < L_PB; ++j) {
         double l[TR], u[TCL];
 #pragma unroll
-        for (int a = 0; a < TR; ++a) l[a] = r0 + a < NFIN ? P[(r0 + a) * L_PB + j] : 0.0;
+        for (int a = 0; a < TR; ++a) l[a] = r0 + a < NFIN ? P[(r0 + a) * L_PP + j] : 0.0;
 #pragma unroll
         for (int q = 0; q < TCL; ++q) u[q] = cb + q < FC ? S[j * FC + cb + q] : 0.0;
 #pragma unroll
@@ -1039,7 +1075,7 @@ __device__ __forceinline__ void l_cr_final(Batch bt, const TI* __restrict__ rows
       for (int a = 0; a < TR; ++a)
 #pragma unroll
         for (int q = 0; q < TCL; ++q)
-          if (r0 + a < NFIN && cb + q < FC) F[(size_t)(r0 + a) * FC + cb + q] -= acc[a][q];
+          if (r0 + a < NFIN && cb + q < FC) F[(size_t)(r0 + a) * FC + cb + q] = fv[a][q] - acc[a][q];
     }
     __syncthreads();
   }
@@ -1048,20 +1084,32 @@ __device__ __forceinline__ void l_cr_final(Batch bt, const TI* __restrict__ rows
   __syncthreads();
   for (int b0 = ((NFIN - 1) / L_PB) * L_PB; b0 >= 0; b0 -= L_PB) {
     const int be = min(b0 + L_PB, NFIN);
-    if (tid < WAVE) {   // the diagonal block, one wave: lane j holds row b0 + j
+    if (tid < WAVE) {   // the diagonal block, one wave: lane j holds row b0 + j (read at once)
       const int r = b0 + tid;
+      double u[L_PB];
+#pragma unroll
+      for (int j = 0; j < L_PB; ++j) u[j] = r < be && b0 + j < be ? F[(size_t)r * FC + b0 + j] : 0.0;
       double v = r < be ? xs[r] : 0.0;
-      for (int c = be - 1; c >= b0; --c) {
-        const double xc = __shfl(v, c - b0) / F[(size_t)c * FC + c];
-        if (r == c) v = xc;
-        else if (r < c) v = fma(-F[(size_t)r * FC + c], xc, v);
+#pragma unroll
+      for (int j = L_PB - 1; j >= 0; --j) {
+        const int c = b0 + j;
+        if (c < be) {
+          const double xc = __shfl(v, j) / __shfl(u[j], j);   // lane j's diagonal entry
+          if (r == c) v = xc;
+          else if (r < c) v = fma(-u[j], xc, v);
+        }
       }
       if (r < be) xs[r] = v;
     }
     __syncthreads();
     for (int r = tid; r < b0; r += nthr) {
+      double u[L_PB];
+#pragma unroll
+      for (int j = 0; j < L_PB; ++j) u[j] = b0 + j < be ? F[(size_t)r * FC + b0 + j] : 0.0;
       double v = xs[r];
-      for (int c = b0; c < be; ++c) v = fma(-F[(size_t)r * FC + c], xs[c], v);
+#pragma unroll
+      for (int j = 0; j < L_PB; ++j)
+        if (b0 + j < be) v = fma(-u[j], xs[b0 + j], v);
       xs[r] = v;
     }
     __syncthreads();
