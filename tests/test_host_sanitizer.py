@@ -2,7 +2,7 @@
 race-detection / sanitizer row: AddressSanitizer and UndefinedBehaviorSanitizer
 on the host code, device code uninstrumented).
 
-build.build_library_sanitized compiles libcolloc's host code with both
+tests/host_check/build_asan.py compiles libcolloc's host code with both
 sanitizers; tests/host_check/asan_driver.cpp links it with the sanitizer
 runtimes and runs the calls that need no GPU: version and error text,
 argument and shape checks, two hiprtc compilations of a generated problem
@@ -14,19 +14,22 @@ Leak detection is off: the HIP runtime keeps allocations until exit.
 """
 import os
 import subprocess
+import sys
 
 import pytest
 
 from colloc_amd import build, emitter, ocp as ocpmod, problems, tables
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "host_check"))
+from build_asan import build_library_sanitized   # noqa: E402
 ROOT = os.path.dirname(HERE)
 CLANG = "/opt/rocm/llvm/bin/clang++"
 
 
 @pytest.mark.skipif(not os.path.exists(CLANG), reason="no ROCm clang")
 def test_host_code_under_address_and_ub_sanitizers(tmp_path):
-    lib = build.build_library_sanitized(str(tmp_path))
+    lib = build_library_sanitized(str(tmp_path))
     drv = str(tmp_path / "asan_driver")
     cmd = [CLANG, "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
            "-I", os.path.join(ROOT, "include"), os.path.join(HERE, "host_check", "asan_driver.cpp"),
