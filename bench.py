@@ -325,7 +325,26 @@ def batch_workload(args):
     # together hold what one handle of --chunk-nodes would
     nstreams = max(1, args.streams)
     chunk_nodes = args.chunk_nodes // nstreams
-    chunks = solver.plan_chunks(nodes, chunk_nodes, len(insts))
+
+    def balanced_chunks(nds, cap, bcap):
+        """Consecutive chunks within cap nodes and bcap instances: as many as
+        solver.plan_chunks makes, rounded up to a multiple of the handles,
+        split at even shares of the nodes so the handles finish together
+        (plan_chunks fills each chunk to the cap and leaves a short last one).
+        Each instance's result does not depend on its chunk."""
+        base = solver.plan_chunks(nds, cap, bcap)
+        k = -(-len(base) // nstreams) * nstreams
+        tot = float(sum(nds))
+        out = [[] for _ in range(k)]
+        run = 0
+        for i, n in enumerate(nds):
+            out[min(k - 1, int((run + 0.5 * n) * k / tot))].append(i)
+            run += n
+        out = [c for c in out if c]
+        if any(sum(nds[i] for i in c) > cap or len(c) > bcap for c in out):
+            return base
+        return out
+    chunks = balanced_chunks(nodes, chunk_nodes, len(insts))
     n_max = max(sum(nodes[i] for i in ch) for ch in chunks)
     b_max = max(len(ch) for ch in chunks)
     handles = [solver.CollocationSolver(bvp, m, n_max=n_max, batch=b_max) for _ in range(nstreams)]
@@ -394,7 +413,7 @@ def batch_workload(args):
             S_.set_batch_mesh([ts_all[i] for i in sel])
             S_.form_initial_input(np.ones((Nt, ny)), np.ones((Nt, nz)), np.ones((len(sel), np_)))
             return solver.newton_stages(S_, cons_all[sel], [h[sel] for h in hom], 0.1, tol, max_iter, stall=stall)
-        for ch, r in run_chunks(solver.plan_chunks([nodes[i] for i in idx], S.n_max, S.batch), work):
+        for ch, r in run_chunks(balanced_chunks([nodes[i] for i in idx], S.n_max, S.batch), work):
             conv[ch] = r["converged"]
             its[:, ch] = r["iterations"]
         return conv, its
